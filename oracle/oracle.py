@@ -1,0 +1,196 @@
+"""ctypes binding of the C oracle (test infrastructure only).
+
+Only the tests, the smoke check and the CPU-baseline leg of bench.py use this
+module; the product never imports it.  The shared library is built by
+``make -C oracle`` (``__graft_entry__.build()`` runs that step).
+"""
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+MAX_PRIM = 16
+
+SLATER, BECKE88, BECKE88_EXCEPT_LDA, VWN5, VWN3, LYP, B3LYP = range(7)
+FUNCTIONAL_IDS = {
+    "DfFunctional_Slater": SLATER,
+    "DfFunctional_Becke88": BECKE88,
+    "DfFunctional_Becke88_ExceptLDA": BECKE88_EXCEPT_LDA,
+    "DfFunctional_VWN": VWN5,
+    "DfFunctional_VWN3": VWN3,
+    "DfFunctional_LYP": LYP,
+    "DfFunctional_B3LYP": B3LYP,
+}
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        d, i64, i32 = ctypes.c_double, ctypes.c_int64, ctypes.c_int
+        P = ctypes.c_void_p
+        L.oracle_functional.argtypes = [i32, i64, P, P]
+        L.oracle_normalize_cgto.argtypes = [i32, i32, P, P, P, P]
+        L.oracle_eval_aos.argtypes = [P, i64, P, P, P, P, P]
+        L.oracle_calc_xc_rks.argtypes = [P, i32, i64, P, P, P, P, d, i32]
+        L.oracle_calc_xc_rks.restype = d
+        L.oracle_lebedev.argtypes = [i32, P, P]
+        L.oracle_lebedev.restype = i32
+        L.oracle_radial_em.argtypes = [d, i32, i32, P, P]
+        L.oracle_bragg_slater.argtypes = [i32]
+        L.oracle_bragg_slater.restype = d
+        L.oracle_o_matrix.argtypes = [i32, P, P, P]
+        L.oracle_becke_weight.argtypes = [i32, P, P, i32, P, i32]
+        L.oracle_becke_weight.restype = d
+        L.oracle_generate_grid.argtypes = [i32, P, P, i32, i32, P, d, P, P, i32]
+        L.oracle_generate_grid.restype = i64
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def functional(fid, inputs):
+    """inputs: (n, 5) rhoA, rhoB, gAA, gAB, gBB -> (n, 6) zk, vrhoA, vrhoB, vgAA, vgAB, vgBB."""
+    x = np.ascontiguousarray(inputs, dtype=np.float64).reshape(-1, 5)
+    out = np.zeros((x.shape[0], 6))
+    lib().oracle_functional(fid, x.shape[0], _p(x), _p(out))
+    return out
+
+
+def normalize_cgto(shell, exps, coefs):
+    e = np.ascontiguousarray(exps, dtype=np.float64)
+    c = np.ascontiguousarray(coefs, dtype=np.float64)
+    oe, oc = np.zeros_like(e), np.zeros_like(c)
+    lib().oracle_normalize_cgto(shell, len(e), _p(e), _p(c), _p(oe), _p(oc))
+    return oe, oc
+
+
+class _Aos(ctypes.Structure):
+    _fields_ = [("n_aos", ctypes.c_int), ("center", ctypes.c_void_p),
+                ("type", ctypes.c_void_p), ("nprim", ctypes.c_void_p),
+                ("exps", ctypes.c_void_p), ("coefs", ctypes.c_void_p)]
+
+
+class AoTable:
+    """Per-AO table (centre, basis type 0..8, normalised primitives)."""
+
+    def __init__(self, center, types, nprim, exps, coefs):
+        self.center = np.ascontiguousarray(center, dtype=np.float64).reshape(-1, 3)
+        self.type = np.ascontiguousarray(types, dtype=np.int32)
+        self.nprim = np.ascontiguousarray(nprim, dtype=np.int32)
+        n = len(self.type)
+        self.exps = np.zeros((n, MAX_PRIM))
+        self.coefs = np.zeros((n, MAX_PRIM))
+        for i in range(n):
+            k = self.nprim[i]
+            self.exps[i, :k] = exps[i][:k]
+            self.coefs[i, :k] = coefs[i][:k]
+        self.n = n
+        self._s = _Aos(n, _p(self.center).value, _p(self.type).value,
+                       _p(self.nprim).value, _p(self.exps).value, _p(self.coefs).value)
+
+    @classmethod
+    def from_basis(cls, basis):
+        """basis: object with per-AO arrays center/type/nprim/exps/coefs."""
+        return cls(basis.ao_center, basis.ao_type, basis.ao_nprim, basis.ao_exps,
+                   basis.ao_coefs)
+
+    def ref(self):
+        return ctypes.byref(self._s)
+
+
+def eval_aos(aos, xyz, grad=True):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    phi = np.zeros((n, aos.n))
+    if grad:
+        gx, gy, gz = (np.zeros((n, aos.n)) for _ in range(3))
+        lib().oracle_eval_aos(aos.ref(), n, _p(xyz), _p(phi), _p(gx), _p(gy), _p(gz))
+        return phi, gx, gy, gz
+    lib().oracle_eval_aos(aos.ref(), n, _p(xyz), _p(phi), None, None, None)
+    return phi
+
+
+def calc_xc_rks(aos, grid, PA, fid=B3LYP, density_cutoff=1e-16, n_threads=0,
+                want_rho=False):
+    """Returns (E_xc, F_alpha dense symmetric, rho (n,4) or None)."""
+    grid = np.ascontiguousarray(grid, dtype=np.float64).reshape(-1, 4)
+    PA = np.ascontiguousarray(PA, dtype=np.float64)
+    F = np.zeros_like(PA)
+    rho = np.zeros((grid.shape[0], 4)) if want_rho else None
+    e = lib().oracle_calc_xc_rks(aos.ref(), fid, grid.shape[0], _p(grid), _p(PA),
+                                 _p(F), _p(rho) if want_rho else None,
+                                 density_cutoff, n_threads)
+    F = np.tril(F)
+    F = F + np.tril(F, -1).T
+    return e, F, rho
+
+
+def lebedev(order):
+    xyz = np.zeros((order, 3))
+    w = np.zeros(order)
+    n = lib().oracle_lebedev(order, _p(xyz), _p(w))
+    if n != order:
+        raise ValueError(f"unsupported Lebedev order {order}")
+    return xyz, w
+
+
+def radial_em(R, Nr, i):
+    r, w = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_radial_em(R, Nr, i, ctypes.byref(r), ctypes.byref(w))
+    return r.value, w.value
+
+
+def bragg_slater(Z):
+    return lib().oracle_bragg_slater(Z)
+
+
+def o_matrix(Z, xyz):
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    O = np.zeros((3, 3))
+    lib().oracle_o_matrix(len(Z), _p(Z), _p(xyz), _p(O))
+    return O
+
+
+def becke_weight(Z, xyz, iatom, r, size_adjust=True):
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    return lib().oracle_becke_weight(len(Z), _p(Z), _p(xyz), iatom, _p(r), int(size_adjust))
+
+
+def generate_grid(Z, xyz, n_rad, n_ang, O=None, weight_cutoff=1e-16):
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    if O is None:
+        O = o_matrix(Z, xyz)
+    O = np.ascontiguousarray(O, dtype=np.float64)
+    L = lib()
+    n = L.oracle_generate_grid(len(Z), _p(Z), _p(xyz), n_rad, n_ang, _p(O),
+                               weight_cutoff, None, None, 0)
+    if n < 0:
+        raise ValueError("unsupported grid")
+    grid = np.zeros((n, 4))
+    atom = np.zeros(n, dtype=np.int32)
+    L.oracle_generate_grid(len(Z), _p(Z), _p(xyz), n_rad, n_ang, _p(O),
+                           weight_cutoff, _p(grid), _p(atom), 0)
+    return grid, atom
+
+
+os.environ.setdefault("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1)))

@@ -1,0 +1,1019 @@
+/*
+ * CPU oracle for the grid XC build -- test infrastructure only (see
+ * pdfxc_oracle.h).  Plain C99 + OpenMP.  Each routine restates the reference
+ * algorithm and cites the ProteinDF lines it follows.  Operation order is kept
+ * close to the reference where that is cheap, so that results agree to a few
+ * ulps rather than only to the physical tolerance.
+ */
+#include "pdfxc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Slater exchange: DfFunctional_Slater.cpp:25-34 (constants), :44-52 (e),    */
+/* :64-76 (derivative).  Slater alpha = 2/3.                                  */
+/* ------------------------------------------------------------------------ */
+static double slater_coef(void) {
+    return -(9.0 / 4.0) * pow((3.0 / (4.0 * M_PI)), (1.0 / 3.0)) * (2.0 / 3.0);
+}
+static double slater_dcoef(void) {
+    return -3.0 * pow((3.0 / (4.0 * M_PI)), (1.0 / 3.0)) * (2.0 / 3.0);
+}
+static double slater_e(double ra, double rb) {
+    return slater_coef() * (pow(ra, 4.0 / 3.0) + pow(rb, 4.0 / 3.0));
+}
+static void slater_d(double ra, double rb, double* va, double* vb) {
+    *va = slater_dcoef() * pow(ra, 1.0 / 3.0);
+    *vb = slater_dcoef() * pow(rb, 1.0 / 3.0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Becke 88: DfFunctional_Becke88.cpp:25-31 (constants), :40-57 (e), :71-108  */
+/* (derivatives), :137-163 (x, g, g'); the "except LDA" variant drops G_PARAM */
+/* from g (DfFunctional_Becke88_ExceptLDA.cpp:26-33).  asinh as log+sqrt      */
+/* (TlMath.h:57-59).                                                          */
+/* ------------------------------------------------------------------------ */
+#define B88_EPS 1.0e-16
+#define B88_B 0.0042
+static double b88_gparam(void) {
+    return -(3.0 / 2.0) * pow(3.0 / (4.0 * M_PI), (1.0 / 3.0));
+}
+static double tl_arcsinh(double x) { return log(x + sqrt(x * x + 1)); }
+static double b88_x(double rho, double gamma) {
+    return sqrt(gamma) / pow(rho, 4.0 / 3.0);
+}
+static double b88_g(double x, int with_lda) {
+    const double bx = B88_B * x;
+    const double ash = tl_arcsinh(x);
+    const double nonlocal = -((bx * x) / (1.0 + 6.0 * bx * ash));
+    return with_lda ? (b88_gparam() - ((bx * x) / (1.0 + 6.0 * bx * ash)))
+                    : nonlocal;
+}
+static double b88_gprime(double x) {
+    const double bx = B88_B * x;
+    const double xx = x * x;
+    const double ash = tl_arcsinh(x);
+    const double M = 1.0 + 6.0 * bx * ash;
+    const double invM = 1.0 / M;
+    const double arg1 = 6.0 * bx * bx * (ash + x / sqrt(xx + 1.0)) * invM * invM;
+    const double arg2 = -2.0 * bx * invM;
+    return arg1 + arg2;
+}
+static double b88_e(double ra, double rb, double gaa, double gbb, int with_lda) {
+    double ta = 0.0, tb = 0.0;
+    if (ra > B88_EPS) {
+        const double xa = b88_x(ra, gaa);
+        ta = pow(ra, 4.0 / 3.0) * b88_g(xa, with_lda);
+    }
+    if (rb > B88_EPS) {
+        const double xb = b88_x(rb, gbb);
+        tb = pow(rb, 4.0 / 3.0) * b88_g(xb, with_lda);
+    }
+    return ta + tb;
+}
+static void b88_d(double ra, double rb, double gaa, double gbb, int with_lda,
+                  double* vra, double* vrb, double* vgaa, double* vgab,
+                  double* vgbb) {
+    *vgab = 0.0;
+    *vra = 0.0;
+    *vgaa = 0.0;
+    if ((ra > B88_EPS) && (gaa > B88_EPS)) {
+        const double xa = b88_x(ra, gaa);
+        const double ga = b88_g(xa, with_lda);
+        const double gpa = b88_gprime(xa);
+        *vra = (4.0 / 3.0) * pow(ra, 1.0 / 3.0) * (ga - xa * gpa);
+        *vgaa = 0.5 * (1.0 / sqrt(gaa)) * gpa;
+    }
+    *vrb = 0.0;
+    *vgbb = 0.0;
+    if ((rb > B88_EPS) && (gbb > B88_EPS)) {
+        const double xb = b88_x(rb, gbb);
+        const double gb = b88_g(xb, with_lda);
+        const double gpb = b88_gprime(xb);
+        *vrb = (4.0 / 3.0) * pow(rb, 1.0 / 3.0) * (gb - xb * gpb);
+        *vgbb = 0.5 * (1.0 / sqrt(gbb)) * gpb;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* VWN correlation: DfFunctional_VWN.cpp:23-46 (parameters), :76-124 (e),     */
+/* :145-260 (eps_c, its x-derivative, g, h, g', h'), :262-346 (derivative).   */
+/* VWN3 replaces the para/ferro sets and keeps the VWN5 anti-ferro set        */
+/* (DfFunctional_VWN3.cpp:23-54).                                             */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    double A, b, c, x0;
+} vwn_set;
+static const vwn_set VWN5_PARA = {0.0310907, 3.72744, 12.9352, -0.10498};
+static const vwn_set VWN5_FERR = {0.01554535, 7.06042, 18.0578, -0.325};
+static const vwn_set VWN3_PARA = {0.0310907, 13.0720, 42.7198, -0.409286};
+static const vwn_set VWN3_FERR = {0.01554535, 20.1231, 101.578, -0.743294};
+static vwn_set vwn_anti(void) {
+    vwn_set s = {-1.0 / (6.0 * M_PI * M_PI), 1.13107, 13.0045, -0.0047584};
+    return s;
+}
+static double vwn_ec_coef(void) { return 4.0 / (9.0 * (pow(2.0, 1.0 / 3.0) - 1.0)); }
+
+static double vwn_eps(vwn_set s, double x) {
+    const double X = x * (x + s.b) + s.c;
+    const double invX = 1.0 / X;
+    const double X0 = s.x0 * (s.x0 + s.b) + s.c;
+    const double invX0 = 1.0 / X0;
+    const double Q = sqrt(4 * s.c - s.b * s.b);
+    const double invQ = 1.0 / Q;
+    const double at = atan(Q / (2.0 * x + s.b));
+    const double arg1 = log(x * x * invX);
+    const double arg2 = 2.0 * s.b * invQ * at;
+    const double xx0 = x - s.x0;
+    const double arg3_1 = log(xx0 * xx0 * invX);
+    const double arg3_2 = 2.0 * (2.0 * s.x0 + s.b) * invQ * at;
+    const double arg3 = -s.b * s.x0 * invX0 * (arg3_1 + arg3_2);
+    return s.A * (arg1 + arg2 + arg3);
+}
+static double vwn_eps_prime(vwn_set s, double x) {
+    const double X = x * (x + s.b) + s.c;
+    const double invX = 1.0 / X;
+    const double X0 = s.x0 * (s.x0 + s.b) + s.c;
+    const double invX0 = 1.0 / X0;
+    const double Q = sqrt(4 * s.c - s.b * s.b);
+    const double x2b = 2.0 * x + s.b;
+    const double inv_x2bQ = 1.0 / (x2b * x2b + Q * Q);
+    const double d1 = 2.0 / x;
+    const double d2 = -x2b * invX;
+    const double d3 = -4.0 * s.b * inv_x2bQ;
+    const double d4 = -s.b * s.x0 * invX0 *
+                      (2.0 / (x - s.x0) - x2b * invX - 4.0 * (2.0 * s.x0 + s.b) * inv_x2bQ);
+    return s.A * (d1 + d2 + d3 + d4);
+}
+static double vwn_gz(double z) {
+    return (9.0 / 8.0) * (pow((1.0 + z), 4.0 / 3.0) + pow((1.0 - z), 4.0 / 3.0) - 2.0);
+}
+static double vwn_gz_prime(double z) {
+    return (3.0 / 2.0) * (pow((1.0 + z), 1.0 / 3.0) - pow((1.0 - z), 1.0 / 3.0));
+}
+static double vwn_epsc(vwn_set para, vwn_set ferr, double x, double zeta) {
+    const double ep = vwn_eps(para, x);
+    const double ef = vwn_eps(ferr, x);
+    const double ea = vwn_eps(vwn_anti(), x);
+    const double g = vwn_gz(zeta);
+    const double z4 = zeta * zeta * zeta * zeta;
+    const double coef2 = 1.0 + (vwn_ec_coef() * (ef - ep) / ea - 1.0) * z4;
+    return ep + ea * g * coef2;
+}
+static double vwn_e(double ra, double rb, int v3) {
+    const vwn_set para = v3 ? VWN3_PARA : VWN5_PARA;
+    const vwn_set ferr = v3 ? VWN3_FERR : VWN5_FERR;
+    const double rho = ra + rb;
+    const double inv = 1.0 / rho;
+    const double x = pow((3.0 / (4.0 * M_PI)) * inv, 1.0 / 6.0);
+    const double zeta = (ra - rb) * inv;
+    return rho * vwn_epsc(para, ferr, x, zeta);
+}
+static void vwn_d(double ra, double rb, int v3, double* va, double* vb) {
+    const vwn_set para = v3 ? VWN3_PARA : VWN5_PARA;
+    const vwn_set ferr = v3 ? VWN3_FERR : VWN5_FERR;
+    const vwn_set anti = vwn_anti();
+    const double C = vwn_ec_coef();
+    *va = 0.0;
+    *vb = 0.0;
+    const double rho = ra + rb;
+    const double inv = 1.0 / rho;
+    const double x = pow((3.0 / (4.0 * M_PI)) * inv, 1.0 / 6.0);
+    const double zeta = (ra - rb) * inv;
+    const double EC = vwn_epsc(para, ferr, x, zeta);
+    const double ep = vwn_eps(para, x), ef = vwn_eps(ferr, x), ea = vwn_eps(anti, x);
+    const double epp = vwn_eps_prime(para, x), efp = vwn_eps_prime(ferr, x),
+                 eap = vwn_eps_prime(anti, x);
+    const double g = vwn_gz(zeta);
+    const double gp = vwn_gz_prime(zeta);
+    const double h = C * ((ef - ep) / ea) - 1.0;
+    const double invea = 1.0 / ea;
+    const double hp = C * invea * (efp - epp - (ef - ep) * invea * eap);
+    const double z3 = zeta * zeta * zeta;
+    const double z4 = z3 * zeta;
+    const double term1 = -x / (6.0 * rho) * (epp + eap * g * (1.0 + h * z4) + ea * g * hp * z4);
+    const double t2c = ea * (gp * (1.0 + h * z4) + 4.0 * g * h * z3);
+    const double dzA = inv * (1.0 - zeta);
+    const double dzB = -inv * (1.0 + zeta);
+    const double dECA = term1 + t2c * dzA;
+    const double dECB = term1 + t2c * dzB;
+    if (ra > 1.0e-20) *va = EC + rho * dECA;
+    if (rb > 1.0e-20) *vb = EC + rho * dECB;
+}
+
+/* ------------------------------------------------------------------------ */
+/* LYP correlation: DfFunctional_LYP.cpp:25-43 (constants), :64-197           */
+/* (derivatives), :288-317 (energy), :346-386 (omega, delta, d/dgamma),       */
+/* :507-594 (mixed second derivatives), :652-674 (omega', delta').            */
+/* ------------------------------------------------------------------------ */
+#define LYP_TOL 1.0e-20
+#define LYP_A 0.04918
+#define LYP_B 0.132
+#define LYP_C 0.2533
+#define LYP_D 0.349
+static double lyp_coef(void) {
+    return pow(2.0, 11.0 / 3.0) * 0.3 * pow((3.0 * M_PI * M_PI), 2.0 / 3.0);
+}
+static double lyp_omega(double rho) {
+    const double m13 = pow(rho, -(1.0 / 3.0));
+    const double m113 = pow(rho, -(11.0 / 3.0));
+    const double t1 = exp(-LYP_C * m13);
+    const double t2 = 1.0 + LYP_D * m13;
+    return t1 * m113 / t2;
+}
+static double lyp_delta(double rho) {
+    const double m13 = pow(rho, -(1.0 / 3.0));
+    return LYP_C * m13 + LYP_D * m13 / (1.0 + LYP_D * m13);
+}
+static double lyp_omega_prime(double rho, double omega) {
+    const double r13 = pow(rho, 1.0 / 3.0);
+    const double rm13 = 1.0 / r13;
+    const double rm43 = (1.0 / rho) * rm13;
+    return -(1.0 / 3.0) * rm43 * omega * (11.0 * r13 - LYP_C - (LYP_D / (1.0 + LYP_D * rm13)));
+}
+static double lyp_delta_prime(double rho, double delta) {
+    const double a11 = 1.0 + LYP_D * pow(rho, -(1.0 / 3.0));
+    const double a1 = ((LYP_D * LYP_D) * pow(rho, -(5.0 / 3.0))) / (a11 * a11);
+    const double a2 = -1.0 / rho * delta;
+    return (1.0 / 3.0) * (a1 + a2);
+}
+static void lyp_dgamma(double rho, double ra, double rb, double omega, double delta,
+                       double* daa, double* dab, double* dbb) {
+    const double rab = ra * rb;
+    const double abw = (LYP_A * LYP_B) * omega;
+    *daa = -abw * ((1.0 / 9.0) * rab * (1.0 - 3.0 * delta - (delta - 11.0) * ra / rho) - rb * rb);
+    *dbb = -abw * ((1.0 / 9.0) * rab * (1.0 - 3.0 * delta - (delta - 11.0) * rb / rho) - ra * ra);
+    *dab = -abw * ((1.0 / 9.0) * rab * (47.0 - 7.0 * delta) - (4.0 / 3.0) * rho * rho);
+}
+static double lyp_e(double ra, double rb, double gaa, double gab, double gbb) {
+    const double rho = ra + rb;
+    const double rab = ra * rb;
+    const double pm13 = pow(rho, -(1.0 / 3.0));
+    const double pa83 = pow(ra, 8.0 / 3.0);
+    const double pb83 = pow(rb, 8.0 / 3.0);
+    const double omega = lyp_omega(rho);
+    const double delta = lyp_delta(rho);
+    double daa, dab, dbb;
+    lyp_dgamma(rho, ra, rb, omega, delta, &daa, &dab, &dbb);
+    const double term1 = -4.0 * LYP_A / (1.0 + LYP_D * pm13) * rab / rho;
+    const double term2 = -lyp_coef() * (LYP_A * LYP_B) * omega * rab * (pa83 + pb83);
+    return term1 + term2 + daa * gaa + dab * gab + dbb * gbb;
+}
+static void lyp_rr(double rho, double ra, double rb, double inv, double rab,
+                   double omega, double delta, double deltap, double* rA_AA,
+                   double* rB_AA, double* rA_AB, double* rB_AB, double* rA_BB,
+                   double* rB_BB) {
+    const double abw = (LYP_A * LYP_B) * omega;
+    const double wpw = -(1.0 / 3.0) * pow(rho, -(4.0 / 3.0)) *
+                       (11.0 * pow(rho, 1.0 / 3.0) - LYP_C -
+                        LYP_D / (1.0 + LYP_D * pow(rho, -(1.0 / 3.0))));
+    double daa, dab, dbb;
+    lyp_dgamma(rho, ra, rb, omega, delta, &daa, &dab, &dbb);
+    {
+        const double t1A = wpw * daa;
+        const double t2A_1 = (1.0 / 9.0) * rb * (1.0 - 3.0 * delta - (delta - 11.0) * ra * inv);
+        const double t2A_2 = -(1.0 / 9.0) * rab * ((3.0 + ra * inv) * deltap + (delta - 11.0) * rb * inv * inv);
+        const double t2A = -abw * (t2A_1 + t2A_2);
+        const double t1B = wpw * dbb;
+        const double t2B_1 = (1.0 / 9.0) * ra * (1.0 - 3.0 * delta - (delta - 11.0) * rb * inv);
+        const double t2B_2 = -(1.0 / 9.0) * rab * ((3.0 + rb * inv) * deltap + (delta - 11.0) * ra * inv * inv);
+        const double t2B = -abw * (t2B_1 + t2B_2);
+        *rA_AA = t1A + t2A;
+        *rB_BB = t1B + t2B;
+    }
+    {
+        const double t1 = wpw * dab;
+        const double t2_1 = (1.0 / 9.0) * (47.0 - 7.0 * delta);
+        const double t2_2 = -(7.0 / 9.0) * rab * deltap - (8.0 / 3.0) * rho;
+        *rA_AB = t1 + (-abw * (t2_1 * rb + t2_2));
+        *rB_AB = t1 + (-abw * (t2_1 * ra + t2_2));
+    }
+    {
+        const double t1A = wpw * dbb;
+        const double t2A_1 = (1.0 / 9.0) * rb * (1.0 - 3.0 * delta - (delta - 11.0) * rb * inv);
+        const double t2A_2 = -(1.0 / 9.0) * rab * ((3.0 + rb * inv) * deltap - (delta - 11.0) * rb * inv * inv);
+        const double t2A_3 = -2.0 * ra;
+        const double t2A = -abw * (t2A_1 + t2A_2 + t2A_3);
+        const double t1B = wpw * daa;
+        const double t2B_1 = (1.0 / 9.0) * ra * (1.0 - 3.0 * delta - (delta - 11.0) * ra * inv);
+        const double t2B_2 = -(1.0 / 9.0) * rab * ((3.0 + ra * inv) * deltap - (delta - 11.0) * ra * inv * inv);
+        const double t2B_3 = -2.0 * rb;
+        const double t2B = -abw * (t2B_1 + t2B_2 + t2B_3);
+        *rA_BB = t1A + t2A;
+        *rB_AA = t1B + t2B;
+    }
+}
+static void lyp_d(double ra, double rb, double gaa, double gab, double gbb,
+                  double* vra, double* vrb, double* vgaa, double* vgab, double* vgbb) {
+    *vra = *vrb = *vgaa = *vgab = *vgbb = 0.0;
+    const double rho = ra + rb;
+    if ((fabs(ra) > LYP_TOL) && (fabs(rb) > LYP_TOL)) {
+        const double rab = ra * rb;
+        const double inv = 1.0 / rho;
+        const double invA = 1.0 / ra;
+        const double invB = 1.0 / rb;
+        const double r13 = pow(rho, 1.0 / 3.0);
+        const double rm13 = 1.0 / r13;
+        const double rm43 = pow(rho, -4.0 / 3.0);
+        const double ra83 = pow(ra, 8.0 / 3.0);
+        const double rb83 = pow(rb, 8.0 / 3.0);
+        const double omega = lyp_omega(rho);
+        const double abw = (LYP_A * LYP_B) * omega;
+        const double omegap = lyp_omega_prime(rho, omega);
+        const double delta = lyp_delta(rho);
+        const double deltap = lyp_delta_prime(rho, delta);
+        double rA_AA, rB_AA, rA_AB, rB_AB, rA_BB, rB_BB;
+        lyp_rr(rho, ra, rb, inv, rab, omega, delta, deltap, &rA_AA, &rB_AA, &rA_AB,
+               &rB_AB, &rA_BB, &rB_BB);
+        const double a1c = -4.0 * LYP_A / (1.0 + LYP_D * rm13) * rab / rho;
+        const double a1A = a1c * ((1.0 / 3.0) * LYP_D * rm43 / (1.0 + LYP_D * rm13) + invA - inv);
+        const double a1B = a1c * ((1.0 / 3.0) * LYP_D * rm43 / (1.0 + LYP_D * rm13) + invB - inv);
+        const double a2A = -lyp_coef() * (LYP_A * LYP_B) *
+                           (omegap * rab * (ra83 + rb83) + omega * rb * ((11.0 / 3.0) * ra83 + rb83));
+        const double a2B = -lyp_coef() * (LYP_A * LYP_B) *
+                           (omegap * rab * (rb83 + ra83) + omega * ra * ((11.0 / 3.0) * rb83 + ra83));
+        if (ra > LYP_TOL) {
+            *vra = a1A + a2A + rA_AA * gaa + rA_AB * gab + rA_BB * gbb;
+            *vgaa = -abw * ((1.0 / 9.0) * rab * (1.0 - 3.0 * delta - (delta - 11.0) * ra / rho) - rb * rb);
+        }
+        if (rb > LYP_TOL) {
+            *vrb = a1B + a2B + rB_AA * gaa + rB_AB * gab + rB_BB * gbb;
+            *vgbb = -abw * ((1.0 / 9.0) * rab * (1.0 - 3.0 * delta - (delta - 11.0) * rb / rho) - ra * ra);
+        }
+        if ((ra > LYP_TOL) && (rb > LYP_TOL)) {
+            *vgab = -abw * ((1.0 / 9.0) * rab * (47.0 - 7.0 * delta) - (4.0 / 3.0) * rho * rho);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* B3LYP: DfFunctional_B3LYP.cpp:25-28 (coefficients), :59-75 (e),            */
+/* :91-141 (derivatives).  0.80 Slater + 0.72 B88(non-LDA) + 0.19 VWN3 +      */
+/* 0.81 LYP.                                                                  */
+/* ------------------------------------------------------------------------ */
+static void b3lyp(double ra, double rb, double gaa, double gab, double gbb, double* o) {
+    const double e = 0.80 * slater_e(ra, rb) + 0.72 * b88_e(ra, rb, gaa, gbb, 0) +
+                     0.19 * vwn_e(ra, rb, 1) + 0.81 * lyp_e(ra, rb, gaa, gab, gbb);
+    double sA, sB, vA, vB, bA, bB, bgaa, bgab, bgbb, lA, lB, lgaa, lgab, lgbb;
+    slater_d(ra, rb, &sA, &sB);
+    vwn_d(ra, rb, 1, &vA, &vB);
+    b88_d(ra, rb, gaa, gbb, 0, &bA, &bB, &bgaa, &bgab, &bgbb);
+    lyp_d(ra, rb, gaa, gab, gbb, &lA, &lB, &lgaa, &lgab, &lgbb);
+    o[0] = e;
+    o[1] = 0.80 * sA + 0.72 * bA + 0.19 * vA + 0.81 * lA;
+    o[2] = 0.80 * sB + 0.72 * bB + 0.19 * vB + 0.81 * lB;
+    o[3] = 0.72 * bgaa + 0.81 * lgaa;
+    o[4] = 0.72 * bgab + 0.81 * lgab;
+    o[5] = 0.72 * bgbb + 0.81 * lgbb;
+}
+
+static void functional_point(int id, const double* x, double* o) {
+    const double ra = x[0], rb = x[1], gaa = x[2], gab = x[3], gbb = x[4];
+    o[0] = o[1] = o[2] = o[3] = o[4] = o[5] = 0.0;
+    switch (id) {
+        case ORACLE_SLATER:
+            o[0] = slater_e(ra, rb);
+            slater_d(ra, rb, &o[1], &o[2]);
+            break;
+        case ORACLE_BECKE88:
+        case ORACLE_BECKE88_EXCEPT_LDA: {
+            const int lda = (id == ORACLE_BECKE88);
+            o[0] = b88_e(ra, rb, gaa, gbb, lda);
+            b88_d(ra, rb, gaa, gbb, lda, &o[1], &o[2], &o[3], &o[4], &o[5]);
+        } break;
+        case ORACLE_VWN5:
+        case ORACLE_VWN3: {
+            const int v3 = (id == ORACLE_VWN3);
+            o[0] = vwn_e(ra, rb, v3);
+            vwn_d(ra, rb, v3, &o[1], &o[2]);
+        } break;
+        case ORACLE_LYP:
+            o[0] = lyp_e(ra, rb, gaa, gab, gbb);
+            lyp_d(ra, rb, gaa, gab, gbb, &o[1], &o[2], &o[3], &o[4], &o[5]);
+            break;
+        case ORACLE_B3LYP:
+            b3lyp(ra, rb, gaa, gab, gbb, o);
+            break;
+        default:
+            break;
+    }
+}
+
+void oracle_functional(int id, int64_t n, const double* in, double* out) {
+    for (int64_t i = 0; i < n; ++i) functional_point(id, in + 5 * i, out + 6 * i);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Basis normalisation: Fl_Gto::getNormalized (Fl_Gto.cpp:644-708),           */
+/* getNormalizedfactor (:558-641), TlOrbitalInfoObject::setCGTO (:38-106).    */
+/* All five d functions use the (l,m,n) = (1,1,0) norm.                       */
+/* ------------------------------------------------------------------------ */
+static double dbfact(int x) {
+    /* TlMath::dbfact: only odd arguments (2l-1 >= -1) reach it here */
+    double a = 1.0;
+    while (x > 1) {
+        a *= (double)x;
+        x -= 2;
+    }
+    return a;
+}
+static double prim_norm(double a, int l, int m, int n) {
+    const double pwr = (double)(l + m + n);
+    double ans = pow(2.0, pwr);
+    ans *= pow(dbfact(2 * l - 1) * dbfact(2 * m - 1) * dbfact(2 * n - 1), -1.0 / 2.0);
+    ans *= pow(2.0 / M_PI, 3.0 / 4.0);
+    ans *= pow(a, (pwr + 3.0 / 2.0) / 2.0);
+    return ans;
+}
+void oracle_normalize_cgto(int shell, int nprim, const double* exps,
+                           const double* coefs, double* out_exps, double* out_coefs) {
+    int l = 0, m = 0, n = 0;
+    if (shell == 1) l = 1;
+    if (shell == 2) l = m = 1;
+    const double pwr = (double)(l + m + n) + 3.0 / 2.0;
+    double ans = 0.0;
+    for (int a = 0; a < nprim; ++a)
+        for (int b = 0; b < nprim; ++b) {
+            double trm = coefs[a] * coefs[b];
+            trm *= prim_norm(exps[a], l, m, n) * prim_norm(exps[b], l, m, n);
+            trm *= pow(exps[a] + exps[b], -pwr);
+            ans += trm;
+        }
+    ans *= dbfact(2 * l - 1) * dbfact(2 * m - 1) * dbfact(2 * n - 1);
+    ans *= pow(2.0, -(double)(l + m + n));
+    ans *= pow(M_PI, 3.0 / 2.0);
+    const double nf = sqrt(1.0 / ans);
+    double e[ORACLE_MAX_PRIM], c[ORACLE_MAX_PRIM];
+    for (int k = 0; k < nprim; ++k) {
+        e[k] = exps[k];
+        c[k] = coefs[k] * prim_norm(exps[k], l, m, n) * nf;
+    }
+    /* stable insertion sort by ascending exponent (cgto_sort_functor) */
+    for (int i = 1; i < nprim; ++i) {
+        const double ei = e[i], ci = c[i];
+        int j = i - 1;
+        while (j >= 0 && e[j] > ei) {
+            e[j + 1] = e[j];
+            c[j + 1] = c[j];
+            --j;
+        }
+        e[j + 1] = ei;
+        c[j + 1] = ci;
+    }
+    for (int k = 0; k < nprim; ++k) {
+        out_exps[k] = e[k];
+        out_coefs[k] = c[k];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* AO values and gradients: DfCalcGridX::getAOs_core / getDAOs_core           */
+/* (DfCalcGridX.cpp:667-769) with getPrefactor (:103-141) and                 */
+/* getPrefactorForDerivative (:143-235).  A primitive is skipped when         */
+/* alpha*r^2 > 30 (TOOBIG, :42).                                              */
+/* ------------------------------------------------------------------------ */
+#define TOOBIG 30.0
+static double prefactor(int t, double x, double y, double z) {
+    const double inv_sqrt3 = 1.0 / sqrt(3.0);
+    switch (t) {
+        case 0: return 1.0;
+        case 1: return x;
+        case 2: return y;
+        case 3: return z;
+        case 4: return x * y;
+        case 5: return z * x;
+        case 6: return y * z;
+        case 7: return 0.5 * (x * x - y * y);
+        case 8: return inv_sqrt3 * (z * z - 0.5 * (x * x + y * y));
+        default: return 0.0;
+    }
+}
+static void prefactor_d(int t, double alpha, double x, double y, double z,
+                        double* px, double* py, double* pz) {
+    const double a2 = 2.0 * alpha;
+    const double inv_sqrt3 = 1.0 / sqrt(3.0);
+    switch (t) {
+        case 0: *px = -a2 * x; *py = -a2 * y; *pz = -a2 * z; break;
+        case 1: *px = -a2 * x * x + 1.0; *py = -a2 * x * y; *pz = -a2 * x * z; break;
+        case 2: *px = -a2 * y * x; *py = -a2 * y * y + 1.0; *pz = -a2 * y * z; break;
+        case 3: *px = -a2 * z * x; *py = -a2 * z * y; *pz = -a2 * z * z + 1.0; break;
+        case 4: {
+            const double xy = x * y;
+            *px = -a2 * xy * x + y; *py = -a2 * xy * y + x; *pz = -a2 * xy * z;
+        } break;
+        case 5: {
+            const double xz = x * z;
+            *px = -a2 * xz * x + z; *py = -a2 * xz * y; *pz = -a2 * xz * z + x;
+        } break;
+        case 6: {
+            const double yz = y * z;
+            *px = -a2 * yz * x; *py = -a2 * yz * y + z; *pz = -a2 * yz * z + y;
+        } break;
+        case 7: {
+            const double xx = x * x, yy = y * y;
+            const double xxX = -a2 * xx * x + 2.0 * x, xxY = -a2 * xx * y, xxZ = -a2 * xx * z;
+            const double yyX = -a2 * yy * x, yyY = -a2 * yy * y + 2.0 * y, yyZ = -a2 * yy * z;
+            *px = 0.5 * (xxX - yyX); *py = 0.5 * (xxY - yyY); *pz = 0.5 * (xxZ - yyZ);
+        } break;
+        case 8: {
+            const double xx = x * x, yy = y * y, zz = z * z;
+            const double xxX = -a2 * xx * x + 2.0 * x, xxY = -a2 * xx * y, xxZ = -a2 * xx * z;
+            const double yyX = -a2 * yy * x, yyY = -a2 * yy * y + 2.0 * y, yyZ = -a2 * yy * z;
+            const double zzX = -a2 * zz * x, zzY = -a2 * zz * y, zzZ = -a2 * zz * z + 2.0 * z;
+            *px = inv_sqrt3 * (zzX - 0.5 * (xxX + yyX));
+            *py = inv_sqrt3 * (zzY - 0.5 * (xxY + yyY));
+            *pz = inv_sqrt3 * (zzZ - 0.5 * (xxZ + yyZ));
+        } break;
+        default: *px = *py = *pz = 0.0; break;
+    }
+}
+static void eval_point(const oracle_aos* a, const double* p, double* phi,
+                       double* gx, double* gy, double* gz) {
+    for (int i = 0; i < a->n_aos; ++i) {
+        const double x = p[0] - a->center[3 * i + 0];
+        const double y = p[1] - a->center[3 * i + 1];
+        const double z = p[2] - a->center[3 * i + 2];
+        const double d2 = x * x + y * y + z * z;
+        const int t = a->type[i];
+        double v = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+        for (int k = 0; k < a->nprim[i]; ++k) {
+            const double al = a->exps[i * ORACLE_MAX_PRIM + k];
+            const double sh = al * d2;
+            if (sh <= TOOBIG) {
+                const double g = a->coefs[i * ORACLE_MAX_PRIM + k] * exp(-sh);
+                v += prefactor(t, x, y, z) * g;
+                if (gx) {
+                    double px, py, pz;
+                    prefactor_d(t, al, x, y, z, &px, &py, &pz);
+                    dx += px * g;
+                    dy += py * g;
+                    dz += pz * g;
+                }
+            }
+        }
+        phi[i] = v;
+        if (gx) {
+            gx[i] = dx;
+            gy[i] = dy;
+            gz[i] = dz;
+        }
+    }
+}
+void oracle_eval_aos(const oracle_aos* aos, int64_t n_pts, const double* xyz,
+                     double* phi, double* gx, double* gy, double* gz) {
+    const int64_t N = aos->n_aos;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t p = 0; p < n_pts; ++p) {
+        eval_point(aos, xyz + 3 * p, phi + p * N, gx ? gx + p * N : NULL,
+                   gy ? gy + p * N : NULL, gz ? gz + p * N : NULL);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* RKS GGA two-pass build: calcRho_GGA_part (DfCalcGridX.cpp:1923-1962) with  */
+/* getRhoAtGridPoint (:865-881) and getGradRhoAtGridPoint (:902-931);         */
+/* buildVxc GGA RKS (:2137-2193) through the non-virtual closed-shell         */
+/* forwarders of DfFunctional_GGA (DfFunctional.h:190-192, :221-232) that call */
+/* the spin-resolved functional with rhoB = rhoA, gAB = gBB = gAA;            */
+/* build_XC_Matrix GGA (:2455-2500).  K is accumulated per thread and summed  */
+/* at the end (the reference adds with omp atomic; the order differs).        */
+/* ------------------------------------------------------------------------ */
+double oracle_calc_xc_rks(const oracle_aos* aos, int functional, int64_t n_pts,
+                          const double* grid, const double* PA, double* FA,
+                          double* rho_out, double density_cutoff, int n_threads) {
+    const int N = aos->n_aos;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = (n_threads > 0) ? n_threads : omp_get_max_threads();
+#else
+    (void)n_threads;
+#endif
+    double* Fpriv = (double*)calloc((size_t)nt * N * N, sizeof(double));
+    double energy = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt) reduction(+ : energy)
+#endif
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double* F = Fpriv + (size_t)tid * N * N;
+        double* phi = (double*)malloc(sizeof(double) * 4 * N);
+        double* gx = phi + N;
+        double* gy = gx + N;
+        double* gz = gy + N;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t g = 0; g < n_pts; ++g) {
+            const double* p = grid + 4 * g;
+            const double w = p[3];
+            eval_point(aos, p, phi, gx, gy, gz);
+            /* rho: triangle loop, getRhoAtGridPoint */
+            double rho = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const double ai = phi[i];
+                const double* Pi = PA + (size_t)i * N;
+                for (int j = 0; j < i; ++j) rho += 2.0 * Pi[j] * ai * phi[j];
+                rho += Pi[i] * ai * ai;
+            }
+            /* grad rho: full square, getGradRhoAtGridPoint */
+            double grx = 0.0, gry = 0.0, grz = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const double ai = phi[i];
+                const double* Pi = PA + (size_t)i * N;
+                for (int j = 0; j < N; ++j) {
+                    const double pij = Pi[j];
+                    grx += pij * ai * gx[j];
+                    gry += pij * ai * gy[j];
+                    grz += pij * ai * gz[j];
+                }
+            }
+            grx *= 2.0;
+            gry *= 2.0;
+            grz *= 2.0;
+            if (rho_out) {
+                rho_out[4 * g + 0] = rho;
+                rho_out[4 * g + 1] = grx;
+                rho_out[4 * g + 2] = gry;
+                rho_out[4 * g + 3] = grz;
+            }
+            if (rho > density_cutoff) {
+                const double gamma = grx * grx + gry * gry + grz * grz;
+                const double in[5] = {rho, rho, gamma, gamma, gamma};
+                double o[6];
+                functional_point(functional, in, o);
+                const double vr = o[1], vgaa = o[3], vgab = o[4];
+                const double coef1 = w * vr;
+                const double vgaa2 = 2.0 * vgaa;
+                const double cx = w * (vgaa2 * grx + vgab * grx);
+                const double cy = w * (vgaa2 * gry + vgab * gry);
+                const double cz = w * (vgaa2 * grz + vgab * grz);
+                for (int i = 0; i < N; ++i) {
+                    const double ai = phi[i];
+                    const double ci = cx * gx[i] + cy * gy[i] + cz * gz[i];
+                    double* Fi = F + (size_t)i * N;
+                    {
+                        double v = coef1 * ai * ai;
+                        v += ai * ci * 2.0;
+                        Fi[i] += v;
+                    }
+                    for (int j = 0; j < i; ++j) {
+                        const double aj = phi[j];
+                        double v = coef1 * ai * aj;
+                        v += ai * (cx * gx[j] + cy * gy[j] + cz * gz[j]);
+                        v += aj * ci;
+                        Fi[j] += v;
+                    }
+                }
+                energy += w * o[0];
+            }
+        }
+        free(phi);
+    }
+    for (int t = 0; t < nt; ++t) {
+        const double* F = Fpriv + (size_t)t * N * N;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j <= i; ++j) FA[(size_t)i * N + j] += F[(size_t)i * N + j];
+    }
+    free(Fpriv);
+    return energy;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Lebedev rules.  Each rule is a list of octahedral orbits (TlLebedevGrid.cpp */
+/* :93-269 lists the six orbit shapes; the per-order parameters :271-, 302    */
+/* at :649-697, 590 at :816-892).  The parameters are the published           */
+/* Lebedev-Laikov values.  The orbit is generated here by applying all 48      */
+/* signed permutations to a seed point and removing duplicates, so the point   */
+/* order differs from the reference (only the summation order changes).        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int type;
+    double a, b, w;
+} leb_orbit;
+
+static const leb_orbit LEB6[] = {{1, 0, 0, 0.1666666666666667}};
+static const leb_orbit LEB38[] = {{1, 0, 0, 0.9523809523809524E-2},
+                                  {3, 0, 0, 0.3214285714285714E-1},
+                                  {5, 0.4597008433809831E+0, 0, 0.2857142857142857E-1}};
+static const leb_orbit LEB86[] = {{1, 0, 0, 0.1154401154401154E-1},
+                                  {3, 0, 0, 0.1194390908585628E-1},
+                                  {4, 0.3696028464541502E+0, 0, 0.1111055571060340E-1},
+                                  {4, 0.6943540066026664E+0, 0, 0.1187650129453714E-1},
+                                  {5, 0.3742430390903412E+0, 0, 0.1181230374690448E-1}};
+static const leb_orbit LEB194[] = {{1, 0, 0, 0.1782340447244611E-2},
+                                   {2, 0, 0, 0.5716905949977102E-2},
+                                   {3, 0, 0, 0.5573383178848738E-2},
+                                   {4, 0.6712973442695226E+0, 0, 0.5608704082587997E-2},
+                                   {4, 0.2892465627575439E+0, 0, 0.5158237711805383E-2},
+                                   {4, 0.4446933178717437E+0, 0, 0.5518771467273614E-2},
+                                   {4, 0.1299335447650067E+0, 0, 0.4106777028169394E-2},
+                                   {5, 0.3457702197611283E+0, 0, 0.5051846064614808E-2},
+                                   {6, 0.1590417105383530E+0, 0.8360360154824589E+0, 0.5530248916233094E-2}};
+static const leb_orbit LEB302[] = {{1, 0, 0, 0.8545911725128148E-3},
+                                   {3, 0, 0, 0.3599119285025571E-2},
+                                   {4, 0.3515640345570105E+0, 0, 0.3449788424305883E-2},
+                                   {4, 0.6566329410219612E+0, 0, 0.3604822601419882E-2},
+                                   {4, 0.4729054132581005E+0, 0, 0.3576729661743367E-2},
+                                   {4, 0.9618308522614784E-1, 0, 0.2352101413689164E-2},
+                                   {4, 0.2219645236294178E+0, 0, 0.3108953122413675E-2},
+                                   {4, 0.7011766416089545E+0, 0, 0.3650045807677255E-2},
+                                   {5, 0.2644152887060663E+0, 0, 0.2982344963171804E-2},
+                                   {5, 0.5718955891878961E+0, 0, 0.3600820932216460E-2},
+                                   {6, 0.2510034751770465E+0, 0.8000727494073952E+0, 0.3571540554273387E-2},
+                                   {6, 0.1233548532583327E+0, 0.4127724083168531E+0, 0.3392312205006170E-2}};
+static const leb_orbit LEB590[] = {{1, 0, 0, 0.3095121295306187E-3},
+                                   {3, 0, 0, 0.1852379698597489E-2},
+                                   {4, 0.7040954938227469E+0, 0, 0.1871790639277744E-2},
+                                   {4, 0.6807744066455243E+0, 0, 0.1858812585438317E-2},
+                                   {4, 0.6372546939258752E+0, 0, 0.1852028828296213E-2},
+                                   {4, 0.5044419707800358E+0, 0, 0.1846715956151242E-2},
+                                   {4, 0.4215761784010967E+0, 0, 0.1818471778162769E-2},
+                                   {4, 0.3317920736472123E+0, 0, 0.1749564657281154E-2},
+                                   {4, 0.2384736701421887E+0, 0, 0.1617210647254411E-2},
+                                   {4, 0.1459036449157763E+0, 0, 0.1384737234851692E-2},
+                                   {4, 0.6095034115507196E-1, 0, 0.9764331165051050E-3},
+                                   {5, 0.6116843442009876E+0, 0, 0.1857161196774078E-2},
+                                   {5, 0.3964755348199858E+0, 0, 0.1705153996395864E-2},
+                                   {5, 0.1724782009907724E+0, 0, 0.1300321685886048E-2},
+                                   {6, 0.5610263808622060E+0, 0.3518280927733519E+0, 0.1842866472905286E-2},
+                                   {6, 0.4742392842551980E+0, 0.2634716655937950E+0, 0.1802658934377451E-2},
+                                   {6, 0.5984126497885380E+0, 0.1816640840360209E+0, 0.1849830560443660E-2},
+                                   {6, 0.3791035407695563E+0, 0.1720795225656878E+0, 0.1713904507106709E-2},
+                                   {6, 0.2778673190586244E+0, 0.8213021581932511E-1, 0.1555213603396808E-2},
+                                   {6, 0.5033564271075117E+0, 0.8999205842074875E-1, 0.1802239128008525E-2}};
+
+static int orbit_points(const leb_orbit* o, double* xyz) {
+    double s[3];
+    switch (o->type) {
+        case 1: s[0] = 1.0; s[1] = 0.0; s[2] = 0.0; break;
+        case 2: s[0] = sqrt(0.5); s[1] = sqrt(0.5); s[2] = 0.0; break;
+        case 3: s[0] = s[1] = s[2] = sqrt(1.0 / 3.0); break;
+        case 4: s[0] = o->a; s[1] = o->a; s[2] = sqrt(1.0 - 2.0 * o->a * o->a); break;
+        case 5: s[0] = o->a; s[1] = sqrt(1.0 - o->a * o->a); s[2] = 0.0; break;
+        case 6: s[0] = o->a; s[1] = o->b; s[2] = sqrt(1.0 - o->a * o->a - o->b * o->b); break;
+        default: return 0;
+    }
+    static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    int n = 0;
+    for (int pi = 0; pi < 6; ++pi)
+        for (int sg = 0; sg < 8; ++sg) {
+            double q[3];
+            for (int d = 0; d < 3; ++d) {
+                const double v = s[perm[pi][d]];
+                q[d] = ((sg >> d) & 1) ? -v : v;
+                if (q[d] == 0.0) q[d] = 0.0; /* fold -0 */
+            }
+            int dup = 0;
+            for (int k = 0; k < n && !dup; ++k)
+                dup = (xyz[3 * k] == q[0] && xyz[3 * k + 1] == q[1] && xyz[3 * k + 2] == q[2]);
+            if (!dup) {
+                xyz[3 * n] = q[0];
+                xyz[3 * n + 1] = q[1];
+                xyz[3 * n + 2] = q[2];
+                ++n;
+            }
+        }
+    return n;
+}
+
+int oracle_lebedev(int order, double* xyz, double* w) {
+    const leb_orbit* t = NULL;
+    int no = 0;
+#define LEB_CASE(N, T) case N: t = T; no = (int)(sizeof(T) / sizeof(T[0])); break;
+    switch (order) {
+        LEB_CASE(6, LEB6)
+        LEB_CASE(38, LEB38)
+        LEB_CASE(86, LEB86)
+        LEB_CASE(194, LEB194)
+        LEB_CASE(302, LEB302)
+        LEB_CASE(590, LEB590)
+        default: return 0;
+    }
+#undef LEB_CASE
+    int n = 0;
+    double buf[48 * 3];
+    for (int k = 0; k < no; ++k) {
+        const int m = orbit_points(&t[k], buf);
+        for (int i = 0; i < m; ++i) {
+            xyz[3 * (n + i)] = buf[3 * i];
+            xyz[3 * (n + i) + 1] = buf[3 * i + 1];
+            xyz[3 * (n + i) + 2] = buf[3 * i + 2];
+            w[n + i] = t[k].w;
+        }
+        n += m;
+    }
+    return (n == order) ? n : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Euler-Maclaurin radial rule (DfGenerateGrid.cpp:1597-1610):                */
+/*   r_i = R i^2/(Nr+1-i)^2,  w_i = 2 R^3 (Nr+1) i^5/(Nr+1-i)^7 * 4 pi.       */
+/* The reference forms i^5 in 32-bit int arithmetic, which wraps for i >= 74; */
+/* the wrap is reproduced with unsigned arithmetic (well defined here).       */
+/* ------------------------------------------------------------------------ */
+void oracle_radial_em(double R, int Nr, int i, double* r, double* w) {
+    const double Nr1i = (double)(Nr + 1 - i);
+    const double inv = 1.0 / Nr1i;
+    *r = R * (double)(i * i) * inv * inv;
+    const uint32_t u = (uint32_t)i;
+    const int32_t i5 = (int32_t)(u * u * u * u * u);
+    *w = 2.0 * R * R * R * (double)(Nr + 1) * (double)i5 * inv * inv * inv * inv *
+         inv * inv * inv * 4.0 * M_PI;
+}
+
+/* Bragg-Slater radii in the units the reference uses them (TlPrdctbl.cpp:150-180) */
+double oracle_bragg_slater(int Z) {
+    static const double t[] = {0.00, 0.25, 0.50, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60,
+                               0.50, 0.50, 1.80, 1.50, 1.25, 1.10, 1.00, 1.00, 1.00,
+                               1.00};
+    if (Z >= 1 && Z < (int)(sizeof(t) / sizeof(t[0]))) return t[Z];
+    return 0.0;
+}
+/* getCovalentRadiiForBecke (DfGenerateGrid.cpp:1958-1968): H uses 0.35 */
+static double becke_radius(int Z) { return (Z == 1) ? 0.35 : oracle_bragg_slater(Z); }
+
+/* 3x3 symmetric eigen-decomposition by cyclic Jacobi rotations. */
+static void jacobi3(double A[3][3], double V[3][3], double ev[3]) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        if (off < 1e-300) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (A[p][q] == 0.0) continue;
+                const double theta = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {
+                    const double akp = A[k][p], akq = A[k][q];
+                    A[k][p] = c * akp - s * akq;
+                    A[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double apk = A[p][k], aqk = A[q][k];
+                    A[p][k] = c * apk - s * aqk;
+                    A[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double vkp = V[k][p], vkq = V[k][q];
+                    V[k][p] = c * vkp - s * vkq;
+                    V[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < 3; ++i) ev[i] = A[i][i];
+}
+
+void oracle_o_matrix(int n_atoms, const int* Z, const double* xyz, double* O) {
+    double T[3] = {0, 0, 0}, sz = 0.0;
+    for (int a = 0; a < n_atoms; ++a) {
+        if (Z[a] <= 0) continue;
+        for (int d = 0; d < 3; ++d) T[d] += Z[a] * xyz[3 * a + d];
+        sz += Z[a];
+    }
+    for (int d = 0; d < 3; ++d) T[d] /= sz;
+    double M[3][3] = {{0}};
+    for (int a = 0; a < n_atoms; ++a) {
+        if (Z[a] <= 0) continue;
+        double r[3];
+        for (int d = 0; d < 3; ++d) r[d] = xyz[3 * a + d] - T[d];
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) M[i][j] += Z[a] * (((i == j) ? r2 : 0.0) - r[i] * r[j]);
+    }
+    double V[3][3], ev[3];
+    jacobi3(M, V, ev);
+    int idx[3] = {0, 1, 2};
+    for (int i = 0; i < 2; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            if (ev[idx[j]] < ev[idx[i]]) {
+                const int t = idx[i];
+                idx[i] = idx[j];
+                idx[j] = t;
+            }
+    for (int c = 0; c < 3; ++c) {
+        const int k = idx[c];
+        int big = 0;
+        for (int r = 1; r < 3; ++r)
+            if (fabs(V[r][k]) > fabs(V[big][k])) big = r;
+        const double sgn = (V[big][k] < 0) ? -1.0 : 1.0;
+        for (int r = 0; r < 3; ++r) O[3 * r + c] = sgn * V[r][k];
+    }
+}
+
+/* Becke cell function product P_m(r) (Ps_uij, DfGenerateGrid.cpp:2043-2086)
+ * and normalisation (calcMultiCenterWeight_Becke, :1811-1876). */
+static double becke_f1(double x) { return 0.5 * x * (3.0 - x * x); }
+static double becke_ps(int n_atoms, const int* Z, const double* xyz, int m,
+                       const double* r, int size_adjust) {
+    const double Rm = becke_radius(Z[m]);
+    const double dxm = r[0] - xyz[3 * m], dym = r[1] - xyz[3 * m + 1], dzm = r[2] - xyz[3 * m + 2];
+    const double rm = sqrt(dxm * dxm + dym * dym + dzm * dzm);
+    double ps = 1.0;
+    for (int n = 0; n < n_atoms; ++n) {
+        if (n == m || Z[n] <= 0) continue;
+        const double Rn = becke_radius(Z[n]);
+        const double dxn = r[0] - xyz[3 * n], dyn = r[1] - xyz[3 * n + 1], dzn = r[2] - xyz[3 * n + 2];
+        const double rn = sqrt(dxn * dxn + dyn * dyn + dzn * dzn);
+        const double ex = xyz[3 * m] - xyz[3 * n], ey = xyz[3 * m + 1] - xyz[3 * n + 1],
+                     ez = xyz[3 * m + 2] - xyz[3 * n + 2];
+        const double inv_dist = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+        double u = (rm - rn) * inv_dist;
+        if (size_adjust) {
+            const double au = (Rm - Rn) / (Rm + Rn);
+            double a = au / (au * au - 1.0);
+            if (a < -0.50) a = -0.50;
+            else if (a > 0.50) a = 0.50;
+            u = u + a * (1.0 - u * u);
+        }
+        const double f3 = becke_f1(becke_f1(becke_f1(u)));
+        ps *= 0.5 * (1.0 - f3);
+    }
+    return ps;
+}
+double oracle_becke_weight(int n_atoms, const int* Z, const double* xyz, int iatom,
+                           const double* r, int size_adjust) {
+    double total = 0.0, mine = 0.0;
+    for (int m = 0; m < n_atoms; ++m) {
+        const double p = becke_ps(n_atoms, Z, xyz, m, r, size_adjust);
+        if (m == iatom) mine = p;
+        total += p;
+    }
+    return mine / total;
+}
+
+int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz, int n_rad,
+                             int n_ang, const double* O, double weight_cutoff,
+                             double* out_grid, int* out_atom, int n_threads) {
+    double* leb = (double*)malloc(sizeof(double) * 4 * (size_t)n_ang);
+    double* lebw = leb + 3 * (size_t)n_ang;
+    if (oracle_lebedev(n_ang, leb, lebw) != n_ang) {
+        free(leb);
+        return -1;
+    }
+    /* each atom fills its own buffer (generateGrid_atom, :934-1091); the
+     * blocks are then concatenated in atom order */
+    const size_t cap = (size_t)n_rad * (size_t)n_ang;
+    double** buf = (double**)calloc((size_t)n_atoms, sizeof(double*));
+    int64_t* cnt = (int64_t*)calloc((size_t)n_atoms, sizeof(int64_t));
+    (void)n_threads;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int a = 0; a < n_atoms; ++a) {
+        int64_t c = 0;
+        if (Z[a] > 0) {
+            double* B = (double*)malloc(sizeof(double) * 4 * cap);
+            buf[a] = B;
+            const double R = oracle_bragg_slater(Z[a]);
+            for (int i = 1; i <= n_rad; ++i) {
+                double ri, wr;
+                oracle_radial_em(R, n_rad, i, &ri, &wr);
+                if (ri > 30.0) continue;
+                for (int k = 0; k < n_ang; ++k) {
+                    double p[3];
+                    for (int d = 0; d < 3; ++d) {
+                        const double v = O[3 * d + 0] * leb[3 * k] + O[3 * d + 1] * leb[3 * k + 1] +
+                                         O[3 * d + 2] * leb[3 * k + 2];
+                        p[d] = v * ri + xyz[3 * a + d];
+                    }
+                    double w = lebw[k] * wr;
+                    w *= oracle_becke_weight(n_atoms, Z, xyz, a, p, 1);
+                    if (fabs(w) > weight_cutoff) {
+                        B[4 * c + 0] = p[0];
+                        B[4 * c + 1] = p[1];
+                        B[4 * c + 2] = p[2];
+                        B[4 * c + 3] = w;
+                        ++c;
+                    }
+                }
+            }
+        }
+        cnt[a] = c;
+    }
+    int64_t total = 0;
+    for (int a = 0; a < n_atoms; ++a) {
+        if (out_grid && cnt[a] > 0) {
+            memcpy(out_grid + 4 * total, buf[a], sizeof(double) * 4 * (size_t)cnt[a]);
+            if (out_atom)
+                for (int64_t k = 0; k < cnt[a]; ++k) out_atom[total + k] = a;
+        }
+        total += cnt[a];
+        free(buf[a]);
+    }
+    free(buf);
+    free(cnt);
+    free(leb);
+    return total;
+}

@@ -1,0 +1,114 @@
+/*
+ * CPU oracle for the grid exchange-correlation build (test infrastructure).
+ *
+ * This is a plain-C restatement of ProteinDF's CPU path, written to check the
+ * HIP engine against.  It is NOT part of the product: only the tests, the
+ * smoke check and the CPU-baseline leg of the benchmark load it.  Every
+ * function cites the reference lines it follows (paths are relative to the
+ * ProteinDF source tree).
+ *
+ * Parity status of each part (what pins it):
+ *   functionals      reference KATs src/unit_test/DfFunctional_*Test.cpp
+ *                    (74 active cases) + central finite differences
+ *   AO normalisation <phi|phi> = 1 by quadrature, brute-force primitive norms
+ *   AO evaluation    closed forms at chosen points, central FD of phi
+ *   rho / K / E      brute force on tiny inputs, <D,K> = dE/de central FD,
+ *                    integral of rho = sum_i P_ii for diagonal P
+ *   grid             Lebedev exactness on spherical harmonics, EM radial
+ *                    closed form, Becke partition of unity
+ *   O matrix signs   parity unpinned (LAPACK dsyev sign convention is
+ *                    implementation-defined; see oracle_o_matrix)
+ */
+#ifndef PDFXC_ORACLE_H
+#define PDFXC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORACLE_MAX_PRIM 16
+
+/* functional ids */
+enum {
+    ORACLE_SLATER = 0,
+    ORACLE_BECKE88 = 1,
+    ORACLE_BECKE88_EXCEPT_LDA = 2,
+    ORACLE_VWN5 = 3,
+    ORACLE_VWN3 = 4,
+    ORACLE_LYP = 5,
+    ORACLE_B3LYP = 6
+};
+
+/* in: n x 5 (rhoA, rhoB, gAA, gAB, gBB); out: n x 6
+ * (zk, vrhoA, vrhoB, vgAA, vgAB, vgBB), spin-resolved entry points. */
+void oracle_functional(int id, int64_t n, const double* in, double* out);
+
+/* Contracted-GTO normalisation (Fl_Gto.cpp:558-708, TlOrbitalInfoObject.cpp:38-106).
+ * shell: 0 s, 1 p, 2 d.  Outputs are sorted by ascending exponent. */
+void oracle_normalize_cgto(int shell, int nprim, const double* exps,
+                           const double* coefs, double* out_exps,
+                           double* out_coefs);
+
+/* AO table: per AO centre (3), basis type 0..8, nprim, exps/coefs
+ * (n_aos x ORACLE_MAX_PRIM, normalised and sorted). */
+typedef struct {
+    int n_aos;
+    const double* center;  /* n_aos x 3 */
+    const int* type;       /* n_aos */
+    const int* nprim;      /* n_aos */
+    const double* exps;    /* n_aos x ORACLE_MAX_PRIM */
+    const double* coefs;   /* n_aos x ORACLE_MAX_PRIM */
+} oracle_aos;
+
+/* phi and grad phi of all AOs at n_pts points (DfCalcGridX.cpp:667-769).
+ * Outputs are n_pts x n_aos, row-major. grad may be NULL. */
+void oracle_eval_aos(const oracle_aos* aos, int64_t n_pts, const double* xyz,
+                     double* phi, double* gx, double* gy, double* gz);
+
+/* RKS GGA build (DfCalcGridX.cpp:2305-2315 with :1923-1962 and :2137-2193,
+ * :2455-2500).  grid: n_pts x 4 (x, y, z, w).  PA: dense N x N alpha density
+ * (symmetric).  FA: dense N x N, the lower triangle is ACCUMULATED (upper
+ * left untouched).  rho_out (optional): n_pts x 4 (rho, drho/dx, dy, dz).
+ * Returns E_xc.  functional: ORACLE_B3LYP (GGA ids only). n_threads<=0: all. */
+double oracle_calc_xc_rks(const oracle_aos* aos, int functional, int64_t n_pts,
+                          const double* grid, const double* PA, double* FA,
+                          double* rho_out, double density_cutoff,
+                          int n_threads);
+
+/* Lebedev rule of the given order (6, 38, 86, 194, 302, 590 supported);
+ * returns number of points written (0 if unsupported). xyz: n x 3, w: n. */
+int oracle_lebedev(int order, double* xyz, double* w);
+
+/* Euler-Maclaurin radial abscissa/weight (DfGenerateGrid.cpp:1597-1610),
+ * including the reference's 32-bit evaluation of i^5. */
+void oracle_radial_em(double R, int Nr, int i, double* r, double* w);
+
+/* Bragg-Slater radius table value as used by the grid (TlPrdctbl.cpp:150-160). */
+double oracle_bragg_slater(int Z);
+
+/* Standard-orientation matrix O (DfGenerateGrid.cpp:1739-1809): eigenvectors
+ * (columns, ascending eigenvalues) of M = sum_A Z_A (|R_A-T|^2 I - (R_A-T)(R_A-T)^T).
+ * Sign convention: the largest-|.| component of each column is positive
+ * (parity unpinned against LAPACK). O is row-major 3x3. */
+void oracle_o_matrix(int n_atoms, const int* Z, const double* xyz, double* O);
+
+/* Becke partition weight P_A / sum_B P_B at point r (DfGenerateGrid.cpp:1811-1876,
+ * :2043-2086) with atomic size adjustments. */
+double oracle_becke_weight(int n_atoms, const int* Z, const double* xyz,
+                           int iatom, const double* r, int size_adjust);
+
+/* Molecular grid (DfGenerateGrid.cpp:730-768, :934-1091) for grid-type USER,
+ * EM radial, no pruning, Becke partitioning with size adjustments, weight
+ * cutoff.  Atom blocks are emitted in atom order.  If out_grid is NULL only
+ * the count is returned.  out_grid: n x 4 (x, y, z, w); out_atom: n. */
+int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz,
+                             int n_rad, int n_ang, const double* O,
+                             double weight_cutoff, double* out_grid,
+                             int* out_atom, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
