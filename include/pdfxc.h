@@ -1,0 +1,148 @@
+/*
+ * pdfxc -- MI355X (gfx950) exchange-correlation quadrature engine for
+ * ProteinDF's grid XC path.  C ABI: plain pointers and sizes, no C++ types.
+ *
+ * The operator this library replaces is
+ *
+ *   double DfCalcGridX::calcXCIntegForFockAndEnergy(
+ *       const TlDenseSymmetricMatrix_Lapack& P_A, DfFunctional_GGA* pFunctional,
+ *       TlDenseSymmetricMatrix_Lapack* pF_A);                // DfCalcGridX.h:203-205
+ *
+ * called from DfXCFunctional::getFxc<> (DfXCFunctional.h:133-190) with
+ * P_A = P/2 and a zeroed F_A.  The engine reads the same inputs (AO table of
+ * TlOrbitalInfo, the grid that DfGenerateGrid writes to grid.mat, the alpha
+ * density in packed lower-triangle order) and returns the same outputs
+ * (E_xc, F_A accumulated in packed lower-triangle order).
+ *
+ * Every entry point returns PDFXC_OK (0) or an error code; pdfxc_last_error()
+ * gives a message for the calling thread.  A context owns one HIP stream and
+ * its device buffers; it is not thread-safe.  Host buffers stay owned by the
+ * caller.
+ */
+#ifndef PDFXC_H
+#define PDFXC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PDFXC_OK = 0,
+    PDFXC_ERR_ARGUMENT = 1,    /* bad size, null pointer, unsupported value */
+    PDFXC_ERR_HIP = 2,         /* HIP runtime error (message has details) */
+    PDFXC_ERR_NO_GRID = 3,     /* build called before a grid was set */
+    PDFXC_ERR_UNSUPPORTED = 4, /* functional / basis type not implemented */
+    PDFXC_ERR_MEMORY = 5       /* device or host allocation failed */
+};
+
+/* functionals (DfXCFunctional.h:145-171 dispatch; B3LYP first) */
+enum { PDFXC_B3LYP = 1 };
+
+/* arithmetic of the two contractions X = Phi.P and K = Phi^T.V */
+enum { PDFXC_FP64 = 0 };
+
+typedef struct pdfxc_opts {
+    double density_cutoff;   /* rho_alpha cutoff, xc-density-threshold (DfCalcGridX.cpp:55-59), 1e-16 */
+    double ao_screen;        /* per-block AO screening bound; 0 keeps every AO the reference
+                                evaluates as non-zero (only the alpha r^2 > 30 primitive cut,
+                                DfCalcGridX.cpp:42, :690-700) */
+    int precision;           /* PDFXC_FP64 */
+    int block_points;        /* points per grid block: 128 */
+    int64_t scratch_bytes;   /* device scratch budget for AO values per launch chunk; 0 = 8 GiB */
+} pdfxc_opts;
+
+typedef struct pdfxc_stats {
+    int64_t n_points;        /* grid points */
+    int64_t n_blocks;        /* grid blocks */
+    int64_t n_chunks;        /* launch chunks per build */
+    double flops;            /* algorithmic sum_b 4 n_b N_b^2 (N_b = screened AOs of block b) */
+    double mean_nao;         /* mean N_b */
+    int max_nao;             /* max N_b */
+    double last_build_ms;    /* device time of the last build (HIP events) */
+    double last_kernel_ms[4];/* per kernel: AO eval, rho+functional+V, K assembly, other */
+} pdfxc_stats;
+
+typedef struct pdfxc_ctx pdfxc_ctx;
+
+/* defaults: density_cutoff 1e-16, ao_screen 0, FP64, 128 points per block */
+void pdfxc_default_opts(pdfxc_opts* opts);
+
+/* Contracted-GTO normalisation as TlOrbitalInfoObject::setCGTO does it
+ * (TlOrbitalInfoObject.cpp:38-106 with Fl_Gto::getNormalizedfactor /
+ * getNormalized, Fl_Gto.cpp:558-708).  shell: 0 s, 1 p, 2 d.  Outputs are
+ * sorted by ascending exponent.  Host only. */
+int pdfxc_normalize_cgto(int shell, int nprim, const double* exps, const double* coefs,
+                         double* out_exps, double* out_coefs);
+
+/* Create a context from the per-AO table TlOrbitalInfo provides
+ * (getPosition, getBasisType, getCgtoContraction, getExponent,
+ * getCoefficient; TlOrbitalInfoObject.cpp:181-259).  ao_exps / ao_coefs are
+ * n_aos x max_prim row-major, already normalised.  The AOs of one shell
+ * (types 0 | 1,2,3 | 4..8 on one centre) must be consecutive. */
+int pdfxc_ctx_create(pdfxc_ctx** ctx, int n_aos, const double* ao_center,
+                     const int* ao_type, const int* ao_nprim, const double* ao_exps,
+                     const double* ao_coefs, int max_prim, const pdfxc_opts* opts);
+int pdfxc_ctx_destroy(pdfxc_ctx* ctx);
+
+/* Grid as grid.mat holds it (CSFD columns x, y, z, weight; DfCalcGridX.h:36-55).
+ * Partitions the points into blocks and screens AOs per block. */
+int pdfxc_set_grid(pdfxc_ctx* ctx, int64_t n_pts, const double* x, const double* y,
+                   const double* z, const double* w);
+
+/* Generate the molecular grid on the device: grid-type USER, Euler-Maclaurin
+ * radial rule, no pruning, Becke partitioning with atomic size adjustments,
+ * |w| > weight_cutoff (DfGenerateGrid.cpp:39-250, :730-768, :934-1091,
+ * :1597-1610, :1715-1876).  Atoms with Z = 0 are dummies.  Sets the grid of
+ * the context and returns the point count. */
+int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
+                        int n_rad, int n_ang, double weight_cutoff, int64_t* n_pts);
+/* Copy the context's grid back in the caller's point order (x, y, z, w
+ * columns, n_pts each). */
+int pdfxc_get_grid(pdfxc_ctx* ctx, double* x, double* y, double* z, double* w);
+
+/* calcXCIntegForFockAndEnergy, RKS GGA (DfCalcGridX.cpp:2305-2315):
+ * P_alpha_packed: N(N+1)/2 doubles, row r, column c <= r (RLHD order).
+ * K_alpha_packed: same layout, ACCUMULATED into (the caller zeroes it).
+ * E_xc: total XC energy.  rho_out (may be NULL): n_pts x 4 (rho_alpha,
+ * d/dx, d/dy, d/dz) in the caller's point order, the values the reference
+ * writes to grid.mat columns 5-8. */
+int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_alpha_packed,
+                    double* K_alpha_packed, double* E_xc, double* rho_out);
+
+/* Same build on device-resident buffers, enqueued on the context's stream
+ * and synchronised before return: dP dense N x N symmetric, dK dense N x N
+ * whose lower triangle is accumulated, dE one double (overwritten). */
+int pdfxc_build_rks_device(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
+                           double* dE);
+
+/* Functional kernel over arbitrary closed-shell points, as the grid path
+ * calls it (DfFunctional.h:190-192, :221-232: rhoB = rhoA, gAB = gBB = gAA).
+ * in: n x 2 (rho_alpha, gamma_alpha_alpha); out: n x 4 (f, df/drhoA,
+ * df/dgammaAA, df/dgammaAB).  Host buffers; runs on the current device. */
+int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* out);
+
+/* AO values and gradients of all AOs at arbitrary points (getAOs / getDAOs,
+ * DfCalcGridX.cpp:655-769).  Outputs n_pts x n_aos row-major. */
+int pdfxc_eval_aos(pdfxc_ctx* ctx, int64_t n_pts, const double* xyz, double* phi,
+                   double* gx, double* gy, double* gz);
+
+int pdfxc_get_stats(pdfxc_ctx* ctx, pdfxc_stats* stats);
+
+/* The context's HIP stream (hipStream_t) for callers that enqueue around
+ * pdfxc_build_rks_device. */
+void* pdfxc_stream(pdfxc_ctx* ctx);
+
+const char* pdfxc_last_error(void);
+
+/* Diagnostics: one v_mfma_f64_16x16x4 product of fixed operands
+ * A[i][k] = i + 100 k, B[k][j] = j - 7 k (i, j < 16, k < 4), written as the
+ * 16 x 16 row-major D the engine's accumulator layout assumes.  Lets a test
+ * confirm the lane layout the kernels rely on. */
+int pdfxc_selftest_mfma(double* out16x16);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

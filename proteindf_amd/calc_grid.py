@@ -1,0 +1,100 @@
+"""Host-side mirror of ProteinDF's grid XC interface over the engine library.
+
+Names, argument meaning and error behaviour follow the reference classes the
+engine replaces, so that callers (and tests) read like the reference's own:
+
+* ``DfFunctional_B3LYP`` -- getFunctional / getDerivativeFunctional for
+  closed-shell points as the grid path calls them (DfFunctional.h:190-192,
+  :221-232), evaluated by the engine's functional kernel.
+* ``DfGenerateGrid`` -- the molecular grid (DfGenerateGrid.cpp:39-250,
+  :730-768), generated on the device.
+* ``DfCalcGridX`` -- calcXCIntegForFockAndEnergy(P_A, functional, F_A)
+  (DfCalcGridX.cpp:2305-2315): P_A and F_A in packed lower-triangle order,
+  F_A accumulated, E_xc returned; getWholeDensity (:2502-2537).
+* ``DfXCFunctional`` -- buildXcMatrix for RKS (DfXCFunctional.cpp:128-152):
+  P_A = P/2, a zeroed F_xc, the B3LYP build, the electron-count check
+  (checkGridAccuracy, :227-248).
+
+Every call goes through the HIP library; nothing here computes on the CPU.
+"""
+import numpy as np
+
+from . import engine
+from .basis import OrbitalInfo, basis_631gd
+
+
+class DfFunctional_B3LYP:
+    functional_id = engine.PDFXC_B3LYP
+
+    def getFunctional(self, rhoA, gammaAA):
+        return engine.eval_functional(rhoA, gammaAA, self.functional_id)[:, 0]
+
+    def getDerivativeFunctional(self, rhoA, gammaAA):
+        o = engine.eval_functional(rhoA, gammaAA, self.functional_id)
+        return o[:, 1], o[:, 2], o[:, 3]
+
+
+def orbital_info(mol, basis_for=basis_631gd):
+    """TlOrbitalInfo for a molecule, normalised by the engine's own routine."""
+    return OrbitalInfo.build(mol, engine.normalize_cgto, basis_for)
+
+
+class DfGenerateGrid:
+    """grid-type USER, EM radial, no pruning, Becke partitioning."""
+
+    def __init__(self, mol, n_rad=75, n_ang=302, weight_cutoff=1e-16):
+        self.mol, self.n_rad, self.n_ang, self.weight_cutoff = mol, n_rad, n_ang, weight_cutoff
+
+    def generate(self, ctx):
+        return ctx.generate_grid(self.mol.Z, self.mol.xyz, self.n_rad, self.n_ang,
+                                 self.weight_cutoff)
+
+
+class DfCalcGridX:
+    def __init__(self, mol, grid=None, n_rad=75, n_ang=302, basis_for=basis_631gd,
+                 density_cutoff=1e-16, ao_screen=0.0, info=None):
+        self.mol = mol
+        self.info = info if info is not None else orbital_info(mol, basis_for)
+        self.m_nNumOfAOs = self.info.n_aos
+        opts = engine.default_opts(density_cutoff=density_cutoff, ao_screen=ao_screen)
+        self.ctx = engine.XcContext(self.info, opts)
+        if grid is None:
+            DfGenerateGrid(mol, n_rad, n_ang).generate(self.ctx)
+        else:
+            self.ctx.set_grid(grid)
+        self._rho = None
+
+    @property
+    def grid(self):
+        return self.ctx.get_grid()
+
+    def calcXCIntegForFockAndEnergy(self, P_A, functional, F_A, keep_rho=False):
+        if not isinstance(functional, DfFunctional_B3LYP):
+            raise NotImplementedError("only B3LYP is implemented")
+        E, _, rho = self.ctx.build_rks(P_A, F_A, want_rho=keep_rho,
+                                       functional=functional.functional_id)
+        if keep_rho:
+            self._rho = rho
+        return E
+
+    def getWholeDensity(self):
+        if self._rho is None:
+            raise RuntimeError("no density kept: call calcXCIntegForFockAndEnergy(..., keep_rho=True)")
+        return float((self.grid[:, 3] * self._rho[:, 0]).sum())
+
+
+class DfXCFunctional:
+    def __init__(self, calc_grid, n_electrons):
+        self.calc_grid = calc_grid
+        self.n_electrons = n_electrons
+        self.XC_energy_ = 0.0
+        self.electrons_on_grid = None
+
+    def buildXcMatrix(self, P_packed):
+        n = self.calc_grid.m_nNumOfAOs
+        P_A = 0.5 * np.asarray(P_packed, dtype=np.float64)
+        Fxc = np.zeros(n * (n + 1) // 2)
+        self.XC_energy_ = self.calc_grid.calcXCIntegForFockAndEnergy(
+            P_A, DfFunctional_B3LYP(), Fxc, keep_rho=True)
+        self.electrons_on_grid = 2.0 * self.calc_grid.getWholeDensity()
+        return Fxc

@@ -1,0 +1,1002 @@
+// Host side of the pdfxc engine: the C ABI, the AO shell table, grid-block
+// partitioning with per-block AO screening, launch-chunk planning, device
+// buffers and grid generation.  Kernels live in pdfxc_kernels.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "pdfxc_internal.hpp"
+
+namespace pdfxc {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return PDFXC_ERR_HIP;
+}
+
+static int arg_error(const std::string& msg) {
+    set_error(msg);
+    return PDFXC_ERR_ARGUMENT;
+}
+
+template <typename T>
+static int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                  " bytes: " + hipGetErrorString(e));
+        return PDFXC_ERR_MEMORY;
+    }
+    return PDFXC_OK;
+}
+
+template <typename T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Contracted-GTO normalisation, as TlOrbitalInfoObject::setCGTO forms the
+// coefficients (TlOrbitalInfoObject.cpp:38-106): c_k = d_k N(a_k) N_c with the
+// primitive norm of Fl_Gto::getNormalized (Fl_Gto.cpp:644-708) and the
+// contraction norm of getNormalizedfactor (:558-641), for the angular
+// exponents (l,m,n) = (0,0,0), (1,0,0), (1,1,0) of s, p, d (the five d
+// functions share the dxy norm).  Primitives end sorted by exponent.
+// ---------------------------------------------------------------------------
+static double odd_factorial2(int x) {  // (x)!! for odd x >= -1
+    double a = 1.0;
+    for (; x > 1; x -= 2) a *= x;
+    return a;
+}
+
+static void lmn_of_shell(int shell, int* l, int* m, int* n) {
+    *l = *m = *n = 0;
+    if (shell >= 1) *l = 1;
+    if (shell >= 2) *m = 1;
+}
+
+static double primitive_norm(double a, int l, int m, int n) {
+    const int L = l + m + n;
+    return std::pow(2.0, (double)L) *
+           std::pow(odd_factorial2(2 * l - 1) * odd_factorial2(2 * m - 1) * odd_factorial2(2 * n - 1),
+                    -0.5) *
+           std::pow(2.0 / kPi, 0.75) * std::pow(a, (L + 1.5) / 2.0);
+}
+
+}  // namespace pdfxc
+
+using namespace pdfxc;
+
+extern "C" {
+
+void pdfxc_default_opts(pdfxc_opts* o) {
+    if (!o) return;
+    o->density_cutoff = 1.0e-16;
+    o->ao_screen = 0.0;
+    o->precision = PDFXC_FP64;
+    o->block_points = kBlockPts;
+    o->scratch_bytes = 0;
+}
+
+const char* pdfxc_last_error(void) { return g_last_error.c_str(); }
+
+int pdfxc_normalize_cgto(int shell, int nprim, const double* exps, const double* coefs,
+                         double* out_exps, double* out_coefs) {
+    if (shell < 0 || shell > 2) {
+        set_error("pdfxc_normalize_cgto: shell must be 0 (s), 1 (p) or 2 (d)");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (nprim <= 0 || nprim > kMaxPrim || !exps || !coefs || !out_exps || !out_coefs)
+        return arg_error("pdfxc_normalize_cgto: bad arguments");
+    int l, m, n;
+    lmn_of_shell(shell, &l, &m, &n);
+    const double pw = (double)(l + m + n) + 1.5;
+    double s = 0.0;
+    for (int a = 0; a < nprim; ++a)
+        for (int b = 0; b < nprim; ++b)
+            s += coefs[a] * coefs[b] * primitive_norm(exps[a], l, m, n) *
+                 primitive_norm(exps[b], l, m, n) * std::pow(exps[a] + exps[b], -pw);
+    s *= odd_factorial2(2 * l - 1) * odd_factorial2(2 * m - 1) * odd_factorial2(2 * n - 1);
+    s *= std::pow(2.0, -(double)(l + m + n));
+    s *= std::pow(kPi, 1.5);
+    const double nc = std::sqrt(1.0 / s);
+    std::vector<int> order(nprim);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return exps[x] < exps[y]; });
+    for (int k = 0; k < nprim; ++k) {
+        const int o = order[k];
+        out_exps[k] = exps[o];
+        out_coefs[k] = coefs[o] * primitive_norm(exps[o], l, m, n) * nc;
+    }
+    return PDFXC_OK;
+}
+
+int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const int* ao_type,
+                     const int* ao_nprim, const double* ao_exps, const double* ao_coefs,
+                     int max_prim, const pdfxc_opts* opts) {
+    if (!out) return arg_error("pdfxc_ctx_create: null output");
+    *out = nullptr;
+    if (n_aos <= 0 || !ao_center || !ao_type || !ao_nprim || !ao_exps || !ao_coefs || max_prim <= 0)
+        return arg_error("pdfxc_ctx_create: bad arguments");
+    Context* c = new Context();
+    pdfxc_default_opts(&c->opts);
+    if (opts) c->opts = *opts;
+    if (c->opts.block_points != kBlockPts) {
+        delete c;
+        set_error("pdfxc_ctx_create: only 128 points per block are supported");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (c->opts.precision != PDFXC_FP64) {
+        delete c;
+        set_error("pdfxc_ctx_create: only the FP64 precision mode is implemented");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    c->n_aos = n_aos;
+    // group AOs into shells: type 0 | 1,2,3 | 4..8 on one centre, same primitives
+    int i = 0;
+    while (i < n_aos) {
+        const int t = ao_type[i];
+        int l, ncomp;
+        if (t == 0) {
+            l = 0;
+            ncomp = 1;
+        } else if (t == 1) {
+            l = 1;
+            ncomp = 3;
+        } else if (t == 4) {
+            l = 2;
+            ncomp = 5;
+        } else {
+            delete c;
+            set_error("pdfxc_ctx_create: AO " + std::to_string(i) + " of type " + std::to_string(t) +
+                      " does not start a shell (s=0, p=1..3, d=4..8 consecutive)");
+            return (t < 0 || t > 8) ? PDFXC_ERR_UNSUPPORTED : PDFXC_ERR_ARGUMENT;
+        }
+        if (i + ncomp > n_aos) {
+            delete c;
+            return arg_error("pdfxc_ctx_create: truncated shell at the end of the AO table");
+        }
+        Shell s;
+        s.first_ao = i;
+        s.l = l;
+        s.nprim = ao_nprim[i];
+        if (s.nprim <= 0 || s.nprim > kMaxPrim || s.nprim > max_prim) {
+            delete c;
+            return arg_error("pdfxc_ctx_create: bad primitive count at AO " + std::to_string(i));
+        }
+        for (int d = 0; d < 3; ++d) s.center[d] = ao_center[3 * i + d];
+        for (int k = 0; k < s.nprim; ++k) {
+            s.exps[k] = ao_exps[(size_t)i * max_prim + k];
+            s.coefs[k] = ao_coefs[(size_t)i * max_prim + k];
+            if (!(s.exps[k] > 0.0)) {
+                delete c;
+                return arg_error("pdfxc_ctx_create: non-positive exponent at AO " + std::to_string(i));
+            }
+        }
+        for (int m = 1; m < ncomp; ++m) {
+            const int j = i + m;
+            bool ok = ao_type[j] == t + m && ao_nprim[j] == s.nprim;
+            for (int d = 0; d < 3 && ok; ++d) ok = ao_center[3 * j + d] == s.center[d];
+            for (int k = 0; k < s.nprim && ok; ++k)
+                ok = ao_exps[(size_t)j * max_prim + k] == s.exps[k] &&
+                     ao_coefs[(size_t)j * max_prim + k] == s.coefs[k];
+            if (!ok) {
+                delete c;
+                return arg_error("pdfxc_ctx_create: AO " + std::to_string(j) +
+                                 " does not continue the shell starting at AO " + std::to_string(i));
+            }
+        }
+        c->shells.push_back(s);
+        i += ncomp;
+    }
+    hipError_t e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
+    if (e != hipSuccess) {
+        const int rc = hip_fail(e, "pdfxc_ctx_create: stream/event creation");
+        delete c;
+        return rc;
+    }
+    std::vector<DeviceShell> ds(c->shells.size());
+    for (size_t k = 0; k < c->shells.size(); ++k) {
+        const Shell& s = c->shells[k];
+        DeviceShell& d = ds[k];
+        std::memset(&d, 0, sizeof(d));
+        d.cx = s.center[0];
+        d.cy = s.center[1];
+        d.cz = s.center[2];
+        d.l = s.l;
+        d.nprim = s.nprim;
+        d.first_ao = s.first_ao;
+        for (int q = 0; q < s.nprim; ++q) {
+            d.exps[q] = s.exps[q];
+            d.coefs[q] = s.coefs[q];
+        }
+    }
+    int rc = dalloc(&c->d_shells, ds.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c->d_P, (size_t)n_aos * n_aos);
+    if (rc == PDFXC_OK) rc = dalloc(&c->d_K, (size_t)n_aos * n_aos);
+    if (rc == PDFXC_OK) rc = dalloc(&c->d_E, 1);
+    if (rc == PDFXC_OK) {
+        e = hipMemcpy(c->d_shells, ds.data(), ds.size() * sizeof(DeviceShell), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_ctx_create: shell upload");
+    }
+    if (rc == PDFXC_OK) {
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->h_pinned),
+                          sizeof(double) * (size_t)n_aos * n_aos, hipHostMallocDefault);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_ctx_create: pinned staging");
+    }
+    if (rc != PDFXC_OK) {
+        pdfxc_ctx_destroy(reinterpret_cast<pdfxc_ctx*>(c));
+        return rc;
+    }
+    *out = reinterpret_cast<pdfxc_ctx*>(c);
+    return PDFXC_OK;
+}
+
+static void free_grid_buffers(Context* c) {
+    dfree(c->d_pts);
+    dfree(c->d_blocks);
+    dfree(c->d_block_shells);
+    dfree(c->d_block_cols);
+    dfree(c->d_block_shell_col);
+    dfree(c->d_scratch);
+    dfree(c->d_coef);
+    dfree(c->d_rho);
+    dfree(c->d_eblock);
+}
+
+int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return PDFXC_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_grid_buffers(c);
+    dfree(c->d_shells);
+    dfree(c->d_P);
+    dfree(c->d_K);
+    dfree(c->d_E);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PDFXC_OK;
+}
+
+void* pdfxc_stream(pdfxc_ctx* ctx) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    return c ? reinterpret_cast<void*>(c->stream) : nullptr;
+}
+
+}  // extern "C"
+
+namespace pdfxc {
+
+// ---------------------------------------------------------------------------
+// Grid blocks: recursive bisection of the point set along its widest extent,
+// cutting at a multiple of 128 so that every block but the last is full.
+// ---------------------------------------------------------------------------
+static void partition_points(const Context& c, std::vector<int64_t>& order,
+                             std::vector<std::pair<int64_t, int64_t>>& leaves) {
+    order.resize(c.n_pts);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<std::pair<int64_t, int64_t>> stack;
+    stack.push_back({0, c.n_pts});
+    const double* X[3] = {c.gx.data(), c.gy.data(), c.gz.data()};
+    while (!stack.empty()) {
+        const auto [lo, hi] = stack.back();
+        stack.pop_back();
+        const int64_t n = hi - lo;
+        if (n <= kBlockPts) {
+            leaves.push_back({lo, hi});
+            continue;
+        }
+        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (int64_t k = lo; k < hi; ++k)
+            for (int d = 0; d < 3; ++d) {
+                const double v = X[d][order[k]];
+                mn[d] = std::min(mn[d], v);
+                mx[d] = std::max(mx[d], v);
+            }
+        int dim = 0;
+        for (int d = 1; d < 3; ++d)
+            if (mx[d] - mn[d] > mx[dim] - mn[dim]) dim = d;
+        const int64_t nch = (n + kBlockPts - 1) / kBlockPts;
+        const int64_t mid = lo + (nch / 2) * kBlockPts;
+        const double* v = X[dim];
+        std::nth_element(order.begin() + lo, order.begin() + mid, order.begin() + hi,
+                         [v](int64_t a, int64_t b) { return v[a] < v[b] || (v[a] == v[b] && a < b); });
+        // right first so that leaves come out left to right
+        stack.push_back({mid, hi});
+        stack.push_back({lo, mid});
+    }
+}
+
+// Upper bound of |phi| and |grad phi| of a shell over r >= d: sum over
+// primitives of |c| (r^l + l r^(l-1) + 2a r^(l+1)) e^(-a r^2), each term at
+// its own maximiser max(d, sqrt(m / 2a)).
+static double shell_bound(const Shell& s, double d) {
+    double b = 0.0;
+    for (int k = 0; k < s.nprim; ++k) {
+        const double a = s.exps[k];
+        auto term = [&](int m) {
+            if (m < 0) return 0.0;
+            const double r = std::max(d, std::sqrt(m / (2.0 * a)));
+            return std::pow(r, (double)m) * std::exp(-a * r * r);
+        };
+        b += std::fabs(s.coefs[k]) * (term(s.l) + 2.0 * s.l * term(s.l - 1) + 2.0 * a * term(s.l + 1));
+    }
+    return b;
+}
+
+static int build_blocks(Context& c) {
+    std::vector<int64_t> order;
+    std::vector<std::pair<int64_t, int64_t>> leaves;
+    partition_points(c, order, leaves);
+    const int nb = (int)leaves.size();
+    const int nsh = (int)c.shells.size();
+    struct Tmp {
+        int64_t lo, hi;
+        std::vector<int> sh;
+        int nao;
+        double cost;
+    };
+    std::vector<Tmp> tmp(nb);
+    const double screen = c.opts.ao_screen;
+    for (int b = 0; b < nb; ++b) {
+        Tmp& t = tmp[b];
+        t.lo = leaves[b].first;
+        t.hi = leaves[b].second;
+        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (int64_t k = t.lo; k < t.hi; ++k) {
+            const int64_t p = order[k];
+            const double v[3] = {c.gx[p], c.gy[p], c.gz[p]};
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = std::min(mn[d], v[d]);
+                mx[d] = std::max(mx[d], v[d]);
+            }
+        }
+        double ctr[3], rad = 0.0;
+        for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (mn[d] + mx[d]);
+        for (int64_t k = t.lo; k < t.hi; ++k) {
+            const int64_t p = order[k];
+            const double dx = c.gx[p] - ctr[0], dy = c.gy[p] - ctr[1], dz = c.gz[p] - ctr[2];
+            rad = std::max(rad, std::sqrt(dx * dx + dy * dy + dz * dz));
+        }
+        rad *= 1.0 + 1e-12;
+        t.nao = 0;
+        for (int s = 0; s < nsh; ++s) {
+            const Shell& S = c.shells[s];
+            const double dx = S.center[0] - ctr[0], dy = S.center[1] - ctr[1], dz = S.center[2] - ctr[2];
+            const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+            const double dmin = std::max(0.0, dist - rad);
+            // the reference skips a primitive with a r^2 > 30: a shell whose most
+            // diffuse primitive is cut at the nearest point is exactly zero here
+            if (S.exps[0] * dmin * dmin > 30.0) continue;
+            if (screen > 0.0 && shell_bound(S, dmin) < screen) continue;
+            t.sh.push_back(s);
+            t.nao += 2 * S.l + 1;
+        }
+        const int ldn = std::max(kColPad, (t.nao + kColPad - 1) / kColPad * kColPad);
+        t.cost = (double)(t.hi - t.lo) * ldn * ldn + 1.0;
+    }
+    // launch order: descending cost (longest blocks first)
+    std::vector<int> lorder(nb);
+    std::iota(lorder.begin(), lorder.end(), 0);
+    std::stable_sort(lorder.begin(), lorder.end(),
+                     [&](int a, int b) { return tmp[a].cost > tmp[b].cost; });
+    c.blocks.assign(nb, BlockDesc());
+    c.block_shells.clear();
+    c.block_cols.clear();
+    c.block_shell_col.clear();
+    c.n_slots = (int64_t)nb * kBlockPts;
+    c.slot_of_point.assign(c.n_pts, -1);
+    std::vector<double> pts(4 * (size_t)c.n_slots, 0.0);
+    c.flops = 0.0;
+    double sum_nao = 0.0;
+    c.max_nao = 0;
+    int64_t scratch_off = 0;
+    for (int li = 0; li < nb; ++li) {
+        const Tmp& t = tmp[lorder[li]];
+        BlockDesc& B = c.blocks[li];
+        B.pt_begin = li * kBlockPts;
+        B.npts = (int)(t.hi - t.lo);
+        B.shell_begin = (int)c.block_shells.size();
+        B.nshell = (int)t.sh.size();
+        B.col_begin = (int)c.block_cols.size();
+        B.nao = t.nao;
+        B.ldn = std::max(kColPad, (t.nao + kColPad - 1) / kColPad * kColPad);
+        B.scratch_off = scratch_off;
+        scratch_off += 4 * (int64_t)B.ldn * kBlockPts;
+        int col = 0;
+        for (int s : t.sh) {
+            c.block_shells.push_back(s);
+            c.block_shell_col.push_back(col);
+            const Shell& S = c.shells[s];
+            for (int m = 0; m < 2 * S.l + 1; ++m) c.block_cols.push_back(S.first_ao + m);
+            col += 2 * S.l + 1;
+        }
+        for (int k = 0; k < kBlockPts; ++k) {
+            const int64_t slot = (int64_t)li * kBlockPts + k;
+            const int64_t p = order[t.lo + std::min<int64_t>(k, B.npts - 1)];
+            pts[slot] = c.gx[p];
+            pts[c.n_slots + slot] = c.gy[p];
+            pts[2 * c.n_slots + slot] = c.gz[p];
+            pts[3 * c.n_slots + slot] = (k < B.npts) ? c.gw[p] : 0.0;
+            if (k < B.npts) c.slot_of_point[p] = slot;
+        }
+        c.flops += 4.0 * B.npts * (double)B.nao * B.nao;
+        sum_nao += B.nao;
+        c.max_nao = std::max(c.max_nao, B.nao);
+    }
+    c.mean_nao = nb ? sum_nao / nb : 0.0;
+    // launch chunks bounded by the scratch budget
+    const int64_t budget = (c.opts.scratch_bytes > 0 ? c.opts.scratch_bytes : (int64_t)8 << 30) /
+                           (int64_t)sizeof(double);
+    c.chunks.clear();
+    int64_t maxchunk = 0;
+    for (int b = 0; b < nb;) {
+        Chunk ch;
+        ch.block_begin = b;
+        int64_t used = 0;
+        while (b < nb) {
+            const int64_t need = 4 * (int64_t)c.blocks[b].ldn * kBlockPts;
+            if (used > 0 && used + need > budget) break;
+            used += need;
+            ++b;
+        }
+        ch.block_end = b;
+        ch.scratch_doubles = used;
+        maxchunk = std::max(maxchunk, used);
+        c.chunks.push_back(ch);
+    }
+    // device buffers
+    free_grid_buffers(&c);
+    int rc = dalloc(&c.d_pts, pts.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_blocks, c.blocks.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_block_shells, c.block_shells.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_block_cols, c.block_cols.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_block_shell_col, c.block_shell_col.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_scratch, (size_t)maxchunk);
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_coef, 4 * (size_t)c.n_slots);
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_rho, 4 * (size_t)c.n_slots);
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_eblock, c.blocks.size());
+    if (rc != PDFXC_OK) return rc;
+    c.scratch_doubles = maxchunk;
+    PDFXC_HIP(hipMemcpy(c.d_pts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice));
+    PDFXC_HIP(hipMemcpy(c.d_blocks, c.blocks.data(), c.blocks.size() * sizeof(BlockDesc),
+                        hipMemcpyHostToDevice));
+    if (!c.block_shells.empty()) {
+        PDFXC_HIP(hipMemcpy(c.d_block_shells, c.block_shells.data(), c.block_shells.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+        PDFXC_HIP(hipMemcpy(c.d_block_shell_col, c.block_shell_col.data(),
+                            c.block_shell_col.size() * sizeof(int), hipMemcpyHostToDevice));
+        PDFXC_HIP(hipMemcpy(c.d_block_cols, c.block_cols.data(), c.block_cols.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    }
+    return PDFXC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grid generation pieces (DfGenerateGrid.cpp).  Lebedev rules are lists of
+// octahedral orbits with the published Lebedev-Laikov parameters (the same
+// rules as TlLebedevGrid.cpp:271-892); each orbit is expanded by applying the
+// 48 signed permutations to its seed point.
+// ---------------------------------------------------------------------------
+struct Orbit {
+    int kind;
+    double a, b, w;
+};
+
+static const std::vector<Orbit>* lebedev_rule(int order) {
+    static const std::vector<Orbit> r6 = {{1, 0, 0, 0.1666666666666667}};
+    static const std::vector<Orbit> r38 = {{1, 0, 0, 0.9523809523809524E-2},
+                                           {3, 0, 0, 0.3214285714285714E-1},
+                                           {5, 0.4597008433809831E+0, 0, 0.2857142857142857E-1}};
+    static const std::vector<Orbit> r86 = {{1, 0, 0, 0.1154401154401154E-1},
+                                           {3, 0, 0, 0.1194390908585628E-1},
+                                           {4, 0.3696028464541502E+0, 0, 0.1111055571060340E-1},
+                                           {4, 0.6943540066026664E+0, 0, 0.1187650129453714E-1},
+                                           {5, 0.3742430390903412E+0, 0, 0.1181230374690448E-1}};
+    static const std::vector<Orbit> r194 = {
+        {1, 0, 0, 0.1782340447244611E-2},
+        {2, 0, 0, 0.5716905949977102E-2},
+        {3, 0, 0, 0.5573383178848738E-2},
+        {4, 0.6712973442695226E+0, 0, 0.5608704082587997E-2},
+        {4, 0.2892465627575439E+0, 0, 0.5158237711805383E-2},
+        {4, 0.4446933178717437E+0, 0, 0.5518771467273614E-2},
+        {4, 0.1299335447650067E+0, 0, 0.4106777028169394E-2},
+        {5, 0.3457702197611283E+0, 0, 0.5051846064614808E-2},
+        {6, 0.1590417105383530E+0, 0.8360360154824589E+0, 0.5530248916233094E-2}};
+    static const std::vector<Orbit> r302 = {
+        {1, 0, 0, 0.8545911725128148E-3},
+        {3, 0, 0, 0.3599119285025571E-2},
+        {4, 0.3515640345570105E+0, 0, 0.3449788424305883E-2},
+        {4, 0.6566329410219612E+0, 0, 0.3604822601419882E-2},
+        {4, 0.4729054132581005E+0, 0, 0.3576729661743367E-2},
+        {4, 0.9618308522614784E-1, 0, 0.2352101413689164E-2},
+        {4, 0.2219645236294178E+0, 0, 0.3108953122413675E-2},
+        {4, 0.7011766416089545E+0, 0, 0.3650045807677255E-2},
+        {5, 0.2644152887060663E+0, 0, 0.2982344963171804E-2},
+        {5, 0.5718955891878961E+0, 0, 0.3600820932216460E-2},
+        {6, 0.2510034751770465E+0, 0.8000727494073952E+0, 0.3571540554273387E-2},
+        {6, 0.1233548532583327E+0, 0.4127724083168531E+0, 0.3392312205006170E-2}};
+    static const std::vector<Orbit> r590 = {
+        {1, 0, 0, 0.3095121295306187E-3},
+        {3, 0, 0, 0.1852379698597489E-2},
+        {4, 0.7040954938227469E+0, 0, 0.1871790639277744E-2},
+        {4, 0.6807744066455243E+0, 0, 0.1858812585438317E-2},
+        {4, 0.6372546939258752E+0, 0, 0.1852028828296213E-2},
+        {4, 0.5044419707800358E+0, 0, 0.1846715956151242E-2},
+        {4, 0.4215761784010967E+0, 0, 0.1818471778162769E-2},
+        {4, 0.3317920736472123E+0, 0, 0.1749564657281154E-2},
+        {4, 0.2384736701421887E+0, 0, 0.1617210647254411E-2},
+        {4, 0.1459036449157763E+0, 0, 0.1384737234851692E-2},
+        {4, 0.6095034115507196E-1, 0, 0.9764331165051050E-3},
+        {5, 0.6116843442009876E+0, 0, 0.1857161196774078E-2},
+        {5, 0.3964755348199858E+0, 0, 0.1705153996395864E-2},
+        {5, 0.1724782009907724E+0, 0, 0.1300321685886048E-2},
+        {6, 0.5610263808622060E+0, 0.3518280927733519E+0, 0.1842866472905286E-2},
+        {6, 0.4742392842551980E+0, 0.2634716655937950E+0, 0.1802658934377451E-2},
+        {6, 0.5984126497885380E+0, 0.1816640840360209E+0, 0.1849830560443660E-2},
+        {6, 0.3791035407695563E+0, 0.1720795225656878E+0, 0.1713904507106709E-2},
+        {6, 0.2778673190586244E+0, 0.8213021581932511E-1, 0.1555213603396808E-2},
+        {6, 0.5033564271075117E+0, 0.8999205842074875E-1, 0.1802239128008525E-2}};
+    switch (order) {
+        case 6: return &r6;
+        case 38: return &r38;
+        case 86: return &r86;
+        case 194: return &r194;
+        case 302: return &r302;
+        case 590: return &r590;
+        default: return nullptr;
+    }
+}
+
+static bool lebedev_points(int order, std::vector<double>& xyz, std::vector<double>& w) {
+    const std::vector<Orbit>* rule = lebedev_rule(order);
+    if (!rule) return false;
+    xyz.clear();
+    w.clear();
+    for (const Orbit& o : *rule) {
+        double s[3];
+        switch (o.kind) {
+            case 1: s[0] = 1.0; s[1] = s[2] = 0.0; break;
+            case 2: s[0] = s[1] = std::sqrt(0.5); s[2] = 0.0; break;
+            case 3: s[0] = s[1] = s[2] = std::sqrt(1.0 / 3.0); break;
+            case 4: s[0] = s[1] = o.a; s[2] = std::sqrt(1.0 - 2.0 * o.a * o.a); break;
+            case 5: s[0] = o.a; s[1] = std::sqrt(1.0 - o.a * o.a); s[2] = 0.0; break;
+            default: s[0] = o.a; s[1] = o.b; s[2] = std::sqrt(1.0 - o.a * o.a - o.b * o.b); break;
+        }
+        const size_t first = w.size();
+        int perm[3] = {0, 1, 2};
+        do {
+            for (int sg = 0; sg < 8; ++sg) {
+                double q[3];
+                for (int d = 0; d < 3; ++d) {
+                    const double v = s[perm[d]];
+                    q[d] = ((sg >> d) & 1) ? -v : v;
+                    if (q[d] == 0.0) q[d] = 0.0;
+                }
+                bool dup = false;
+                for (size_t k = first; k < w.size() && !dup; ++k)
+                    dup = xyz[3 * k] == q[0] && xyz[3 * k + 1] == q[1] && xyz[3 * k + 2] == q[2];
+                if (!dup) {
+                    xyz.insert(xyz.end(), q, q + 3);
+                    w.push_back(o.w);
+                }
+            }
+        } while (std::next_permutation(perm, perm + 3));
+    }
+    return (int)w.size() == order;
+}
+
+// Euler-Maclaurin radial abscissa and weight (DfGenerateGrid.cpp:1597-1610);
+// i^5 is formed in 32-bit two's-complement as the reference does.
+static void radial_em(double R, int Nr, int i, double* r, double* w) {
+    const double inv = 1.0 / (double)(Nr + 1 - i);
+    *r = R * (double)(i * i) * inv * inv;
+    const uint32_t u = (uint32_t)i;
+    const int32_t i5 = (int32_t)(u * u * u * u * u);
+    *w = 2.0 * R * R * R * (double)(Nr + 1) * (double)i5 * inv * inv * inv * inv * inv * inv * inv *
+         4.0 * kPi;
+}
+
+// Bragg-Slater radii as the grid uses them (TlPrdctbl.cpp:150-180)
+static double bragg_slater(int Z) {
+    static const double t[] = {0.00, 0.25, 0.50, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60, 0.50,
+                               0.50, 1.80, 1.50, 1.25, 1.10, 1.00, 1.00, 1.00, 1.00};
+    return (Z >= 1 && Z < (int)(sizeof(t) / sizeof(t[0]))) ? t[Z] : 0.0;
+}
+
+// Standard orientation (DfGenerateGrid.cpp:1739-1809): eigenvectors of the
+// charge-weighted inertia-like tensor as columns, ascending eigenvalues; the
+// largest-magnitude component of each column is made positive.
+static void orientation(int na, const int* Z, const double* xyz, double O[9]) {
+    double T[3] = {0, 0, 0}, sz = 0.0;
+    for (int a = 0; a < na; ++a) {
+        if (Z[a] <= 0) continue;
+        for (int d = 0; d < 3; ++d) T[d] += Z[a] * xyz[3 * a + d];
+        sz += Z[a];
+    }
+    for (int d = 0; d < 3; ++d) T[d] /= sz;
+    double M[3][3] = {{0}};
+    for (int a = 0; a < na; ++a) {
+        if (Z[a] <= 0) continue;
+        double r[3];
+        for (int d = 0; d < 3; ++d) r[d] = xyz[3 * a + d] - T[d];
+        const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) M[i][j] += Z[a] * ((i == j ? r2 : 0.0) - r[i] * r[j]);
+    }
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        const double off = M[0][1] * M[0][1] + M[0][2] * M[0][2] + M[1][2] * M[1][2];
+        if (off < 1e-300) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (M[p][q] == 0.0) continue;
+                const double th = (M[q][q] - M[p][p]) / (2.0 * M[p][q]);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < 3; ++k) {
+                    const double a = M[k][p], b = M[k][q];
+                    M[k][p] = cs * a - sn * b;
+                    M[k][q] = sn * a + cs * b;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double a = M[p][k], b = M[q][k];
+                    M[p][k] = cs * a - sn * b;
+                    M[q][k] = sn * a + cs * b;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double a = V[k][p], b = V[k][q];
+                    V[k][p] = cs * a - sn * b;
+                    V[k][q] = sn * a + cs * b;
+                }
+            }
+    }
+    int idx[3] = {0, 1, 2};
+    std::sort(idx, idx + 3, [&](int a, int b) { return M[a][a] < M[b][b]; });
+    for (int col = 0; col < 3; ++col) {
+        const int k = idx[col];
+        int big = 0;
+        for (int r = 1; r < 3; ++r)
+            if (std::fabs(V[r][k]) > std::fabs(V[big][k])) big = r;
+        const double sg = V[big][k] < 0 ? -1.0 : 1.0;
+        for (int r = 0; r < 3; ++r) O[3 * r + col] = sg * V[r][k];
+    }
+}
+
+}  // namespace pdfxc
+
+extern "C" {
+
+int pdfxc_set_grid(pdfxc_ctx* ctx, int64_t n_pts, const double* x, const double* y, const double* z,
+                   const double* w) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_set_grid: null context");
+    if (n_pts <= 0 || !x || !y || !z || !w) return arg_error("pdfxc_set_grid: bad arguments");
+    if (n_pts > ((int64_t)1 << 31) - kBlockPts)
+        return arg_error("pdfxc_set_grid: more than 2^31 points");
+    c->n_pts = n_pts;
+    c->gx.assign(x, x + n_pts);
+    c->gy.assign(y, y + n_pts);
+    c->gz.assign(z, z + n_pts);
+    c->gw.assign(w, w + n_pts);
+    for (int64_t i = 0; i < n_pts; ++i)
+        if (!std::isfinite(c->gx[i]) || !std::isfinite(c->gy[i]) || !std::isfinite(c->gz[i]) ||
+            !std::isfinite(c->gw[i]))
+            return arg_error("pdfxc_set_grid: non-finite grid value at point " + std::to_string(i));
+    return build_blocks(*c);
+}
+
+int pdfxc_get_grid(pdfxc_ctx* ctx, double* x, double* y, double* z, double* w) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_get_grid: null context");
+    if (c->n_pts == 0) {
+        set_error("pdfxc_get_grid: no grid");
+        return PDFXC_ERR_NO_GRID;
+    }
+    if (x) std::copy(c->gx.begin(), c->gx.end(), x);
+    if (y) std::copy(c->gy.begin(), c->gy.end(), y);
+    if (z) std::copy(c->gz.begin(), c->gz.end(), z);
+    if (w) std::copy(c->gw.begin(), c->gw.end(), w);
+    return PDFXC_OK;
+}
+
+int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz, int n_rad,
+                        int n_ang, double weight_cutoff, int64_t* n_out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_generate_grid: null context");
+    if (n_atoms <= 0 || !Z || !xyz || n_rad <= 0) return arg_error("pdfxc_generate_grid: bad arguments");
+    std::vector<double> leb, lebw;
+    if (!lebedev_points(n_ang, leb, lebw)) {
+        set_error("pdfxc_generate_grid: unsupported Lebedev order " + std::to_string(n_ang) +
+                  " (6, 38, 86, 194, 302, 590)");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    for (int a = 0; a < n_atoms; ++a)
+        if (Z[a] < 0 || (Z[a] > 0 && bragg_slater(Z[a]) == 0.0)) {
+            set_error("pdfxc_generate_grid: no Bragg-Slater radius for Z = " + std::to_string(Z[a]));
+            return PDFXC_ERR_UNSUPPORTED;
+        }
+    double O[9];
+    orientation(n_atoms, Z, xyz, O);
+    // single-centre points and weights, atom by atom, radial shell by shell
+    std::vector<double> pts, w;
+    std::vector<int> atom;
+    for (int a = 0; a < n_atoms; ++a) {
+        if (Z[a] <= 0) continue;
+        const double R = bragg_slater(Z[a]);
+        for (int i = 1; i <= n_rad; ++i) {
+            double ri, wr;
+            radial_em(R, n_rad, i, &ri, &wr);
+            if (ri > 30.0) continue;
+            for (int k = 0; k < n_ang; ++k) {
+                for (int d = 0; d < 3; ++d) {
+                    const double v = O[3 * d] * leb[3 * k] + O[3 * d + 1] * leb[3 * k + 1] +
+                                     O[3 * d + 2] * leb[3 * k + 2];
+                    pts.push_back(v * ri + xyz[3 * a + d]);
+                }
+                w.push_back(lebw[k] * wr);
+                atom.push_back(a);
+            }
+        }
+    }
+    const int64_t n = (int64_t)w.size();
+    // atom table for the Becke kernel: (x, y, z, Z), 1/R_mn, size adjustments a_mn
+    // (getCovalentRadiiForBecke: H 0.35, else Bragg-Slater; |a| <= 1/2)
+    std::vector<double> at(4 * (size_t)n_atoms + 2 * (size_t)n_atoms * n_atoms, 0.0);
+    for (int m = 0; m < n_atoms; ++m) {
+        for (int d = 0; d < 3; ++d) at[4 * m + d] = xyz[3 * m + d];
+        at[4 * m + 3] = Z[m];
+    }
+    double* invd = at.data() + 4 * n_atoms;
+    double* amat = invd + (size_t)n_atoms * n_atoms;
+    auto brad = [](int z) { return z == 1 ? 0.35 : bragg_slater(z); };
+    for (int m = 0; m < n_atoms; ++m)
+        for (int k = 0; k < n_atoms; ++k) {
+            if (m == k) continue;
+            const double dx = xyz[3 * m] - xyz[3 * k], dy = xyz[3 * m + 1] - xyz[3 * k + 1],
+                         dz = xyz[3 * m + 2] - xyz[3 * k + 2];
+            const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist < 1e-10 && Z[m] > 0 && Z[k] > 0)
+                return arg_error("pdfxc_generate_grid: atoms " + std::to_string(m) + " and " +
+                                 std::to_string(k) + " coincide");
+            invd[(size_t)m * n_atoms + k] = 1.0 / dist;
+            if (Z[m] > 0 && Z[k] > 0) {
+                const double Rm = brad(Z[m]), Rk = brad(Z[k]);
+                const double au = (Rm - Rk) / (Rm + Rk);
+                double aa = au / (au * au - 1.0);
+                aa = std::min(0.5, std::max(-0.5, aa));
+                amat[(size_t)m * n_atoms + k] = aa;
+            }
+        }
+    double *d_at = nullptr, *d_p = nullptr, *d_w = nullptr;
+    int* d_a = nullptr;
+    int rc = dalloc(&d_at, at.size());
+    if (rc == PDFXC_OK) rc = dalloc(&d_p, pts.size());
+    if (rc == PDFXC_OK) rc = dalloc(&d_w, w.size());
+    if (rc == PDFXC_OK) rc = dalloc(&d_a, atom.size());
+    hipError_t e = hipSuccess;
+    if (rc == PDFXC_OK) {
+        e = hipMemcpyAsync(d_at, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_p, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_a, atom.data(), atom.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_generate_grid: upload");
+    }
+    if (rc == PDFXC_OK) rc = launch_becke(d_at, n_atoms, n, d_p, d_a, d_w, c->stream);
+    if (rc == PDFXC_OK) {
+        e = hipMemcpyAsync(w.data(), d_w, w.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_generate_grid: Becke weights");
+    }
+    dfree(d_at);
+    dfree(d_p);
+    dfree(d_w);
+    dfree(d_a);
+    if (rc != PDFXC_OK) return rc;
+    // weight screening (screeningGridsByWeight0, DfGenerateGrid.cpp:1979-2004)
+    std::vector<double> X, Y, Zc, W;
+    for (int64_t i = 0; i < n; ++i)
+        if (std::fabs(w[i]) > weight_cutoff) {
+            X.push_back(pts[3 * i]);
+            Y.push_back(pts[3 * i + 1]);
+            Zc.push_back(pts[3 * i + 2]);
+            W.push_back(w[i]);
+        }
+    if (n_out) *n_out = (int64_t)W.size();
+    return pdfxc_set_grid(ctx, (int64_t)W.size(), X.data(), Y.data(), Zc.data(), W.data());
+}
+
+static int build_device(Context* c, const double* dP, double* dK, double* dE) {
+    if (c->n_pts == 0 || c->blocks.empty()) {
+        set_error("build: no grid set");
+        return PDFXC_ERR_NO_GRID;
+    }
+    double* saveP = c->d_P;
+    double* saveK = c->d_K;
+    c->d_P = const_cast<double*>(dP);
+    c->d_K = dK;
+    int rc = PDFXC_OK;
+    float t[4] = {0, 0, 0, 0};
+    PDFXC_HIP(hipEventRecord(c->ev[0], c->stream));
+    for (const Chunk& ch : c->chunks) {
+        PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
+        rc = launch_ao_eval(*c, ch);
+        if (rc) break;
+        PDFXC_HIP(hipEventRecord(c->ev[2], c->stream));
+        rc = launch_rho_vxc(*c, ch);
+        if (rc) break;
+        PDFXC_HIP(hipEventRecord(c->ev[3], c->stream));
+        rc = launch_kmat(*c, ch);
+        if (rc) break;
+        PDFXC_HIP(hipEventRecord(c->ev[4], c->stream));
+        PDFXC_HIP(hipEventSynchronize(c->ev[4]));
+        float a, b, d;
+        PDFXC_HIP(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+        PDFXC_HIP(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
+        PDFXC_HIP(hipEventElapsedTime(&d, c->ev[3], c->ev[4]));
+        t[0] += a;
+        t[1] += b;
+        t[2] += d;
+    }
+    if (rc == PDFXC_OK) rc = launch_energy_sum(*c, dE);
+    c->d_P = saveP;
+    c->d_K = saveK;
+    if (rc) return rc;
+    PDFXC_HIP(hipEventRecord(c->ev[5], c->stream));
+    PDFXC_HIP(hipEventSynchronize(c->ev[5]));
+    float tot;
+    PDFXC_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
+    c->last_build_ms = tot;
+    for (int k = 0; k < 3; ++k) c->last_kernel_ms[k] = t[k];
+    c->last_kernel_ms[3] = tot - (t[0] + t[1] + t[2]);
+    return PDFXC_OK;
+}
+
+int pdfxc_build_rks_device(pdfxc_ctx* ctx, int functional, const double* dP, double* dK, double* dE) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_build_rks_device: null context");
+    if (functional != PDFXC_B3LYP) {
+        set_error("pdfxc_build_rks_device: only B3LYP is implemented");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (!dP || !dK || !dE) return arg_error("pdfxc_build_rks_device: null buffer");
+    return build_device(c, dP, dK, dE);
+}
+
+int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_packed, double* K_packed,
+                    double* E_xc, double* rho_out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_build_rks: null context");
+    if (functional != PDFXC_B3LYP) {
+        set_error("pdfxc_build_rks: only B3LYP is implemented");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (!P_packed || !K_packed || !E_xc) return arg_error("pdfxc_build_rks: null buffer");
+    const int N = c->n_aos;
+    double* H = c->h_pinned;
+    for (int r = 0, k = 0; r < N; ++r)
+        for (int q = 0; q <= r; ++q, ++k) {
+            H[(size_t)r * N + q] = P_packed[k];
+            H[(size_t)q * N + r] = P_packed[k];
+        }
+    PDFXC_HIP(hipMemcpyAsync(c->d_P, H, sizeof(double) * (size_t)N * N, hipMemcpyHostToDevice, c->stream));
+    PDFXC_HIP(hipMemsetAsync(c->d_K, 0, sizeof(double) * (size_t)N * N, c->stream));
+    int rc = build_device(c, c->d_P, c->d_K, c->d_E);
+    if (rc) return rc;
+    double E = 0.0;
+    PDFXC_HIP(hipMemcpyAsync(H, c->d_K, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost, c->stream));
+    PDFXC_HIP(hipMemcpyAsync(&E, c->d_E, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PDFXC_HIP(hipStreamSynchronize(c->stream));
+    for (int r = 0, k = 0; r < N; ++r)
+        for (int q = 0; q <= r; ++q, ++k) K_packed[k] += H[(size_t)r * N + q];
+    *E_xc = E;
+    if (rho_out) {
+        std::vector<double> rs(4 * (size_t)c->n_slots);
+        PDFXC_HIP(hipMemcpy(rs.data(), c->d_rho, rs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < c->n_pts; ++p) {
+            const int64_t s = c->slot_of_point[p];
+            for (int q = 0; q < 4; ++q) rho_out[4 * p + q] = rs[4 * s + q];
+        }
+    }
+    return PDFXC_OK;
+}
+
+int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* out) {
+    if (functional != PDFXC_B3LYP) {
+        set_error("pdfxc_eval_functional: only B3LYP is implemented");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (n < 0 || (n > 0 && (!in || !out))) return arg_error("pdfxc_eval_functional: bad arguments");
+    if (n == 0) return PDFXC_OK;
+    double *di = nullptr, *dout = nullptr;
+    int rc = dalloc(&di, 2 * (size_t)n);
+    if (rc == PDFXC_OK) rc = dalloc(&dout, 4 * (size_t)n);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipMemcpy(di, in, 2 * n * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_functional: upload");
+    }
+    if (rc == PDFXC_OK) rc = launch_eval_functional(n, di, dout, nullptr);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipMemcpy(out, dout, 4 * n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_functional: download");
+    }
+    dfree(di);
+    dfree(dout);
+    return rc;
+}
+
+int pdfxc_eval_aos(pdfxc_ctx* ctx, int64_t n, const double* xyz, double* phi, double* gx, double* gy,
+                   double* gz) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_eval_aos: null context");
+    if (n < 0 || (n > 0 && (!xyz || !phi))) return arg_error("pdfxc_eval_aos: bad arguments");
+    if (n == 0) return PDFXC_OK;
+    const size_t plane = (size_t)n * c->n_aos;
+    double *dx = nullptr, *dout = nullptr;
+    int rc = dalloc(&dx, 3 * (size_t)n);
+    if (rc == PDFXC_OK) rc = dalloc(&dout, 4 * plane);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipMemcpy(dx, xyz, 3 * n * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_aos: upload");
+    }
+    if (rc == PDFXC_OK) rc = launch_eval_aos_points(*c, n, dx, dout, c->stream);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        double* outs[4] = {phi, gx, gy, gz};
+        for (int q = 0; q < 4 && e == hipSuccess; ++q)
+            if (outs[q]) e = hipMemcpy(outs[q], dout + q * plane, plane * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_aos: download");
+    }
+    dfree(dx);
+    dfree(dout);
+    return rc;
+}
+
+int pdfxc_get_stats(pdfxc_ctx* ctx, pdfxc_stats* s) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c || !s) return arg_error("pdfxc_get_stats: bad arguments");
+    s->n_points = c->n_pts;
+    s->n_blocks = (int64_t)c->blocks.size();
+    s->n_chunks = (int64_t)c->chunks.size();
+    s->flops = c->flops;
+    s->mean_nao = c->mean_nao;
+    s->max_nao = c->max_nao;
+    s->last_build_ms = c->last_build_ms;
+    for (int k = 0; k < 4; ++k) s->last_kernel_ms[k] = c->last_kernel_ms[k];
+    return PDFXC_OK;
+}
+
+int pdfxc_selftest_mfma(double* out16x16) {
+    if (!out16x16) return arg_error("pdfxc_selftest_mfma: null output");
+    double* d = nullptr;
+    int rc = dalloc(&d, 256);
+    if (rc) return rc;
+    rc = launch_mfma_selftest(d, nullptr);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipMemcpy(out16x16, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_selftest_mfma");
+    }
+    dfree(d);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// Internal declarations shared by the host engine (pdfxc_host.cpp) and the
+// kernels (pdfxc_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pdfxc.h"
+
+namespace pdfxc {
+
+constexpr int kBlockPts = 128;  // grid points per block (rows of the MFMA tiles)
+constexpr int kColPad = 16;     // AO columns of a block are padded to a multiple of this
+constexpr int kMaxPrim = 16;
+constexpr double kPi = 3.14159265358979323846;
+
+// One shell = the 2l+1 consecutive AOs that share a centre and primitives.
+struct Shell {
+    int first_ao;  // global index of the first AO
+    int l;         // 0 s, 1 p, 2 d
+    int nprim;
+    double center[3];
+    double exps[kMaxPrim];   // ascending
+    double coefs[kMaxPrim];  // normalised
+};
+
+// Device-side view of one grid block.
+struct BlockDesc {
+    int64_t scratch_off;  // offset (doubles) of this block's AO values in the chunk scratch
+    int pt_begin;         // first point (in block-sorted order; blocks are padded to 128 slots)
+    int npts;             // real points (<= 128); slots npts..127 have weight 0
+    int shell_begin;      // offset into the block-shell list
+    int nshell;
+    int nao;              // screened AO columns N_b
+    int ldn;              // N_b padded to a multiple of kColPad
+    int col_begin;        // offset into the block-AO-index list
+};
+
+struct DeviceShell {
+    double cx, cy, cz;
+    int l, nprim;
+    int first_ao;
+    int pad;
+    double exps[kMaxPrim];
+    double coefs[kMaxPrim];
+};
+
+struct Chunk {
+    int block_begin, block_end;  // range in launch order
+    int64_t scratch_doubles;
+};
+
+struct Context {
+    pdfxc_opts opts;
+    int n_aos = 0;
+    std::vector<Shell> shells;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+
+    // grid (host copies in caller order)
+    int64_t n_pts = 0;
+    std::vector<double> gx, gy, gz, gw;
+
+    // blocks
+    std::vector<BlockDesc> blocks;     // launch order (descending cost)
+    std::vector<int> block_shells;     // global shell ids per block
+    std::vector<int> block_cols;       // global AO index per block column
+    std::vector<int> block_shell_col;  // local first column of each block shell
+    std::vector<int64_t> slot_of_point;  // caller point -> slot (block*128 + k)
+    std::vector<Chunk> chunks;
+    double flops = 0.0;
+    double mean_nao = 0.0;
+    int max_nao = 0;
+
+    // device buffers
+    DeviceShell* d_shells = nullptr;
+    double* d_pts = nullptr;  // 4 x n_slots SoA: x, y, z, w
+    int64_t n_slots = 0;
+    BlockDesc* d_blocks = nullptr;
+    int* d_block_shells = nullptr;
+    int* d_block_cols = nullptr;
+    int* d_block_shell_col = nullptr;
+    double* d_scratch = nullptr;  // AO values of one chunk
+    int64_t scratch_doubles = 0;
+    double* d_coef = nullptr;     // per slot: (w vrho / 2, cx, cy, cz)
+    double* d_rho = nullptr;      // per slot: rho, grad rho (4)
+    double* d_eblock = nullptr;   // per block energy
+    double* d_P = nullptr;        // dense N x N
+    double* d_K = nullptr;        // dense N x N
+    double* d_E = nullptr;        // 1 double
+    double* h_pinned = nullptr;   // packed staging (pinned)
+
+    double last_build_ms = 0.0;
+    double last_kernel_ms[4] = {0, 0, 0, 0};
+};
+
+// error helpers
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define PDFXC_HIP(call)                                          \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ::pdfxc::hip_fail(e_, #call); \
+    } while (0)
+
+// kernel launchers (pdfxc_kernels.hip)
+int launch_ao_eval(Context& c, const Chunk& ch);
+int launch_rho_vxc(Context& c, const Chunk& ch);
+int launch_kmat(Context& c, const Chunk& ch);
+int launch_energy_sum(Context& c, double* dE);
+int launch_eval_functional(int64_t n, const double* d_in, double* d_out, hipStream_t s);
+int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, double* d_out,
+                           hipStream_t s);
+int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,
+                 const int* d_atom_of_pt, double* d_w, hipStream_t s);
+int launch_mfma_selftest(double* d_out, hipStream_t s);
+
+}  // namespace pdfxc

@@ -1,0 +1,203 @@
+"""ctypes binding of the engine library (include/pdfxc.h).
+
+The library is built in-tree by ``python -m proteindf_amd.build`` (or
+``__graft_entry__.build()``) into ``proteindf_amd/_lib/libpdfxc.so``.  There
+is no fallback: if the library is missing or fails to load, every entry point
+raises.
+"""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libpdfxc.so"
+
+PDFXC_OK = 0
+PDFXC_B3LYP = 1
+PDFXC_FP64 = 0
+ERROR_NAMES = {1: "ARGUMENT", 2: "HIP", 3: "NO_GRID", 4: "UNSUPPORTED", 5: "MEMORY"}
+
+# every symbol declared in include/pdfxc.h
+EXPORTED = [
+    "pdfxc_default_opts", "pdfxc_normalize_cgto", "pdfxc_ctx_create", "pdfxc_ctx_destroy",
+    "pdfxc_set_grid", "pdfxc_generate_grid", "pdfxc_get_grid", "pdfxc_build_rks",
+    "pdfxc_build_rks_device", "pdfxc_eval_functional", "pdfxc_eval_aos", "pdfxc_get_stats",
+    "pdfxc_stream", "pdfxc_last_error", "pdfxc_selftest_mfma",
+]
+
+
+class PdfxcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pdfxc error {code} ({ERROR_NAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("density_cutoff", ctypes.c_double), ("ao_screen", ctypes.c_double),
+                ("precision", ctypes.c_int), ("block_points", ctypes.c_int),
+                ("scratch_bytes", ctypes.c_int64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("n_points", ctypes.c_int64), ("n_blocks", ctypes.c_int64),
+                ("n_chunks", ctypes.c_int64), ("flops", ctypes.c_double),
+                ("mean_nao", ctypes.c_double), ("max_nao", ctypes.c_int),
+                ("last_build_ms", ctypes.c_double), ("last_kernel_ms", ctypes.c_double * 4)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"engine library not built: {LIB_PATH} (run python -m proteindf_amd.build)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        P, i32, i64, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+        L.pdfxc_default_opts.argtypes = [P]
+        L.pdfxc_default_opts.restype = None
+        L.pdfxc_normalize_cgto.argtypes = [i32, i32, P, P, P, P]
+        L.pdfxc_ctx_create.argtypes = [P, i32, P, P, P, P, P, i32, P]
+        L.pdfxc_ctx_destroy.argtypes = [P]
+        L.pdfxc_set_grid.argtypes = [P, i64, P, P, P, P]
+        L.pdfxc_generate_grid.argtypes = [P, i32, P, P, i32, i32, d, P]
+        L.pdfxc_get_grid.argtypes = [P, P, P, P, P]
+        L.pdfxc_build_rks.argtypes = [P, i32, P, P, P, P]
+        L.pdfxc_build_rks_device.argtypes = [P, i32, P, P, P]
+        L.pdfxc_eval_functional.argtypes = [i32, i64, P, P]
+        L.pdfxc_eval_aos.argtypes = [P, i64, P, P, P, P, P]
+        L.pdfxc_get_stats.argtypes = [P, P]
+        L.pdfxc_stream.argtypes = [P]
+        L.pdfxc_stream.restype = P
+        L.pdfxc_last_error.argtypes = []
+        L.pdfxc_last_error.restype = ctypes.c_char_p
+        L.pdfxc_selftest_mfma.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(code):
+    if code != PDFXC_OK:
+        raise PdfxcError(code, lib().pdfxc_last_error().decode())
+
+
+def default_opts(**kw):
+    o = Opts()
+    lib().pdfxc_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def normalize_cgto(shell, exps, coefs):
+    e = np.ascontiguousarray(exps, dtype=np.float64)
+    c = np.ascontiguousarray(coefs, dtype=np.float64)
+    oe, oc = np.zeros_like(e), np.zeros_like(c)
+    check(lib().pdfxc_normalize_cgto(shell, len(e), _p(e), _p(c), _p(oe), _p(oc)))
+    return oe, oc
+
+
+def eval_functional(rho_a, gamma_aa, functional=PDFXC_B3LYP):
+    """Closed-shell functional kernel: returns (n, 4) f, df/drhoA, df/dgAA, df/dgAB."""
+    x = np.ascontiguousarray(np.stack([np.ravel(rho_a), np.ravel(gamma_aa)], 1), dtype=np.float64)
+    out = np.zeros((x.shape[0], 4))
+    check(lib().pdfxc_eval_functional(functional, x.shape[0], _p(x), _p(out)))
+    return out
+
+
+def selftest_mfma():
+    out = np.zeros((16, 16))
+    check(lib().pdfxc_selftest_mfma(_p(out)))
+    return out
+
+
+class XcContext:
+    """One engine context: AO table + grid + device buffers (pdfxc_ctx)."""
+
+    def __init__(self, info, opts=None, max_prim=None):
+        n = info.n_aos
+        if max_prim is None:
+            max_prim = int(max(info.ao_nprim))
+        self.n_aos = n
+        E, C = info.padded(max_prim)
+        self._center = np.ascontiguousarray(info.ao_center, dtype=np.float64)
+        self._type = np.ascontiguousarray(info.ao_type, dtype=np.int32)
+        self._nprim = np.ascontiguousarray(info.ao_nprim, dtype=np.int32)
+        self._exps = np.ascontiguousarray(E)
+        self._coefs = np.ascontiguousarray(C)
+        self.opts = opts if opts is not None else default_opts()
+        h = ctypes.c_void_p()
+        check(lib().pdfxc_ctx_create(ctypes.byref(h), n, _p(self._center), _p(self._type),
+                                     _p(self._nprim), _p(self._exps), _p(self._coefs), max_prim,
+                                     ctypes.byref(self.opts)))
+        self.h = h
+        self.n_pts = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pdfxc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_grid(self, grid):
+        g = np.asarray(grid, dtype=np.float64).reshape(-1, 4)
+        cols = [np.ascontiguousarray(g[:, k]) for k in range(4)]
+        check(lib().pdfxc_set_grid(self.h, g.shape[0], *[_p(c) for c in cols]))
+        self.n_pts = g.shape[0]
+
+    def generate_grid(self, Z, xyz, n_rad, n_ang, weight_cutoff=1e-16):
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = ctypes.c_int64()
+        check(lib().pdfxc_generate_grid(self.h, len(Z), _p(Z), _p(xyz), n_rad, n_ang,
+                                        weight_cutoff, ctypes.byref(n)))
+        self.n_pts = n.value
+        return n.value
+
+    def get_grid(self):
+        cols = [np.zeros(self.n_pts) for _ in range(4)]
+        check(lib().pdfxc_get_grid(self.h, *[_p(c) for c in cols]))
+        return np.stack(cols, 1)
+
+    def build_rks(self, P_alpha_packed, K_alpha_packed=None, want_rho=False,
+                  functional=PDFXC_B3LYP):
+        P = np.ascontiguousarray(P_alpha_packed, dtype=np.float64)
+        n = self.n_aos
+        if P.shape != (n * (n + 1) // 2,):
+            raise ValueError("P_alpha_packed must hold N(N+1)/2 values")
+        K = np.zeros_like(P) if K_alpha_packed is None else K_alpha_packed
+        if K.dtype != np.float64 or not K.flags.c_contiguous or K.shape != P.shape:
+            raise ValueError("K_alpha_packed must be a contiguous float64 array of N(N+1)/2 values")
+        E = ctypes.c_double()
+        rho = np.zeros((self.n_pts, 4)) if want_rho else None
+        check(lib().pdfxc_build_rks(self.h, functional, _p(P), _p(K), ctypes.byref(E), _p(rho)))
+        return E.value, K, rho
+
+    def build_rks_device(self, dP_ptr, dK_ptr, dE_ptr, functional=PDFXC_B3LYP):
+        check(lib().pdfxc_build_rks_device(self.h, functional, ctypes.c_void_p(dP_ptr),
+                                           ctypes.c_void_p(dK_ptr), ctypes.c_void_p(dE_ptr)))
+
+    def eval_aos(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        outs = [np.zeros((n, self.n_aos)) for _ in range(4)]
+        check(lib().pdfxc_eval_aos(self.h, n, _p(xyz), *[_p(o) for o in outs]))
+        return tuple(outs)
+
+    def stats(self):
+        s = Stats()
+        check(lib().pdfxc_get_stats(self.h, ctypes.byref(s)))
+        return s
+
+    def stream(self):
+        return lib().pdfxc_stream(self.h)
