@@ -142,6 +142,11 @@ const char* pdfxc_last_error(void);
  * confirm the lane layout the kernels rely on. */
 int pdfxc_selftest_mfma(double* out16x16);
 
+/* Diagnostics: a plain streaming copy of n doubles between device buffers
+ * (reads 8n bytes, writes 8n bytes), the reference point for HBM traffic
+ * counters of the other kernels. */
+int pdfxc_selftest_copy(const double* d_src, double* d_dst, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -999,4 +999,11 @@ int pdfxc_selftest_mfma(double* out16x16) {
     return rc;
 }
 
+int pdfxc_selftest_copy(const double* d_src, double* d_dst, int64_t n) {
+    if (n < 0 || (n > 0 && (!d_src || !d_dst))) return arg_error("pdfxc_selftest_copy: bad arguments");
+    int rc = launch_copy(d_src, d_dst, n, nullptr);
+    if (rc == PDFXC_OK) PDFXC_HIP(hipDeviceSynchronize());
+    return rc;
+}
+
 }  // extern "C"

@@ -119,5 +119,6 @@ int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, dou
 int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,
                  const int* d_atom_of_pt, double* d_w, hipStream_t s);
 int launch_mfma_selftest(double* d_out, hipStream_t s);
+int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 
 }  // namespace pdfxc

@@ -565,6 +565,20 @@ __global__ void k_mfma_selftest(double* out) {
     for (int r = 0; r < 4; ++r) out[(4 * r + (l >> 4)) * 16 + (l & 15)] = acc[r];
 }
 
+// plain streaming copy: the reference point for HBM traffic counters
+__global__ __launch_bounds__(256) void k_copy(const double* __restrict__ src, double* __restrict__ dst,
+                                              int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return PDFXC_OK;
+    const int64_t g = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)g), dim3(256), 0, s, src, dst, n);
+    return check_launch("k_copy");
+}
+
 int launch_mfma_selftest(double* d_out, hipStream_t s) {
     hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, s, d_out);
     return check_launch("k_mfma_selftest");

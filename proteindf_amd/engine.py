@@ -22,7 +22,7 @@ EXPORTED = [
     "pdfxc_default_opts", "pdfxc_normalize_cgto", "pdfxc_ctx_create", "pdfxc_ctx_destroy",
     "pdfxc_set_grid", "pdfxc_generate_grid", "pdfxc_get_grid", "pdfxc_build_rks",
     "pdfxc_build_rks_device", "pdfxc_eval_functional", "pdfxc_eval_aos", "pdfxc_get_stats",
-    "pdfxc_stream", "pdfxc_last_error", "pdfxc_selftest_mfma",
+    "pdfxc_stream", "pdfxc_last_error", "pdfxc_selftest_mfma", "pdfxc_selftest_copy",
 ]
 
 
@@ -73,6 +73,7 @@ def lib():
         L.pdfxc_last_error.argtypes = []
         L.pdfxc_last_error.restype = ctypes.c_char_p
         L.pdfxc_selftest_mfma.argtypes = [P]
+        L.pdfxc_selftest_copy.argtypes = [P, P, i64]
         _lib = L
     return _lib
 
