@@ -50,18 +50,18 @@ typedef struct pdfxc_opts {
                                 DfCalcGridX.cpp:42, :690-700) */
     int precision;           /* PDFXC_FP64 */
     int block_points;        /* points per grid block: 128 */
-    int64_t scratch_bytes;   /* device scratch budget for AO values per launch chunk; 0 = 8 GiB */
+    int64_t scratch_bytes;   /* reserved, must be 0 (AO values are never stored in HBM) */
 } pdfxc_opts;
 
 typedef struct pdfxc_stats {
     int64_t n_points;        /* grid points */
     int64_t n_blocks;        /* grid blocks */
-    int64_t n_chunks;        /* launch chunks per build */
+    int64_t n_chunks;        /* kernel launches per build (one per kernel instantiation class) */
     double flops;            /* algorithmic sum_b 4 n_b N_b^2 (N_b = screened AOs of block b) */
     double mean_nao;         /* mean N_b */
     int max_nao;             /* max N_b */
     double last_build_ms;    /* device time of the last build (HIP events) */
-    double last_kernel_ms[4];/* per kernel: AO eval, rho+functional+V, K assembly, other */
+    double last_kernel_ms[4];/* per kernel: k_rho (rho + functional), k_kmat (K_xc), energy sum, unused */
 } pdfxc_stats;
 
 typedef struct pdfxc_ctx pdfxc_ctx;

@@ -3,7 +3,8 @@
 hipcc compiles the host engine and the gfx950 kernels into one shared library
 with a plain C ABI (include/pdfxc.h).  Packed FP32 instructions are disabled
 (-packed-fp32-ops); FP64 atomics use the hardware instruction
-(-munsafe-fp-atomics).
+(-munsafe-fp-atomics); MFMA accumulators stay in VGPRs
+(-amdgpu-mfma-vgpr-form) so the accumulate loops do not copy them to AGPRs.
 
     python -m proteindf_amd.build
 """
@@ -33,6 +34,7 @@ def build(verbose=False):
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
            "-munsafe-fp-atomics", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-mllvm", "-amdgpu-mfma-vgpr-form=1",
            "-Wall", "-Wno-unused-result", f"-I{ROOT / 'include'}", "-o", str(tmp)]
     cmd += [str(s) for s in SOURCES]
     if verbose:

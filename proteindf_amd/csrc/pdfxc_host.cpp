@@ -1,5 +1,5 @@
 // Host side of the pdfxc engine: the C ABI, the AO shell table, grid-block
-// partitioning with per-block AO screening, launch-chunk planning, device
+// partitioning with per-block AO screening, per-class work lists, device
 // buffers and grid generation.  Kernels live in pdfxc_kernels.hip.
 #include <algorithm>
 #include <cmath>
@@ -42,7 +42,7 @@ static int dalloc(T** p, size_t count) {
 
 template <typename T>
 static void dfree(T*& p) {
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFree(reinterpret_cast<void*>(p));
     p = nullptr;
 }
 
@@ -200,6 +200,13 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
         c->shells.push_back(s);
         i += ncomp;
     }
+    {
+        const int rc0 = configure_kernels();
+        if (rc0 != PDFXC_OK) {
+            delete c;
+            return rc0;
+        }
+    }
     hipError_t e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&c->ev[k]);
@@ -224,7 +231,28 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
             d.coefs[q] = s.coefs[q];
         }
     }
+    int maxp = 1;
+    for (const Shell& sh : c->shells) maxp = std::max(maxp, sh.nprim);
+    c->rec_stride = 4 + 2 * maxp;
+    std::vector<double> rec(c->shells.size() * (size_t)c->rec_stride, 0.0);
+    for (size_t k = 0; k < c->shells.size(); ++k) {
+        const Shell& sh = c->shells[k];
+        double* r = rec.data() + k * c->rec_stride;
+        r[0] = sh.center[0];
+        r[1] = sh.center[1];
+        r[2] = sh.center[2];
+        r[3] = (double)(sh.l + 4 * sh.nprim);
+        for (int q = 0; q < sh.nprim; ++q) {
+            r[4 + 2 * q] = sh.exps[q];
+            r[5 + 2 * q] = sh.coefs[q];
+        }
+    }
     int rc = dalloc(&c->d_shells, ds.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c->d_shrec, rec.size());
+    if (rc == PDFXC_OK) {
+        e = hipMemcpy(c->d_shrec, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_ctx_create: shell records upload");
+    }
     if (rc == PDFXC_OK) rc = dalloc(&c->d_P, (size_t)n_aos * n_aos);
     if (rc == PDFXC_OK) rc = dalloc(&c->d_K, (size_t)n_aos * n_aos);
     if (rc == PDFXC_OK) rc = dalloc(&c->d_E, 1);
@@ -251,7 +279,10 @@ static void free_grid_buffers(Context* c) {
     dfree(c->d_block_shells);
     dfree(c->d_block_cols);
     dfree(c->d_block_shell_col);
-    dfree(c->d_scratch);
+    dfree(c->d_block_slab);
+    dfree(c->d_block_sorder);
+    dfree(c->d_rho_work);
+    dfree(c->d_k_work);
     dfree(c->d_coef);
     dfree(c->d_rho);
     dfree(c->d_eblock);
@@ -263,6 +294,7 @@ int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_grid_buffers(c);
     dfree(c->d_shells);
+    dfree(c->d_shrec);
     dfree(c->d_P);
     dfree(c->d_K);
     dfree(c->d_E);
@@ -400,13 +432,14 @@ static int build_blocks(Context& c) {
     c.block_shells.clear();
     c.block_cols.clear();
     c.block_shell_col.clear();
+    c.block_slab.clear();
+    c.block_sorder.clear();
     c.n_slots = (int64_t)nb * kBlockPts;
     c.slot_of_point.assign(c.n_pts, -1);
     std::vector<double> pts(4 * (size_t)c.n_slots, 0.0);
     c.flops = 0.0;
     double sum_nao = 0.0;
     c.max_nao = 0;
-    int64_t scratch_off = 0;
     for (int li = 0; li < nb; ++li) {
         const Tmp& t = tmp[lorder[li]];
         BlockDesc& B = c.blocks[li];
@@ -417,15 +450,38 @@ static int build_blocks(Context& c) {
         B.col_begin = (int)c.block_cols.size();
         B.nao = t.nao;
         B.ldn = std::max(kColPad, (t.nao + kColPad - 1) / kColPad * kColPad);
-        B.scratch_off = scratch_off;
-        scratch_off += 4 * (int64_t)B.ldn * kBlockPts;
+        B.slab_begin = (int)c.block_slab.size();
+        if (B.ldn > 1024) {
+            set_error("set_grid: a block keeps " + std::to_string(B.nao) +
+                      " AOs; more than 1024 per block is not supported (raise ao_screen)");
+            return PDFXC_ERR_UNSUPPORTED;
+        }
         int col = 0;
+        std::vector<int> ends;
         for (int s : t.sh) {
             c.block_shells.push_back(s);
             c.block_shell_col.push_back(col);
             const Shell& S = c.shells[s];
             for (int m = 0; m < 2 * S.l + 1; ++m) c.block_cols.push_back(S.first_ao + m);
             col += 2 * S.l + 1;
+            ends.push_back(col);
+        }
+        // evaluation order: shells of equal (l, nprim) side by side, so that a
+        // wave's lanes mostly run the same code path
+        {
+            std::vector<int> so(B.nshell);
+            std::iota(so.begin(), so.end(), 0);
+            std::stable_sort(so.begin(), so.end(), [&](int a, int b) {
+                const Shell& A = c.shells[t.sh[a]];
+                const Shell& Bs = c.shells[t.sh[b]];
+                return A.l != Bs.l ? A.l < Bs.l : A.nprim < Bs.nprim;
+            });
+            c.block_sorder.insert(c.block_sorder.end(), so.begin(), so.end());
+        }
+        // first local shell whose columns reach 16 s, for s = 0 .. ldn/16
+        for (int s = 0, sh = 0; s <= B.ldn / 16; ++s) {
+            while (sh < B.nshell && ends[sh] <= 16 * s) ++sh;
+            c.block_slab.push_back(sh);
         }
         for (int k = 0; k < kBlockPts; ++k) {
             const int64_t slot = (int64_t)li * kBlockPts + k;
@@ -441,26 +497,80 @@ static int build_blocks(Context& c) {
         c.max_nao = std::max(c.max_nao, B.nao);
     }
     c.mean_nao = nb ? sum_nao / nb : 0.0;
-    // launch chunks bounded by the scratch budget
-    const int64_t budget = (c.opts.scratch_bytes > 0 ? c.opts.scratch_bytes : (int64_t)8 << 30) /
-                           (int64_t)sizeof(double);
-    c.chunks.clear();
-    int64_t maxchunk = 0;
-    for (int b = 0; b < nb;) {
-        Chunk ch;
-        ch.block_begin = b;
-        int64_t used = 0;
-        while (b < nb) {
-            const int64_t need = 4 * (int64_t)c.blocks[b].ldn * kBlockPts;
-            if (used > 0 && used + need > budget) break;
-            used += need;
-            ++b;
+
+    // k_rho work lists by template class: ldn <= 128, 256 (32 points per
+    // workgroup), <= 512, 1024 (16 points per workgroup)
+    std::vector<int> rho_work;
+    c.rho_launches.clear();
+    const int RS = c.rec_stride;
+    for (int cls = 0; cls < 4; ++cls) {
+        const int lim = 128 << cls;
+        const int lo_lim = cls ? (64 << cls) : 0;
+        const int pb = (cls < 2) ? 32 : 16;
+        LaunchRange r{cls, (int)rho_work.size(), 0, 0};
+        size_t maxdyn = 0;
+        for (int b = 0; b < nb; ++b) {
+            const BlockDesc& B = c.blocks[b];
+            if (B.ldn > lo_lim && B.ldn <= lim) {
+                const size_t lds = sizeof(double) * ((size_t)std::max(B.ldn, 256) * pb + (size_t)B.nshell * RS) +
+                                   sizeof(int) * (size_t)(B.ldn + 3 * B.nshell + B.ldn / 16 + 1);
+                if (lds > (size_t)kMaxDynLds) {
+                    set_error("set_grid: a block with " + std::to_string(B.nao) + " AOs and " +
+                              std::to_string(B.nshell) + " shells needs more LDS than a workgroup has "
+                              "(raise ao_screen)");
+                    return PDFXC_ERR_UNSUPPORTED;
+                }
+                rho_work.push_back(b);
+                maxdyn = std::max(maxdyn, lds);
+            }
         }
-        ch.block_end = b;
-        ch.scratch_doubles = used;
-        maxchunk = std::max(maxchunk, used);
-        c.chunks.push_back(ch);
+        r.count = (int)rho_work.size() - r.offset;
+        r.lds = maxdyn;
+        if (r.count) c.rho_launches.push_back(r);
     }
+    // k_kmat work: lower tiles split into items of at most 4 x MAXT tiles
+    std::vector<KWork> kw[2];
+    size_t klds[2] = {0, 0};
+    for (int b = 0; b < nb; ++b) {
+        const BlockDesc& B = c.blocks[b];
+        const int T = B.ldn / 16;
+        const int nt = T * (T + 1) / 2;
+        const int cls = (nt <= 32) ? 0 : 1;
+        const int per = (cls == 0) ? 32 : 64;
+        for (int t0 = 0; t0 < nt; t0 += per) {
+            KWork w;
+            w.block = b;
+            w.t0 = t0;
+            w.t1 = std::min(nt, t0 + per);
+            int I = 0;
+            while ((I + 1) * (I + 2) / 2 <= w.t1 - 1) ++I;
+            w.ncols = 16 * (I + 1);
+            int nsh = 0;
+            while (nsh < B.nshell && c.block_shell_col[B.shell_begin + nsh] < w.ncols) ++nsh;
+            w.nsh = nsh;
+            size_t lds = 0;
+            for (int ps = 16; ps >= 4; ps /= 2) {
+                w.ps = ps;
+                lds = sizeof(double) * (2 * (size_t)ps * (w.ncols + 2) + (size_t)nsh * RS) +
+                      sizeof(int) * (size_t)(w.ncols + 2 * nsh);
+                if (lds <= (size_t)kMaxDynLds) break;
+            }
+            if (lds > (size_t)kMaxDynLds) {
+                set_error("set_grid: block too wide for the K kernel's LDS");
+                return PDFXC_ERR_UNSUPPORTED;
+            }
+            klds[cls] = std::max(klds[cls], lds);
+            kw[cls].push_back(w);
+        }
+    }
+    std::vector<KWork> kall;
+    c.kmat_launches.clear();
+    for (int cls = 0; cls < 2; ++cls) {
+        if (kw[cls].empty()) continue;
+        c.kmat_launches.push_back(LaunchRange{cls, (int)kall.size(), (int)kw[cls].size(), klds[cls]});
+        kall.insert(kall.end(), kw[cls].begin(), kw[cls].end());
+    }
+
     // device buffers
     free_grid_buffers(&c);
     int rc = dalloc(&c.d_pts, pts.size());
@@ -468,23 +578,29 @@ static int build_blocks(Context& c) {
     if (rc == PDFXC_OK) rc = dalloc(&c.d_block_shells, c.block_shells.size());
     if (rc == PDFXC_OK) rc = dalloc(&c.d_block_cols, c.block_cols.size());
     if (rc == PDFXC_OK) rc = dalloc(&c.d_block_shell_col, c.block_shell_col.size());
-    if (rc == PDFXC_OK) rc = dalloc(&c.d_scratch, (size_t)maxchunk);
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_block_slab, c.block_slab.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_block_sorder, c.block_sorder.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_rho_work, rho_work.size());
+    if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&c.d_k_work), kall.size() * sizeof(KWork));
     if (rc == PDFXC_OK) rc = dalloc(&c.d_coef, 4 * (size_t)c.n_slots);
     if (rc == PDFXC_OK) rc = dalloc(&c.d_rho, 4 * (size_t)c.n_slots);
-    if (rc == PDFXC_OK) rc = dalloc(&c.d_eblock, c.blocks.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_eblock, 8 * c.blocks.size());
     if (rc != PDFXC_OK) return rc;
-    c.scratch_doubles = maxchunk;
-    PDFXC_HIP(hipMemcpy(c.d_pts, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice));
-    PDFXC_HIP(hipMemcpy(c.d_blocks, c.blocks.data(), c.blocks.size() * sizeof(BlockDesc),
-                        hipMemcpyHostToDevice));
+    auto up = [](void* d, const void* h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
+    PDFXC_HIP(up(c.d_pts, pts.data(), pts.size() * sizeof(double)));
+    PDFXC_HIP(up(c.d_blocks, c.blocks.data(), c.blocks.size() * sizeof(BlockDesc)));
+    PDFXC_HIP(up(c.d_block_slab, c.block_slab.data(), c.block_slab.size() * sizeof(int)));
+    PDFXC_HIP(up(c.d_rho_work, rho_work.data(), rho_work.size() * sizeof(int)));
+    PDFXC_HIP(up(c.d_k_work, kall.data(), kall.size() * sizeof(KWork)));
     if (!c.block_shells.empty()) {
-        PDFXC_HIP(hipMemcpy(c.d_block_shells, c.block_shells.data(), c.block_shells.size() * sizeof(int),
-                            hipMemcpyHostToDevice));
-        PDFXC_HIP(hipMemcpy(c.d_block_shell_col, c.block_shell_col.data(),
-                            c.block_shell_col.size() * sizeof(int), hipMemcpyHostToDevice));
-        PDFXC_HIP(hipMemcpy(c.d_block_cols, c.block_cols.data(), c.block_cols.size() * sizeof(int),
-                            hipMemcpyHostToDevice));
+        PDFXC_HIP(up(c.d_block_shells, c.block_shells.data(), c.block_shells.size() * sizeof(int)));
+        PDFXC_HIP(up(c.d_block_shell_col, c.block_shell_col.data(), c.block_shell_col.size() * sizeof(int)));
+        PDFXC_HIP(up(c.d_block_cols, c.block_cols.data(), c.block_cols.size() * sizeof(int)));
+        PDFXC_HIP(up(c.d_block_sorder, c.block_sorder.data(), c.block_sorder.size() * sizeof(int)));
     }
+    PDFXC_HIP(hipMemset(c.d_coef, 0, 4 * (size_t)c.n_slots * sizeof(double)));
+    PDFXC_HIP(hipMemset(c.d_rho, 0, 4 * (size_t)c.n_slots * sizeof(double)));
+    PDFXC_HIP(hipMemset(c.d_eblock, 0, 8 * c.blocks.size() * sizeof(double)));
     return PDFXC_OK;
 }
 
@@ -835,39 +951,28 @@ static int build_device(Context* c, const double* dP, double* dK, double* dE) {
     c->d_P = const_cast<double*>(dP);
     c->d_K = dK;
     int rc = PDFXC_OK;
-    float t[4] = {0, 0, 0, 0};
     PDFXC_HIP(hipEventRecord(c->ev[0], c->stream));
-    for (const Chunk& ch : c->chunks) {
-        PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
-        rc = launch_ao_eval(*c, ch);
-        if (rc) break;
-        PDFXC_HIP(hipEventRecord(c->ev[2], c->stream));
-        rc = launch_rho_vxc(*c, ch);
-        if (rc) break;
-        PDFXC_HIP(hipEventRecord(c->ev[3], c->stream));
-        rc = launch_kmat(*c, ch);
-        if (rc) break;
-        PDFXC_HIP(hipEventRecord(c->ev[4], c->stream));
-        PDFXC_HIP(hipEventSynchronize(c->ev[4]));
-        float a, b, d;
-        PDFXC_HIP(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
-        PDFXC_HIP(hipEventElapsedTime(&b, c->ev[2], c->ev[3]));
-        PDFXC_HIP(hipEventElapsedTime(&d, c->ev[3], c->ev[4]));
-        t[0] += a;
-        t[1] += b;
-        t[2] += d;
-    }
+    for (const LaunchRange& r : c->rho_launches)
+        if (rc == PDFXC_OK) rc = launch_rho(*c, r);
+    PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
+    for (const LaunchRange& r : c->kmat_launches)
+        if (rc == PDFXC_OK) rc = launch_kmat(*c, r);
+    PDFXC_HIP(hipEventRecord(c->ev[2], c->stream));
     if (rc == PDFXC_OK) rc = launch_energy_sum(*c, dE);
+    PDFXC_HIP(hipEventRecord(c->ev[3], c->stream));
     c->d_P = saveP;
     c->d_K = saveK;
     if (rc) return rc;
-    PDFXC_HIP(hipEventRecord(c->ev[5], c->stream));
-    PDFXC_HIP(hipEventSynchronize(c->ev[5]));
-    float tot;
-    PDFXC_HIP(hipEventElapsedTime(&tot, c->ev[0], c->ev[5]));
-    c->last_build_ms = tot;
-    for (int k = 0; k < 3; ++k) c->last_kernel_ms[k] = t[k];
-    c->last_kernel_ms[3] = tot - (t[0] + t[1] + t[2]);
+    PDFXC_HIP(hipEventSynchronize(c->ev[3]));
+    float t1, t2, t3;
+    PDFXC_HIP(hipEventElapsedTime(&t1, c->ev[0], c->ev[1]));
+    PDFXC_HIP(hipEventElapsedTime(&t2, c->ev[1], c->ev[2]));
+    PDFXC_HIP(hipEventElapsedTime(&t3, c->ev[2], c->ev[3]));
+    c->last_kernel_ms[0] = t1;
+    c->last_kernel_ms[1] = t2;
+    c->last_kernel_ms[2] = t3;
+    c->last_kernel_ms[3] = 0.0;
+    c->last_build_ms = t1 + t2 + t3;
     return PDFXC_OK;
 }
 
@@ -976,7 +1081,7 @@ int pdfxc_get_stats(pdfxc_ctx* ctx, pdfxc_stats* s) {
     if (!c || !s) return arg_error("pdfxc_get_stats: bad arguments");
     s->n_points = c->n_pts;
     s->n_blocks = (int64_t)c->blocks.size();
-    s->n_chunks = (int64_t)c->chunks.size();
+    s->n_chunks = (int64_t)(c->rho_launches.size() + c->kmat_launches.size());
     s->flops = c->flops;
     s->mean_nao = c->mean_nao;
     s->max_nao = c->max_nao;

@@ -27,16 +27,31 @@ struct Shell {
     double coefs[kMaxPrim];  // normalised
 };
 
+constexpr int kMaxDynLds = 160 * 1024 - 16 * 1024;  // dynamic LDS ceiling per workgroup (gfx950: 160 KiB total)
+
 // Device-side view of one grid block.
 struct BlockDesc {
-    int64_t scratch_off;  // offset (doubles) of this block's AO values in the chunk scratch
-    int pt_begin;         // first point (in block-sorted order; blocks are padded to 128 slots)
-    int npts;             // real points (<= 128); slots npts..127 have weight 0
-    int shell_begin;      // offset into the block-shell list
+    int pt_begin;     // first slot (block-sorted order; every block owns 128 slots)
+    int npts;         // real points (<= 128); slots npts..127 have weight 0
+    int shell_begin;  // offset into the block-shell list
     int nshell;
-    int nao;              // screened AO columns N_b
-    int ldn;              // N_b padded to a multiple of kColPad
-    int col_begin;        // offset into the block-AO-index list
+    int nao;          // screened AO columns N_b
+    int ldn;          // N_b padded to a multiple of kColPad
+    int col_begin;    // offset into the block-AO-index list
+    int slab_begin;   // offset into the per-16-column first-shell table (ldn/16 + 1 entries)
+};
+
+// One k_kmat work item: lower 16 x 16 tiles [t0, t1) of a block.
+struct KWork {
+    int block, t0, t1, ncols, nsh, ps;
+};
+
+// A launch of one kernel instantiation over a contiguous range of a work list.
+struct LaunchRange {
+    int cls;     // template instantiation
+    int offset;  // first work item
+    int count;   // work items (blocks for k_rho, KWork for k_kmat)
+    size_t lds;  // dynamic LDS bytes
 };
 
 struct DeviceShell {
@@ -46,11 +61,6 @@ struct DeviceShell {
     int pad;
     double exps[kMaxPrim];
     double coefs[kMaxPrim];
-};
-
-struct Chunk {
-    int block_begin, block_end;  // range in launch order
-    int64_t scratch_doubles;
 };
 
 struct Context {
@@ -70,25 +80,31 @@ struct Context {
     std::vector<int> block_shells;     // global shell ids per block
     std::vector<int> block_cols;       // global AO index per block column
     std::vector<int> block_shell_col;  // local first column of each block shell
+    std::vector<int> block_slab;       // first local shell reaching column 16 s, per block
+    std::vector<int> block_sorder;     // local shells of each block ordered by (l, nprim)
     std::vector<int64_t> slot_of_point;  // caller point -> slot (block*128 + k)
-    std::vector<Chunk> chunks;
+    std::vector<LaunchRange> rho_launches, kmat_launches;
     double flops = 0.0;
     double mean_nao = 0.0;
     int max_nao = 0;
 
     // device buffers
     DeviceShell* d_shells = nullptr;
+    double* d_shrec = nullptr;  // per shell: cx, cy, cz, l + 4 nprim, (exp, coef) x maxprim
+    int rec_stride = 0;
     double* d_pts = nullptr;  // 4 x n_slots SoA: x, y, z, w
     int64_t n_slots = 0;
     BlockDesc* d_blocks = nullptr;
     int* d_block_shells = nullptr;
     int* d_block_cols = nullptr;
     int* d_block_shell_col = nullptr;
-    double* d_scratch = nullptr;  // AO values of one chunk
-    int64_t scratch_doubles = 0;
+    int* d_block_slab = nullptr;
+    int* d_block_sorder = nullptr;
+    int* d_rho_work = nullptr;    // block indices grouped by k_rho class
+    void* d_k_work = nullptr;     // KWork items grouped by k_kmat class
     double* d_coef = nullptr;     // per slot: (w vrho / 2, cx, cy, cz)
     double* d_rho = nullptr;      // per slot: rho, grad rho (4)
-    double* d_eblock = nullptr;   // per block energy
+    double* d_eblock = nullptr;   // per block x 8 sub-blocks: energy
     double* d_P = nullptr;        // dense N x N
     double* d_K = nullptr;        // dense N x N
     double* d_E = nullptr;        // 1 double
@@ -109,9 +125,9 @@ int hip_fail(hipError_t e, const char* what);
     } while (0)
 
 // kernel launchers (pdfxc_kernels.hip)
-int launch_ao_eval(Context& c, const Chunk& ch);
-int launch_rho_vxc(Context& c, const Chunk& ch);
-int launch_kmat(Context& c, const Chunk& ch);
+int launch_rho(Context& c, const LaunchRange& r);
+int launch_kmat(Context& c, const LaunchRange& r);
+int configure_kernels();
 int launch_energy_sum(Context& c, double* dE);
 int launch_eval_functional(int64_t n, const double* d_in, double* d_out, hipStream_t s);
 int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, double* d_out,

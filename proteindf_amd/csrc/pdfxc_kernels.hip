@@ -1,14 +1,16 @@
 // HIP kernels of the grid XC build for gfx950 (CDNA4), FP64 path.
 //
-// Per grid block b (128 points, N_b screened AOs, ldn = N_b padded to 16):
-//   k_ao_eval   phi, dphi/dx, dphi/dy, dphi/dz of the block's AOs   -> scratch
-//               [4][ldn][128] (AO-major, points contiguous)
-//   k_rho_vxc   X = Phi_b P_bb on v_mfma_f64_16x16x4 with the row dots
-//               rho = sum_j X_pj phi_pj, grad rho = 2 sum_j X_pj grad phi_pj fused,
-//               then the functional per point and V = (w vrho/2) phi + c . grad phi
-//               written over the dphi/dx plane; per-block energy sum.
-//   k_kmat      K_bb += Phi_b^T V_b + V_b^T Phi_b on MFMA for the lower tile
-//               pairs, scattered into the dense lower triangle of K.
+// Per grid block b (128 points, N_b screened AOs, ldn = N_b padded to 16),
+// AO values are evaluated where they are consumed and never stored in HBM:
+//   k_rho   for 32 (or 16) points of a block: X = Phi_b P_bb on
+//           v_mfma_f64_16x16x4 with X held in accumulators, Phi streamed in
+//           16-column slabs through LDS; then rho = sum_j X_pj phi_pj and
+//           grad rho = 2 sum_j X_pj grad phi_pj with phi, grad phi
+//           re-evaluated per 64-column chunk, then the B3LYP point terms.
+//   k_kmat  for a range of lower 16 x 16 tiles of K_bb (held in
+//           accumulators): Phi_b^T V_b + V_b^T Phi_b over the block's points
+//           in 16-point slabs, phi / grad phi / V evaluated into LDS, then
+//           scattered into the dense lower triangle of K.
 // The reference runs the same quantities point by point
 // (DfCalcGridX.cpp:1923-1962, :2137-2193, :2455-2500).
 #include <hip/hip_runtime.h>
@@ -80,164 +82,271 @@ __device__ inline void shell_values(const DeviceShell& S, double x, double y, do
     }
 }
 
-__global__ __launch_bounds__(256) void k_ao_eval(const BlockDesc* __restrict__ blocks, int b0,
-                                                 const double* __restrict__ pts, int64_t n_slots,
-                                                 const DeviceShell* __restrict__ shells,
-                                                 const int* __restrict__ bshells,
-                                                 const int* __restrict__ bshell_col,
-                                                 double* __restrict__ scratch,
-                                                 int64_t chunk_off) {
-    const BlockDesc B = blocks[b0 + blockIdx.x];
-    double* phi = scratch + (B.scratch_off - chunk_off);
-    const int64_t plane = (int64_t)B.ldn * kBlockPts;
-    const int p = threadIdx.x & (kBlockPts - 1);
-    const int half = threadIdx.x >> 7;
-    const int64_t slot = B.pt_begin + p;
-    const double px = pts[slot], py = pts[n_slots + slot], pz = pts[2 * n_slots + slot];
-    for (int s = half; s < B.nshell; s += 2) {
-        const DeviceShell& S = shells[bshells[B.shell_begin + s]];
-        const int col = bshell_col[B.shell_begin + s];
-        double v[5], vx[5], vy[5], vz[5];
-        shell_values(S, px - S.cx, py - S.cy, pz - S.cz, v, vx, vy, vz);
-        const int nc = 2 * S.l + 1;
-        for (int m = 0; m < nc; ++m) {
-            const int64_t o = (int64_t)(col + m) * kBlockPts + p;
-            phi[o] = v[m];
-            phi[plane + o] = vx[m];
-            phi[2 * plane + o] = vy[m];
-            phi[3 * plane + o] = vz[m];
+// ---------------------------------------------------------------------------
+// Shell records.  Every shell has a record of RS doubles in global memory:
+// [cx, cy, cz, l + 4 nprim, e0, c0, e1, c1, ...]; a block copies the records
+// it needs into LDS once, so that AO evaluation reads no global memory.
+// ---------------------------------------------------------------------------
+__device__ inline void rec_values(const double* rec, double px, double py, double pz, double* v,
+                                  double* vx, double* vy, double* vz) {
+    const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+    const int meta = (int)rec[3];
+    const int l = meta & 3, nprim = meta >> 2;
+    const double r2 = x * x + y * y + z * z;
+    double R0 = 0.0, R1 = 0.0;
+    for (int k = 0; k < nprim; ++k) {
+        const double a = rec[4 + 2 * k];
+        const double sh = a * r2;
+        if (sh <= kTooBig) {
+            const double g = rec[5 + 2 * k] * exp(-sh);
+            R0 += g;
+            R1 += -2.0 * a * g;
         }
     }
-    for (int c = B.nao + half; c < B.ldn; c += 2) {
-        const int64_t o = (int64_t)c * kBlockPts + p;
-        phi[o] = 0.0;
-        phi[plane + o] = 0.0;
-        phi[2 * plane + o] = 0.0;
-        phi[3 * plane + o] = 0.0;
+    if (l == 0) {
+        v[0] = R0;
+        vx[0] = x * R1;
+        vy[0] = y * R1;
+        vz[0] = z * R1;
+    } else if (l == 1) {
+        v[0] = x * R0;
+        v[1] = y * R0;
+        v[2] = z * R0;
+        vx[0] = R0 + x * x * R1;
+        vy[0] = x * y * R1;
+        vz[0] = x * z * R1;
+        vx[1] = y * x * R1;
+        vy[1] = R0 + y * y * R1;
+        vz[1] = y * z * R1;
+        vx[2] = z * x * R1;
+        vy[2] = z * y * R1;
+        vz[2] = R0 + z * z * R1;
+    } else {
+        // 4 xy, 5 xz, 6 yz, 7 (x^2-y^2)/2, 8 (z^2-(x^2+y^2)/2)/sqrt3
+        const double f[5] = {x * y, z * x, y * z, 0.5 * (x * x - y * y),
+                             kInvSqrt3 * (z * z - 0.5 * (x * x + y * y))};
+        const double fx[5] = {y, z, 0.0, x, -kInvSqrt3 * x};
+        const double fy[5] = {x, 0.0, z, -y, -kInvSqrt3 * y};
+        const double fz[5] = {0.0, x, y, 0.0, 2.0 * kInvSqrt3 * z};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            v[m] = f[m] * R0;
+            vx[m] = fx[m] * R0 + f[m] * x * R1;
+            vy[m] = fy[m] * R0 + f[m] * y * R1;
+            vz[m] = fz[m] * R0 + f[m] * z * R1;
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// X = Phi_b P_bb and the density at the block's points, then the functional
-// (buildVxc, DfCalcGridX.cpp:2137-2193) and V_b.  4 waves; wave w owns rows
-// 32w..32w+31 (two 16-row MFMA tiles) of a 128 x 64 output tile, the
-// workgroup sweeps the 64-column tiles of X.
+// rho pass for PB = 16 RT points of a block: X^T = P_bb Phi^T on
+// v_mfma_f64_16x16x4 (rows = AOs, columns = points) with X^T held in
+// accumulators; wave w owns the AO tiles w, w + 4, ....  Phi of all ldn
+// columns is evaluated into LDS once; the A operand comes from P in global
+// memory (L2), prefetched one group of G k-steps ahead.  Epilogue: phi and
+// grad phi per 64-column chunk (one AO tile per wave) evaluated into the same
+// LDS and dotted with X^T: rho = sum_j X_pj phi_pj, grad rho = 2 sum_j X_pj
+// grad phi_pj (calcRho_GGA_part, DfCalcGridX.cpp:1923-1962); then the B3LYP
+// point terms (buildVxc, :2137-2193): coef = (w vrho / 2, w (2 vgAA + vgAB)
+// grad rho) and the weighted energy.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rho_vxc(const BlockDesc* __restrict__ blocks, int b0,
-                                                 const double* __restrict__ pts, int64_t n_slots,
-                                                 const double* __restrict__ P, int N,
-                                                 const int* __restrict__ bcols,
-                                                 double* __restrict__ scratch, int64_t chunk_off,
-                                                 double* __restrict__ coef_out,
-                                                 double* __restrict__ rho_out,
-                                                 double* __restrict__ eblock, double dens_cut) {
-    extern __shared__ int s_cols[];  // ldn column -> global AO (or -1)
-    __shared__ double s_rho[kBlockPts][4];
-    __shared__ double s_coef[kBlockPts][4];
-    __shared__ double s_e[kBlockPts];
+template <int RT, int MAXI, int G>
+__global__ __launch_bounds__(256) void k_rho(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
+                                             const double* __restrict__ pts, int64_t n_slots,
+                                             const double* __restrict__ shrec, int RS,
+                                             const int* __restrict__ bshells, const int* __restrict__ bshell_col,
+                                             const int* __restrict__ bcols, const int* __restrict__ bslab,
+                                             const int* __restrict__ bsorder, const double* __restrict__ P, int N,
+                                             double* __restrict__ coef_out, double* __restrict__ rho_out,
+                                             double* __restrict__ ewg, double dens_cut) {
+    constexpr int PB = 16 * RT;
+    constexpr int SUBS = kBlockPts / PB;
+    __shared__ double s_pt[PB][4];
+    __shared__ double s_red[4][PB][4];
+    __shared__ double s_e[PB];
+    extern __shared__ double s_dyn[];
 
-    const int bidx = b0 + blockIdx.x;
+    const int bidx = work[blockIdx.x / SUBS];
+    const int sub = blockIdx.x % SUBS;
     const BlockDesc B = blocks[bidx];
-    const double* phi = scratch + (B.scratch_off - chunk_off);
-    const int64_t plane = (int64_t)B.ldn * kBlockPts;
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int li = lane & 15;
-    const int lq = lane >> 4;
-    const int ldn = B.ldn;
-
-    for (int c = tid; c < ldn; c += 256) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
+    const int p0 = sub * PB;
+    const int64_t wg_slot = (int64_t)bidx * 8 + sub;
+    if (p0 >= B.npts) {  // no real point in this part of the block (uniform exit, before any barrier)
+        if (tid == 0) ewg[wg_slot] = 0.0;
+        return;
+    }
+    const int lane = tid & 63, li = lane & 15, lq = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ldn = B.ldn, nao = B.nao, nsh = B.nshell;
+    const int T = ldn >> 4;
+    // LDS: phi [max(ldn, 256)][PB] | records [nsh][RS] | cols [ldn] | shcol [nsh] | sorder [nsh] | slab [T+1]
+    double* s_phi = s_dyn;
+    double* s_rec = s_phi + (size_t)max(ldn, 4 * 64) * PB;
+    int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
+    int* s_shcol = s_cols + ldn;
+    int* s_sord = s_shcol + nsh;
+    int* s_slab = s_sord + nsh;
+    for (int c = tid; c < ldn; c += 256) s_cols[c] = (c < nao) ? bcols[B.col_begin + c] : -1;
+    for (int c = tid; c < nsh; c += 256) {
+        s_shcol[c] = bshell_col[B.shell_begin + c];
+        s_sord[c] = bsorder[B.shell_begin + c];
+    }
+    for (int c = tid; c <= T; c += 256) s_slab[c] = bslab[B.slab_begin + c];
+    for (int idx = tid; idx < nsh * RS; idx += 256) {
+        const int k = idx / RS;
+        s_rec[idx] = shrec[(int64_t)bshells[B.shell_begin + k] * RS + (idx - k * RS)];
+    }
+    if (tid < PB) {
+        const int64_t slot = B.pt_begin + p0 + tid;
+        s_pt[tid][0] = pts[slot];
+        s_pt[tid][1] = pts[n_slots + slot];
+        s_pt[tid][2] = pts[2 * n_slots + slot];
+        s_pt[tid][3] = pts[3 * n_slots + slot];
+    }
     __syncthreads();
 
-    const int row0 = wave * 32;
-    double part[2][4][4];
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) part[rt][r][q] = 0.0;
+    // phase 1: phi of all columns at the PB points, shells in (l, nprim) order
+    for (int it = tid; it < nsh * PB; it += 256) {
+        const int p = it % PB;
+        const int k = s_sord[it / PB];
+        double v[5], vx[5], vy[5], vz[5];
+        rec_values(s_rec + k * RS, s_pt[p][0], s_pt[p][1], s_pt[p][2], v, vx, vy, vz);
+        const int c0 = s_shcol[k];
+        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+        for (int m = 0; m < nc; ++m) s_phi[(c0 + m) * PB + p] = v[m];
+    }
+    for (int it = tid; it < (ldn - nao) * PB; it += 256) s_phi[nao * PB + it] = 0.0;
+    __syncthreads();
 
-    for (int j0 = 0; j0 < ldn; j0 += 64) {
-        const int nct = min(4, (ldn - j0) >> 4);
-        int gcol[4];
+    // phase 2: X^T = P_bb Phi^T, A fragments from P, B fragments from LDS
+    v4d acc[MAXI][RT];
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) gcol[ct] = (ct < nct) ? s_cols[j0 + ct * 16 + li] : -1;
-        v4d acc[2][4];
+    for (int i = 0; i < MAXI; ++i)
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < RT; ++rt) acc[i][rt] = v4d{0.0, 0.0, 0.0, 0.0};
+    int gi[MAXI];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < MAXI; ++i) {
+        const int I = wave + 4 * i;
+        gi[i] = (I < T) ? s_cols[I * 16 + li] : -1;
+    }
+    const int nsteps = T * 4;
+    const int ngroups = (nsteps + G - 1) / G;
+    double aC[G][MAXI], aN[G][MAXI];
+    auto load_a = [&](int g, double (&a)[G][MAXI]) {
+#pragma unroll
+        for (int d = 0; d < G; ++d) {
+            const int step = g * G + d;
+            const int gk = (step < nsteps) ? s_cols[4 * step + lq] : -1;
+            const double* prow = P + (int64_t)(gk < 0 ? 0 : gk) * N;
+#pragma unroll
+            for (int i = 0; i < MAXI; ++i) a[d][i] = (gk >= 0 && gi[i] >= 0) ? prow[gi[i]] : 0.0;
+        }
+    };
+    load_a(0, aC);
+    for (int g = 0; g < ngroups; ++g) {
+        if (g + 1 < ngroups) load_a(g + 1, aN);
+#pragma unroll
+        for (int d = 0; d < G; ++d) {
+            const int step = g * G + d;
+            if (step < nsteps) {
+                double b[RT];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) b[rt] = s_phi[(4 * step + lq) * PB + rt * 16 + li];
+#pragma unroll
+                for (int i = 0; i < MAXI; ++i)
+#pragma unroll
+                    for (int rt = 0; rt < RT; ++rt) acc[i][rt] = mfma16x16x4(aC[d][i], b[rt], acc[i][rt]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < G; ++d)
+#pragma unroll
+            for (int i = 0; i < MAXI; ++i) aC[d][i] = aN[d][i];
+    }
 
-        for (int k0 = 0; k0 < ldn; k0 += 4) {
-            const int kk = k0 + lq;
-            const double a0 = phi[(int64_t)kk * kBlockPts + row0 + li];
-            const double a1 = phi[(int64_t)kk * kBlockPts + row0 + 16 + li];
-            const int gk = s_cols[kk];
-            const double* Prow = P + (int64_t)(gk < 0 ? 0 : gk) * N;
+    // phase 3: epilogue per 64-column chunk; E[q][col][p] reuses the phi region
+    double part[RT][4];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                if (ct < nct) {
-                    const double b = (gk >= 0 && gcol[ct] >= 0) ? Prow[gcol[ct]] : 0.0;
-                    acc[0][ct] = mfma16x16x4(a0, b, acc[0][ct]);
-                    acc[1][ct] = mfma16x16x4(a1, b, acc[1][ct]);
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part[rt][q] = 0.0;
+    double* E = s_phi;
+    constexpr int EQ = 64 * PB;  // doubles per E plane
+#pragma unroll
+    for (int c = 0; c < MAXI; ++c) {
+        const int lo = c * 64;
+        if (lo < ldn) {
+            const int hi = min(lo + 64, ldn);
+            const int first = s_slab[lo >> 4];
+            int last = first;
+            while (last < nsh && s_shcol[last] < hi) ++last;
+            __syncthreads();
+            for (int it = tid; it < (last - first) * PB; it += 256) {
+                const int p = it % PB;
+                const int k = first + it / PB;
+                double v[5], vx[5], vy[5], vz[5];
+                rec_values(s_rec + k * RS, s_pt[p][0], s_pt[p][1], s_pt[p][2], v, vx, vy, vz);
+                const int c0 = s_shcol[k];
+                const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+                for (int m = 0; m < nc; ++m) {
+                    const int col = c0 + m;
+                    if (col >= lo && col < hi) {
+                        double* e = E + (col - lo) * PB + p;
+                        e[0] = v[m];
+                        e[EQ] = vx[m];
+                        e[2 * EQ] = vy[m];
+                        e[3 * EQ] = vz[m];
+                    }
+                }
+            }
+            __syncthreads();
+            const int I = wave + 4 * c;
+            if (I < T) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int cl = wave * 16 + 4 * r + lq;  // column within the chunk
+                    if (s_cols[lo + cl] >= 0) {
+#pragma unroll
+                        for (int rt = 0; rt < RT; ++rt) {
+                            const double x = acc[c][rt][r];
+                            const double* e = E + cl * PB + rt * 16 + li;
+                            part[rt][0] += x * e[0];
+                            part[rt][1] += x * e[EQ];
+                            part[rt][2] += x * e[2 * EQ];
+                            part[rt][3] += x * e[3 * EQ];
+                        }
+                    }
                 }
             }
         }
-        // fused row dots with phi and grad phi of the same columns
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-            if (ct < nct) {
-                const int64_t cbase = (int64_t)(j0 + ct * 16 + li) * kBlockPts;
-#pragma unroll
-                for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int row = row0 + rt * 16 + 4 * r + lq;
-                        const double x = acc[rt][ct][r];
-                        part[rt][r][0] += x * phi[cbase + row];
-                        part[rt][r][1] += x * phi[plane + cbase + row];
-                        part[rt][r][2] += x * phi[2 * plane + cbase + row];
-                        part[rt][r][3] += x * phi[3 * plane + cbase + row];
-                    }
-            }
-        }
     }
-    // reduce over the 16 lanes that share rows (same lq)
+    // reduce over the four lanes that share a point (lq), then over waves
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int q = 0; q < 4; ++q) {
+            double v = part[rt][q];
+            v += __shfl_xor(v, 16);
+            v += __shfl_xor(v, 32);
+            part[rt][q] = v;
+        }
+    if (lq == 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                double v = part[rt][r][q];
-                v += __shfl_xor(v, 1);
-                v += __shfl_xor(v, 2);
-                v += __shfl_xor(v, 4);
-                v += __shfl_xor(v, 8);
-                part[rt][r][q] = v;
-            }
-    if (li == 0) {
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = row0 + rt * 16 + 4 * r + lq;
-                s_rho[row][0] = part[rt][r][0];
-                s_rho[row][1] = 2.0 * part[rt][r][1];
-                s_rho[row][2] = 2.0 * part[rt][r][2];
-                s_rho[row][3] = 2.0 * part[rt][r][3];
-            }
+            for (int q = 0; q < 4; ++q) s_red[wave][rt * 16 + li][q] = part[rt][q];
     }
     __syncthreads();
-
-    if (tid < kBlockPts) {
+    if (tid < PB) {
         const int p = tid;
-        const int64_t slot = B.pt_begin + p;
-        const double w = pts[3 * n_slots + slot];
-        const double rho = s_rho[p][0];
-        const double gx = s_rho[p][1], gy = s_rho[p][2], gz = s_rho[p][3];
+        double rq[4];
+        for (int q = 0; q < 4; ++q) rq[q] = s_red[0][p][q] + s_red[1][p][q] + s_red[2][p][q] + s_red[3][p][q];
+        const double rho = rq[0];
+        const double gx = 2.0 * rq[1], gy = 2.0 * rq[2], gz = 2.0 * rq[3];
+        const int64_t slot = B.pt_begin + p0 + p;
+        const double w = s_pt[p][3];
         if (rho_out) {
             rho_out[4 * slot + 0] = rho;
             rho_out[4 * slot + 1] = gx;
@@ -245,7 +354,7 @@ __global__ __launch_bounds__(256) void k_rho_vxc(const BlockDesc* __restrict__ b
             rho_out[4 * slot + 3] = gz;
         }
         double a = 0.0, cx = 0.0, cy = 0.0, cz = 0.0, e = 0.0;
-        if (p < B.npts && rho > dens_cut) {
+        if (p0 + p < B.npts && rho > dens_cut) {
             const double gamma = gx * gx + gy * gy + gz * gz;
             const XcPoint f = b3lyp::eval_rks(rho, gamma);
             // build_XC_Matrix GGA (DfCalcGridX.cpp:2455-2500): coef1 = w vrho,
@@ -257,122 +366,168 @@ __global__ __launch_bounds__(256) void k_rho_vxc(const BlockDesc* __restrict__ b
             cz = w * (c2 * gz + f.vgab * gz);
             e = w * f.e;
         }
-        s_coef[p][0] = a;
-        s_coef[p][1] = cx;
-        s_coef[p][2] = cy;
-        s_coef[p][3] = cz;
+        coef_out[4 * slot + 0] = a;
+        coef_out[4 * slot + 1] = cx;
+        coef_out[4 * slot + 2] = cy;
+        coef_out[4 * slot + 3] = cz;
         s_e[p] = e;
-        if (coef_out) {
-            coef_out[4 * slot + 0] = a;
-            coef_out[4 * slot + 1] = cx;
-            coef_out[4 * slot + 2] = cy;
-            coef_out[4 * slot + 3] = cz;
-        }
     }
     __syncthreads();
-    // deterministic block energy sum
-    for (int s = kBlockPts / 2; s > 0; s >>= 1) {
-        if (tid < s) s_e[tid] += s_e[tid + s];
-        __syncthreads();
-    }
-    if (tid == 0) eblock[bidx] = s_e[0];
-
-    // V = a phi + c . grad phi, written over the d/dx plane
-    double* V = const_cast<double*>(phi) + plane;
-    const int p = tid & (kBlockPts - 1);
-    const double a = s_coef[p][0], cx = s_coef[p][1], cy = s_coef[p][2], cz = s_coef[p][3];
-    for (int c = tid >> 7; c < ldn; c += 2) {
-        const int64_t o = (int64_t)c * kBlockPts + p;
-        const double v = a * phi[o] + cx * V[o] + cy * phi[2 * plane + o] + cz * phi[3 * plane + o];
-        V[o] = v;
+    if (tid == 0) {
+        double e = 0.0;
+        for (int p = 0; p < PB; ++p) e += s_e[p];
+        ewg[wg_slot] = e;
     }
 }
 
 // ---------------------------------------------------------------------------
-// K_bb += Phi^T V + V^T Phi over the block's points, lower tile pairs only,
-// scattered into K (lower triangle, global AO indices ascending within a
-// block).  Wave w of 4 owns a 32 x 32 quarter of a 64 x 64 tile pair; the
-// reduction runs over the 128 points in slabs of 16, each lane loading 4
-// consecutive points of one AO row (the MFMA k index is a permutation of the
-// points, the same for both operands).
+// K pass: K_bb += Phi^T V + V^T Phi for the lower 16 x 16 tiles t0 <= t < t1
+// of one block (t = I (I + 1) / 2 + J), held in accumulators; wave w owns a
+// contiguous quarter of them, padded to MAXT with a repeated tile that is not
+// scattered, so the accumulate loop has no branches.  The reduction streams
+// the block's points in slabs of PS: phi and grad phi of the needed columns
+// are evaluated into LDS from the LDS shell records and V = (w vrho / 2) phi
+// + c . grad phi formed with the per-point coefficients (build_XC_Matrix GGA,
+// DfCalcGridX.cpp:2455-2500).  Tiles go into the dense lower triangle of K
+// with FP64 atomics.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_kmat(const BlockDesc* __restrict__ blocks, int b0,
-                                              const int* __restrict__ bcols,
-                                              const double* __restrict__ scratch, int64_t chunk_off,
-                                              double* __restrict__ K, int N) {
-    extern __shared__ int s_cols[];
-    const BlockDesc B = blocks[b0 + blockIdx.x];
-    const double* phi = scratch + (B.scratch_off - chunk_off);
-    const int64_t plane = (int64_t)B.ldn * kBlockPts;
-    const double* V = phi + plane;
+template <int MAXT>
+__global__ __launch_bounds__(256) void k_kmat(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+                                              const double* __restrict__ pts, int64_t n_slots,
+                                              const double* __restrict__ shrec, int RS,
+                                              const int* __restrict__ bshells, const int* __restrict__ bshell_col,
+                                              const int* __restrict__ bcols, const int* __restrict__ bsorder,
+                                              const double* __restrict__ coef, double* __restrict__ K, int N) {
+    __shared__ double s_pt[kBlockPts][3];
+    __shared__ double s_cf[kBlockPts][4];
+    extern __shared__ double s_dynd[];
+    const KWork W = work[blockIdx.x];
+    const BlockDesc B = blocks[W.block];
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int li = lane & 15;
-    const int lq = lane >> 4;
-    const int ldn = B.ldn;
-    for (int c = tid; c < ldn; c += 256) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
-    __syncthreads();
-
-    const int ntile = (ldn + 63) >> 6;
-    const int wi = wave >> 1, wj = wave & 1;
-    for (int I = 0; I < ntile; ++I) {
-        for (int J = 0; J <= I; ++J) {
-            const int r0 = I * 64 + wi * 32;
-            const int c0 = J * 64 + wj * 32;
-            if (r0 >= ldn || c0 >= ldn) continue;  // whole quarter outside the padded width
-            if (I == J && wj > wi) continue;        // strictly upper quarter of a diagonal pair
-            const bool rv[2] = {r0 < ldn, r0 + 16 < ldn};
-            const bool cv[2] = {c0 < ldn, c0 + 16 < ldn};
-            v4d acc[2][2];
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-            for (int t = 0; t < kBlockPts / 16; ++t) {
-                const int pbase = 16 * t + 4 * lq;
-                double4 fI[2], vI[2], fJ[2], vJ[2];
-#pragma unroll
-                for (int a = 0; a < 2; ++a) {
-                    const int64_t oi = (int64_t)(r0 + a * 16 + li) * kBlockPts + pbase;
-                    const int64_t oj = (int64_t)(c0 + a * 16 + li) * kBlockPts + pbase;
-                    const double4 z = {0.0, 0.0, 0.0, 0.0};
-                    fI[a] = rv[a] ? *reinterpret_cast<const double4*>(phi + oi) : z;
-                    vI[a] = rv[a] ? *reinterpret_cast<const double4*>(V + oi) : z;
-                    fJ[a] = cv[a] ? *reinterpret_cast<const double4*>(phi + oj) : z;
-                    vJ[a] = cv[a] ? *reinterpret_cast<const double4*>(V + oj) : z;
-                }
-#pragma unroll
-                for (int a = 0; a < 2; ++a)
-#pragma unroll
-                    for (int b = 0; b < 2; ++b) {
-                        if (!(rv[a] && cv[b])) continue;
-                        acc[a][b] = mfma16x16x4(fI[a].x, vJ[b].x, acc[a][b]);
-                        acc[a][b] = mfma16x16x4(vI[a].x, fJ[b].x, acc[a][b]);
-                        acc[a][b] = mfma16x16x4(fI[a].y, vJ[b].y, acc[a][b]);
-                        acc[a][b] = mfma16x16x4(vI[a].y, fJ[b].y, acc[a][b]);
-                        acc[a][b] = mfma16x16x4(fI[a].z, vJ[b].z, acc[a][b]);
-                        acc[a][b] = mfma16x16x4(vI[a].z, fJ[b].z, acc[a][b]);
-                        acc[a][b] = mfma16x16x4(fI[a].w, vJ[b].w, acc[a][b]);
-                        acc[a][b] = mfma16x16x4(vI[a].w, fJ[b].w, acc[a][b]);
-                    }
+    const int lane = tid & 63, li = lane & 15, lq = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ncols = W.ncols;  // multiple of 16, <= ldn
+    const int LD = ncols + 2;
+    const int PS = W.ps;
+    const int nsh = W.nsh;
+    // LDS: phi [PS][LD] | V [PS][LD] | records [nsh][RS] | cols [ncols] | shcol [nsh] | shid [nsh]
+    double* s_phi = s_dynd;
+    double* s_v = s_phi + PS * LD;
+    double* s_rec = s_v + PS * LD;
+    int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
+    int* s_shcol = s_cols + ncols;
+    int* s_shid = s_shcol + nsh;
+    for (int c = tid; c < ncols; c += 256) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
+    // shells this item needs (local index < nsh), in (l, nprim) order, compacted by wave 0
+    if (wave == 0) {
+        int base = 0;
+        for (int i0 = 0; i0 < B.nshell; i0 += 64) {
+            const int i = i0 + lane;
+            const int sh = (i < B.nshell) ? bsorder[B.shell_begin + i] : B.nshell;
+            const bool keep = sh < nsh;
+            const unsigned long long m = __ballot(keep);
+            const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+            if (keep) {
+                s_shid[pos] = bshells[B.shell_begin + sh];
+                s_shcol[pos] = bshell_col[B.shell_begin + sh];
             }
+            base += __popcll(m);
+        }
+    }
+    for (int p = tid; p < kBlockPts; p += 256) {
+        const int64_t slot = B.pt_begin + p;
+        s_pt[p][0] = pts[slot];
+        s_pt[p][1] = pts[n_slots + slot];
+        s_pt[p][2] = pts[2 * n_slots + slot];
+        s_cf[p][0] = coef[4 * slot + 0];
+        s_cf[p][1] = coef[4 * slot + 1];
+        s_cf[p][2] = coef[4 * slot + 2];
+        s_cf[p][3] = coef[4 * slot + 3];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nsh * RS; idx += 256) {
+        const int k = idx / RS;
+        s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
+    }
+    // this wave's tiles: contiguous run [tb, te), offsets of rows (I) and columns (J)
+    const int q4 = (W.t1 - W.t0 + 3) >> 2;
+    const int tb = W.t0 + wave * q4;
+    const int te = min(W.t1, tb + q4);
+    int offI[MAXT], offJ[MAXT];
+    {
+        int t = min(tb, W.t1 - 1);
+        int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while (I * (I + 1) / 2 > t) --I;
+        while ((I + 1) * (I + 2) / 2 <= t) ++I;
+        int J = t - I * (I + 1) / 2;
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    if (!(rv[a] && cv[b])) continue;
-                    const int lc = c0 + b * 16 + li;
-                    const int gj = s_cols[lc];
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int lr = r0 + a * 16 + 4 * r + lq;
-                        const int gi = s_cols[lr];
-                        if (gi >= 0 && gj >= 0 && lr >= lc) {
-                            atomicAdd(K + (int64_t)gi * N + gj, acc[a][b][r]);
-                        }
-                    }
+        for (int m = 0; m < MAXT; ++m) {
+            offI[m] = I * 16;
+            offJ[m] = J * 16;
+            if (tb + m + 1 < te) {  // advance to the next tile of the run; padding repeats the last
+                if (J == I) {
+                    ++I;
+                    J = 0;
+                } else {
+                    ++J;
                 }
+            }
+        }
+    }
+    v4d acc[MAXT];
+#pragma unroll
+    for (int m = 0; m < MAXT; ++m) acc[m] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int npts_pad = (B.npts + PS - 1) / PS * PS;
+    for (int pb = 0; pb < npts_pad; pb += PS) {
+        __syncthreads();
+        for (int it = tid; it < nsh * PS; it += 256) {
+            const int p = it % PS;
+            const int k = it / PS;
+            const int pp = pb + p;
+            double v[5], vx[5], vy[5], vz[5];
+            rec_values(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], v, vx, vy, vz);
+            const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
+            const int c0 = s_shcol[k];
+            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+            for (int m = 0; m < nc; ++m) {
+                const int c = c0 + m;
+                if (c < ncols) {
+                    s_phi[p * LD + c] = v[m];
+                    s_v[p * LD + c] = a * v[m] + cx * vx[m] + cy * vy[m] + cz * vz[m];
+                }
+            }
+        }
+        for (int it = tid; it < (ncols - min(ncols, B.nao)) * PS; it += 256) {
+            const int c = B.nao + it / PS;
+            const int p = it % PS;
+            s_phi[p * LD + c] = 0.0;
+            s_v[p * LD + c] = 0.0;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < PS / 4; ++ks) {
+            const double* fr = s_phi + (4 * ks + lq) * LD + li;
+            const double* vr = s_v + (4 * ks + lq) * LD + li;
+#pragma unroll
+            for (int m = 0; m < MAXT; ++m) {
+                const double a1 = fr[offI[m]], b1 = vr[offJ[m]];
+                const double a2 = vr[offI[m]], b2 = fr[offJ[m]];
+                acc[m] = mfma16x16x4(a1, b1, acc[m]);
+                acc[m] = mfma16x16x4(a2, b2, acc[m]);
+            }
+        }
+    }
+    // scatter: tile element (4 r + lq, li) -> K[cols[I 16 + 4 r + lq]][cols[J 16 + li]]
+#pragma unroll
+    for (int m = 0; m < MAXT; ++m) {
+        if (tb + m < te) {
+            const int lc = offJ[m] + li;
+            const int gj = s_cols[lc];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int lr = offI[m] + 4 * r + lq;
+                const int gi = s_cols[lr];
+                if (gi >= 0 && gj >= 0 && lr >= lc) atomicAdd(K + (int64_t)gi * N + gj, acc[m][r]);
+            }
         }
     }
 }
@@ -477,46 +632,63 @@ static int check_launch(const char* what) {
     return PDFXC_OK;
 }
 
-static int max_ldn(const Context& c, const Chunk& ch) {
-    int m = kColPad;
-    for (int b = ch.block_begin; b < ch.block_end; ++b) m = std::max(m, c.blocks[b].ldn);
-    return m;
+template <int RT, int MAXI, int G>
+static int rho_one(Context& c, const LaunchRange& r) {
+    const int subs = kBlockPts / (16 * RT);
+    hipLaunchKernelGGL((k_rho<RT, MAXI, G>), dim3(r.count * subs), dim3(256), r.lds, c.stream,
+                       c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
+                       c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_slab,
+                       c.d_block_sorder, c.d_P, c.n_aos, c.d_coef, c.d_rho, c.d_eblock,
+                       c.opts.density_cutoff);
+    return check_launch("k_rho");
 }
 
-int launch_ao_eval(Context& c, const Chunk& ch) {
-    const int nb = ch.block_end - ch.block_begin;
-    if (nb <= 0) return PDFXC_OK;
-    const int64_t off = c.blocks[ch.block_begin].scratch_off;
-    hipLaunchKernelGGL(k_ao_eval, dim3(nb), dim3(256), 0, c.stream, c.d_blocks, ch.block_begin,
-                       c.d_pts, c.n_slots, c.d_shells, c.d_block_shells, c.d_block_shell_col,
-                       c.d_scratch, off);
-    return check_launch("k_ao_eval");
+int launch_rho(Context& c, const LaunchRange& r) {
+    if (r.count <= 0) return PDFXC_OK;
+    switch (r.cls) {
+        case 0: return rho_one<2, 2, 4>(c, r);
+        case 1: return rho_one<2, 4, 4>(c, r);
+        case 2: return rho_one<1, 8, 2>(c, r);
+        case 3: return rho_one<1, 16, 1>(c, r);
+        default: set_error("k_rho: bad class"); return PDFXC_ERR_ARGUMENT;
+    }
 }
 
-int launch_rho_vxc(Context& c, const Chunk& ch) {
-    const int nb = ch.block_end - ch.block_begin;
-    if (nb <= 0) return PDFXC_OK;
-    const int64_t off = c.blocks[ch.block_begin].scratch_off;
-    const size_t lds = sizeof(int) * (size_t)max_ldn(c, ch);
-    hipLaunchKernelGGL(k_rho_vxc, dim3(nb), dim3(256), lds, c.stream, c.d_blocks, ch.block_begin,
-                       c.d_pts, c.n_slots, c.d_P, c.n_aos, c.d_block_cols, c.d_scratch, off,
-                       c.d_coef, c.d_rho, c.d_eblock, c.opts.density_cutoff);
-    return check_launch("k_rho_vxc");
-}
-
-int launch_kmat(Context& c, const Chunk& ch) {
-    const int nb = ch.block_end - ch.block_begin;
-    if (nb <= 0) return PDFXC_OK;
-    const int64_t off = c.blocks[ch.block_begin].scratch_off;
-    const size_t lds = sizeof(int) * (size_t)max_ldn(c, ch);
-    hipLaunchKernelGGL(k_kmat, dim3(nb), dim3(256), lds, c.stream, c.d_blocks, ch.block_begin,
-                       c.d_block_cols, c.d_scratch, off, c.d_K, c.n_aos);
+template <int MAXT>
+static int kmat_one(Context& c, const LaunchRange& r) {
+    hipLaunchKernelGGL((k_kmat<MAXT>), dim3(r.count), dim3(256), r.lds, c.stream,
+                       reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
+                       c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col,
+                       c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
     return check_launch("k_kmat");
+}
+
+int launch_kmat(Context& c, const LaunchRange& r) {
+    if (r.count <= 0) return PDFXC_OK;
+    switch (r.cls) {
+        case 0: return kmat_one<8>(c, r);
+        case 1: return kmat_one<16>(c, r);
+        default: set_error("k_kmat: bad class"); return PDFXC_ERR_ARGUMENT;
+    }
+}
+
+int configure_kernels() {
+    // the block kernels use up to 160 KiB of LDS per workgroup (static + dynamic);
+    // HIP on gfx950 needs no opt-in, but the device must offer it
+    int dev = 0, lds = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute(MaxSharedMemoryPerBlock)");
+    if (lds < 160 * 1024) {
+        set_error("device offers " + std::to_string(lds) + " bytes of LDS per workgroup; the kernels need 160 KiB (gfx950)");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    return PDFXC_OK;
 }
 
 int launch_energy_sum(Context& c, double* dE) {
     hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(256), 0, c.stream, c.d_eblock,
-                       (int)c.blocks.size(), dE);
+                       (int)c.blocks.size() * 8, dE);
     return check_launch("k_energy_sum");
 }
 
