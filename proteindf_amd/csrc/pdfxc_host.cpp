@@ -536,7 +536,7 @@ static int build_blocks(Context& c) {
         const int T = B.ldn / 16;
         const int nt = T * (T + 1) / 2;
         const int cls = (nt <= 32) ? 0 : 1;
-        const int per = (cls == 0) ? 32 : 64;
+        const int per = (cls == 0) ? 32 : 96;
         for (int t0 = 0; t0 < nt; t0 += per) {
             KWork w;
             w.block = b;
@@ -549,9 +549,10 @@ static int build_blocks(Context& c) {
             while (nsh < B.nshell && c.block_shell_col[B.shell_begin + nsh] < w.ncols) ++nsh;
             w.nsh = nsh;
             size_t lds = 0;
+            const int LD = (w.ncols & 31) ? w.ncols : w.ncols + 16;
             for (int ps = 16; ps >= 4; ps /= 2) {
                 w.ps = ps;
-                lds = sizeof(double) * (2 * (size_t)ps * (w.ncols + 2) + (size_t)nsh * RS) +
+                lds = sizeof(double) * (4 * (size_t)ps * LD + (size_t)nsh * RS) +
                       sizeof(int) * (size_t)(w.ncols + 2 * nsh);
                 if (lds <= (size_t)kMaxDynLds) break;
             }

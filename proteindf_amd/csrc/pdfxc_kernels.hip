@@ -382,17 +382,20 @@ __global__ __launch_bounds__(256) void k_rho(const int* __restrict__ work, const
 
 // ---------------------------------------------------------------------------
 // K pass: K_bb += Phi^T V + V^T Phi for the lower 16 x 16 tiles t0 <= t < t1
-// of one block (t = I (I + 1) / 2 + J), held in accumulators; wave w owns a
-// contiguous quarter of them, padded to MAXT with a repeated tile that is not
-// scattered, so the accumulate loop has no branches.  The reduction streams
-// the block's points in slabs of PS: phi and grad phi of the needed columns
-// are evaluated into LDS from the LDS shell records and V = (w vrho / 2) phi
-// + c . grad phi formed with the per-point coefficients (build_XC_Matrix GGA,
-// DfCalcGridX.cpp:2455-2500).  Tiles go into the dense lower triangle of K
-// with FP64 atomics.
+// of one block (t = I (I + 1) / 2 + J).  Eight waves: waves 0-3 (consumers)
+// hold the tiles in accumulators -- each a contiguous quarter, padded to MAXT
+// with a repeated tile that is not scattered, so the accumulate loop has no
+// branches -- and run the MFMAs on point slab s while waves 4-7 (producers)
+// evaluate slab s + 1 into the other half of a double-buffered LDS area:
+// phi and grad phi of the needed columns from the LDS shell records, and
+// V = (w vrho / 2) phi + c . grad phi with the per-point coefficients
+// (build_XC_Matrix GGA, DfCalcGridX.cpp:2455-2500).  Slab rows are padded to
+// a stride of 16 (mod 32) doubles so that the 16-lane groups of an MFMA
+// operand read fall on disjoint LDS banks.  Tiles go into the dense lower
+// triangle of K with FP64 atomics.
 // ---------------------------------------------------------------------------
 template <int MAXT>
-__global__ __launch_bounds__(256) void k_kmat(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                               const double* __restrict__ pts, int64_t n_slots,
                                               const double* __restrict__ shrec, int RS,
                                               const int* __restrict__ bshells, const int* __restrict__ bshell_col,
@@ -406,20 +409,21 @@ __global__ __launch_bounds__(256) void k_kmat(const KWork* __restrict__ work, co
     const int tid = threadIdx.x;
     const int lane = tid & 63, li = lane & 15, lq = lane >> 4;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool producer = wave >= 4;
     const int ncols = W.ncols;  // multiple of 16, <= ldn
-    const int LD = ncols + 2;
+    const int LD = (ncols & 31) ? ncols : ncols + 16;
     const int PS = W.ps;
     const int nsh = W.nsh;
-    // LDS: phi [PS][LD] | V [PS][LD] | records [nsh][RS] | cols [ncols] | shcol [nsh] | shid [nsh]
-    double* s_phi = s_dynd;
-    double* s_v = s_phi + PS * LD;
-    double* s_rec = s_v + PS * LD;
+    const int SLAB = PS * LD;  // doubles per array per buffer
+    // LDS: [2 buffers][phi, V][PS][LD] | records [nsh][RS] | cols [ncols] | shcol [nsh] | shid [nsh]
+    double* s_buf = s_dynd;
+    double* s_rec = s_buf + 4 * SLAB;
     int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
     int* s_shcol = s_cols + ncols;
     int* s_shid = s_shcol + nsh;
-    for (int c = tid; c < ncols; c += 256) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
-    // shells this item needs (local index < nsh), in (l, nprim) order, compacted by wave 0
-    if (wave == 0) {
+    for (int c = tid; c < ncols; c += 512) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
+    // shells this item needs (local index < nsh), in (l, nprim) order, compacted by wave 4
+    if (wave == 4) {
         int base = 0;
         for (int i0 = 0; i0 < B.nshell; i0 += 64) {
             const int i = i0 + lane;
@@ -434,7 +438,7 @@ __global__ __launch_bounds__(256) void k_kmat(const KWork* __restrict__ work, co
             base += __popcll(m);
         }
     }
-    for (int p = tid; p < kBlockPts; p += 256) {
+    for (int p = tid; p < kBlockPts; p += 512) {
         const int64_t slot = B.pt_begin + p;
         s_pt[p][0] = pts[slot];
         s_pt[p][1] = pts[n_slots + slot];
@@ -445,26 +449,69 @@ __global__ __launch_bounds__(256) void k_kmat(const KWork* __restrict__ work, co
         s_cf[p][3] = coef[4 * slot + 3];
     }
     __syncthreads();
-    for (int idx = tid; idx < nsh * RS; idx += 256) {
+    for (int idx = tid; idx < nsh * RS; idx += 512) {
         const int k = idx / RS;
         s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
     }
-    // this wave's tiles: contiguous run [tb, te), offsets of rows (I) and columns (J)
+    __syncthreads();
+
+    const int ptid = tid - 256;  // producer thread index
+    auto produce = [&](int pb, double* phi, double* vv) {
+        for (int it = ptid; it < nsh * PS; it += 256) {
+            const int p = it % PS;
+            const int k = it / PS;
+            const int pp = pb + p;
+            double v[5], vx[5], vy[5], vz[5];
+            rec_values(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], v, vx, vy, vz);
+            const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
+            const int c0 = s_shcol[k];
+            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+            for (int m = 0; m < nc; ++m) {
+                const int c = c0 + m;
+                if (c < ncols) {
+                    phi[p * LD + c] = v[m];
+                    vv[p * LD + c] = a * v[m] + cx * vx[m] + cy * vy[m] + cz * vz[m];
+                }
+            }
+        }
+        for (int it = ptid; it < (ncols - min(ncols, B.nao)) * PS; it += 256) {
+            const int c = B.nao + it / PS;
+            const int p = it % PS;
+            phi[p * LD + c] = 0.0;
+            vv[p * LD + c] = 0.0;
+        }
+    };
+
+    const int nslab = (B.npts + PS - 1) / PS;
+    if (producer) {
+        // producers: slab s + 1 while the consumers multiply slab s; the
+        // number of barriers matches the consumer branch
+        produce(0, s_buf, s_buf + SLAB);
+        __syncthreads();
+        for (int s = 0; s < nslab; ++s) {
+            if (s + 1 < nslab) {
+                double* nxt = s_buf + ((s + 1) & 1) * 2 * SLAB;
+                produce((s + 1) * PS, nxt, nxt + SLAB);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    // consumers: a contiguous run [tb, te) of tiles, row (I) and column (J) offsets
     const int q4 = (W.t1 - W.t0 + 3) >> 2;
     const int tb = W.t0 + wave * q4;
     const int te = min(W.t1, tb + q4);
     int offI[MAXT], offJ[MAXT];
     {
-        int t = min(tb, W.t1 - 1);
-        int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-        while (I * (I + 1) / 2 > t) --I;
+        const int t = min(tb, W.t1 - 1);
+        int I = 0;
         while ((I + 1) * (I + 2) / 2 <= t) ++I;
         int J = t - I * (I + 1) / 2;
 #pragma unroll
         for (int m = 0; m < MAXT; ++m) {
             offI[m] = I * 16;
             offJ[m] = J * 16;
-            if (tb + m + 1 < te) {  // advance to the next tile of the run; padding repeats the last
+            if (tb + m + 1 < te) {  // advance along the run; padding repeats the last tile
                 if (J == I) {
                     ++I;
                     J = 0;
@@ -477,44 +524,27 @@ __global__ __launch_bounds__(256) void k_kmat(const KWork* __restrict__ work, co
     v4d acc[MAXT];
 #pragma unroll
     for (int m = 0; m < MAXT; ++m) acc[m] = v4d{0.0, 0.0, 0.0, 0.0};
-    const int npts_pad = (B.npts + PS - 1) / PS * PS;
-    for (int pb = 0; pb < npts_pad; pb += PS) {
-        __syncthreads();
-        for (int it = tid; it < nsh * PS; it += 256) {
-            const int p = it % PS;
-            const int k = it / PS;
-            const int pp = pb + p;
-            double v[5], vx[5], vy[5], vz[5];
-            rec_values(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], v, vx, vy, vz);
-            const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
-            const int c0 = s_shcol[k];
-            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-            for (int m = 0; m < nc; ++m) {
-                const int c = c0 + m;
-                if (c < ncols) {
-                    s_phi[p * LD + c] = v[m];
-                    s_v[p * LD + c] = a * v[m] + cx * vx[m] + cy * vy[m] + cz * vz[m];
-                }
-            }
-        }
-        for (int it = tid; it < (ncols - min(ncols, B.nao)) * PS; it += 256) {
-            const int c = B.nao + it / PS;
-            const int p = it % PS;
-            s_phi[p * LD + c] = 0.0;
-            s_v[p * LD + c] = 0.0;
-        }
-        __syncthreads();
+    __syncthreads();
+    for (int s = 0; s < nslab; ++s) {
+        const double* cur = s_buf + (s & 1) * 2 * SLAB;
         for (int ks = 0; ks < PS / 4; ++ks) {
-            const double* fr = s_phi + (4 * ks + lq) * LD + li;
-            const double* vr = s_v + (4 * ks + lq) * LD + li;
+            const double* fr = cur + (4 * ks + lq) * LD + li;
+            const double* vr = cur + SLAB + (4 * ks + lq) * LD + li;
+            // groups of 4 tiles: bounded operand lookahead keeps the
+            // accumulators and operands within the register budget
 #pragma unroll
-            for (int m = 0; m < MAXT; ++m) {
-                const double a1 = fr[offI[m]], b1 = vr[offJ[m]];
-                const double a2 = vr[offI[m]], b2 = fr[offJ[m]];
-                acc[m] = mfma16x16x4(a1, b1, acc[m]);
-                acc[m] = mfma16x16x4(a2, b2, acc[m]);
+            for (int m0 = 0; m0 < MAXT; m0 += 4) {
+#pragma unroll
+                for (int m = m0; m < m0 + 4 && m < MAXT; ++m) {
+                    const double a1 = fr[offI[m]], b1 = vr[offJ[m]];
+                    const double a2 = vr[offI[m]], b2 = fr[offJ[m]];
+                    acc[m] = mfma16x16x4(a1, b1, acc[m]);
+                    acc[m] = mfma16x16x4(a2, b2, acc[m]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
+        __syncthreads();
     }
     // scatter: tile element (4 r + lq, li) -> K[cols[I 16 + 4 r + lq]][cols[J 16 + li]]
 #pragma unroll
@@ -656,7 +686,7 @@ int launch_rho(Context& c, const LaunchRange& r) {
 
 template <int MAXT>
 static int kmat_one(Context& c, const LaunchRange& r) {
-    hipLaunchKernelGGL((k_kmat<MAXT>), dim3(r.count), dim3(256), r.lds, c.stream,
+    hipLaunchKernelGGL((k_kmat<MAXT>), dim3(r.count), dim3(512), r.lds, c.stream,
                        reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
                        c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col,
                        c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
@@ -667,7 +697,7 @@ int launch_kmat(Context& c, const LaunchRange& r) {
     if (r.count <= 0) return PDFXC_OK;
     switch (r.cls) {
         case 0: return kmat_one<8>(c, r);
-        case 1: return kmat_one<16>(c, r);
+        case 1: return kmat_one<24>(c, r);
         default: set_error("k_kmat: bad class"); return PDFXC_ERR_ARGUMENT;
     }
 }
