@@ -279,7 +279,6 @@ static void free_grid_buffers(Context* c) {
     dfree(c->d_block_shells);
     dfree(c->d_block_cols);
     dfree(c->d_block_shell_col);
-    dfree(c->d_block_slab);
     dfree(c->d_block_sorder);
     dfree(c->d_rho_work);
     dfree(c->d_k_work);
@@ -432,7 +431,6 @@ static int build_blocks(Context& c) {
     c.block_shells.clear();
     c.block_cols.clear();
     c.block_shell_col.clear();
-    c.block_slab.clear();
     c.block_sorder.clear();
     c.n_slots = (int64_t)nb * kBlockPts;
     c.slot_of_point.assign(c.n_pts, -1);
@@ -450,21 +448,18 @@ static int build_blocks(Context& c) {
         B.col_begin = (int)c.block_cols.size();
         B.nao = t.nao;
         B.ldn = std::max(kColPad, (t.nao + kColPad - 1) / kColPad * kColPad);
-        B.slab_begin = (int)c.block_slab.size();
         if (B.ldn > 1024) {
             set_error("set_grid: a block keeps " + std::to_string(B.nao) +
                       " AOs; more than 1024 per block is not supported (raise ao_screen)");
             return PDFXC_ERR_UNSUPPORTED;
         }
         int col = 0;
-        std::vector<int> ends;
         for (int s : t.sh) {
             c.block_shells.push_back(s);
             c.block_shell_col.push_back(col);
             const Shell& S = c.shells[s];
             for (int m = 0; m < 2 * S.l + 1; ++m) c.block_cols.push_back(S.first_ao + m);
             col += 2 * S.l + 1;
-            ends.push_back(col);
         }
         // evaluation order: shells of equal (l, nprim) side by side, so that a
         // wave's lanes mostly run the same code path
@@ -477,11 +472,6 @@ static int build_blocks(Context& c) {
                 return A.l != Bs.l ? A.l < Bs.l : A.nprim < Bs.nprim;
             });
             c.block_sorder.insert(c.block_sorder.end(), so.begin(), so.end());
-        }
-        // first local shell whose columns reach 16 s, for s = 0 .. ldn/16
-        for (int s = 0, sh = 0; s <= B.ldn / 16; ++s) {
-            while (sh < B.nshell && ends[sh] <= 16 * s) ++sh;
-            c.block_slab.push_back(sh);
         }
         for (int k = 0; k < kBlockPts; ++k) {
             const int64_t slot = (int64_t)li * kBlockPts + k;
@@ -498,22 +488,23 @@ static int build_blocks(Context& c) {
     }
     c.mean_nao = nb ? sum_nao / nb : 0.0;
 
-    // k_rho work lists by template class: ldn <= 128, 256 (32 points per
-    // workgroup), <= 512, 1024 (16 points per workgroup)
+    // k_rho work lists by template class: ldn <= 128 (64 points per
+    // workgroup), <= 256 (32), <= 512, <= 1024 (16)
     std::vector<int> rho_work;
     c.rho_launches.clear();
     const int RS = c.rec_stride;
     for (int cls = 0; cls < 4; ++cls) {
         const int lim = 128 << cls;
         const int lo_lim = cls ? (64 << cls) : 0;
-        const int pb = (cls < 2) ? 32 : 16;
+        const size_t pb = (size_t)rho_points_per_workgroup(cls);
         LaunchRange r{cls, (int)rho_work.size(), 0, 0};
         size_t maxdyn = 0;
         for (int b = 0; b < nb; ++b) {
             const BlockDesc& B = c.blocks[b];
             if (B.ldn > lo_lim && B.ldn <= lim) {
-                const size_t lds = sizeof(double) * ((size_t)std::max(B.ldn, 256) * pb + (size_t)B.nshell * RS) +
-                                   sizeof(int) * (size_t)(B.ldn + 3 * B.nshell + B.ldn / 16 + 1);
+                const size_t lds = sizeof(double) * (std::max<size_t>((size_t)B.ldn * pb, 4 * 512) +
+                                                     (size_t)B.nshell * RS) +
+                                   sizeof(int) * (size_t)(B.ldn + 2 * B.nshell);
                 if (lds > (size_t)kMaxDynLds) {
                     set_error("set_grid: a block with " + std::to_string(B.nao) + " AOs and " +
                               std::to_string(B.nshell) + " shells needs more LDS than a workgroup has "
@@ -579,18 +570,16 @@ static int build_blocks(Context& c) {
     if (rc == PDFXC_OK) rc = dalloc(&c.d_block_shells, c.block_shells.size());
     if (rc == PDFXC_OK) rc = dalloc(&c.d_block_cols, c.block_cols.size());
     if (rc == PDFXC_OK) rc = dalloc(&c.d_block_shell_col, c.block_shell_col.size());
-    if (rc == PDFXC_OK) rc = dalloc(&c.d_block_slab, c.block_slab.size());
     if (rc == PDFXC_OK) rc = dalloc(&c.d_block_sorder, c.block_sorder.size());
     if (rc == PDFXC_OK) rc = dalloc(&c.d_rho_work, rho_work.size());
     if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&c.d_k_work), kall.size() * sizeof(KWork));
     if (rc == PDFXC_OK) rc = dalloc(&c.d_coef, 4 * (size_t)c.n_slots);
     if (rc == PDFXC_OK) rc = dalloc(&c.d_rho, 4 * (size_t)c.n_slots);
-    if (rc == PDFXC_OK) rc = dalloc(&c.d_eblock, 8 * c.blocks.size());
+    if (rc == PDFXC_OK) rc = dalloc(&c.d_eblock, (size_t)(c.n_slots + 255) / 256);
     if (rc != PDFXC_OK) return rc;
     auto up = [](void* d, const void* h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
     PDFXC_HIP(up(c.d_pts, pts.data(), pts.size() * sizeof(double)));
     PDFXC_HIP(up(c.d_blocks, c.blocks.data(), c.blocks.size() * sizeof(BlockDesc)));
-    PDFXC_HIP(up(c.d_block_slab, c.block_slab.data(), c.block_slab.size() * sizeof(int)));
     PDFXC_HIP(up(c.d_rho_work, rho_work.data(), rho_work.size() * sizeof(int)));
     PDFXC_HIP(up(c.d_k_work, kall.data(), kall.size() * sizeof(KWork)));
     if (!c.block_shells.empty()) {
@@ -601,7 +590,7 @@ static int build_blocks(Context& c) {
     }
     PDFXC_HIP(hipMemset(c.d_coef, 0, 4 * (size_t)c.n_slots * sizeof(double)));
     PDFXC_HIP(hipMemset(c.d_rho, 0, 4 * (size_t)c.n_slots * sizeof(double)));
-    PDFXC_HIP(hipMemset(c.d_eblock, 0, 8 * c.blocks.size() * sizeof(double)));
+    PDFXC_HIP(hipMemset(c.d_eblock, 0, (size_t)(c.n_slots + 255) / 256 * sizeof(double)));
     return PDFXC_OK;
 }
 
@@ -955,6 +944,7 @@ static int build_device(Context* c, const double* dP, double* dK, double* dE) {
     PDFXC_HIP(hipEventRecord(c->ev[0], c->stream));
     for (const LaunchRange& r : c->rho_launches)
         if (rc == PDFXC_OK) rc = launch_rho(*c, r);
+    if (rc == PDFXC_OK) rc = launch_xc_points(*c);
     PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
     for (const LaunchRange& r : c->kmat_launches)
         if (rc == PDFXC_OK) rc = launch_kmat(*c, r);

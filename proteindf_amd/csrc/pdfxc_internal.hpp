@@ -38,7 +38,7 @@ struct BlockDesc {
     int nao;          // screened AO columns N_b
     int ldn;          // N_b padded to a multiple of kColPad
     int col_begin;    // offset into the block-AO-index list
-    int slab_begin;   // offset into the per-16-column first-shell table (ldn/16 + 1 entries)
+    int pad;
 };
 
 // One k_kmat work item: lower 16 x 16 tiles [t0, t1) of a block.
@@ -80,7 +80,6 @@ struct Context {
     std::vector<int> block_shells;     // global shell ids per block
     std::vector<int> block_cols;       // global AO index per block column
     std::vector<int> block_shell_col;  // local first column of each block shell
-    std::vector<int> block_slab;       // first local shell reaching column 16 s, per block
     std::vector<int> block_sorder;     // local shells of each block ordered by (l, nprim)
     std::vector<int64_t> slot_of_point;  // caller point -> slot (block*128 + k)
     std::vector<LaunchRange> rho_launches, kmat_launches;
@@ -98,13 +97,12 @@ struct Context {
     int* d_block_shells = nullptr;
     int* d_block_cols = nullptr;
     int* d_block_shell_col = nullptr;
-    int* d_block_slab = nullptr;
     int* d_block_sorder = nullptr;
     int* d_rho_work = nullptr;    // block indices grouped by k_rho class
     void* d_k_work = nullptr;     // KWork items grouped by k_kmat class
     double* d_coef = nullptr;     // per slot: (w vrho / 2, cx, cy, cz)
     double* d_rho = nullptr;      // per slot: rho, grad rho (4)
-    double* d_eblock = nullptr;   // per block x 8 sub-blocks: energy
+    double* d_eblock = nullptr;   // per 256 slots: energy (k_xc_points partial sums)
     double* d_P = nullptr;        // dense N x N
     double* d_K = nullptr;        // dense N x N
     double* d_E = nullptr;        // 1 double
@@ -126,6 +124,8 @@ int hip_fail(hipError_t e, const char* what);
 
 // kernel launchers (pdfxc_kernels.hip)
 int launch_rho(Context& c, const LaunchRange& r);
+int rho_points_per_workgroup(int cls);
+int launch_xc_points(Context& c);
 int launch_kmat(Context& c, const LaunchRange& r);
 int configure_kernels();
 int launch_energy_sum(Context& c, double* dE);

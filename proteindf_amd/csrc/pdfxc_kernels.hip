@@ -139,109 +139,71 @@ __device__ inline void rec_values(const double* rec, double px, double py, doubl
 }
 
 // ---------------------------------------------------------------------------
-// rho pass for PB = 16 RT points of a block: X^T = P_bb Phi^T on
-// v_mfma_f64_16x16x4 (rows = AOs, columns = points) with X^T held in
-// accumulators; wave w owns the AO tiles w, w + 4, ....  Phi of all ldn
-// columns is evaluated into LDS once; the A operand comes from P in global
-// memory (L2), prefetched one group of G k-steps ahead.  Epilogue: phi and
-// grad phi per 64-column chunk (one AO tile per wave) evaluated into the same
-// LDS and dotted with X^T: rho = sum_j X_pj phi_pj, grad rho = 2 sum_j X_pj
-// grad phi_pj (calcRho_GGA_part, DfCalcGridX.cpp:1923-1962); then the B3LYP
-// point terms (buildVxc, :2137-2193): coef = (w vrho / 2, w (2 vgAA + vgAB)
-// grad rho) and the weighted energy.
+// LDS layout of the per-point AO planes of k_rho: element (column c, point p)
+// of a [ldn][PB] plane sits at c PB + (p ^ 16 (c & 1)).  The swizzle puts the
+// four 16-lane groups of an MFMA B-operand read (columns 4 s + 0..3) on
+// alternating bank halves.
 // ---------------------------------------------------------------------------
-template <int RT, int MAXI, int G>
-__global__ __launch_bounds__(256) void k_rho(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
-                                             const double* __restrict__ pts, int64_t n_slots,
-                                             const double* __restrict__ shrec, int RS,
-                                             const int* __restrict__ bshells, const int* __restrict__ bshell_col,
-                                             const int* __restrict__ bcols, const int* __restrict__ bslab,
-                                             const int* __restrict__ bsorder, const double* __restrict__ P, int N,
-                                             double* __restrict__ coef_out, double* __restrict__ rho_out,
-                                             double* __restrict__ ewg, double dens_cut) {
-    constexpr int PB = 16 * RT;
-    constexpr int SUBS = kBlockPts / PB;
-    __shared__ double s_pt[PB][4];
-    __shared__ double s_red[4][PB][4];
-    __shared__ double s_e[PB];
-    extern __shared__ double s_dyn[];
+template <int PB>
+__device__ inline int plane_at(int c, int p) {
+    if constexpr (PB >= 32) return c * PB + (p ^ ((c & 1) << 4));
+    return c * PB + p;
+}
 
-    const int bidx = work[blockIdx.x / SUBS];
-    const int sub = blockIdx.x % SUBS;
-    const BlockDesc B = blocks[bidx];
-    const int tid = threadIdx.x;
-    const int p0 = sub * PB;
-    const int64_t wg_slot = (int64_t)bidx * 8 + sub;
-    if (p0 >= B.npts) {  // no real point in this part of the block (uniform exit, before any barrier)
-        if (tid == 0) ewg[wg_slot] = 0.0;
-        return;
+// values only (first pass of k_rho)
+__device__ inline void rec_values0(const double* rec, double px, double py, double pz, double* v) {
+    const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+    const int meta = (int)rec[3];
+    const int l = meta & 3, nprim = meta >> 2;
+    const double r2 = x * x + y * y + z * z;
+    double R0 = 0.0;
+    for (int k = 0; k < nprim; ++k) {
+        const double a = rec[4 + 2 * k];
+        const double sh = a * r2;
+        if (sh <= kTooBig) R0 += rec[5 + 2 * k] * exp(-sh);
     }
-    const int lane = tid & 63, li = lane & 15, lq = lane >> 4;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int ldn = B.ldn, nao = B.nao, nsh = B.nshell;
-    const int T = ldn >> 4;
-    // LDS: phi [max(ldn, 256)][PB] | records [nsh][RS] | cols [ldn] | shcol [nsh] | sorder [nsh] | slab [T+1]
-    double* s_phi = s_dyn;
-    double* s_rec = s_phi + (size_t)max(ldn, 4 * 64) * PB;
-    int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
-    int* s_shcol = s_cols + ldn;
-    int* s_sord = s_shcol + nsh;
-    int* s_slab = s_sord + nsh;
-    for (int c = tid; c < ldn; c += 256) s_cols[c] = (c < nao) ? bcols[B.col_begin + c] : -1;
-    for (int c = tid; c < nsh; c += 256) {
-        s_shcol[c] = bshell_col[B.shell_begin + c];
-        s_sord[c] = bsorder[B.shell_begin + c];
+    if (l == 0) {
+        v[0] = R0;
+    } else if (l == 1) {
+        v[0] = x * R0;
+        v[1] = y * R0;
+        v[2] = z * R0;
+    } else {
+        v[0] = x * y * R0;
+        v[1] = z * x * R0;
+        v[2] = y * z * R0;
+        v[3] = 0.5 * (x * x - y * y) * R0;
+        v[4] = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)) * R0;
     }
-    for (int c = tid; c <= T; c += 256) s_slab[c] = bslab[B.slab_begin + c];
-    for (int idx = tid; idx < nsh * RS; idx += 256) {
-        const int k = idx / RS;
-        s_rec[idx] = shrec[(int64_t)bshells[B.shell_begin + k] * RS + (idx - k * RS)];
-    }
-    if (tid < PB) {
-        const int64_t slot = B.pt_begin + p0 + tid;
-        s_pt[tid][0] = pts[slot];
-        s_pt[tid][1] = pts[n_slots + slot];
-        s_pt[tid][2] = pts[2 * n_slots + slot];
-        s_pt[tid][3] = pts[3 * n_slots + slot];
-    }
-    __syncthreads();
+}
 
-    // phase 1: phi of all columns at the PB points, shells in (l, nprim) order
-    for (int it = tid; it < nsh * PB; it += 256) {
-        const int p = it % PB;
-        const int k = s_sord[it / PB];
-        double v[5], vx[5], vy[5], vz[5];
-        rec_values(s_rec + k * RS, s_pt[p][0], s_pt[p][1], s_pt[p][2], v, vx, vy, vz);
-        const int c0 = s_shcol[k];
-        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-        for (int m = 0; m < nc; ++m) s_phi[(c0 + m) * PB + p] = v[m];
-    }
-    for (int it = tid; it < (ldn - nao) * PB; it += 256) s_phi[nao * PB + it] = 0.0;
-    __syncthreads();
-
-    // phase 2: X^T = P_bb Phi^T, A fragments from P, B fragments from LDS
-    v4d acc[MAXI][RT];
+// X^T tiles of one wave: rows I = wI + WI i (i < NI), point tiles
+// wP RT .. wP RT + RT - 1; A = P_bb from global memory (any valid AO index
+// stands in for a padding column: its Phi row is zero), B = Phi from LDS.
+// After a barrier (every wave has stopped reading Phi) the tiles are stored
+// over the Phi plane as X^T[col][p].
+template <int PB, int RT, int NI, int G>
+__device__ inline void rho_tiles(int wI, int WI, int wP, int T, int lane, const int* s_cols,
+                                 double* s_phi, const double* __restrict__ P, int N) {
+    const int li = lane & 15, lq = lane >> 4;
+    v4d acc[NI][RT];
 #pragma unroll
-    for (int i = 0; i < MAXI; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) acc[i][rt] = v4d{0.0, 0.0, 0.0, 0.0};
-    int gi[MAXI];
+    int gi[NI];
 #pragma unroll
-    for (int i = 0; i < MAXI; ++i) {
-        const int I = wave + 4 * i;
-        gi[i] = (I < T) ? s_cols[I * 16 + li] : -1;
-    }
+    for (int i = 0; i < NI; ++i) gi[i] = max(s_cols[(wI + WI * i) * 16 + li], 0);
     const int nsteps = T * 4;
     const int ngroups = (nsteps + G - 1) / G;
-    double aC[G][MAXI], aN[G][MAXI];
-    auto load_a = [&](int g, double (&a)[G][MAXI]) {
+    double aC[G][NI], aN[G][NI];
+    auto load_a = [&](int g, double (&a)[G][NI]) {
 #pragma unroll
         for (int d = 0; d < G; ++d) {
-            const int step = g * G + d;
-            const int gk = (step < nsteps) ? s_cols[4 * step + lq] : -1;
-            const double* prow = P + (int64_t)(gk < 0 ? 0 : gk) * N;
+            const int step = min(g * G + d, nsteps - 1);
+            const double* prow = P + (int64_t)max(s_cols[4 * step + lq], 0) * N;
 #pragma unroll
-            for (int i = 0; i < MAXI; ++i) a[d][i] = (gk >= 0 && gi[i] >= 0) ? prow[gi[i]] : 0.0;
+            for (int i = 0; i < NI; ++i) a[d][i] = prow[gi[i]];
         }
     };
     load_a(0, aC);
@@ -251,11 +213,12 @@ __global__ __launch_bounds__(256) void k_rho(const int* __restrict__ work, const
         for (int d = 0; d < G; ++d) {
             const int step = g * G + d;
             if (step < nsteps) {
+                const int c = 4 * step + lq;
                 double b[RT];
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) b[rt] = s_phi[(4 * step + lq) * PB + rt * 16 + li];
+                for (int rt = 0; rt < RT; ++rt) b[rt] = s_phi[plane_at<PB>(c, (wP * RT + rt) * 16 + li)];
 #pragma unroll
-                for (int i = 0; i < MAXI; ++i)
+                for (int i = 0; i < NI; ++i)
 #pragma unroll
                     for (int rt = 0; rt < RT; ++rt) acc[i][rt] = mfma16x16x4(aC[d][i], b[rt], acc[i][rt]);
             }
@@ -263,102 +226,189 @@ __global__ __launch_bounds__(256) void k_rho(const int* __restrict__ work, const
 #pragma unroll
         for (int d = 0; d < G; ++d)
 #pragma unroll
-            for (int i = 0; i < MAXI; ++i) aC[d][i] = aN[d][i];
+            for (int i = 0; i < NI; ++i) aC[d][i] = aN[d][i];
     }
-
-    // phase 3: epilogue per 64-column chunk; E[q][col][p] reuses the phi region
-    double part[RT][4];
+    __syncthreads();
+    // tile element (4 r + lq, li) = X^T[I 16 + 4 r + lq][point tile 16 + li]
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) part[rt][q] = 0.0;
-    double* E = s_phi;
-    constexpr int EQ = 64 * PB;  // doubles per E plane
-#pragma unroll
-    for (int c = 0; c < MAXI; ++c) {
-        const int lo = c * 64;
-        if (lo < ldn) {
-            const int hi = min(lo + 64, ldn);
-            const int first = s_slab[lo >> 4];
-            int last = first;
-            while (last < nsh && s_shcol[last] < hi) ++last;
-            __syncthreads();
-            for (int it = tid; it < (last - first) * PB; it += 256) {
-                const int p = it % PB;
-                const int k = first + it / PB;
-                double v[5], vx[5], vy[5], vz[5];
-                rec_values(s_rec + k * RS, s_pt[p][0], s_pt[p][1], s_pt[p][2], v, vx, vy, vz);
-                const int c0 = s_shcol[k];
-                const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-                for (int m = 0; m < nc; ++m) {
-                    const int col = c0 + m;
-                    if (col >= lo && col < hi) {
-                        double* e = E + (col - lo) * PB + p;
-                        e[0] = v[m];
-                        e[EQ] = vx[m];
-                        e[2 * EQ] = vy[m];
-                        e[3 * EQ] = vz[m];
-                    }
-                }
-            }
-            __syncthreads();
-            const int I = wave + 4 * c;
-            if (I < T) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int cl = wave * 16 + 4 * r + lq;  // column within the chunk
-                    if (s_cols[lo + cl] >= 0) {
-#pragma unroll
-                        for (int rt = 0; rt < RT; ++rt) {
-                            const double x = acc[c][rt][r];
-                            const double* e = E + cl * PB + rt * 16 + li;
-                            part[rt][0] += x * e[0];
-                            part[rt][1] += x * e[EQ];
-                            part[rt][2] += x * e[2 * EQ];
-                            part[rt][3] += x * e[3 * EQ];
-                        }
-                    }
-                }
-            }
-        }
-    }
-    // reduce over the four lanes that share a point (lq), then over waves
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double v = part[rt][q];
-            v += __shfl_xor(v, 16);
-            v += __shfl_xor(v, 32);
-            part[rt][q] = v;
-        }
-    if (lq == 0) {
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) s_red[wave][rt * 16 + li][q] = part[rt][q];
+            for (int r = 0; r < 4; ++r)
+                s_phi[plane_at<PB>((wI + WI * i) * 16 + 4 * r + lq, (wP * RT + rt) * 16 + li)] = acc[i][rt][r];
+}
+
+// ---------------------------------------------------------------------------
+// rho pass for PB points of a block (512 threads, 8 waves):
+//   1. Phi of all ldn columns at the PB points into an LDS plane; thread
+//      (point p, shell lane kk) evaluates shells kk, kk + 512/PB, ... in
+//      (l, nprim) order.
+//   2. X^T = P_bb Phi^T on v_mfma_f64_16x16x4, X^T in accumulators: the
+//      waves form a WI x (8/WI) grid over (AO tile rows, point tiles); wave
+//      (wI, wP) owns the exact number of rows T - wI over WI rounded up, so no
+//      MFMA runs on padding tiles.  X^T is then stored over the Phi plane.
+//   3. phi and grad phi re-evaluated with the same thread mapping and dotted
+//      with X: rho = sum_j X_pj phi_pj, grad rho = 2 sum_j X_pj grad phi_pj
+//      (calcRho_GGA_part, DfCalcGridX.cpp:1923-1962); the partial sums of
+//      the shell lanes are added in a fixed order.
+// rho_out receives (rho, grad rho) per slot; the functional runs over all
+// points in k_xc_points.
+// ---------------------------------------------------------------------------
+template <int PB, int WI, int NIMAX, int G>
+__global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
+                                             const double* __restrict__ pts, int64_t n_slots,
+                                             const double* __restrict__ shrec, int RS,
+                                             const int* __restrict__ bshells, const int* __restrict__ bshell_col,
+                                             const int* __restrict__ bcols, const int* __restrict__ bsorder,
+                                             const double* __restrict__ P, int N, double* __restrict__ rho_out) {
+    constexpr int SUBS = kBlockPts / PB;
+    constexpr int NSL = 512 / PB;  // shell lanes per point
+    constexpr int WP = 8 / WI;
+    constexpr int RT = PB / 16 / WP;  // point tiles per wave
+    static_assert(RT * 16 * WP == PB, "point tiles must cover the points");
+    __shared__ double s_pt[PB][3];
+    extern __shared__ double s_dyn[];
+
+    const int bidx = work[blockIdx.x / SUBS];
+    const int sub = blockIdx.x % SUBS;
+    const BlockDesc B = blocks[bidx];
+    const int p0 = sub * PB;
+    if (p0 >= B.npts) return;  // uniform exit before any barrier: no real point here
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int ldn = B.ldn, nao = B.nao, nsh = B.nshell;
+    const int T = ldn >> 4;
+    // LDS: plane [max(ldn PB, 4 x 512)] | records [nsh][RS] | cols [ldn] | shcol [nsh] | sorder [nsh]
+    double* s_phi = s_dyn;
+    double* s_rec = s_phi + max(ldn * PB, 4 * 512);
+    int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
+    int* s_shcol = s_cols + ldn;
+    int* s_sord = s_shcol + nsh;
+    for (int c = tid; c < ldn; c += 512) s_cols[c] = (c < nao) ? bcols[B.col_begin + c] : -1;
+    for (int c = tid; c < nsh; c += 512) {
+        s_shcol[c] = bshell_col[B.shell_begin + c];
+        s_sord[c] = bsorder[B.shell_begin + c];
+    }
+    for (int idx = tid; idx < nsh * RS; idx += 512) {
+        const int k = idx / RS;
+        s_rec[idx] = shrec[(int64_t)bshells[B.shell_begin + k] * RS + (idx - k * RS)];
+    }
+    if (tid < PB) {
+        const int64_t slot = B.pt_begin + p0 + tid;
+        s_pt[tid][0] = pts[slot];
+        s_pt[tid][1] = pts[n_slots + slot];
+        s_pt[tid][2] = pts[2 * n_slots + slot];
     }
     __syncthreads();
-    if (tid < PB) {
-        const int p = tid;
-        double rq[4];
-        for (int q = 0; q < 4; ++q) rq[q] = s_red[0][p][q] + s_red[1][p][q] + s_red[2][p][q] + s_red[3][p][q];
-        const double rho = rq[0];
-        const double gx = 2.0 * rq[1], gy = 2.0 * rq[2], gz = 2.0 * rq[3];
-        const int64_t slot = B.pt_begin + p0 + p;
-        const double w = s_pt[p][3];
-        if (rho_out) {
-            rho_out[4 * slot + 0] = rho;
-            rho_out[4 * slot + 1] = gx;
-            rho_out[4 * slot + 2] = gy;
-            rho_out[4 * slot + 3] = gz;
+
+    const int p = tid & (PB - 1);
+    const int kk = tid / PB;
+    const double px = s_pt[p][0], py = s_pt[p][1], pz = s_pt[p][2];
+    // 1. Phi plane
+    for (int j = kk; j < nsh; j += NSL) {
+        const int k = s_sord[j];
+        double v[5];
+        rec_values0(s_rec + k * RS, px, py, pz, v);
+        const int c0 = s_shcol[k];
+        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+        for (int m = 0; m < nc; ++m) s_phi[plane_at<PB>(c0 + m, p)] = v[m];
+    }
+    for (int it = tid; it < (ldn - nao) * PB; it += 512) s_phi[nao * PB + it] = 0.0;
+    __syncthreads();
+
+    // 2. X^T = P_bb Phi^T, stored over the plane
+    {
+        const int wI = wave % WI, wP = wave / WI;
+        const int nI = (T > wI) ? (T - wI + WI - 1) / WI : 0;
+        switch (nI) {
+            case 1: rho_tiles<PB, RT, 1, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N); break;
+            case 2:
+                if constexpr (NIMAX >= 2) rho_tiles<PB, RT, 2, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
+                break;
+            case 3:
+                if constexpr (NIMAX >= 3) rho_tiles<PB, RT, 3, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
+                break;
+            case 4:
+                if constexpr (NIMAX >= 4) rho_tiles<PB, RT, 4, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
+                break;
+            case 5:
+                if constexpr (NIMAX >= 5) rho_tiles<PB, RT, 5, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
+                break;
+            case 6:
+                if constexpr (NIMAX >= 6) rho_tiles<PB, RT, 6, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
+                break;
+            case 7:
+                if constexpr (NIMAX >= 7) rho_tiles<PB, RT, 7, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
+                break;
+            case 8:
+                if constexpr (NIMAX >= 8) rho_tiles<PB, RT, 8, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
+                break;
+            default: __syncthreads(); break;  // no rows: only the barrier before the store
         }
-        double a = 0.0, cx = 0.0, cy = 0.0, cz = 0.0, e = 0.0;
-        if (p0 + p < B.npts && rho > dens_cut) {
+    }
+    __syncthreads();
+
+    // 3. rho and grad rho: phi, grad phi dotted with X
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    for (int j = kk; j < nsh; j += NSL) {
+        const int k = s_sord[j];
+        double v[5], vx[5], vy[5], vz[5];
+        rec_values(s_rec + k * RS, px, py, pz, v, vx, vy, vz);
+        const int c0 = s_shcol[k];
+        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+        for (int m = 0; m < nc; ++m) {
+            const double x = s_phi[plane_at<PB>(c0 + m, p)];
+            r0 += x * v[m];
+            r1 += x * vx[m];
+            r2 += x * vy[m];
+            r3 += x * vz[m];
+        }
+    }
+    __syncthreads();  // X no longer read: the plane holds the partial sums
+    double* s_part = s_phi;  // [4][NSL][PB]
+    s_part[(0 * NSL + kk) * PB + p] = r0;
+    s_part[(1 * NSL + kk) * PB + p] = r1;
+    s_part[(2 * NSL + kk) * PB + p] = r2;
+    s_part[(3 * NSL + kk) * PB + p] = r3;
+    __syncthreads();
+    if (tid < 4 * PB) {
+        const int q = tid / PB;
+        const int pp = tid & (PB - 1);
+        double s = 0.0;
+        for (int k2 = 0; k2 < NSL; ++k2) s += s_part[(q * NSL + k2) * PB + pp];
+        const int64_t slot = B.pt_begin + p0 + pp;
+        rho_out[4 * slot + q] = (q == 0) ? s : 2.0 * s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// B3LYP point terms for every slot (buildVxc, DfCalcGridX.cpp:2137-2193, and
+// the GGA coefficients of build_XC_Matrix, :2455-2500): coef = (w vrho / 2,
+// w (2 vgAA + vgAB) grad rho), zero where rho <= the density cut-off or
+// w = 0 (padding slots); per-workgroup sums of w e_xc in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_xc_points(int64_t n_slots, const double* __restrict__ pts,
+                                                   const double* __restrict__ rho, double* __restrict__ coef,
+                                                   double* __restrict__ epart, double dens_cut) {
+    __shared__ double s_e[256];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double e = 0.0;
+    if (i < n_slots) {
+        const double w = pts[3 * n_slots + i];
+        const double r = rho[4 * i], gx = rho[4 * i + 1], gy = rho[4 * i + 2], gz = rho[4 * i + 3];
+        double a = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        if (w != 0.0 && r > dens_cut) {
             const double gamma = gx * gx + gy * gy + gz * gz;
-            const XcPoint f = b3lyp::eval_rks(rho, gamma);
-            // build_XC_Matrix GGA (DfCalcGridX.cpp:2455-2500): coef1 = w vrho,
-            // coef2 = w (2 vgAA + vgAB) grad rho; V carries half of coef1
+            const XcPoint f = b3lyp::eval_rks(r, gamma);
             a = 0.5 * (w * f.vr);
             const double c2 = 2.0 * f.vgaa;
             cx = w * (c2 * gx + f.vgab * gx);
@@ -366,18 +416,18 @@ __global__ __launch_bounds__(256) void k_rho(const int* __restrict__ work, const
             cz = w * (c2 * gz + f.vgab * gz);
             e = w * f.e;
         }
-        coef_out[4 * slot + 0] = a;
-        coef_out[4 * slot + 1] = cx;
-        coef_out[4 * slot + 2] = cy;
-        coef_out[4 * slot + 3] = cz;
-        s_e[p] = e;
+        coef[4 * i + 0] = a;
+        coef[4 * i + 1] = cx;
+        coef[4 * i + 2] = cy;
+        coef[4 * i + 3] = cz;
     }
+    s_e[threadIdx.x] = e;
     __syncthreads();
-    if (tid == 0) {
-        double e = 0.0;
-        for (int p = 0; p < PB; ++p) e += s_e[p];
-        ewg[wg_slot] = e;
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) s_e[threadIdx.x] += s_e[threadIdx.x + k];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) epart[blockIdx.x] = s_e[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -662,26 +712,35 @@ static int check_launch(const char* what) {
     return PDFXC_OK;
 }
 
-template <int RT, int MAXI, int G>
+template <int PB, int WI, int NIMAX, int G>
 static int rho_one(Context& c, const LaunchRange& r) {
-    const int subs = kBlockPts / (16 * RT);
-    hipLaunchKernelGGL((k_rho<RT, MAXI, G>), dim3(r.count * subs), dim3(256), r.lds, c.stream,
+    const int subs = kBlockPts / PB;
+    hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G>), dim3(r.count * subs), dim3(512), r.lds, c.stream,
                        c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
-                       c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_slab,
-                       c.d_block_sorder, c.d_P, c.n_aos, c.d_coef, c.d_rho, c.d_eblock,
-                       c.opts.density_cutoff);
+                       c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
+                       c.n_aos, c.d_rho);
     return check_launch("k_rho");
 }
+
+// k_rho classes by padded block width ldn: <= 128, 256, 512, 1024
+int rho_points_per_workgroup(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 : 16; }
 
 int launch_rho(Context& c, const LaunchRange& r) {
     if (r.count <= 0) return PDFXC_OK;
     switch (r.cls) {
-        case 0: return rho_one<2, 2, 4>(c, r);
-        case 1: return rho_one<2, 4, 4>(c, r);
-        case 2: return rho_one<1, 8, 2>(c, r);
-        case 3: return rho_one<1, 16, 1>(c, r);
+        case 0: return rho_one<64, 4, 2, 4>(c, r);
+        case 1: return rho_one<32, 8, 2, 4>(c, r);
+        case 2: return rho_one<16, 8, 4, 4>(c, r);
+        case 3: return rho_one<16, 8, 8, 2>(c, r);
         default: set_error("k_rho: bad class"); return PDFXC_ERR_ARGUMENT;
     }
+}
+
+int launch_xc_points(Context& c) {
+    const int64_t g = (c.n_slots + 255) / 256;
+    hipLaunchKernelGGL(k_xc_points, dim3((unsigned)g), dim3(256), 0, c.stream, c.n_slots, c.d_pts, c.d_rho,
+                       c.d_coef, c.d_eblock, c.opts.density_cutoff);
+    return check_launch("k_xc_points");
 }
 
 template <int MAXT>
@@ -718,7 +777,7 @@ int configure_kernels() {
 
 int launch_energy_sum(Context& c, double* dE) {
     hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(256), 0, c.stream, c.d_eblock,
-                       (int)c.blocks.size() * 8, dE);
+                       (int)((c.n_slots + 255) / 256), dE);
     return check_launch("k_energy_sum");
 }
 
