@@ -19,7 +19,7 @@ ROOT = PKG.parent
 SRC = PKG / "csrc"
 OUT = PKG / "_lib" / "libpdfxc.so"
 SOURCES = [SRC / "pdfxc_host.cpp", SRC / "pdfxc_kernels.hip"]
-HEADERS = [SRC / "pdfxc_internal.hpp", SRC / "xc_functional.hpp", ROOT / "include" / "pdfxc.h"]
+HEADERS = [SRC / "pdfxc_internal.hpp", SRC / "pdfxc_device.hpp", SRC / "xc_functional.hpp", ROOT / "include" / "pdfxc.h"]
 
 
 def hipcc():

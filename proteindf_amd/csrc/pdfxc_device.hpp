@@ -1,0 +1,107 @@
+// Device helpers shared by the FP64 kernels (pdfxc_kernels.hip) and the
+// split-precision kernels (pdfxc_split.hip): MFMA wrappers and AO
+// evaluation from LDS shell records.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace pdfxc {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// MFMA f64 16x16x4 accumulator layout (checked on gfx950 by
+// pdfxc_selftest_mfma): register r of lane l holds D[4 r + l / 16][l % 16];
+// operand A[i][k] comes from lane i + 16 k, operand B[k][j] from lane j + 16 k.
+__device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr double kTooBig = 30.0;  // DfCalcGridX::TOOBIG, primitives with alpha r^2 > 30 are skipped
+constexpr double kInvSqrt3 = 0.57735026918962576451;
+
+// ---------------------------------------------------------------------------
+// Shell records.  Every shell has a record of RS doubles in global memory:
+// [cx, cy, cz, l + 4 nprim, e0, c0, e1, c1, ...]; a block copies the records
+// it needs into LDS once, so that AO evaluation reads no global memory.
+// ---------------------------------------------------------------------------
+__device__ inline void rec_values(const double* rec, double px, double py, double pz, double* v,
+                                  double* vx, double* vy, double* vz) {
+    const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+    const int meta = (int)rec[3];
+    const int l = meta & 3, nprim = meta >> 2;
+    const double r2 = x * x + y * y + z * z;
+    double R0 = 0.0, R1 = 0.0;
+    for (int k = 0; k < nprim; ++k) {
+        const double a = rec[4 + 2 * k];
+        const double sh = a * r2;
+        if (sh <= kTooBig) {
+            const double g = rec[5 + 2 * k] * exp(-sh);
+            R0 += g;
+            R1 += -2.0 * a * g;
+        }
+    }
+    if (l == 0) {
+        v[0] = R0;
+        vx[0] = x * R1;
+        vy[0] = y * R1;
+        vz[0] = z * R1;
+    } else if (l == 1) {
+        v[0] = x * R0;
+        v[1] = y * R0;
+        v[2] = z * R0;
+        vx[0] = R0 + x * x * R1;
+        vy[0] = x * y * R1;
+        vz[0] = x * z * R1;
+        vx[1] = y * x * R1;
+        vy[1] = R0 + y * y * R1;
+        vz[1] = y * z * R1;
+        vx[2] = z * x * R1;
+        vy[2] = z * y * R1;
+        vz[2] = R0 + z * z * R1;
+    } else {
+        // 4 xy, 5 xz, 6 yz, 7 (x^2-y^2)/2, 8 (z^2-(x^2+y^2)/2)/sqrt3
+        const double f[5] = {x * y, z * x, y * z, 0.5 * (x * x - y * y),
+                             kInvSqrt3 * (z * z - 0.5 * (x * x + y * y))};
+        const double fx[5] = {y, z, 0.0, x, -kInvSqrt3 * x};
+        const double fy[5] = {x, 0.0, z, -y, -kInvSqrt3 * y};
+        const double fz[5] = {0.0, x, y, 0.0, 2.0 * kInvSqrt3 * z};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            v[m] = f[m] * R0;
+            vx[m] = fx[m] * R0 + f[m] * x * R1;
+            vy[m] = fy[m] * R0 + f[m] * y * R1;
+            vz[m] = fz[m] * R0 + f[m] * z * R1;
+        }
+    }
+}
+
+// values only (first pass of k_rho)
+__device__ inline void rec_values0(const double* rec, double px, double py, double pz, double* v) {
+    const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+    const int meta = (int)rec[3];
+    const int l = meta & 3, nprim = meta >> 2;
+    const double r2 = x * x + y * y + z * z;
+    double R0 = 0.0;
+    for (int k = 0; k < nprim; ++k) {
+        const double a = rec[4 + 2 * k];
+        const double sh = a * r2;
+        if (sh <= kTooBig) R0 += rec[5 + 2 * k] * exp(-sh);
+    }
+    if (l == 0) {
+        v[0] = R0;
+    } else if (l == 1) {
+        v[0] = x * R0;
+        v[1] = y * R0;
+        v[2] = z * R0;
+    } else {
+        v[0] = x * y * R0;
+        v[1] = z * x * R0;
+        v[2] = y * z * R0;
+        v[3] = 0.5 * (x * x - y * y) * R0;
+        v[4] = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)) * R0;
+    }
+}
+
+}  // namespace pdfxc

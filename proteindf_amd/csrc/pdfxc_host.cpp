@@ -519,7 +519,7 @@ static int build_blocks(Context& c) {
         r.lds = maxdyn;
         if (r.count) c.rho_launches.push_back(r);
     }
-    // k_kmat work: lower tiles split into items of at most 4 x MAXT tiles
+    // k_kmat work: lower tiles split into items of at most 96 tiles
     std::vector<KWork> kw[2];
     size_t klds[2] = {0, 0};
     for (int b = 0; b < nb; ++b) {
@@ -527,7 +527,7 @@ static int build_blocks(Context& c) {
         const int T = B.ldn / 16;
         const int nt = T * (T + 1) / 2;
         const int cls = (nt <= 32) ? 0 : 1;
-        const int per = (cls == 0) ? 32 : 96;
+        const int per = (cls == 0) ? 32 : 96;  // at most 8 x 12 tiles (the kernel's consumer limit)
         for (int t0 = 0; t0 < nt; t0 += per) {
             KWork w;
             w.block = b;

@@ -17,22 +17,11 @@
 
 #include <algorithm>
 
+#include "pdfxc_device.hpp"
 #include "pdfxc_internal.hpp"
 #include "xc_functional.hpp"
 
 namespace pdfxc {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-// MFMA f64 16x16x4 accumulator layout (checked on gfx950 by
-// pdfxc_selftest_mfma): register r of lane l holds D[4 r + l / 16][l % 16];
-// operand A[i][k] comes from lane i + 16 k, operand B[k][j] from lane j + 16 k.
-__device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-constexpr double kTooBig = 30.0;  // DfCalcGridX::TOOBIG, primitives with alpha r^2 > 30 are skipped
-constexpr double kInvSqrt3 = 0.57735026918962576451;
 
 // ---------------------------------------------------------------------------
 // AO values and gradients for one block (getAOs_core / getDAOs_core,
@@ -83,62 +72,6 @@ __device__ inline void shell_values(const DeviceShell& S, double x, double y, do
 }
 
 // ---------------------------------------------------------------------------
-// Shell records.  Every shell has a record of RS doubles in global memory:
-// [cx, cy, cz, l + 4 nprim, e0, c0, e1, c1, ...]; a block copies the records
-// it needs into LDS once, so that AO evaluation reads no global memory.
-// ---------------------------------------------------------------------------
-__device__ inline void rec_values(const double* rec, double px, double py, double pz, double* v,
-                                  double* vx, double* vy, double* vz) {
-    const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
-    const int meta = (int)rec[3];
-    const int l = meta & 3, nprim = meta >> 2;
-    const double r2 = x * x + y * y + z * z;
-    double R0 = 0.0, R1 = 0.0;
-    for (int k = 0; k < nprim; ++k) {
-        const double a = rec[4 + 2 * k];
-        const double sh = a * r2;
-        if (sh <= kTooBig) {
-            const double g = rec[5 + 2 * k] * exp(-sh);
-            R0 += g;
-            R1 += -2.0 * a * g;
-        }
-    }
-    if (l == 0) {
-        v[0] = R0;
-        vx[0] = x * R1;
-        vy[0] = y * R1;
-        vz[0] = z * R1;
-    } else if (l == 1) {
-        v[0] = x * R0;
-        v[1] = y * R0;
-        v[2] = z * R0;
-        vx[0] = R0 + x * x * R1;
-        vy[0] = x * y * R1;
-        vz[0] = x * z * R1;
-        vx[1] = y * x * R1;
-        vy[1] = R0 + y * y * R1;
-        vz[1] = y * z * R1;
-        vx[2] = z * x * R1;
-        vy[2] = z * y * R1;
-        vz[2] = R0 + z * z * R1;
-    } else {
-        // 4 xy, 5 xz, 6 yz, 7 (x^2-y^2)/2, 8 (z^2-(x^2+y^2)/2)/sqrt3
-        const double f[5] = {x * y, z * x, y * z, 0.5 * (x * x - y * y),
-                             kInvSqrt3 * (z * z - 0.5 * (x * x + y * y))};
-        const double fx[5] = {y, z, 0.0, x, -kInvSqrt3 * x};
-        const double fy[5] = {x, 0.0, z, -y, -kInvSqrt3 * y};
-        const double fz[5] = {0.0, x, y, 0.0, 2.0 * kInvSqrt3 * z};
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            v[m] = f[m] * R0;
-            vx[m] = fx[m] * R0 + f[m] * x * R1;
-            vy[m] = fy[m] * R0 + f[m] * y * R1;
-            vz[m] = fz[m] * R0 + f[m] * z * R1;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // LDS layout of the per-point AO planes of k_rho: element (column c, point p)
 // of a [ldn][PB] plane sits at c PB + (p ^ 16 (c & 1)).  The swizzle puts the
 // four 16-lane groups of an MFMA B-operand read (columns 4 s + 0..3) on
@@ -148,33 +81,6 @@ template <int PB>
 __device__ inline int plane_at(int c, int p) {
     if constexpr (PB >= 32) return c * PB + (p ^ ((c & 1) << 4));
     return c * PB + p;
-}
-
-// values only (first pass of k_rho)
-__device__ inline void rec_values0(const double* rec, double px, double py, double pz, double* v) {
-    const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
-    const int meta = (int)rec[3];
-    const int l = meta & 3, nprim = meta >> 2;
-    const double r2 = x * x + y * y + z * z;
-    double R0 = 0.0;
-    for (int k = 0; k < nprim; ++k) {
-        const double a = rec[4 + 2 * k];
-        const double sh = a * r2;
-        if (sh <= kTooBig) R0 += rec[5 + 2 * k] * exp(-sh);
-    }
-    if (l == 0) {
-        v[0] = R0;
-    } else if (l == 1) {
-        v[0] = x * R0;
-        v[1] = y * R0;
-        v[2] = z * R0;
-    } else {
-        v[0] = x * y * R0;
-        v[1] = z * x * R0;
-        v[2] = y * z * R0;
-        v[3] = 0.5 * (x * x - y * y) * R0;
-        v[4] = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)) * R0;
-    }
 }
 
 // X^T tiles of one wave: rows I = wI + WI i (i < NI), point tiles
@@ -431,21 +337,90 @@ __global__ __launch_bounds__(256) void k_xc_points(int64_t n_slots, const double
 }
 
 // ---------------------------------------------------------------------------
-// K pass: K_bb += Phi^T V + V^T Phi for the lower 16 x 16 tiles t0 <= t < t1
-// of one block (t = I (I + 1) / 2 + J).  Eight waves: waves 0-3 (consumers)
-// hold the tiles in accumulators -- each a contiguous quarter, padded to MAXT
-// with a repeated tile that is not scattered, so the accumulate loop has no
-// branches -- and run the MFMAs on point slab s while waves 4-7 (producers)
-// evaluate slab s + 1 into the other half of a double-buffered LDS area:
-// phi and grad phi of the needed columns from the LDS shell records, and
-// V = (w vrho / 2) phi + c . grad phi with the per-point coefficients
-// (build_XC_Matrix GGA, DfCalcGridX.cpp:2455-2500).  Slab rows are padded to
-// a stride of 16 (mod 32) doubles so that the 16-lane groups of an MFMA
-// operand read fall on disjoint LDS banks.  Tiles go into the dense lower
-// triangle of K with FP64 atomics.
+// K pass consumer: NT accumulator tiles of K_bb (a contiguous run [tb, te)
+// of lower tiles, t = I (I + 1) / 2 + J, rounded up to a multiple of 4 with
+// repeats of the last tile that are not scattered) over the point slabs the
+// producer waves fill.  Slab element (point p, column c) sits at
+// p LD + (c ^ (p & 15)): the XOR keeps the 16 lanes of an operand read on
+// one 128-byte run while spreading a producer's column writes over the banks.
 // ---------------------------------------------------------------------------
-template <int MAXT>
-__global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+template <int NT>
+__device__ inline void kmat_consume(int tb, int te, int nslab, int PS, int LD, int SLAB, int lane,
+                                    const double* s_buf, const int* s_cols, double* __restrict__ K, int N) {
+    const int li = lane & 15, lq = lane >> 4;
+    int offI[NT], offJ[NT];
+    {
+        int I = 0;
+        while ((I + 1) * (I + 2) / 2 <= tb) ++I;
+        int J = tb - I * (I + 1) / 2;
+#pragma unroll
+        for (int m = 0; m < NT; ++m) {
+            offI[m] = I * 16;
+            offJ[m] = J * 16;
+            if (tb + m + 1 < te) {  // advance along the run; padding repeats the last tile
+                if (J == I) {
+                    ++I;
+                    J = 0;
+                } else {
+                    ++J;
+                }
+            }
+        }
+    }
+    v4d acc[NT];
+#pragma unroll
+    for (int m = 0; m < NT; ++m) acc[m] = v4d{0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+    for (int s = 0; s < nslab; ++s) {
+        const double* cur = s_buf + (s & 1) * 2 * SLAB;
+        for (int ks = 0; ks < PS / 4; ++ks) {
+            const int p = 4 * ks + lq;
+            const double* fr = cur + p * LD + (li ^ (p & 15));
+            const double* vr = fr + SLAB;
+            // groups of 4 tiles: bounded operand lookahead keeps the
+            // accumulators and operands within the register budget
+#pragma unroll
+            for (int m0 = 0; m0 < NT; m0 += 4) {
+#pragma unroll
+                for (int m = m0; m < m0 + 4; ++m) {
+                    const double a1 = fr[offI[m]], b1 = vr[offJ[m]];
+                    const double a2 = vr[offI[m]], b2 = fr[offJ[m]];
+                    acc[m] = mfma16x16x4(a1, b1, acc[m]);
+                    acc[m] = mfma16x16x4(a2, b2, acc[m]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        __syncthreads();
+    }
+    // scatter: tile element (4 r + lq, li) -> K[cols[I 16 + 4 r + lq]][cols[J 16 + li]]
+#pragma unroll
+    for (int m = 0; m < NT; ++m) {
+        if (tb + m < te) {
+            const int lc = offJ[m] + li;
+            const int gj = s_cols[lc];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int lr = offI[m] + 4 * r + lq;
+                const int gi = s_cols[lr];
+                if (gi >= 0 && gj >= 0 && lr >= lc) atomicAdd(K + (int64_t)gi * N + gj, acc[m][r]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K pass: K_bb += Phi^T V + V^T Phi for the lower 16 x 16 tiles t0 <= t < t1
+// (at most 96) of one block.  Twelve waves: waves 0-7 (consumers, two per
+// SIMD) each hold a contiguous eighth of the tiles in accumulators and run
+// the MFMAs on point slab s while waves 8-11 (producers) evaluate slab s + 1
+// into the other
+// half of a double-buffered LDS area: phi and grad phi of the needed columns
+// from the LDS shell records, and V = (w vrho / 2) phi + c . grad phi with
+// the per-point coefficients (build_XC_Matrix GGA, DfCalcGridX.cpp:2455-2500).
+// Tiles go into the dense lower triangle of K with FP64 atomics.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                               const double* __restrict__ pts, int64_t n_slots,
                                               const double* __restrict__ shrec, int RS,
                                               const int* __restrict__ bshells, const int* __restrict__ bshell_col,
@@ -457,9 +432,9 @@ __global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, co
     const KWork W = work[blockIdx.x];
     const BlockDesc B = blocks[W.block];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, li = lane & 15, lq = lane >> 4;
+    const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const bool producer = wave >= 4;
+    const bool producer = wave >= 8;
     const int ncols = W.ncols;  // multiple of 16, <= ldn
     const int LD = (ncols & 31) ? ncols : ncols + 16;
     const int PS = W.ps;
@@ -471,9 +446,9 @@ __global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, co
     int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
     int* s_shcol = s_cols + ncols;
     int* s_shid = s_shcol + nsh;
-    for (int c = tid; c < ncols; c += 512) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
-    // shells this item needs (local index < nsh), in (l, nprim) order, compacted by wave 4
-    if (wave == 4) {
+    for (int c = tid; c < ncols; c += 768) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
+    // shells this item needs (local index < nsh), in (l, nprim) order, compacted by wave 8
+    if (wave == 8) {
         int base = 0;
         for (int i0 = 0; i0 < B.nshell; i0 += 64) {
             const int i = i0 + lane;
@@ -488,7 +463,7 @@ __global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, co
             base += __popcll(m);
         }
     }
-    for (int p = tid; p < kBlockPts; p += 512) {
+    for (int p = tid; p < kBlockPts; p += 768) {
         const int64_t slot = B.pt_begin + p;
         s_pt[p][0] = pts[slot];
         s_pt[p][1] = pts[n_slots + slot];
@@ -499,13 +474,13 @@ __global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, co
         s_cf[p][3] = coef[4 * slot + 3];
     }
     __syncthreads();
-    for (int idx = tid; idx < nsh * RS; idx += 512) {
+    for (int idx = tid; idx < nsh * RS; idx += 768) {
         const int k = idx / RS;
         s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
     }
     __syncthreads();
 
-    const int ptid = tid - 256;  // producer thread index
+    const int ptid = tid - 512;  // producer thread index
     auto produce = [&](int pb, double* phi, double* vv) {
         for (int it = ptid; it < nsh * PS; it += 256) {
             const int p = it % PS;
@@ -516,19 +491,20 @@ __global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, co
             const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
             const int c0 = s_shcol[k];
             const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+            const int sw = p & 15;
             for (int m = 0; m < nc; ++m) {
                 const int c = c0 + m;
                 if (c < ncols) {
-                    phi[p * LD + c] = v[m];
-                    vv[p * LD + c] = a * v[m] + cx * vx[m] + cy * vy[m] + cz * vz[m];
+                    phi[p * LD + (c ^ sw)] = v[m];
+                    vv[p * LD + (c ^ sw)] = a * v[m] + cx * vx[m] + cy * vy[m] + cz * vz[m];
                 }
             }
         }
         for (int it = ptid; it < (ncols - min(ncols, B.nao)) * PS; it += 256) {
             const int c = B.nao + it / PS;
             const int p = it % PS;
-            phi[p * LD + c] = 0.0;
-            vv[p * LD + c] = 0.0;
+            phi[p * LD + (c ^ (p & 15))] = 0.0;
+            vv[p * LD + (c ^ (p & 15))] = 0.0;
         }
     };
 
@@ -547,68 +523,19 @@ __global__ __launch_bounds__(512) void k_kmat(const KWork* __restrict__ work, co
         }
         return;
     }
-    // consumers: a contiguous run [tb, te) of tiles, row (I) and column (J) offsets
-    const int q4 = (W.t1 - W.t0 + 3) >> 2;
-    const int tb = W.t0 + wave * q4;
-    const int te = min(W.t1, tb + q4);
-    int offI[MAXT], offJ[MAXT];
-    {
-        const int t = min(tb, W.t1 - 1);
-        int I = 0;
-        while ((I + 1) * (I + 2) / 2 <= t) ++I;
-        int J = t - I * (I + 1) / 2;
-#pragma unroll
-        for (int m = 0; m < MAXT; ++m) {
-            offI[m] = I * 16;
-            offJ[m] = J * 16;
-            if (tb + m + 1 < te) {  // advance along the run; padding repeats the last tile
-                if (J == I) {
-                    ++I;
-                    J = 0;
-                } else {
-                    ++J;
-                }
-            }
-        }
-    }
-    v4d acc[MAXT];
-#pragma unroll
-    for (int m = 0; m < MAXT; ++m) acc[m] = v4d{0.0, 0.0, 0.0, 0.0};
-    __syncthreads();
-    for (int s = 0; s < nslab; ++s) {
-        const double* cur = s_buf + (s & 1) * 2 * SLAB;
-        for (int ks = 0; ks < PS / 4; ++ks) {
-            const double* fr = cur + (4 * ks + lq) * LD + li;
-            const double* vr = cur + SLAB + (4 * ks + lq) * LD + li;
-            // groups of 4 tiles: bounded operand lookahead keeps the
-            // accumulators and operands within the register budget
-#pragma unroll
-            for (int m0 = 0; m0 < MAXT; m0 += 4) {
-#pragma unroll
-                for (int m = m0; m < m0 + 4 && m < MAXT; ++m) {
-                    const double a1 = fr[offI[m]], b1 = vr[offJ[m]];
-                    const double a2 = vr[offI[m]], b2 = fr[offJ[m]];
-                    acc[m] = mfma16x16x4(a1, b1, acc[m]);
-                    acc[m] = mfma16x16x4(a2, b2, acc[m]);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-        }
-        __syncthreads();
-    }
-    // scatter: tile element (4 r + lq, li) -> K[cols[I 16 + 4 r + lq]][cols[J 16 + li]]
-#pragma unroll
-    for (int m = 0; m < MAXT; ++m) {
-        if (tb + m < te) {
-            const int lc = offJ[m] + li;
-            const int gj = s_cols[lc];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int lr = offI[m] + 4 * r + lq;
-                const int gi = s_cols[lr];
-                if (gi >= 0 && gj >= 0 && lr >= lc) atomicAdd(K + (int64_t)gi * N + gj, acc[m][r]);
-            }
-        }
+    // consumers: a contiguous run [tb, te) of the item's tiles, dispatched on
+    // the run length rounded up to 4 (one barrier before the slab loop, one per slab)
+    const int q8 = (W.t1 - W.t0 + 7) >> 3;
+    const int tstart = W.t0 + wave * q8;
+    const int tend = min(W.t1, tstart + q8);
+    const bool has = tstart < tend;
+    const int tb = has ? tstart : W.t1 - 1;  // a wave without tiles keeps valid offsets
+    const int te = has ? tend : tb;          // and scatters nothing
+    const int ngrp = has ? (tend - tstart + 3) >> 2 : 1;
+    switch (ngrp) {
+        case 1: kmat_consume<4>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K, N); break;
+        case 2: kmat_consume<8>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K, N); break;
+        default: kmat_consume<12>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K, N); break;
     }
 }
 
@@ -743,9 +670,8 @@ int launch_xc_points(Context& c) {
     return check_launch("k_xc_points");
 }
 
-template <int MAXT>
 static int kmat_one(Context& c, const LaunchRange& r) {
-    hipLaunchKernelGGL((k_kmat<MAXT>), dim3(r.count), dim3(512), r.lds, c.stream,
+    hipLaunchKernelGGL(k_kmat, dim3(r.count), dim3(768), r.lds, c.stream,
                        reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
                        c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col,
                        c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
@@ -755,8 +681,8 @@ static int kmat_one(Context& c, const LaunchRange& r) {
 int launch_kmat(Context& c, const LaunchRange& r) {
     if (r.count <= 0) return PDFXC_OK;
     switch (r.cls) {
-        case 0: return kmat_one<8>(c, r);
-        case 1: return kmat_one<24>(c, r);
+        case 0:
+        case 1: return kmat_one(c, r);
         default: set_error("k_kmat: bad class"); return PDFXC_ERR_ARGUMENT;
     }
 }
