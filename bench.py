@@ -2,7 +2,12 @@
 
 Workload (default): alpha-helical H-(Gly)_10-OH, 73 atoms, 6-31G(d) with 5
 spherical d (638 AOs), 75 radial x 302 Lebedev Becke grid (~1.48M points),
-128-point grid blocks with per-block AO screening at 1e-12, FP64 contractions.
+128-point grid blocks with per-block AO screening at 1e-12.  Each run times
+the FP64 build and the split-precision builds (bf16 slices on MFMA, FP32
+accumulation, FP64 elsewhere) on the same grid and P; value is the mode named
+by --precision (default: split-BF16 hi+mid+lo), and "modes" lists every mode's
+time, throughput and, for the split modes, |dE_xc| and max|dK_xc| against the
+FP64 build.
 P is the deterministic synthetic guess of proteindf_amd.guess (the reference's
 own SCF is outside the XC path).  Inputs (grid, blocks, P) are resident in HBM
 before the timed region; a step is one pdfxc_build_rks_device call.
@@ -14,6 +19,7 @@ baseline runs the C oracle (the reference algorithm: all AOs at every point)
 on a bounded sample of grid points and scales to the whole grid.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload gly10|h2o8]
+                    [--precision fp64|bf16x2|bf16x3] [--modes fp64,bf16x2,bf16x3]
 """
 import argparse
 import ctypes
@@ -30,7 +36,6 @@ sys.path.insert(0, str(ROOT))
 
 from proteindf_amd import calc_grid, engine, guess, molecules  # noqa: E402
 
-PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
@@ -99,12 +104,57 @@ def cpu_baseline(info, grid, PA, flops_total, target_s=12.0):
                       f"scaled to a {t_full:.0f} s whole-grid build"}
 
 
+MODES = {"fp64": engine.PDFXC_FP64, "bf16x2": engine.PDFXC_SPLIT_BF16, "bf16x3": engine.PDFXC_SPLIT_BF16X3}
+PEAK_TFLOPS = {"fp64": 78.6, "bf16x2": 2500.0, "bf16x3": 2500.0}
+E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 build
+
+
+def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup):
+    """Build a context with the given precision on the same grid, time steps
+    builds on HBM-resident P/K, return (ms per build, stats, E, K dense)."""
+    N = info.n_aos
+    ctx = engine.XcContext(info, engine.default_opts(ao_screen=ao_screen, precision=precision))
+    ctx.set_grid(grid)
+    dP, dK, dE = hip.malloc(PA.nbytes), hip.malloc(PA.nbytes), hip.malloc(8)
+    hip.h2d(dP, PA)
+    hip.memset(dK, PA.nbytes)
+    for _ in range(warmup):
+        ctx.build_rks_device(dP, dK, dE)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.build_rks_device(dP, dK, dE)
+    hip.sync()
+    dt = (time.perf_counter() - t0) / max(steps, 1)
+    st = ctx.stats()
+    # one more build into a zeroed K for the result
+    hip.memset(dK, PA.nbytes)
+    ctx.build_rks_device(dP, dK, dE)
+    E = np.zeros(1)
+    K = np.zeros((N, N))
+    hip.d2h(E, dE)
+    hip.d2h(K, dK)
+    K = np.tril(K)
+    K = K + np.tril(K, -1).T
+    # the host-buffer entry point gives the same energy
+    E_host, _, _ = ctx.build_rks(guess.pack_lower(PA))
+    if abs(E[0] - E_host) > 1e-10 * max(1.0, abs(E_host)):
+        raise RuntimeError(f"device/host build energies differ: {E[0]} vs {E_host}")
+    ctx.close()
+    for p in (dP, dK, dE):
+        hip.free(p)
+    return dt, st, E[0], K
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="gly10", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="bf16x3", choices=sorted(MODES),
+                    help="mode whose throughput is the reported value (the others are measured beside it)")
+    ap.add_argument("--modes", default="fp64,bf16x2,bf16x3", help="comma list of modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
@@ -121,64 +171,67 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU per process; the build does not "
                                    "shard across GPUs (use --gpus 1)"}), flush=True)
         sys.exit(2)
+    modes = [m for m in args.modes.split(",") if m]
+    if "fp64" not in modes:
+        modes.insert(0, "fp64")  # the split errors are stated against the FP64 build
+    if args.precision not in modes:
+        modes.append(args.precision)
     wl = WORKLOADS[args.workload]
     mol = wl["make"]()
     info = calc_grid.orbital_info(mol)
     N = info.n_aos
-    opts = engine.default_opts(ao_screen=wl["ao_screen"])
-    ctx = engine.XcContext(info, opts)
+    gctx = engine.XcContext(info, engine.default_opts(ao_screen=wl["ao_screen"]))
     t0 = time.perf_counter()
-    n_pts = ctx.generate_grid(mol.Z, mol.xyz, wl["n_rad"], wl["n_ang"])
+    n_pts = gctx.generate_grid(mol.Z, mol.xyz, wl["n_rad"], wl["n_ang"])
     t_grid = time.perf_counter() - t0
-    st = ctx.stats()
-    print(f"[bench] {wl['label']}: {mol.n_atoms} atoms, {N} AOs, {n_pts} points, {st.n_blocks} blocks, "
-          f"mean N_b {st.mean_nao:.1f}, max N_b {st.max_nao}, {st.n_chunks} launches, "
-          f"grid+blocks {t_grid:.2f} s", file=sys.stderr, flush=True)
+    grid = gctx.get_grid()
+    st0 = gctx.stats()
+    gctx.close()
+    print(f"[bench] {wl['label']}: {mol.n_atoms} atoms, {N} AOs, {n_pts} points, {st0.n_blocks} blocks, "
+          f"mean N_b {st0.mean_nao:.1f}, max N_b {st0.max_nao}, grid generation {t_grid:.2f} s",
+          file=sys.stderr, flush=True)
 
-    P = guess.density_guess(mol, info, seed=0, eps=0.05)
-    PA = np.ascontiguousarray(0.5 * P)
+    PA = np.ascontiguousarray(0.5 * guess.density_guess(mol, info, seed=0, eps=0.05))
     hip = Hip()
-    dP = hip.malloc(PA.nbytes)
-    dK = hip.malloc(PA.nbytes)
-    dE = hip.malloc(8)
-    hip.h2d(dP, PA)
-    hip.memset(dK, PA.nbytes)
+    res = {}
+    for m in modes:
+        dt, st, E, K = time_mode(hip, info, grid, PA, MODES[m], wl["ao_screen"], args.steps, args.warmup)
+        res[m] = dict(dt=dt, st=st, E=E, K=K, k_ms=list(st.last_kernel_ms))
+        print(f"[bench] {m:7s} E_xc {E:.10f}  build {dt * 1e3:.2f} ms  kernels ms: "
+              + ", ".join(f"{t:.2f}" for t in res[m]["k_ms"][:3]), file=sys.stderr, flush=True)
+    ref = res["fp64"]
+    summary = {}
+    for m in modes:
+        r = res[m]
+        e = {"ms": r["dt"] * 1e3, "gflops": r["st"].flops / r["dt"] / 1e9}
+        if m != "fp64":
+            e["abs_dE"] = abs(r["E"] - ref["E"])
+            e["max_abs_dK"] = float(np.abs(r["K"] - ref["K"]).max())
+            e["speedup_vs_fp64"] = ref["dt"] / r["dt"]
+            e["meets_1e-6_1e-7"] = bool(e["abs_dE"] <= E_TARGET and e["max_abs_dK"] <= K_TARGET)
+        summary[m] = e
+    head = res[args.precision]
+    st = head["st"]
+    dt = head["dt"]
 
-    for _ in range(args.warmup):
-        ctx.build_rks_device(dP, dK, dE)
-    hip.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.build_rks_device(dP, dK, dE)
-    hip.sync()
-    dt = (time.perf_counter() - t0) / max(args.steps, 1)
-    st = ctx.stats()
-    E = np.zeros(1)
-    hip.d2h(E, dE)
-    # cross-check the resident-path energy against the host-buffer entry point
-    E_host, _, _ = ctx.build_rks(guess.pack_lower(PA))
-    if abs(E[0] - E_host) > 1e-10:
-        raise RuntimeError(f"device/host build energies differ: {E[0]} vs {E_host}")
-
-    # dominant kernel: per-launch algorithmic flops over its HIP-event time
-    k_ms = list(st.last_kernel_ms)
-    names = ["k_rho", "k_kmat", "k_energy_sum", "unused"]
+    # dominant kernel of the reported mode: per-build algorithmic flops of its
+    # contraction (half of 4 n_b N_b^2) over its HIP-event time
+    k_ms = head["k_ms"]
+    names = ["k_rho", "k_kmat"] if args.precision == "fp64" else ["k_rho_split", "k_mfull_split+k_ksym"]
     dom = int(np.argmax(k_ms[:2]))
-    gemm_flops = st.flops / 2.0  # each contraction is half of 4 n_b N_b^2
-    achieved = gemm_flops / (k_ms[dom] / 1e3) / 1e12
-    roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
-    print(f"[bench] E_xc {E[0]:.10f}  build {dt * 1e3:.2f} ms  kernels ms: "
-          + ", ".join(f"{n} {t:.2f}" for n, t in zip(names, k_ms)), file=sys.stderr, flush=True)
+    achieved = st.flops / 2.0 / (k_ms[dom] / 1e3) / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": peak,
+            "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(info, ctx.get_grid(), PA, st.flops)
+        cpu = cpu_baseline(info, grid, PA, st.flops)
 
-    gflops = st.flops / dt / 1e9
+    label = {"fp64": "FP64", "bf16x2": "split-BF16 hi+lo", "bf16x3": "split-BF16 hi+mid+lo"}[args.precision]
     line = {
-        "metric": "K_xc+E_xc build GFLOP/s (algorithmic 4 n_b N_b^2), FP64 GPU",
-        "value": gflops,
+        "metric": f"K_xc+E_xc build GFLOP/s (algorithmic 4 n_b N_b^2), {label} GPU",
+        "value": st.flops / dt / 1e9,
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -188,19 +241,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "bf16 slices, f32 accumulate, f64 AO/functional/sums",
         "data": "synthetic (NeRF alpha-helix geometry, deterministic density guess)",
-        "config": {"workload": wl["label"], "n_atoms": mol.n_atoms, "n_aos": N, "n_points": n_pts,
-                   "n_blocks": int(st.n_blocks), "block_points": 128, "ao_screen": wl["ao_screen"],
-                   "mean_block_aos": round(st.mean_nao, 1), "algorithmic_gflop": st.flops / 1e9,
-                   "E_xc": E[0]},
+        "config": {"workload": wl["label"], "precision": args.precision, "n_atoms": mol.n_atoms, "n_aos": N,
+                   "n_points": n_pts, "n_blocks": int(st.n_blocks), "block_points": 128,
+                   "ao_screen": wl["ao_screen"], "mean_block_aos": round(st.mean_nao, 1),
+                   "algorithmic_gflop": st.flops / 1e9, "E_xc": head["E"]},
+        "modes": summary,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    ctx.close()
-    for p in (dP, dK, dE):
-        hip.free(p)
 
 
 if __name__ == "__main__":

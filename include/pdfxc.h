@@ -40,15 +40,19 @@ enum {
 /* functionals (DfXCFunctional.h:145-171 dispatch; B3LYP first) */
 enum { PDFXC_B3LYP = 1 };
 
-/* arithmetic of the two contractions X = Phi.P and K = Phi^T.V */
-enum { PDFXC_FP64 = 0 };
+/* arithmetic of the two contractions X = Phi.P and K = Phi^T.V:
+ * FP64 MFMA, or bf16 MFMA with FP32 accumulation on operands split into
+ * bf16 slices (hi + lo: 3 slice products; hi + mid + lo: 6).  AO values,
+ * the functional, the rho / grad rho dots and the K_xc sums stay FP64 in
+ * every mode.  No reference counterpart (the reference is FP64 only). */
+enum { PDFXC_FP64 = 0, PDFXC_SPLIT_BF16 = 1, PDFXC_SPLIT_BF16X3 = 2 };
 
 typedef struct pdfxc_opts {
     double density_cutoff;   /* rho_alpha cutoff, xc-density-threshold (DfCalcGridX.cpp:55-59), 1e-16 */
     double ao_screen;        /* per-block AO screening bound; 0 keeps every AO the reference
                                 evaluates as non-zero (only the alpha r^2 > 30 primitive cut,
                                 DfCalcGridX.cpp:42, :690-700) */
-    int precision;           /* PDFXC_FP64 */
+    int precision;           /* PDFXC_FP64, PDFXC_SPLIT_BF16, PDFXC_SPLIT_BF16X3 */
     int block_points;        /* points per grid block: 128 */
     int64_t scratch_bytes;   /* reserved, must be 0 (AO values are never stored in HBM) */
 } pdfxc_opts;
@@ -141,6 +145,13 @@ const char* pdfxc_last_error(void);
  * 16 x 16 row-major D the engine's accumulator layout assumes.  Lets a test
  * confirm the lane layout the kernels rely on. */
 int pdfxc_selftest_mfma(double* out16x16);
+
+/* Diagnostics: one v_mfma_f32_16x16x32_bf16 product of the integer operands
+ * A[i][k] = ((3 i + 5 k) mod 11) - 5, B[k][j] = ((7 k + 2 j) mod 13) - 6
+ * (i, j < 16, k < 32; exact in bf16 and FP32), written as the 16 x 16
+ * row-major D the split-precision kernels' operand and accumulator layout
+ * assumes. */
+int pdfxc_selftest_mfma_split(float* out16x16);
 
 /* Diagnostics: a plain streaming copy of n doubles between device buffers
  * (reads 8n bytes, writes 8n bytes), the reference point for HBM traffic

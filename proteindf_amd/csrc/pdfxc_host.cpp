@@ -137,10 +137,11 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
         set_error("pdfxc_ctx_create: only 128 points per block are supported");
         return PDFXC_ERR_UNSUPPORTED;
     }
-    if (c->opts.precision != PDFXC_FP64) {
+    if (c->opts.precision != PDFXC_FP64 && c->opts.precision != PDFXC_SPLIT_BF16 &&
+        c->opts.precision != PDFXC_SPLIT_BF16X3) {
         delete c;
-        set_error("pdfxc_ctx_create: only the FP64 precision mode is implemented");
-        return PDFXC_ERR_UNSUPPORTED;
+        set_error("pdfxc_ctx_create: unknown precision mode " + std::to_string(c->opts.precision));
+        return PDFXC_ERR_ARGUMENT;
     }
     c->n_aos = n_aos;
     // group AOs into shells: type 0 | 1,2,3 | 4..8 on one centre, same primitives
@@ -256,6 +257,12 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
     if (rc == PDFXC_OK) rc = dalloc(&c->d_P, (size_t)n_aos * n_aos);
     if (rc == PDFXC_OK) rc = dalloc(&c->d_K, (size_t)n_aos * n_aos);
     if (rc == PDFXC_OK) rc = dalloc(&c->d_E, 1);
+    if (rc == PDFXC_OK && c->opts.precision != PDFXC_FP64) {
+        const int S = (c->opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
+        c->split_nc = (n_aos + 7) / 8 * 8;
+        rc = dalloc(&c->d_Ps, (size_t)S * n_aos * c->split_nc);
+        if (rc == PDFXC_OK) rc = dalloc(&c->d_M, (size_t)n_aos * n_aos);
+    }
     if (rc == PDFXC_OK) {
         e = hipMemcpy(c->d_shells, ds.data(), ds.size() * sizeof(DeviceShell), hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = hip_fail(e, "pdfxc_ctx_create: shell upload");
@@ -285,6 +292,11 @@ static void free_grid_buffers(Context* c) {
     dfree(c->d_coef);
     dfree(c->d_rho);
     dfree(c->d_eblock);
+    dfree(c->d_sblocks);
+    dfree(c->d_chunks);
+    dfree(c->d_sshell_col);
+    dfree(c->d_srho_work);
+    dfree(c->d_sk_work);
 }
 
 int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
@@ -297,6 +309,8 @@ int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
     dfree(c->d_P);
     dfree(c->d_K);
     dfree(c->d_E);
+    dfree(c->d_Ps);
+    dfree(c->d_M);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -563,6 +577,85 @@ static int build_blocks(Context& c) {
         kall.insert(kall.end(), kw[cls].begin(), kw[cls].end());
     }
 
+    // split precision: chunked column layout, k_rho_split classes (ldnp <= 256:
+    // 64 points per workgroup, <= 512: 32, <= 1024: 16), M window items
+    std::vector<int> srho_work;
+    std::vector<SKWork> skw;
+    size_t sk_lds = 0;
+    c.sblocks.clear();
+    c.chunks.clear();
+    c.sshell_col.clear();
+    c.srho_launches.clear();
+    c.sk_launches.clear();
+    if (c.opts.precision != PDFXC_FP64) {
+        const int S = (c.opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
+        c.sblocks.resize(nb);
+        for (int b = 0; b < nb; ++b) {
+            const BlockDesc& B = c.blocks[b];
+            SplitBlock& SB = c.sblocks[b];
+            SB.chunk_begin = (int)c.chunks.size();
+            for (int k = 0; k < B.nshell; ++k) {
+                const Shell& Sh = c.shells[c.block_shells[B.shell_begin + k]];
+                for (int g = Sh.first_ao / 8; g <= (Sh.first_ao + 2 * Sh.l) / 8; ++g)
+                    if ((int)c.chunks.size() == SB.chunk_begin || c.chunks.back() != g) c.chunks.push_back(g);
+            }
+            SB.nchunk = (int)c.chunks.size() - SB.chunk_begin;
+            SB.ldnp = std::max(32, (8 * SB.nchunk + 31) / 32 * 32);
+            SB.pad = 0;
+            int pos = 0;
+            for (int k = 0; k < B.nshell; ++k) {
+                const Shell& Sh = c.shells[c.block_shells[B.shell_begin + k]];
+                while (c.chunks[SB.chunk_begin + pos] != Sh.first_ao / 8) ++pos;
+                c.sshell_col.push_back(8 * pos + Sh.first_ao % 8);
+            }
+            if (SB.ldnp > 1024) {
+                set_error("set_grid: a block keeps " + std::to_string(8 * SB.nchunk) +
+                          " chunked AO columns; the split kernels take at most 1024 (raise ao_screen)");
+                return PDFXC_ERR_UNSUPPORTED;
+            }
+        }
+        for (int cls = 0; cls < 3; ++cls) {
+            const int lim = 256 << cls;
+            const int lo_lim = cls ? (128 << cls) : 0;
+            const size_t pb = (size_t)split_points_per_workgroup(cls);
+            LaunchRange r{cls, (int)srho_work.size(), 0, 0};
+            for (int b = 0; b < nb; ++b) {
+                const SplitBlock& SB = c.sblocks[b];
+                if (SB.ldnp <= lo_lim || SB.ldnp > lim) continue;
+                const size_t plane = ((size_t)S * pb * (SB.ldnp + 8) * 2 + 15) / 16 * 16;
+                const size_t lds = plane + sizeof(double) * (size_t)c.blocks[b].nshell * RS +
+                                   sizeof(int) * (size_t)(SB.ldnp / 8 + 2 * c.blocks[b].nshell);
+                if (lds > (size_t)kMaxDynLds) {
+                    set_error("set_grid: a block with " + std::to_string(c.blocks[b].nao) +
+                              " AOs needs more LDS than a workgroup has in the split rho kernel (raise ao_screen)");
+                    return PDFXC_ERR_UNSUPPORTED;
+                }
+                srho_work.push_back(b);
+                r.lds = std::max(r.lds, lds);
+            }
+            r.count = (int)srho_work.size() - r.offset;
+            if (r.count) c.srho_launches.push_back(r);
+        }
+        for (int b = 0; b < nb; ++b) {
+            const int T = c.sblocks[b].ldnp / 16;
+            for (int I0 = 0; I0 < T; I0 += 16)
+                for (int J0 = 0; J0 < T; J0 += 16) {
+                    SKWork w{b, I0, J0, std::min(16, T - I0), std::min(16, T - J0), 0};
+                    const size_t slab = ((size_t)S * 16 * (w.nI + w.nJ) * 32 * 2 + 15) / 16 * 16;
+                    const size_t lds = slab + sizeof(double) * (size_t)c.blocks[b].nshell * RS +
+                                       sizeof(int) * (size_t)(2 * c.blocks[b].nshell + 16 * (w.nI + w.nJ));
+                    if (lds > (size_t)kMaxDynLds) {
+                        set_error("set_grid: a block with " + std::to_string(c.blocks[b].nshell) +
+                                  " shells needs more LDS than a workgroup has in the split K kernel (raise ao_screen)");
+                        return PDFXC_ERR_UNSUPPORTED;
+                    }
+                    sk_lds = std::max(sk_lds, lds);
+                    skw.push_back(w);
+                }
+        }
+        if (!skw.empty()) c.sk_launches.push_back(LaunchRange{0, 0, (int)skw.size(), sk_lds});
+    }
+
     // device buffers
     free_grid_buffers(&c);
     int rc = dalloc(&c.d_pts, pts.size());
@@ -576,6 +669,13 @@ static int build_blocks(Context& c) {
     if (rc == PDFXC_OK) rc = dalloc(&c.d_coef, 4 * (size_t)c.n_slots);
     if (rc == PDFXC_OK) rc = dalloc(&c.d_rho, 4 * (size_t)c.n_slots);
     if (rc == PDFXC_OK) rc = dalloc(&c.d_eblock, (size_t)(c.n_slots + 255) / 256);
+    if (rc == PDFXC_OK && c.opts.precision != PDFXC_FP64) {
+        rc = dalloc(&c.d_sblocks, c.sblocks.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_chunks, c.chunks.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_sshell_col, c.sshell_col.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_srho_work, srho_work.size());
+        if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&c.d_sk_work), skw.size() * sizeof(SKWork));
+    }
     if (rc != PDFXC_OK) return rc;
     auto up = [](void* d, const void* h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
     PDFXC_HIP(up(c.d_pts, pts.data(), pts.size() * sizeof(double)));
@@ -587,6 +687,13 @@ static int build_blocks(Context& c) {
         PDFXC_HIP(up(c.d_block_shell_col, c.block_shell_col.data(), c.block_shell_col.size() * sizeof(int)));
         PDFXC_HIP(up(c.d_block_cols, c.block_cols.data(), c.block_cols.size() * sizeof(int)));
         PDFXC_HIP(up(c.d_block_sorder, c.block_sorder.data(), c.block_sorder.size() * sizeof(int)));
+    }
+    if (c.opts.precision != PDFXC_FP64) {
+        PDFXC_HIP(up(c.d_sblocks, c.sblocks.data(), c.sblocks.size() * sizeof(SplitBlock)));
+        PDFXC_HIP(up(c.d_chunks, c.chunks.data(), c.chunks.size() * sizeof(int)));
+        PDFXC_HIP(up(c.d_sshell_col, c.sshell_col.data(), c.sshell_col.size() * sizeof(int)));
+        PDFXC_HIP(up(c.d_srho_work, srho_work.data(), srho_work.size() * sizeof(int)));
+        PDFXC_HIP(up(c.d_sk_work, skw.data(), skw.size() * sizeof(SKWork)));
     }
     PDFXC_HIP(hipMemset(c.d_coef, 0, 4 * (size_t)c.n_slots * sizeof(double)));
     PDFXC_HIP(hipMemset(c.d_rho, 0, 4 * (size_t)c.n_slots * sizeof(double)));
@@ -941,13 +1048,22 @@ static int build_device(Context* c, const double* dP, double* dK, double* dE) {
     c->d_P = const_cast<double*>(dP);
     c->d_K = dK;
     int rc = PDFXC_OK;
+    const bool split = c->opts.precision != PDFXC_FP64;
     PDFXC_HIP(hipEventRecord(c->ev[0], c->stream));
-    for (const LaunchRange& r : c->rho_launches)
-        if (rc == PDFXC_OK) rc = launch_rho(*c, r);
+    if (split) {
+        rc = launch_split_rho(*c);
+    } else {
+        for (const LaunchRange& r : c->rho_launches)
+            if (rc == PDFXC_OK) rc = launch_rho(*c, r);
+    }
     if (rc == PDFXC_OK) rc = launch_xc_points(*c);
     PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
-    for (const LaunchRange& r : c->kmat_launches)
-        if (rc == PDFXC_OK) rc = launch_kmat(*c, r);
+    if (split) {
+        if (rc == PDFXC_OK) rc = launch_split_kmat(*c);
+    } else {
+        for (const LaunchRange& r : c->kmat_launches)
+            if (rc == PDFXC_OK) rc = launch_kmat(*c, r);
+    }
     PDFXC_HIP(hipEventRecord(c->ev[2], c->stream));
     if (rc == PDFXC_OK) rc = launch_energy_sum(*c, dE);
     PDFXC_HIP(hipEventRecord(c->ev[3], c->stream));
@@ -1072,7 +1188,9 @@ int pdfxc_get_stats(pdfxc_ctx* ctx, pdfxc_stats* s) {
     if (!c || !s) return arg_error("pdfxc_get_stats: bad arguments");
     s->n_points = c->n_pts;
     s->n_blocks = (int64_t)c->blocks.size();
-    s->n_chunks = (int64_t)(c->rho_launches.size() + c->kmat_launches.size());
+    s->n_chunks = (c->opts.precision != PDFXC_FP64)
+                      ? (int64_t)(c->srho_launches.size() + c->sk_launches.size() + 3)
+                      : (int64_t)(c->rho_launches.size() + c->kmat_launches.size());
     s->flops = c->flops;
     s->mean_nao = c->mean_nao;
     s->max_nao = c->max_nao;
@@ -1090,6 +1208,20 @@ int pdfxc_selftest_mfma(double* out16x16) {
     if (rc == PDFXC_OK) {
         hipError_t e = hipMemcpy(out16x16, d, 256 * sizeof(double), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(e, "pdfxc_selftest_mfma");
+    }
+    dfree(d);
+    return rc;
+}
+
+int pdfxc_selftest_mfma_split(float* out16x16) {
+    if (!out16x16) return arg_error("pdfxc_selftest_mfma_split: null output");
+    float* d = nullptr;
+    int rc = dalloc(&d, 256);
+    if (rc) return rc;
+    rc = launch_mfma_split_selftest(d, nullptr);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipMemcpy(out16x16, d, 256 * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_selftest_mfma_split");
     }
     dfree(d);
     return rc;

@@ -41,6 +41,18 @@ struct BlockDesc {
     int pad;
 };
 
+// Split-precision column layout of a block: nchunk 8-aligned runs of global
+// AO indices (chunk ids in the chunk list from chunk_begin), ldnp = 8 nchunk
+// rounded up to 32.
+struct SplitBlock {
+    int chunk_begin, nchunk, ldnp, pad;
+};
+
+// One k_mfull_split work item: tiles [I0, I0 + nI) x [J0, J0 + nJ) of M_bb.
+struct SKWork {
+    int block, I0, J0, nI, nJ, pad;
+};
+
 // One k_kmat work item: lower 16 x 16 tiles [t0, t1) of a block.
 struct KWork {
     int block, t0, t1, ncols, nsh, ps;
@@ -108,6 +120,20 @@ struct Context {
     double* d_E = nullptr;        // 1 double
     double* h_pinned = nullptr;   // packed staging (pinned)
 
+    // split precision (opts.precision != PDFXC_FP64)
+    int split_nc = 0;                   // N rounded up to 8: row length of the P slices
+    std::vector<SplitBlock> sblocks;
+    std::vector<int> chunks;            // per block: chunk ids (global AO / 8), ascending
+    std::vector<int> sshell_col;        // per block shell: local column in the chunk layout
+    std::vector<LaunchRange> srho_launches, sk_launches;
+    unsigned short* d_Ps = nullptr;     // S x N x split_nc bf16 slices of P
+    double* d_M = nullptr;              // dense N x N: M = Phi^T V
+    SplitBlock* d_sblocks = nullptr;
+    int* d_chunks = nullptr;
+    int* d_sshell_col = nullptr;
+    int* d_srho_work = nullptr;
+    void* d_sk_work = nullptr;
+
     double last_build_ms = 0.0;
     double last_kernel_ms[4] = {0, 0, 0, 0};
 };
@@ -135,6 +161,11 @@ int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, dou
 int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,
                  const int* d_atom_of_pt, double* d_w, hipStream_t s);
 int launch_mfma_selftest(double* d_out, hipStream_t s);
+// split precision (pdfxc_split.hip)
+int split_points_per_workgroup(int cls);
+int launch_split_rho(Context& c);   // P slices + k_rho_split
+int launch_split_kmat(Context& c);  // M = Phi^T V, K += M + M^T
+int launch_mfma_split_selftest(float* d_out, hipStream_t s);
 int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 
 }  // namespace pdfxc

@@ -15,6 +15,9 @@ LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libpdfxc.so"
 PDFXC_OK = 0
 PDFXC_B3LYP = 1
 PDFXC_FP64 = 0
+PDFXC_SPLIT_BF16 = 1
+PDFXC_SPLIT_BF16X3 = 2
+PRECISIONS = {"fp64": PDFXC_FP64, "bf16x2": PDFXC_SPLIT_BF16, "bf16x3": PDFXC_SPLIT_BF16X3}
 ERROR_NAMES = {1: "ARGUMENT", 2: "HIP", 3: "NO_GRID", 4: "UNSUPPORTED", 5: "MEMORY"}
 
 # every symbol declared in include/pdfxc.h
@@ -22,7 +25,8 @@ EXPORTED = [
     "pdfxc_default_opts", "pdfxc_normalize_cgto", "pdfxc_ctx_create", "pdfxc_ctx_destroy",
     "pdfxc_set_grid", "pdfxc_generate_grid", "pdfxc_get_grid", "pdfxc_build_rks",
     "pdfxc_build_rks_device", "pdfxc_eval_functional", "pdfxc_eval_aos", "pdfxc_get_stats",
-    "pdfxc_stream", "pdfxc_last_error", "pdfxc_selftest_mfma", "pdfxc_selftest_copy",
+    "pdfxc_stream", "pdfxc_last_error", "pdfxc_selftest_mfma", "pdfxc_selftest_mfma_split",
+    "pdfxc_selftest_copy",
 ]
 
 
@@ -73,6 +77,7 @@ def lib():
         L.pdfxc_last_error.argtypes = []
         L.pdfxc_last_error.restype = ctypes.c_char_p
         L.pdfxc_selftest_mfma.argtypes = [P]
+        L.pdfxc_selftest_mfma_split.argtypes = [P]
         L.pdfxc_selftest_copy.argtypes = [P, P, i64]
         _lib = L
     return _lib
@@ -114,6 +119,12 @@ def eval_functional(rho_a, gamma_aa, functional=PDFXC_B3LYP):
 def selftest_mfma():
     out = np.zeros((16, 16))
     check(lib().pdfxc_selftest_mfma(_p(out)))
+    return out
+
+
+def selftest_mfma_split():
+    out = np.zeros((16, 16), dtype=np.float32)
+    check(lib().pdfxc_selftest_mfma_split(_p(out)))
     return out
 
 

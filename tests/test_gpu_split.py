@@ -1,0 +1,96 @@
+"""GPU checks of the split-precision path (bf16 slices on MFMA, FP32
+accumulation; everything else FP64).  The reference is FP64 only: the split
+builds are compared with the FP64 GPU build (itself pinned to the oracle in
+test_gpu_engine.py) and with the oracle directly on small grids, and the
+tolerances are the measured error levels of each mode with headroom."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from proteindf_amd import calc_grid, engine, guess, molecules
+
+pytestmark = pytest.mark.gpu
+
+# |dE| and max|dK| bounds per mode on the (H2O)_8 / 6-31G(d) / 75 x 302 build
+BOUNDS = {engine.PDFXC_SPLIT_BF16: (2e-4, 2e-5), engine.PDFXC_SPLIT_BF16X3: (1e-6, 1e-6)}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_mfma_split_operand_and_accumulator_layout():
+    d = engine.selftest_mfma_split()
+    i = np.arange(16)[:, None]
+    k = np.arange(32)[None, :]
+    A = ((3 * i + 5 * k) % 11) - 5
+    kk = np.arange(32)[:, None]
+    j = np.arange(16)[None, :]
+    Bm = ((7 * kk + 2 * j) % 13) - 6
+    np.testing.assert_array_equal(d, (A @ Bm).astype(np.float32))
+
+
+def _build(info, grid, PA, precision, ao_screen=0.0):
+    ctx = engine.XcContext(info, engine.default_opts(precision=precision, ao_screen=ao_screen))
+    ctx.set_grid(grid)
+    E, K, rho = ctx.build_rks(guess.pack_lower(PA), want_rho=True)
+    ctx.close()
+    return E, guess.unpack_lower(K, info.n_aos), rho
+
+
+@pytest.mark.parametrize("precision", [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3])
+def test_split_build_h2o8_against_fp64(precision):
+    mol = molecules.water_cluster(8, 0)
+    info = calc_grid.orbital_info(mol)
+    ctx = engine.XcContext(info)
+    ctx.generate_grid(mol.Z, mol.xyz, 75, 302)
+    grid = ctx.get_grid()
+    ctx.close()
+    PA = 0.5 * guess.density_guess(mol, info, seed=0, eps=0.05)
+    E0, K0, rho0 = _build(info, grid, PA, engine.PDFXC_FP64)
+    E1, K1, rho1 = _build(info, grid, PA, precision)
+    dE, dK = abs(E1 - E0), np.abs(K1 - K0).max()
+    drho = np.abs(rho1[:, 0] - rho0[:, 0]).max() / np.abs(rho0[:, 0]).max()
+    say(f"precision {precision}: E {E1:.12f} fp64 {E0:.12f} |dE| {dE:.3e} max|dK| {dK:.3e} "
+        f"max|drho|/max rho {drho:.3e} max|K| {np.abs(K0).max():.3f}")
+    eb, kb = BOUNDS[precision]
+    assert dE <= eb
+    assert dK <= kb
+    np.testing.assert_allclose(K1, K1.T, atol=0)  # symmetric by construction
+
+
+@pytest.mark.parametrize("precision", [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3])
+def test_split_ragged_grids_against_oracle(precision):
+    mol = molecules.water_cluster(2, seed=5)
+    info = calc_grid.orbital_info(mol)
+    grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 40, 194)
+    PA = 0.5 * guess.density_guess(mol, info, seed=1, eps=0.05)
+    rng = np.random.default_rng(1)
+    aos = oracle.AoTable.from_basis(info)
+    for npts in (1, 33, 127, 128, 129, 1000):
+        sub = grid[np.sort(rng.choice(len(grid), npts, replace=False))]
+        E, K, rho = _build(info, sub, PA, precision)
+        e_ref, K_ref, rho_ref = oracle.calc_xc_rks(aos, sub, PA, want_rho=True)
+        dE, dK = abs(E - e_ref), np.abs(K - K_ref).max()
+        say(f"precision {precision} npts {npts}: |dE| {dE:.3e} max|dK| {dK:.3e}")
+        eb, kb = BOUNDS[precision]
+        assert dE <= eb and dK <= kb
+        np.testing.assert_allclose(rho[:, 0], rho_ref[:, 0], rtol=2e-4 if precision == 1 else 2e-6,
+                                   atol=1e-12)
+
+
+def test_split_zero_density_and_accumulate():
+    mol = molecules.water_cluster(1, seed=6)
+    info = calc_grid.orbital_info(mol)
+    grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 30, 86)
+    ctx = engine.XcContext(info, engine.default_opts(precision=engine.PDFXC_SPLIT_BF16X3))
+    ctx.set_grid(grid)
+    n = info.n_aos
+    E, K, _ = ctx.build_rks(np.zeros(n * (n + 1) // 2))
+    assert E == 0.0 and not np.any(K)
+    PA = 0.5 * guess.density_guess(mol, info, seed=2, eps=0.05)
+    E1, K1, _ = ctx.build_rks(guess.pack_lower(PA))
+    E2, K2, _ = ctx.build_rks(guess.pack_lower(PA), np.full(n * (n + 1) // 2, 0.25))
+    ctx.close()
+    assert E1 == E2
+    np.testing.assert_allclose(K2 - 0.25, K1, rtol=0, atol=1e-14)
