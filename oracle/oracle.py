@@ -56,6 +56,11 @@ def lib():
         L.oracle_becke_weight.restype = d
         L.oracle_generate_grid.argtypes = [i32, P, P, i32, i32, P, d, P, P, i32]
         L.oracle_generate_grid.restype = i64
+        L.oracle_eval_ao_hessians.argtypes = [P, i64, P, P]
+        L.oracle_becke_weight_gradient.argtypes = [i32, P, P, i32, P, P]
+        L.oracle_becke_weight_gradient.restype = d
+        L.oracle_xc_gradient_rks.argtypes = [P, P, i32, i64, P, P, i32, P, P, P, d, P, P, i32]
+        L.oracle_xc_gradient_rks.restype = d
         _lib = L
     return _lib
 
@@ -139,6 +144,41 @@ def calc_xc_rks(aos, grid, PA, fid=B3LYP, density_cutoff=1e-16, n_threads=0,
     F = np.tril(F)
     F = F + np.tril(F, -1).T
     return e, F, rho
+
+
+def eval_ao_hessians(aos, xyz):
+    """(n, N, 6) second derivatives xx, xy, xz, yy, yz, zz."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    h = np.zeros((xyz.shape[0], aos.n, 6))
+    lib().oracle_eval_ao_hessians(aos.ref(), xyz.shape[0], _p(xyz), _p(h))
+    return h
+
+
+def becke_weight_gradient(Z, xyz, iatom, r):
+    """(omega_A, (n_atoms, 3) grad_B omega_A with the point fixed)."""
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    out = np.zeros((len(Z), 3))
+    om = lib().oracle_becke_weight_gradient(len(Z), _p(Z), _p(xyz), iatom, _p(r), _p(out))
+    return om, out
+
+
+def xc_gradient_rks(aos, ao_atom, grid, pt_atom, Z, xyz, PA, fid=B3LYP, density_cutoff=1e-16,
+                    n_threads=0):
+    """Returns (E_xc, grad_functional (n_atoms, 3), grad_weight (n_atoms, 3))."""
+    grid = np.ascontiguousarray(grid, dtype=np.float64).reshape(-1, 4)
+    pt_atom = np.ascontiguousarray(pt_atom, dtype=np.int32)
+    ao_atom = np.ascontiguousarray(ao_atom, dtype=np.int32)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    PA = np.ascontiguousarray(PA, dtype=np.float64)
+    gf = np.zeros((len(Z), 3))
+    gw = np.zeros((len(Z), 3))
+    e = lib().oracle_xc_gradient_rks(aos.ref(), _p(ao_atom), fid, grid.shape[0], _p(grid), _p(pt_atom),
+                                     len(Z), _p(Z), _p(xyz), _p(PA), density_cutoff, _p(gf), _p(gw),
+                                     n_threads)
+    return e, gf, gw
 
 
 def lebedev(order):

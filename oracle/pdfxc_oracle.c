@@ -1017,3 +1017,300 @@ int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz, int n
     free(leb);
     return total;
 }
+
+/* ------------------------------------------------------------------------ */
+/* XC nuclear gradient, RKS GGA (energyGradient / energyGradient_part,        */
+/* DfCalcGridX.cpp:1215-1241, :1243-1415).                                    */
+/*                                                                            */
+/* Functional term, as the reference forms it per grid point of atom A with   */
+/* weight w, for every AO i (on atom B = atom(i)):                            */
+/*   g_i = -2 w [ v_rho dphi_i X_i + sum_d c_d (d dphi_i/dr_d X_i             */
+/*                + dphi_i Y_di) ],  X = P phi, Y_d = P dphi/dr_d,            */
+/*   c_d = 2 v_gAA drho_d + v_gAB drho_d,                                     */
+/* added to atom B and subtracted from atom A (the point moves with A), then  */
+/* doubled for RKS (:1234).  AO second derivatives: getD2AOs_core (:771-863)  */
+/* with getPrefactorForSecondDerivative (:237-385), written here through the  */
+/* product rule on (angular polynomial) x exp(-a r^2).                        */
+/*                                                                            */
+/* Weight term: sum over B != A of w_single e_xc grad_B omega_A, added to B   */
+/* and subtracted from A (:1386-1404).  The reference's JGP_t                 */
+/* (DfGenerateGrid.cpp:2355-2372) only evaluates for |mu| > 1, which never   */
+/* happens, so its weight term is zero; this oracle uses the exact            */
+/* derivative of the size-adjusted Becke weight instead (pinned by finite     */
+/* differences of E_xc on a grid that moves rigidly with the atoms).          */
+/* ------------------------------------------------------------------------ */
+static void angular_poly(int t, double x, double y, double z, double* f, double g[3], double h[6]) {
+    const double s3 = 1.0 / sqrt(3.0);
+    for (int k = 0; k < 3; ++k) g[k] = 0.0;
+    for (int k = 0; k < 6; ++k) h[k] = 0.0;  /* xx xy xz yy yz zz */
+    switch (t) {
+        case 0: *f = 1.0; break;
+        case 1: *f = x; g[0] = 1.0; break;
+        case 2: *f = y; g[1] = 1.0; break;
+        case 3: *f = z; g[2] = 1.0; break;
+        case 4: *f = x * y; g[0] = y; g[1] = x; h[1] = 1.0; break;
+        case 5: *f = z * x; g[0] = z; g[2] = x; h[2] = 1.0; break;
+        case 6: *f = y * z; g[1] = z; g[2] = y; h[4] = 1.0; break;
+        case 7: *f = 0.5 * (x * x - y * y); g[0] = x; g[1] = -y; h[0] = 1.0; h[3] = -1.0; break;
+        case 8:
+            *f = s3 * (z * z - 0.5 * (x * x + y * y));
+            g[0] = -s3 * x; g[1] = -s3 * y; g[2] = 2.0 * s3 * z;
+            h[0] = -s3; h[3] = -s3; h[5] = 2.0 * s3;
+            break;
+        default: *f = 0.0; break;
+    }
+}
+/* phi, grad phi and the 6 second derivatives of every AO at point p */
+static void eval_point_d2(const oracle_aos* a, const double* p, double* phi, double* g3, double* h6) {
+    static const int ia[6] = {0, 0, 0, 1, 1, 2}, ib[6] = {0, 1, 2, 1, 2, 2};
+    for (int i = 0; i < a->n_aos; ++i) {
+        const double r[3] = {p[0] - a->center[3 * i + 0], p[1] - a->center[3 * i + 1],
+                             p[2] - a->center[3 * i + 2]};
+        const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        double f, fg[3], fh[6];
+        angular_poly(a->type[i], r[0], r[1], r[2], &f, fg, fh);
+        double v = 0.0, gv[3] = {0, 0, 0}, hv[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < a->nprim[i]; ++k) {
+            const double al = a->exps[i * ORACLE_MAX_PRIM + k];
+            const double sh = al * d2;
+            if (sh > TOOBIG) continue;
+            const double e = a->coefs[i * ORACLE_MAX_PRIM + k] * exp(-sh);
+            v += f * e;
+            for (int d = 0; d < 3; ++d) gv[d] += (fg[d] - 2.0 * al * r[d] * f) * e;
+            for (int q = 0; q < 6; ++q) {
+                const int x = ia[q], y = ib[q];
+                const double t = fh[q] - 2.0 * al * (r[x] * fg[y] + r[y] * fg[x]) +
+                                 f * (4.0 * al * al * r[x] * r[y] - (x == y ? 2.0 * al : 0.0));
+                hv[q] += t * e;
+            }
+        }
+        phi[i] = v;
+        for (int d = 0; d < 3; ++d) g3[3 * i + d] = gv[d];
+        for (int q = 0; q < 6; ++q) h6[6 * i + q] = hv[q];
+    }
+}
+void oracle_eval_ao_hessians(const oracle_aos* aos, int64_t n_pts, const double* xyz, double* h6) {
+    const int N = aos->n_aos;
+#pragma omp parallel
+    {
+        double* phi = (double*)malloc(sizeof(double) * 4 * (size_t)N);
+        double* g3 = phi + N;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t p = 0; p < n_pts; ++p) eval_point_d2(aos, xyz + 3 * p, phi, g3, h6 + (size_t)p * 6 * N);
+        free(phi);
+    }
+}
+
+/* Becke step s(nu) = (1 - f(f(f(nu)))) / 2 and ds/dnu */
+static void becke_s(double nu, double* s, double* ds) {
+    const double f1 = becke_f1(nu), f2 = becke_f1(f1), f3 = becke_f1(f2);
+    *s = 0.5 * (1.0 - f3);
+    *ds = -0.5 * (1.5 * (1.0 - f2 * f2)) * (1.5 * (1.0 - f1 * f1)) * (1.5 * (1.0 - nu * nu));
+}
+static double becke_a(int Zi, int Zj) {
+    const double Ri = becke_radius(Zi), Rj = becke_radius(Zj);
+    const double au = (Ri - Rj) / (Ri + Rj);
+    double a = au / (au * au - 1.0);
+    if (a < -0.50) a = -0.50;
+    else if (a > 0.50) a = 0.50;
+    return a;
+}
+/* product of row i of s without column j */
+static double prod_except(const double* s, int n, int i, int j) {
+    double q = 1.0;
+    for (int m = 0; m < n; ++m)
+        if (m != j) q *= s[i * n + m];
+    return q;
+}
+/* grad_B omega_A at point r for every B (point held fixed); out: n_atoms x 3.
+ * Returns omega_A. */
+static double becke_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
+                                    double* s, double* dsm, double* Pv, double* out) {
+    const int n = n_atoms;
+    double* rr = Pv + n;
+    for (int i = 0; i < n; ++i) {
+        const double dx = r[0] - xyz[3 * i], dy = r[1] - xyz[3 * i + 1], dz = r[2] - xyz[3 * i + 2];
+        rr[i] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    /* s_ij and ds_ij/dmu_ij */
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            s[i * n + j] = 1.0;
+            dsm[i * n + j] = 0.0;
+            if (i == j || Z[i] <= 0 || Z[j] <= 0) continue;
+            const double ex = xyz[3 * i] - xyz[3 * j], ey = xyz[3 * i + 1] - xyz[3 * j + 1],
+                         ez = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double R = sqrt(ex * ex + ey * ey + ez * ez);
+            const double mu = (rr[i] - rr[j]) / R;
+            const double a = becke_a(Z[i], Z[j]);
+            double sv, ds;
+            becke_s(mu + a * (1.0 - mu * mu), &sv, &ds);
+            s[i * n + j] = sv;
+            dsm[i * n + j] = ds * (1.0 - 2.0 * a * mu);
+        }
+    double Zs = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double p = (Z[i] > 0) ? 1.0 : 0.0;
+        for (int j = 0; j < n; ++j) p *= s[i * n + j];
+        Pv[i] = p;
+        Zs += p;
+    }
+    const double omega = Pv[A] / Zs;
+    for (int B = 0; B < n; ++B) {
+        double* o = out + 3 * B;
+        o[0] = o[1] = o[2] = 0.0;
+        if (B == A || Z[B] <= 0) continue;
+        const double uB[3] = {(xyz[3 * B] - r[0]) / rr[B], (xyz[3 * B + 1] - r[1]) / rr[B],
+                              (xyz[3 * B + 2] - r[2]) / rr[B]};
+        double gPA[3] = {0, 0, 0}, gZ[3] = {0, 0, 0};
+        for (int i = 0; i < n; ++i) {
+            if (Z[i] <= 0) continue;
+            if (i == B) {
+                /* grad_B P_B = sum_j prod_{k != j} s_Bk  ds_Bj grad_B mu_Bj */
+                for (int j = 0; j < n; ++j) {
+                    if (j == B || Z[j] <= 0) continue;
+                    const double R[3] = {xyz[3 * B] - xyz[3 * j], xyz[3 * B + 1] - xyz[3 * j + 1],
+                                         xyz[3 * B + 2] - xyz[3 * j + 2]};
+                    const double Rl = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+                    const double mu = (rr[B] - rr[j]) / Rl;
+                    const double c = prod_except(s, n, B, j) * dsm[B * n + j];
+                    for (int d = 0; d < 3; ++d) gZ[d] += c * (uB[d] - mu * R[d] / Rl) / Rl;
+                }
+            } else {
+                /* grad_B P_i = prod_{k != B} s_ik  ds_iB grad_B mu_iB */
+                const double R[3] = {xyz[3 * B] - xyz[3 * i], xyz[3 * B + 1] - xyz[3 * i + 1],
+                                     xyz[3 * B + 2] - xyz[3 * i + 2]};
+                const double Rl = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+                const double mu = (rr[i] - rr[B]) / Rl;
+                const double c = prod_except(s, n, i, B) * dsm[i * n + B];
+                for (int d = 0; d < 3; ++d) {
+                    const double gm = (-uB[d] - mu * R[d] / Rl) / Rl;
+                    gZ[d] += c * gm;
+                    if (i == A) gPA[d] += c * gm;
+                }
+            }
+        }
+        for (int d = 0; d < 3; ++d) o[d] = (gPA[d] - omega * gZ[d]) / Zs;
+    }
+    return omega;
+}
+
+double oracle_becke_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
+                                    double* out) {
+    const int n = n_atoms;
+    double* s = (double*)malloc(sizeof(double) * (2 * (size_t)n * n + 2 * (size_t)n));
+    const double om = becke_weight_gradient(n, Z, xyz, A, r, s, s + (size_t)n * n, s + 2 * (size_t)n * n, out);
+    free(s);
+    return om;
+}
+
+double oracle_xc_gradient_rks(const oracle_aos* aos, const int* ao_atom, int functional, int64_t n_pts,
+                              const double* grid, const int* pt_atom, int n_atoms, const int* Z,
+                              const double* xyz, const double* PA, double density_cutoff,
+                              double* grad_f, double* grad_w, int n_threads) {
+    const int N = aos->n_aos;
+    const int na = n_atoms;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = (n_threads > 0) ? n_threads : omp_get_max_threads();
+#else
+    (void)n_threads;
+#endif
+    double* acc = (double*)calloc((size_t)nt * 6 * na, sizeof(double));
+    double energy = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt) reduction(+ : energy)
+#endif
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double* gf = acc + (size_t)tid * 6 * na;
+        double* gw = gf + 3 * na;
+        double* phi = (double*)malloc(sizeof(double) * 10 * (size_t)N);
+        double* g3 = phi + N;
+        double* h6 = g3 + 3 * (size_t)N;
+        double* s = (double*)malloc(sizeof(double) * (2 * (size_t)na * na + 2 * (size_t)na + 3 * (size_t)na));
+        double* dsm = s + (size_t)na * na;
+        double* Pv = dsm + (size_t)na * na;
+        double* gom = Pv + 2 * (size_t)na;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t g = 0; g < n_pts; ++g) {
+            const double* p = grid + 4 * g;
+            const double w = p[3];
+            const int A = pt_atom[g];
+            eval_point_d2(aos, p, phi, g3, h6);
+            /* rho (getRhoAtGridPoint) and grad rho (getGradRhoAtGridPoint) */
+            double rho = 0.0;
+            for (int i = 0; i < N; ++i) {
+                const double* Pi = PA + (size_t)i * N;
+                for (int j = 0; j < i; ++j) rho += 2.0 * Pi[j] * phi[i] * phi[j];
+                rho += Pi[i] * phi[i] * phi[i];
+            }
+            double gr[3] = {0, 0, 0};
+            for (int i = 0; i < N; ++i) {
+                const double* Pi = PA + (size_t)i * N;
+                for (int j = 0; j < N; ++j)
+                    for (int d = 0; d < 3; ++d) gr[d] += Pi[j] * phi[i] * g3[3 * j + d];
+            }
+            for (int d = 0; d < 3; ++d) gr[d] *= 2.0;
+            if (!(rho > density_cutoff)) continue;
+            const double gamma = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
+            const double in[5] = {rho, rho, gamma, gamma, gamma};
+            double o[6];
+            functional_point(functional, in, o);
+            energy += w * o[0];
+            const double vr = o[1];
+            const double c[3] = {2.0 * o[3] * gr[0] + o[4] * gr[0], 2.0 * o[3] * gr[1] + o[4] * gr[1],
+                                 2.0 * o[3] * gr[2] + o[4] * gr[2]};
+            /* functional term */
+            for (int i = 0; i < N; ++i) {
+                double rao = 0.0, rd[3] = {0, 0, 0};
+                const double* Pi = PA + (size_t)i * N;
+                for (int j = 0; j < N; ++j) {
+                    rao += Pi[j] * phi[j];
+                    for (int d = 0; d < 3; ++d) rd[d] += Pi[j] * g3[3 * j + d];
+                }
+                const double* di = g3 + 3 * i;
+                const double* hi = h6 + 6 * i;
+                /* H[a][b] of AO i: xx xy xz yy yz zz */
+                const double H[3][3] = {{hi[0], hi[1], hi[2]}, {hi[1], hi[3], hi[4]}, {hi[2], hi[4], hi[5]}};
+                const int B = ao_atom[i];
+                for (int a = 0; a < 3; ++a) {
+                    double v = vr * di[a] * rao;
+                    for (int d = 0; d < 3; ++d) v += c[d] * (H[a][d] * rao + di[a] * rd[d]);
+                    v *= -2.0 * w;
+                    gf[3 * B + a] += v;
+                    gf[3 * A + a] -= v;
+                }
+            }
+            /* weight term */
+            const double omega = becke_weight_gradient(na, Z, xyz, A, p, s, dsm, Pv, gom);
+            const double ws = w / omega;
+            for (int B = 0; B < na; ++B) {
+                if (B == A) continue;
+                for (int a = 0; a < 3; ++a) {
+                    const double v = ws * o[0] * gom[3 * B + a];
+                    gw[3 * B + a] += v;
+                    gw[3 * A + a] -= v;
+                }
+            }
+        }
+        free(phi);
+        free(s);
+    }
+    for (int k = 0; k < 3 * na; ++k) {
+        double f = 0.0, wv = 0.0;
+        for (int t = 0; t < nt; ++t) {
+            f += acc[(size_t)t * 6 * na + k];
+            wv += acc[(size_t)t * 6 * na + 3 * na + k];
+        }
+        grad_f[k] = 2.0 * f; /* rks (:1234) */
+        grad_w[k] = wv;
+    }
+    free(acc);
+    return energy;
+}

@@ -108,6 +108,29 @@ int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz,
                              double weight_cutoff, double* out_grid,
                              int* out_atom, int n_threads);
 
+/* AO second derivatives (getD2AOs_core, DfCalcGridX.cpp:771-863): h6 is
+ * n_pts x n_aos x 6 (xx, xy, xz, yy, yz, zz). */
+void oracle_eval_ao_hessians(const oracle_aos* aos, int64_t n_pts, const double* xyz, double* h6);
+
+/* grad_B omega_A (size-adjusted Becke partition of atom A) at point r for
+ * every atom B, point held fixed: out n_atoms x 3 (row A is zero).
+ * Returns omega_A. */
+double oracle_becke_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
+                                    double* out);
+
+/* XC nuclear gradient, RKS GGA (energyGradient, DfCalcGridX.cpp:1215-1415)
+ * on the given grid: grid n_pts x 4 (x, y, z, w), pt_atom[p] = the atom the
+ * point belongs to (the point moves with it), ao_atom[i] = atom of AO i,
+ * Z / xyz the atoms (Becke partition with size adjustments).  grad_f: the
+ * functional term (doubled for RKS as the reference does); grad_w: the
+ * partition-weight term with the exact Becke derivative (the reference's
+ * term is identically zero, see the .c file).  Both n_atoms x 3, dE/dR.
+ * Returns E_xc. */
+double oracle_xc_gradient_rks(const oracle_aos* aos, const int* ao_atom, int functional, int64_t n_pts,
+                              const double* grid, const int* pt_atom, int n_atoms, const int* Z,
+                              const double* xyz, const double* PA, double density_cutoff,
+                              double* grad_f, double* grad_w, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif
