@@ -121,6 +121,33 @@ int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_alpha_packed
 int pdfxc_build_rks_device(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
                            double* dE);
 
+/* Update mode (xc-update = yes; DfXCFunctional.cpp:134-136 passes P_A =
+ * dP/2 and calcRho_GGA_part adds rho[dP_A] to the grid's stored rho,
+ * DfCalcGridX.cpp:1925-1961): rho and grad rho of dP_alpha are ADDED to the
+ * context's device-resident values (the previous build's, or zero after
+ * set_grid / generate_grid), and K_xc, E_xc follow from the accumulated rho
+ * exactly as in pdfxc_build_rks.  Same buffers and layouts as
+ * pdfxc_build_rks / pdfxc_build_rks_device. */
+int pdfxc_build_rks_update(pdfxc_ctx* ctx, int functional, const double* dP_alpha_packed,
+                           double* K_alpha_packed, double* E_xc, double* rho_out);
+int pdfxc_build_rks_update_device(pdfxc_ctx* ctx, int functional, const double* d_dP, double* dK,
+                                  double* dE);
+
+/* XC nuclear gradient, RKS GGA (DfCalcGridX::energyGradient,
+ * DfCalcGridX.cpp:1215-1415), on the context's grid with its points moving
+ * rigidly with their atoms: point_atom[p] (caller point order) gives each
+ * point's atom, or NULL when pdfxc_generate_grid made the grid (its own
+ * assignment is used); AO centres must coincide with atoms.  Outputs dE/dR,
+ * n_atoms x 3 each: grad_functional = the reference's functional term
+ * (AO-centre and point-translation parts, doubled for RKS); grad_weight = the
+ * derivative of the size-adjusted Becke partition weights (the reference's
+ * own weight term evaluates to zero, DfGenerateGrid.cpp:2355-2372).  Their
+ * sum is the gradient of E_xc.  E_xc (may be NULL) is returned.  FP64
+ * contractions whatever the context's precision. */
+int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z, const double* xyz,
+                       const int* point_atom, const double* P_alpha_packed, double* grad_functional,
+                       double* grad_weight, double* E_xc);
+
 /* Functional kernel over arbitrary closed-shell points, as the grid path
  * calls it (DfFunctional.h:190-192, :221-232: rhoB = rhoA, gAB = gBB = gAA).
  * in: n x 2 (rho_alpha, gamma_alpha_alpha); out: n x 4 (f, df/drhoA,

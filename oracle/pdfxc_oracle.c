@@ -1140,8 +1140,8 @@ static double becke_weight_gradient(int n_atoms, const int* Z, const double* xyz
             if (i == j || Z[i] <= 0 || Z[j] <= 0) continue;
             const double ex = xyz[3 * i] - xyz[3 * j], ey = xyz[3 * i + 1] - xyz[3 * j + 1],
                          ez = xyz[3 * i + 2] - xyz[3 * j + 2];
-            const double R = sqrt(ex * ex + ey * ey + ez * ez);
-            const double mu = (rr[i] - rr[j]) / R;
+            const double inv_dist = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+            const double mu = (rr[i] - rr[j]) * inv_dist;  /* as becke_ps forms it */
             const double a = becke_a(Z[i], Z[j]);
             double sv, ds;
             becke_s(mu + a * (1.0 - mu * mu), &sv, &ds);
@@ -1287,8 +1287,9 @@ double oracle_xc_gradient_rks(const oracle_aos* aos, const int* ao_atom, int fun
                     gf[3 * A + a] -= v;
                 }
             }
-            /* weight term */
+            /* weight term (a cell weight that rounds to 0 has ds/dnu = 0 there too) */
             const double omega = becke_weight_gradient(na, Z, xyz, A, p, s, dsm, Pv, gom);
+            if (!(omega > 0.0)) continue;
             const double ws = w / omega;
             for (int B = 0; B < na; ++B) {
                 if (B == A) continue;

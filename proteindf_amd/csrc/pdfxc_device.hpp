@@ -104,4 +104,62 @@ __device__ inline void rec_values0(const double* rec, double px, double py, doub
     }
 }
 
+// Radial sums of a shell record at a point: R0 = sum c e^{-a r^2},
+// R1 = sum -2a c e^{-a r^2}, R2 = sum 4a^2 c e^{-a r^2} (TOOBIG cut as in
+// getD2AOs_core, DfCalcGridX.cpp:771-863); returns l and the offset (x, y, z).
+__device__ inline int rec_radial2(const double* rec, double px, double py, double pz, double* r3, double& R0,
+                                  double& R1, double& R2) {
+    r3[0] = px - rec[0];
+    r3[1] = py - rec[1];
+    r3[2] = pz - rec[2];
+    const int meta = (int)rec[3];
+    const int nprim = meta >> 2;
+    const double r2 = r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
+    R0 = R1 = R2 = 0.0;
+    for (int k = 0; k < nprim; ++k) {
+        const double a = rec[4 + 2 * k];
+        const double sh = a * r2;
+        if (sh <= kTooBig) {
+            const double g = rec[5 + 2 * k] * exp(-sh);
+            R0 += g;
+            R1 += -2.0 * a * g;
+            R2 += 4.0 * a * a * g;
+        }
+    }
+    return meta & 3;
+}
+
+// Angular polynomial f of component m of an l shell, its gradient and its
+// second derivatives (xx, xy, xz, yy, yz, zz): s; x, y, z; xy, zx, yz,
+// (x^2 - y^2)/2, (z^2 - (x^2 + y^2)/2)/sqrt3 (getPrefactor, :103-141).
+__device__ inline void angular2(int l, int m, const double* r, double& f, double* fg, double* fh) {
+    const double x = r[0], y = r[1], z = r[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) fg[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) fh[k] = 0.0;
+    if (l == 0) {
+        f = 1.0;
+    } else if (l == 1) {
+        f = r[m];
+        fg[m] = 1.0;
+    } else {
+        switch (m) {
+            case 0: f = x * y; fg[0] = y; fg[1] = x; fh[1] = 1.0; break;
+            case 1: f = z * x; fg[0] = z; fg[2] = x; fh[2] = 1.0; break;
+            case 2: f = y * z; fg[1] = z; fg[2] = y; fh[4] = 1.0; break;
+            case 3: f = 0.5 * (x * x - y * y); fg[0] = x; fg[1] = -y; fh[0] = 1.0; fh[3] = -1.0; break;
+            default:
+                f = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y));
+                fg[0] = -kInvSqrt3 * x;
+                fg[1] = -kInvSqrt3 * y;
+                fg[2] = 2.0 * kInvSqrt3 * z;
+                fh[0] = -kInvSqrt3;
+                fh[3] = -kInvSqrt3;
+                fh[5] = 2.0 * kInvSqrt3;
+                break;
+        }
+    }
+}
+
 }  // namespace pdfxc

@@ -904,6 +904,7 @@ int pdfxc_set_grid(pdfxc_ctx* ctx, int64_t n_pts, const double* x, const double*
     if (n_pts > ((int64_t)1 << 31) - kBlockPts)
         return arg_error("pdfxc_set_grid: more than 2^31 points");
     c->n_pts = n_pts;
+    c->gen_atom.clear();
     c->gx.assign(x, x + n_pts);
     c->gy.assign(y, y + n_pts);
     c->gz.assign(z, z + n_pts);
@@ -1027,18 +1028,22 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
     if (rc != PDFXC_OK) return rc;
     // weight screening (screeningGridsByWeight0, DfGenerateGrid.cpp:1979-2004)
     std::vector<double> X, Y, Zc, W;
+    std::vector<int> kept_atom;
     for (int64_t i = 0; i < n; ++i)
         if (std::fabs(w[i]) > weight_cutoff) {
             X.push_back(pts[3 * i]);
             Y.push_back(pts[3 * i + 1]);
             Zc.push_back(pts[3 * i + 2]);
             W.push_back(w[i]);
+            kept_atom.push_back(atom[i]);
         }
     if (n_out) *n_out = (int64_t)W.size();
-    return pdfxc_set_grid(ctx, (int64_t)W.size(), X.data(), Y.data(), Zc.data(), W.data());
+    rc = pdfxc_set_grid(ctx, (int64_t)W.size(), X.data(), Y.data(), Zc.data(), W.data());
+    if (rc == PDFXC_OK) c->gen_atom = std::move(kept_atom);
+    return rc;
 }
 
-static int build_device(Context* c, const double* dP, double* dK, double* dE) {
+static int build_device(Context* c, const double* dP, double* dK, double* dE, int accumulate_rho = 0) {
     if (c->n_pts == 0 || c->blocks.empty()) {
         set_error("build: no grid set");
         return PDFXC_ERR_NO_GRID;
@@ -1049,6 +1054,7 @@ static int build_device(Context* c, const double* dP, double* dK, double* dE) {
     c->d_K = dK;
     int rc = PDFXC_OK;
     const bool split = c->opts.precision != PDFXC_FP64;
+    c->rho_accumulate = accumulate_rho;
     PDFXC_HIP(hipEventRecord(c->ev[0], c->stream));
     if (split) {
         rc = launch_split_rho(*c);
@@ -1069,6 +1075,7 @@ static int build_device(Context* c, const double* dP, double* dK, double* dE) {
     PDFXC_HIP(hipEventRecord(c->ev[3], c->stream));
     c->d_P = saveP;
     c->d_K = saveK;
+    c->rho_accumulate = 0;
     if (rc) return rc;
     PDFXC_HIP(hipEventSynchronize(c->ev[3]));
     float t1, t2, t3;
@@ -1094,6 +1101,32 @@ int pdfxc_build_rks_device(pdfxc_ctx* ctx, int functional, const double* dP, dou
     return build_device(c, dP, dK, dE);
 }
 
+int pdfxc_build_rks_update_device(pdfxc_ctx* ctx, int functional, const double* d_dP, double* dK, double* dE) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_build_rks_update_device: null context");
+    if (functional != PDFXC_B3LYP) {
+        set_error("pdfxc_build_rks_update_device: only B3LYP is implemented");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (!d_dP || !dK || !dE) return arg_error("pdfxc_build_rks_update_device: null buffer");
+    return build_device(c, d_dP, dK, dE, 1);
+}
+
+static int build_host(Context* c, const double* P_packed, double* K_packed, double* E_xc, double* rho_out,
+                      int accumulate_rho);
+
+int pdfxc_build_rks_update(pdfxc_ctx* ctx, int functional, const double* dP_packed, double* K_packed,
+                           double* E_xc, double* rho_out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_build_rks_update: null context");
+    if (functional != PDFXC_B3LYP) {
+        set_error("pdfxc_build_rks_update: only B3LYP is implemented");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (!dP_packed || !K_packed || !E_xc) return arg_error("pdfxc_build_rks_update: null buffer");
+    return build_host(c, dP_packed, K_packed, E_xc, rho_out, 1);
+}
+
 int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_packed, double* K_packed,
                     double* E_xc, double* rho_out) {
     Context* c = reinterpret_cast<Context*>(ctx);
@@ -1103,6 +1136,15 @@ int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_packed, doub
         return PDFXC_ERR_UNSUPPORTED;
     }
     if (!P_packed || !K_packed || !E_xc) return arg_error("pdfxc_build_rks: null buffer");
+    return build_host(c, P_packed, K_packed, E_xc, rho_out, 0);
+}
+
+static int build_host(Context* c, const double* P_packed, double* K_packed, double* E_xc, double* rho_out,
+                      int accumulate_rho) {
+    if (c->n_pts == 0 || c->blocks.empty()) {
+        set_error("build: no grid set");
+        return PDFXC_ERR_NO_GRID;
+    }
     const int N = c->n_aos;
     double* H = c->h_pinned;
     for (int r = 0, k = 0; r < N; ++r)
@@ -1112,7 +1154,7 @@ int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_packed, doub
         }
     PDFXC_HIP(hipMemcpyAsync(c->d_P, H, sizeof(double) * (size_t)N * N, hipMemcpyHostToDevice, c->stream));
     PDFXC_HIP(hipMemsetAsync(c->d_K, 0, sizeof(double) * (size_t)N * N, c->stream));
-    int rc = build_device(c, c->d_P, c->d_K, c->d_E);
+    int rc = build_device(c, c->d_P, c->d_K, c->d_E, accumulate_rho);
     if (rc) return rc;
     double E = 0.0;
     PDFXC_HIP(hipMemcpyAsync(H, c->d_K, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost, c->stream));
@@ -1129,6 +1171,152 @@ int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_packed, doub
             for (int q = 0; q < 4; ++q) rho_out[4 * p + q] = rs[4 * s + q];
         }
     }
+    return PDFXC_OK;
+}
+
+int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z, const double* xyz,
+                       const int* point_atom, const double* P_packed, double* grad_functional,
+                       double* grad_weight, double* E_xc) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_gradient_rks: null context");
+    if (functional != PDFXC_B3LYP) {
+        set_error("pdfxc_gradient_rks: only B3LYP is implemented");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    if (n_atoms <= 0 || !Z || !xyz || !P_packed || !grad_functional || !grad_weight)
+        return arg_error("pdfxc_gradient_rks: bad arguments");
+    if (c->n_pts == 0 || c->blocks.empty()) {
+        set_error("pdfxc_gradient_rks: no grid set");
+        return PDFXC_ERR_NO_GRID;
+    }
+    if (!point_atom && c->gen_atom.size() != (size_t)c->n_pts)
+        return arg_error("pdfxc_gradient_rks: the grid was not made by pdfxc_generate_grid; pass point_atom");
+    const int N = c->n_aos;
+    const int na = n_atoms;
+    // atom of every shell: the atom at its centre
+    std::vector<int> shell_atom(c->shells.size(), -1);
+    for (size_t k = 0; k < c->shells.size(); ++k) {
+        const double* ctr = c->shells[k].center;
+        for (int a = 0; a < na; ++a) {
+            const double dx = ctr[0] - xyz[3 * a], dy = ctr[1] - xyz[3 * a + 1], dz = ctr[2] - xyz[3 * a + 2];
+            if (dx * dx + dy * dy + dz * dz < 1e-16) {
+                shell_atom[k] = a;
+                break;
+            }
+        }
+        if (shell_atom[k] < 0)
+            return arg_error("pdfxc_gradient_rks: the centre of AO " + std::to_string(c->shells[k].first_ao) +
+                             " is not at any atom");
+    }
+    std::vector<int> slot_atom((size_t)c->n_slots, -1);
+    for (int64_t p = 0; p < c->n_pts; ++p) {
+        const int a = point_atom ? point_atom[p] : c->gen_atom[p];
+        if (a < 0 || a >= na || Z[a] <= 0)
+            return arg_error("pdfxc_gradient_rks: point " + std::to_string(p) + " has no valid atom");
+        slot_atom[c->slot_of_point[p]] = a;
+    }
+    // Becke table (x, y, z, Z | 1/R_mn | a_mn) as the grid generator builds it
+    std::vector<double> at(4 * (size_t)na + 2 * (size_t)na * na, 0.0);
+    for (int m = 0; m < na; ++m) {
+        for (int d = 0; d < 3; ++d) at[4 * m + d] = xyz[3 * m + d];
+        at[4 * m + 3] = Z[m];
+    }
+    {
+        double* invd = at.data() + 4 * na;
+        double* amat = invd + (size_t)na * na;
+        auto brad = [](int z) { return z == 1 ? 0.35 : bragg_slater(z); };
+        for (int m = 0; m < na; ++m)
+            for (int k = 0; k < na; ++k) {
+                if (m == k) continue;
+                const double dx = xyz[3 * m] - xyz[3 * k], dy = xyz[3 * m + 1] - xyz[3 * k + 1],
+                             dz = xyz[3 * m + 2] - xyz[3 * k + 2];
+                const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (dist < 1e-10 && Z[m] > 0 && Z[k] > 0)
+                    return arg_error("pdfxc_gradient_rks: atoms " + std::to_string(m) + " and " +
+                                     std::to_string(k) + " coincide");
+                invd[(size_t)m * na + k] = 1.0 / dist;
+                if (Z[m] > 0 && Z[k] > 0) {
+                    if (bragg_slater(Z[m]) == 0.0 || bragg_slater(Z[k]) == 0.0) {
+                        set_error("pdfxc_gradient_rks: no Bragg-Slater radius for an atom");
+                        return PDFXC_ERR_UNSUPPORTED;
+                    }
+                    const double Rm = brad(Z[m]), Rk = brad(Z[k]);
+                    const double au = (Rm - Rk) / (Rm + Rk);
+                    amat[(size_t)m * na + k] = std::min(0.5, std::max(-0.5, au / (au * au - 1.0)));
+                }
+            }
+    }
+    // functional-term launches: the k_rho block lists, 16 points per workgroup
+    const int RS = c->rec_stride;
+    std::vector<LaunchRange> gl;
+    for (const LaunchRange& r : c->rho_launches) {
+        LaunchRange g = r;
+        g.lds = 0;
+        std::vector<int> ids(r.count);
+        PDFXC_HIP(hipMemcpy(ids.data(), c->d_rho_work + r.offset, sizeof(int) * r.count, hipMemcpyDeviceToHost));
+        for (int b : ids) {
+            const BlockDesc& B = c->blocks[b];
+            const size_t lds = sizeof(double) * (2 * (size_t)B.ldn * 16 + (size_t)B.nshell * RS + 3 * (size_t)na) +
+                               sizeof(int) * (size_t)(B.ldn + 3 * B.nshell);
+            if (r.cls > 2 || lds > (size_t)kMaxDynLds) {
+                set_error("pdfxc_gradient_rks: a block with " + std::to_string(B.nao) + " AOs and " +
+                          std::to_string(na) + " atoms does not fit the gradient kernel (raise ao_screen)");
+                return PDFXC_ERR_UNSUPPORTED;
+            }
+            g.lds = std::max(g.lds, lds);
+        }
+        gl.push_back(g);
+    }
+    int *d_shat = nullptr, *d_slat = nullptr;
+    double *d_at = nullptr, *d_ewf = nullptr, *d_g = nullptr;
+    int rc = dalloc(&d_shat, shell_atom.size());
+    if (rc == PDFXC_OK) rc = dalloc(&d_slat, slot_atom.size());
+    if (rc == PDFXC_OK) rc = dalloc(&d_at, at.size());
+    if (rc == PDFXC_OK) rc = dalloc(&d_ewf, (size_t)c->n_slots);
+    if (rc == PDFXC_OK) rc = dalloc(&d_g, 6 * (size_t)na);
+    hipError_t e = hipSuccess;
+    if (rc == PDFXC_OK) {
+        e = hipMemcpy(d_shat, shell_atom.data(), sizeof(int) * shell_atom.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_slat, slot_atom.data(), sizeof(int) * slot_atom.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d_at, at.data(), sizeof(double) * at.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(d_g, 0, sizeof(double) * 6 * (size_t)na);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_gradient_rks: upload");
+    }
+    if (rc == PDFXC_OK) {
+        double* H = c->h_pinned;
+        for (int r = 0, k = 0; r < N; ++r)
+            for (int q = 0; q <= r; ++q, ++k) {
+                H[(size_t)r * N + q] = P_packed[k];
+                H[(size_t)q * N + r] = P_packed[k];
+            }
+        e = hipMemcpyAsync(c->d_P, H, sizeof(double) * (size_t)N * N, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_gradient_rks: P upload");
+    }
+    // rho, the point terms (with w e_xc per slot), the two gradient terms, E_xc
+    for (const LaunchRange& r : c->rho_launches)
+        if (rc == PDFXC_OK) rc = launch_rho(*c, r);
+    if (rc == PDFXC_OK) rc = launch_xc_points(*c, d_ewf);
+    for (const LaunchRange& r : gl)
+        if (rc == PDFXC_OK) rc = launch_grad_fn(*c, r, d_shat, d_slat, na, d_g);
+    if (rc == PDFXC_OK) rc = launch_grad_w(*c, d_at, na, d_slat, d_ewf, d_g);
+    if (rc == PDFXC_OK) rc = launch_energy_sum(*c, c->d_E);
+    std::vector<double> g(6 * (size_t)na);
+    double E = 0.0;
+    if (rc == PDFXC_OK) {
+        e = hipMemcpyAsync(g.data(), d_g, sizeof(double) * g.size(), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&E, c->d_E, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_gradient_rks: results");
+    }
+    dfree(d_shat);
+    dfree(d_slat);
+    dfree(d_at);
+    dfree(d_ewf);
+    dfree(d_g);
+    if (rc != PDFXC_OK) return rc;
+    std::copy(g.begin(), g.begin() + 3 * na, grad_functional);
+    std::copy(g.begin() + 3 * na, g.end(), grad_weight);
+    if (E_xc) *E_xc = E;
     return PDFXC_OK;
 }
 

@@ -86,6 +86,7 @@ struct Context {
     // grid (host copies in caller order)
     int64_t n_pts = 0;
     std::vector<double> gx, gy, gz, gw;
+    std::vector<int> gen_atom;  // atom of each point when pdfxc_generate_grid made the grid
 
     // blocks
     std::vector<BlockDesc> blocks;     // launch order (descending cost)
@@ -134,6 +135,8 @@ struct Context {
     int* d_srho_work = nullptr;
     void* d_sk_work = nullptr;
 
+    int rho_accumulate = 0;             // update mode: the rho kernels add rho[dP] to d_rho
+
     double last_build_ms = 0.0;
     double last_kernel_ms[4] = {0, 0, 0, 0};
 };
@@ -151,7 +154,7 @@ int hip_fail(hipError_t e, const char* what);
 // kernel launchers (pdfxc_kernels.hip)
 int launch_rho(Context& c, const LaunchRange& r);
 int rho_points_per_workgroup(int cls);
-int launch_xc_points(Context& c);
+int launch_xc_points(Context& c, double* d_ewf = nullptr);
 int launch_kmat(Context& c, const LaunchRange& r);
 int configure_kernels();
 int launch_energy_sum(Context& c, double* dE);
@@ -166,6 +169,11 @@ int split_points_per_workgroup(int cls);
 int launch_split_rho(Context& c);   // P slices + k_rho_split
 int launch_split_kmat(Context& c);  // M = Phi^T V, K += M + M^T
 int launch_mfma_split_selftest(float* d_out, hipStream_t s);
+// XC nuclear gradient (pdfxc_grad.hip); d_gacc: functional term [3 n_atoms] then weight term [3 n_atoms]
+int launch_grad_fn(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,
+                   int n_atoms, double* d_gacc);
+int launch_grad_w(Context& c, const double* d_atoms, int n_atoms, const int* d_slot_atom, const double* d_ewf,
+                  double* d_gacc);
 int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 
 }  // namespace pdfxc

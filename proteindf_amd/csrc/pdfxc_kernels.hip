@@ -158,8 +158,9 @@ __device__ inline void rho_tiles(int wI, int WI, int wP, int T, int lane, const 
 //      with X: rho = sum_j X_pj phi_pj, grad rho = 2 sum_j X_pj grad phi_pj
 //      (calcRho_GGA_part, DfCalcGridX.cpp:1923-1962); the partial sums of
 //      the shell lanes are added in a fixed order.
-// rho_out receives (rho, grad rho) per slot; the functional runs over all
-// points in k_xc_points.
+// rho_out receives (rho, grad rho) per slot -- or adds them (update mode,
+// calcRho_GGA_part's accumulation, DfCalcGridX.cpp:1925-1961); the
+// functional runs over all points in k_xc_points.
 // ---------------------------------------------------------------------------
 template <int PB, int WI, int NIMAX, int G>
 __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
@@ -167,7 +168,8 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
                                              const double* __restrict__ shrec, int RS,
                                              const int* __restrict__ bshells, const int* __restrict__ bshell_col,
                                              const int* __restrict__ bcols, const int* __restrict__ bsorder,
-                                             const double* __restrict__ P, int N, double* __restrict__ rho_out) {
+                                             const double* __restrict__ P, int N, double* __restrict__ rho_out,
+                                             int accumulate) {
     constexpr int SUBS = kBlockPts / PB;
     constexpr int NSL = 512 / PB;  // shell lanes per point
     constexpr int WP = 8 / WI;
@@ -292,7 +294,8 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
         double s = 0.0;
         for (int k2 = 0; k2 < NSL; ++k2) s += s_part[(q * NSL + k2) * PB + pp];
         const int64_t slot = B.pt_begin + p0 + pp;
-        rho_out[4 * slot + q] = (q == 0) ? s : 2.0 * s;
+        const double v = (q == 0) ? s : 2.0 * s;
+        rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;  // update mode adds rho[dP]
     }
 }
 
@@ -300,11 +303,13 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
 // B3LYP point terms for every slot (buildVxc, DfCalcGridX.cpp:2137-2193, and
 // the GGA coefficients of build_XC_Matrix, :2455-2500): coef = (w vrho / 2,
 // w (2 vgAA + vgAB) grad rho), zero where rho <= the density cut-off or
-// w = 0 (padding slots); per-workgroup sums of w e_xc in a fixed order.
+// w = 0 (padding slots); per-workgroup sums of w e_xc in a fixed order, and
+// w e_xc per slot when ewf is given.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_xc_points(int64_t n_slots, const double* __restrict__ pts,
                                                    const double* __restrict__ rho, double* __restrict__ coef,
-                                                   double* __restrict__ epart, double dens_cut) {
+                                                   double* __restrict__ epart, double* __restrict__ ewf,
+                                                   double dens_cut) {
     __shared__ double s_e[256];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     double e = 0.0;
@@ -326,6 +331,7 @@ __global__ __launch_bounds__(256) void k_xc_points(int64_t n_slots, const double
         coef[4 * i + 1] = cx;
         coef[4 * i + 2] = cy;
         coef[4 * i + 3] = cz;
+        if (ewf) ewf[i] = e;
     }
     s_e[threadIdx.x] = e;
     __syncthreads();
@@ -438,6 +444,7 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
     const int ncols = W.ncols;  // multiple of 16, <= ldn
     const int LD = (ncols & 31) ? ncols : ncols + 16;
     const int PS = W.ps;
+    const int ps_log = 31 - __builtin_clz(PS);
     const int nsh = W.nsh;
     const int SLAB = PS * LD;  // doubles per array per buffer
     // LDS: [2 buffers][phi, V][PS][LD] | records [nsh][RS] | cols [ncols] | shcol [nsh] | shid [nsh]
@@ -483,8 +490,8 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
     const int ptid = tid - 512;  // producer thread index
     auto produce = [&](int pb, double* phi, double* vv) {
         for (int it = ptid; it < nsh * PS; it += 256) {
-            const int p = it % PS;
-            const int k = it / PS;
+            const int p = it & (PS - 1);  // PS is a power of two (16, 8 or 4)
+            const int k = it >> ps_log;
             const int pp = pb + p;
             double v[5], vx[5], vy[5], vz[5];
             rec_values(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], v, vx, vy, vz);
@@ -501,8 +508,8 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
             }
         }
         for (int it = ptid; it < (ncols - min(ncols, B.nao)) * PS; it += 256) {
-            const int c = B.nao + it / PS;
-            const int p = it % PS;
+            const int c = B.nao + (it >> ps_log);
+            const int p = it & (PS - 1);
             phi[p * LD + (c ^ (p & 15))] = 0.0;
             vv[p * LD + (c ^ (p & 15))] = 0.0;
         }
@@ -645,7 +652,7 @@ static int rho_one(Context& c, const LaunchRange& r) {
     hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G>), dim3(r.count * subs), dim3(512), r.lds, c.stream,
                        c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
                        c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
-                       c.n_aos, c.d_rho);
+                       c.n_aos, c.d_rho, c.rho_accumulate);
     return check_launch("k_rho");
 }
 
@@ -663,10 +670,10 @@ int launch_rho(Context& c, const LaunchRange& r) {
     }
 }
 
-int launch_xc_points(Context& c) {
+int launch_xc_points(Context& c, double* d_ewf) {
     const int64_t g = (c.n_slots + 255) / 256;
     hipLaunchKernelGGL(k_xc_points, dim3((unsigned)g), dim3(256), 0, c.stream, c.n_slots, c.d_pts, c.d_rho,
-                       c.d_coef, c.d_eblock, c.opts.density_cutoff);
+                       c.d_coef, c.d_eblock, d_ewf, c.opts.density_cutoff);
     return check_launch("k_xc_points");
 }
 

@@ -154,7 +154,7 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
                                                    const int* __restrict__ bshells, const int* __restrict__ bsorder,
                                                    const int* __restrict__ sshell_col, const int* __restrict__ chunks,
                                                    const unsigned short* __restrict__ Ps, int N, int Nc,
-                                                   double* __restrict__ rho_out) {
+                                                   double* __restrict__ rho_out, int accumulate) {
     constexpr int SUBS = kBlockPts / PB;
     constexpr int NSL = 512 / PB;  // shell lanes per point
     __shared__ double s_pt[PB][3];
@@ -285,10 +285,9 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
     }
     if (kk == 0) {
         const int64_t slot = B.pt_begin + p0 + p;
-        rho_out[4 * slot + 0] = r0;
-        rho_out[4 * slot + 1] = 2.0 * r1;
-        rho_out[4 * slot + 2] = 2.0 * r2;
-        rho_out[4 * slot + 3] = 2.0 * r3;
+        const double v[4] = {r0, 2.0 * r1, 2.0 * r2, 2.0 * r3};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v[q] : v[q];
     }
 }
 
@@ -508,7 +507,7 @@ static int rho_split_one(Context& c, const LaunchRange& r) {
     hipLaunchKernelGGL((k_rho_split<S, PB, NJMAX>), dim3(r.count * subs), dim3(512), r.lds, c.stream,
                        c.d_srho_work + r.offset, c.d_blocks, c.d_sblocks, c.d_pts, c.n_slots, c.d_shrec,
                        c.rec_stride, c.d_block_shells, c.d_block_sorder, c.d_sshell_col, c.d_chunks, c.d_Ps,
-                       c.n_aos, c.split_nc, c.d_rho);
+                       c.n_aos, c.split_nc, c.d_rho, c.rho_accumulate);
     return check("k_rho_split");
 }
 

@@ -25,7 +25,8 @@ EXPORTED = [
     "pdfxc_default_opts", "pdfxc_normalize_cgto", "pdfxc_ctx_create", "pdfxc_ctx_destroy",
     "pdfxc_set_grid", "pdfxc_generate_grid", "pdfxc_get_grid", "pdfxc_build_rks",
     "pdfxc_build_rks_device", "pdfxc_eval_functional", "pdfxc_eval_aos", "pdfxc_get_stats",
-    "pdfxc_stream", "pdfxc_last_error", "pdfxc_selftest_mfma", "pdfxc_selftest_mfma_split",
+    "pdfxc_stream", "pdfxc_last_error", "pdfxc_gradient_rks", "pdfxc_build_rks_update",
+    "pdfxc_build_rks_update_device", "pdfxc_selftest_mfma", "pdfxc_selftest_mfma_split",
     "pdfxc_selftest_copy",
 ]
 
@@ -77,6 +78,9 @@ def lib():
         L.pdfxc_last_error.argtypes = []
         L.pdfxc_last_error.restype = ctypes.c_char_p
         L.pdfxc_selftest_mfma.argtypes = [P]
+        L.pdfxc_gradient_rks.argtypes = [P, i32, i32, P, P, P, P, P, P, P]
+        L.pdfxc_build_rks_update.argtypes = [P, i32, P, P, P, P]
+        L.pdfxc_build_rks_update_device.argtypes = [P, i32, P, P, P]
         L.pdfxc_selftest_mfma_split.argtypes = [P]
         L.pdfxc_selftest_copy.argtypes = [P, P, i64]
         _lib = L
@@ -193,6 +197,31 @@ class XcContext:
         E = ctypes.c_double()
         rho = np.zeros((self.n_pts, 4)) if want_rho else None
         check(lib().pdfxc_build_rks(self.h, functional, _p(P), _p(K), ctypes.byref(E), _p(rho)))
+        return E.value, K, rho
+
+    def gradient_rks(self, Z, xyz, P_alpha_packed, point_atom=None, functional=PDFXC_B3LYP):
+        """(E_xc, grad_functional (n_atoms, 3), grad_weight (n_atoms, 3)); dE/dR."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        P = np.ascontiguousarray(P_alpha_packed, dtype=np.float64)
+        pa = None if point_atom is None else np.ascontiguousarray(point_atom, dtype=np.int32)
+        gf = np.zeros((len(Z), 3))
+        gw = np.zeros((len(Z), 3))
+        E = ctypes.c_double()
+        check(lib().pdfxc_gradient_rks(self.h, functional, len(Z), _p(Z), _p(xyz), _p(pa), _p(P), _p(gf),
+                                       _p(gw), ctypes.byref(E)))
+        return E.value, gf, gw
+
+    def build_rks_update(self, dP_alpha_packed, K_alpha_packed=None, want_rho=False, functional=PDFXC_B3LYP):
+        """Update mode: adds rho[dP_alpha] to the stored rho, then K and E from it."""
+        P = np.ascontiguousarray(dP_alpha_packed, dtype=np.float64)
+        n = self.n_aos
+        if P.shape != (n * (n + 1) // 2,):
+            raise ValueError("dP_alpha_packed must hold N(N+1)/2 values")
+        K = np.zeros_like(P) if K_alpha_packed is None else K_alpha_packed
+        E = ctypes.c_double()
+        rho = np.zeros((self.n_pts, 4)) if want_rho else None
+        check(lib().pdfxc_build_rks_update(self.h, functional, _p(P), _p(K), ctypes.byref(E), _p(rho)))
         return E.value, K, rho
 
     def build_rks_device(self, dP_ptr, dK_ptr, dE_ptr, functional=PDFXC_B3LYP):

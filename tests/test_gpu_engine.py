@@ -253,3 +253,27 @@ def test_reference_style_dfxcfunctional_electron_count():
     assert abs(xc.electrons_on_grid - mol.n_electrons) < 1e-4 * mol.n_electrons
     assert np.isfinite(xc.XC_energy_) and xc.XC_energy_ < 0
     assert np.all(np.isfinite(F))
+
+
+def test_update_mode_accumulates_rho_from_density_differences():
+    # xc-update = yes: rho[P2] = rho[P1] + rho[P2 - P1] (calcRho_GGA_part adds,
+    # DfCalcGridX.cpp:1925-1961); K and E then follow from the accumulated rho
+    mol, info = water(2, seed=9)
+    grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 40, 194)
+    P1 = 0.5 * guess.density_guess(mol, info, seed=1, eps=0.05)
+    P2 = 0.5 * guess.density_guess(mol, info, seed=2, eps=0.08)
+    ctx = engine.XcContext(info)
+    ctx.set_grid(grid)
+    E0, K0, rho0 = ctx.build_rks_update(guess.pack_lower(P1), want_rho=True)  # stored rho starts at zero
+    E1, K1, rho1 = ctx.build_rks(guess.pack_lower(P1), want_rho=True)
+    assert abs(E0 - E1) <= 1e-12 * abs(E1)
+    np.testing.assert_allclose(rho0, rho1, rtol=1e-14, atol=1e-18)
+    Eu, Ku, rhou = ctx.build_rks_update(guess.pack_lower(P2 - P1), want_rho=True)
+    E2, K2, rho2 = ctx.build_rks(guess.pack_lower(P2), want_rho=True)
+    ctx.close()
+    say("update mode: |dE|", abs(Eu - E2), "max|dK|", np.abs(Ku - K2).max())
+    assert abs(Eu - E2) <= 1e-11
+    np.testing.assert_allclose(Ku, K2, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(rhou, rho2, rtol=1e-12, atol=1e-16)
+    e_ref, K_ref, _ = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), grid, P2)
+    assert abs(Eu - e_ref) <= 1e-10
