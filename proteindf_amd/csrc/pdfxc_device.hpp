@@ -21,6 +21,34 @@ __device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
 constexpr double kTooBig = 30.0;  // DfCalcGridX::TOOBIG, primitives with alpha r^2 > 30 are skipped
 constexpr double kInvSqrt3 = 0.57735026918962576451;
 
+// e^{-x} for 0 <= x <= kTooBig: Cody-Waite reduction x = n ln2 + r with
+// |r| <= ln2 / 2 (ln2 split in a 32-bit-exact high part and a low part),
+// e^{-r} by its Taylor series to degree 13 in Horner form (truncation
+// < 5e-18 relative), then scaling by 2^{-n}.  Written for the primitive
+// exponentials of the AO values; agrees with the libm result to a few ulp.
+__device__ inline double exp_neg(double x) {
+    constexpr double kLog2e = 1.4426950408889634074;
+    constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000
+    constexpr double kLn2Lo = 1.90821492927058770002e-10;
+    const double n = __builtin_rint(x * kLog2e);
+    const double r = -__builtin_fma(-n, kLn2Lo, __builtin_fma(-n, kLn2Hi, x));  // -(x - n ln2)
+    double p = 1.0 / 6227020800.0;                        // 1/13!
+    p = __builtin_fma(p, r, 1.0 / 479001600.0);           // 1/12!
+    p = __builtin_fma(p, r, 1.0 / 39916800.0);
+    p = __builtin_fma(p, r, 1.0 / 3628800.0);
+    p = __builtin_fma(p, r, 1.0 / 362880.0);
+    p = __builtin_fma(p, r, 1.0 / 40320.0);
+    p = __builtin_fma(p, r, 1.0 / 5040.0);
+    p = __builtin_fma(p, r, 1.0 / 720.0);
+    p = __builtin_fma(p, r, 1.0 / 120.0);
+    p = __builtin_fma(p, r, 1.0 / 24.0);
+    p = __builtin_fma(p, r, 1.0 / 6.0);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    return __builtin_amdgcn_ldexp(p, -(int)n);
+}
+
 // ---------------------------------------------------------------------------
 // Shell records.  Every shell has a record of RS doubles in global memory:
 // [cx, cy, cz, l + 4 nprim, e0, c0, e1, c1, ...]; a block copies the records
@@ -33,14 +61,14 @@ __device__ inline void rec_values(const double* rec, double px, double py, doubl
     const int l = meta & 3, nprim = meta >> 2;
     const double r2 = x * x + y * y + z * z;
     double R0 = 0.0, R1 = 0.0;
+    // exponents ascend, so alpha r^2 > 30 holds for every primitive after the first that exceeds it
     for (int k = 0; k < nprim; ++k) {
         const double a = rec[4 + 2 * k];
         const double sh = a * r2;
-        if (sh <= kTooBig) {
-            const double g = rec[5 + 2 * k] * exp(-sh);
-            R0 += g;
-            R1 += -2.0 * a * g;
-        }
+        if (sh > kTooBig) break;
+        const double g = rec[5 + 2 * k] * exp_neg(sh);
+        R0 += g;
+        R1 += -2.0 * a * g;
     }
     if (l == 0) {
         v[0] = R0;
@@ -84,10 +112,11 @@ __device__ inline void rec_values0(const double* rec, double px, double py, doub
     const int l = meta & 3, nprim = meta >> 2;
     const double r2 = x * x + y * y + z * z;
     double R0 = 0.0;
-    for (int k = 0; k < nprim; ++k) {
+    for (int k = 0; k < nprim; ++k) {  // ascending exponents (see rec_values)
         const double a = rec[4 + 2 * k];
         const double sh = a * r2;
-        if (sh <= kTooBig) R0 += rec[5 + 2 * k] * exp(-sh);
+        if (sh > kTooBig) break;
+        R0 += rec[5 + 2 * k] * exp_neg(sh);
     }
     if (l == 0) {
         v[0] = R0;
@@ -116,15 +145,14 @@ __device__ inline int rec_radial2(const double* rec, double px, double py, doubl
     const int nprim = meta >> 2;
     const double r2 = r3[0] * r3[0] + r3[1] * r3[1] + r3[2] * r3[2];
     R0 = R1 = R2 = 0.0;
-    for (int k = 0; k < nprim; ++k) {
+    for (int k = 0; k < nprim; ++k) {  // ascending exponents (see rec_values)
         const double a = rec[4 + 2 * k];
         const double sh = a * r2;
-        if (sh <= kTooBig) {
-            const double g = rec[5 + 2 * k] * exp(-sh);
-            R0 += g;
-            R1 += -2.0 * a * g;
-            R2 += 4.0 * a * a * g;
-        }
+        if (sh > kTooBig) break;
+        const double g = rec[5 + 2 * k] * exp_neg(sh);
+        R0 += g;
+        R1 += -2.0 * a * g;
+        R2 += 4.0 * a * a * g;
     }
     return meta & 3;
 }

@@ -198,6 +198,23 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
                                  " does not continue the shell starting at AO " + std::to_string(i));
             }
         }
+        // primitives in ascending exponent order, as setCGTO keeps them
+        // (TlOrbitalInfoObject.cpp:38-106): the kernels stop at the first
+        // primitive past the alpha r^2 > 30 cut
+        {
+            int ord[kMaxPrim];
+            for (int k = 0; k < s.nprim; ++k) ord[k] = k;
+            std::stable_sort(ord, ord + s.nprim, [&](int x, int y) { return s.exps[x] < s.exps[y]; });
+            double e2[kMaxPrim], c2[kMaxPrim];
+            for (int k = 0; k < s.nprim; ++k) {
+                e2[k] = s.exps[ord[k]];
+                c2[k] = s.coefs[ord[k]];
+            }
+            for (int k = 0; k < s.nprim; ++k) {
+                s.exps[k] = e2[k];
+                s.coefs[k] = c2[k];
+            }
+        }
         c->shells.push_back(s);
         i += ncomp;
     }

@@ -33,14 +33,13 @@ __device__ inline void shell_values(const DeviceShell& S, double x, double y, do
                                     double* vx, double* vy, double* vz) {
     const double r2 = x * x + y * y + z * z;
     double R0 = 0.0, R1 = 0.0;
-    for (int k = 0; k < S.nprim; ++k) {
+    for (int k = 0; k < S.nprim; ++k) {  // exponents ascend: the first cut primitive ends the sum
         const double a = S.exps[k];
         const double sh = a * r2;
-        if (sh <= kTooBig) {
-            const double g = S.coefs[k] * exp(-sh);
-            R0 += g;
-            R1 += -2.0 * a * g;
-        }
+        if (sh > kTooBig) break;
+        const double g = S.coefs[k] * exp_neg(sh);
+        R0 += g;
+        R1 += -2.0 * a * g;
     }
     if (S.l == 0) {
         v[0] = R0;
