@@ -277,8 +277,33 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
     if (rc == PDFXC_OK && c->opts.precision != PDFXC_FP64) {
         const int S = (c->opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
         c->split_nc = (n_aos + 7) / 8 * 8;
+        // split AO order: atoms (runs of shells on one centre) in input order,
+        // each atom's shells by most diffuse exponent first, so that the shells
+        // a block keeps of a distant atom form one run of columns
+        {
+            std::vector<int> order(c->shells.size());
+            std::iota(order.begin(), order.end(), 0);
+            size_t a0 = 0;
+            while (a0 < c->shells.size()) {
+                size_t a1 = a0 + 1;
+                while (a1 < c->shells.size() &&
+                       std::equal(c->shells[a1].center, c->shells[a1].center + 3, c->shells[a0].center))
+                    ++a1;
+                std::stable_sort(order.begin() + a0, order.begin() + a1,
+                                 [&](int x, int y) { return c->shells[x].exps[0] < c->shells[y].exps[0]; });
+                a0 = a1;
+            }
+            c->split_perm.clear();
+            for (int k : order)
+                for (int m = 0; m < 2 * c->shells[k].l + 1; ++m) c->split_perm.push_back(c->shells[k].first_ao + m);
+        }
         rc = dalloc(&c->d_Ps, (size_t)S * n_aos * c->split_nc);
         if (rc == PDFXC_OK) rc = dalloc(&c->d_M, (size_t)n_aos * n_aos);
+        if (rc == PDFXC_OK) rc = dalloc(&c->d_perm, (size_t)n_aos);
+        if (rc == PDFXC_OK) {
+            e = hipMemcpy(c->d_perm, c->split_perm.data(), sizeof(int) * n_aos, hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = hip_fail(e, "pdfxc_ctx_create: split order upload");
+        }
     }
     if (rc == PDFXC_OK) {
         e = hipMemcpy(c->d_shells, ds.data(), ds.size() * sizeof(DeviceShell), hipMemcpyHostToDevice);
@@ -328,6 +353,7 @@ int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
     dfree(c->d_E);
     dfree(c->d_Ps);
     dfree(c->d_M);
+    dfree(c->d_perm);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -607,26 +633,33 @@ static int build_blocks(Context& c) {
     if (c.opts.precision != PDFXC_FP64) {
         const int S = (c.opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
         c.sblocks.resize(nb);
+        std::vector<int> ipos(c.n_aos);
+        for (int q = 0; q < c.n_aos; ++q) ipos[c.split_perm[q]] = q;
+        std::vector<int> ch;
         for (int b = 0; b < nb; ++b) {
             const BlockDesc& B = c.blocks[b];
             SplitBlock& SB = c.sblocks[b];
             SB.chunk_begin = (int)c.chunks.size();
+            ch.clear();
             for (int k = 0; k < B.nshell; ++k) {
                 const Shell& Sh = c.shells[c.block_shells[B.shell_begin + k]];
-                for (int g = Sh.first_ao / 8; g <= (Sh.first_ao + 2 * Sh.l) / 8; ++g)
-                    if ((int)c.chunks.size() == SB.chunk_begin || c.chunks.back() != g) c.chunks.push_back(g);
+                const int q0 = ipos[Sh.first_ao];
+                for (int g = q0 / 4; g <= (q0 + 2 * Sh.l) / 4; ++g) ch.push_back(g);
             }
-            SB.nchunk = (int)c.chunks.size() - SB.chunk_begin;
-            SB.ldnp = std::max(32, (8 * SB.nchunk + 31) / 32 * 32);
+            std::sort(ch.begin(), ch.end());
+            ch.erase(std::unique(ch.begin(), ch.end()), ch.end());
+            c.chunks.insert(c.chunks.end(), ch.begin(), ch.end());
+            SB.nchunk = (int)ch.size();
+            SB.ldnp = std::max(32, (4 * SB.nchunk + 31) / 32 * 32);
             SB.pad = 0;
-            int pos = 0;
             for (int k = 0; k < B.nshell; ++k) {
                 const Shell& Sh = c.shells[c.block_shells[B.shell_begin + k]];
-                while (c.chunks[SB.chunk_begin + pos] != Sh.first_ao / 8) ++pos;
-                c.sshell_col.push_back(8 * pos + Sh.first_ao % 8);
+                const int q0 = ipos[Sh.first_ao];
+                const int pos = (int)(std::lower_bound(ch.begin(), ch.end(), q0 / 4) - ch.begin());
+                c.sshell_col.push_back(4 * pos + q0 % 4);
             }
             if (SB.ldnp > 1024) {
-                set_error("set_grid: a block keeps " + std::to_string(8 * SB.nchunk) +
+                set_error("set_grid: a block keeps " + std::to_string(4 * SB.nchunk) +
                           " chunked AO columns; the split kernels take at most 1024 (raise ao_screen)");
                 return PDFXC_ERR_UNSUPPORTED;
             }
@@ -639,9 +672,10 @@ static int build_blocks(Context& c) {
             for (int b = 0; b < nb; ++b) {
                 const SplitBlock& SB = c.sblocks[b];
                 if (SB.ldnp <= lo_lim || SB.ldnp > lim) continue;
-                const size_t plane = ((size_t)S * pb * (SB.ldnp + 8) * 2 + 15) / 16 * 16;
+                const size_t plane = std::max(((size_t)S * pb * (SB.ldnp + 8) * 2 + 15) / 16 * 16,
+                                              (size_t)4 * 512 * sizeof(double));
                 const size_t lds = plane + sizeof(double) * (size_t)c.blocks[b].nshell * RS +
-                                   sizeof(int) * (size_t)(SB.ldnp / 8 + 2 * c.blocks[b].nshell);
+                                   sizeof(int) * (size_t)(SB.ldnp / 4 + 2 * c.blocks[b].nshell);
                 if (lds > (size_t)kMaxDynLds) {
                     set_error("set_grid: a block with " + std::to_string(c.blocks[b].nao) +
                               " AOs needs more LDS than a workgroup has in the split rho kernel (raise ao_screen)");

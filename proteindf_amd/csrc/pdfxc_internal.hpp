@@ -41,9 +41,9 @@ struct BlockDesc {
     int pad;
 };
 
-// Split-precision column layout of a block: nchunk 8-aligned runs of global
-// AO indices (chunk ids in the chunk list from chunk_begin), ldnp = 8 nchunk
-// rounded up to 32.
+// Split-precision column layout of a block: nchunk 4-aligned runs of AO
+// positions in the split order (chunk ids in the chunk list from
+// chunk_begin), ldnp = 4 nchunk rounded up to 32.
 struct SplitBlock {
     int chunk_begin, nchunk, ldnp, pad;
 };
@@ -123,11 +123,13 @@ struct Context {
 
     // split precision (opts.precision != PDFXC_FP64)
     int split_nc = 0;                   // N rounded up to 8: row length of the P slices
+    std::vector<int> split_perm;        // split AO order: position -> AO (each atom's shells diffuse first)
+    int* d_perm = nullptr;
     std::vector<SplitBlock> sblocks;
-    std::vector<int> chunks;            // per block: chunk ids (global AO / 8), ascending
+    std::vector<int> chunks;            // per block: chunk ids (split-order AO position / 4), ascending
     std::vector<int> sshell_col;        // per block shell: local column in the chunk layout
     std::vector<LaunchRange> srho_launches, sk_launches;
-    unsigned short* d_Ps = nullptr;     // S x N x split_nc bf16 slices of P
+    unsigned short* d_Ps = nullptr;     // S x N x split_nc bf16 slices of P in the split AO order
     double* d_M = nullptr;              // dense N x N: M = Phi^T V
     SplitBlock* d_sblocks = nullptr;
     int* d_chunks = nullptr;

@@ -6,10 +6,12 @@
 // leave) and the slice products h_a h'_b with a + b < S accumulated in FP32:
 // S = 2 is hi + lo (3 products), S = 3 is hi + mid + lo (6 products).  AO
 // values, the functional, rho / grad rho dots and the K_xc scatter stay
-// FP64.  Block columns are 8-aligned runs of global AO indices ("chunks") so
-// that the P operand is one 16-byte load per lane; AOs of a chunk that the
-// block's screening dropped are zero columns.
-//   k_psplit      P (dense FP64 N x N) -> S bf16 slice planes [S][N][Nc]
+// FP64.  AOs are renumbered once per context (each atom's shells most
+// diffuse first, so the shells a block keeps of a distant atom are one run)
+// and block columns are 4-aligned runs of that order ("chunks"), so that the
+// P operand is two 8-byte loads per lane; AOs of a chunk that the block's
+// screening dropped are zero columns.
+//   k_psplit      P (dense FP64 N x N) -> S bf16 slice planes [S][N][Nc], split order
 //   k_rho_split   X = Phi_b P_bb for PB points, then rho, grad rho
 //                 (calcRho_GGA_part, DfCalcGridX.cpp:1923-1962)
 //   k_mfull_split M_bb = Phi_b^T V_b over a block's points, tiles of a 16 x 16
@@ -23,6 +25,7 @@
 namespace pdfxc {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // v_mfma_f32_16x16x32_bf16 (checked on gfx950 by pdfxc_selftest_mfma_split):
@@ -64,17 +67,17 @@ struct SlicePairs<3> {
 };
 
 // ---------------------------------------------------------------------------
-// P slices: Ps[s][r][c] for r < N, c < Nc (Nc = N rounded up to 8; columns
-// c >= N are zero)
+// P slices in the split order: Ps[s][r][c] = slice s of P[perm r][perm c]
+// for r < N, c < Nc (Nc = N rounded up to 8; columns c >= N are zero)
 // ---------------------------------------------------------------------------
 template <int S>
-__global__ __launch_bounds__(256) void k_psplit(const double* __restrict__ P, int N, int Nc,
-                                                unsigned short* __restrict__ Ps) {
+__global__ __launch_bounds__(256) void k_psplit(const double* __restrict__ P, const int* __restrict__ perm, int N,
+                                                int Nc, unsigned short* __restrict__ Ps) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t plane = (int64_t)N * Nc;
     if (i >= plane) return;
     const int r = (int)(i / Nc), c = (int)(i % Nc);
-    const double x = (c < N) ? P[(int64_t)r * N + c] : 0.0;
+    const double x = (c < N) ? P[(int64_t)perm[r] * N + perm[c]] : 0.0;
     unsigned short h[S];
     split_bf16<S>(x, h);
 #pragma unroll
@@ -86,9 +89,9 @@ __global__ __launch_bounds__(256) void k_psplit(const double* __restrict__ P, in
 // threads).  LDS plane A[s][p][c] (bf16, row stride ldnp + 8) holds the Phi
 // slices; X = Phi P_bb accumulates in FP32 with wave w owning the AO column
 // tiles J = w, w + 8, ... and all PB / 16 point tiles; the B operand (P_bb
-// rows = P rows by symmetry) is loaded per lane as 8 consecutive AOs of one
-// chunk.  X is then stored over the plane (FP32, row stride ldnp + 4) and
-// dotted with phi, grad phi in FP64.
+// rows = P rows by symmetry) is loaded per lane as the 8 AOs of two chunks.
+// X is then stored over the plane (FP32, row stride ldnp + 4) and dotted
+// with phi, grad phi in FP64.
 // ---------------------------------------------------------------------------
 template <int S, int PB, int NJ>
 __device__ inline void rho_split_tiles(int wave, int lane, int ldnp, int nch, const int* s_chunk,
@@ -98,28 +101,35 @@ __device__ inline void rho_split_tiles(int wave, int lane, int ldnp, int nch, co
     const int li = lane & 15, lq = lane >> 4;
     const int RA = ldnp + 8;
     const int64_t plane = (int64_t)N * Nc;
-    f32x4 acc[NP][NJ];
+    // each K step's slice products accumulate in FP32 from zero and are then
+    // added to FP64 accumulators, so FP32 rounding acts on one step's sum only
+    v4d acc[NP][NJ];
 #pragma unroll
     for (int ip = 0; ip < NP; ++ip)
 #pragma unroll
-        for (int jj = 0; jj < NJ; ++jj) acc[ip][jj] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int jj = 0; jj < NJ; ++jj) acc[ip][jj] = v4d{0.0, 0.0, 0.0, 0.0};
     int64_t prow[NJ];
 #pragma unroll
     for (int jj = 0; jj < NJ; ++jj) {
         const int c = (wave + 8 * jj) * 16 + li;
-        const int ch = c >> 3;
-        const int g = s_chunk[ch] * 8 + (c & 7);
+        const int ch = c >> 2;
+        const int g = s_chunk[ch] * 4 + (c & 3);
         prow[jj] = (int64_t)((ch < nch && g < N) ? g : 0) * Nc;  // padding columns: any row (Phi is 0 there)
     }
     const int nks = ldnp / 32;
     for (int ks = 0; ks < nks; ++ks) {
-        const int koff = s_chunk[ks * 4 + lq] * 8;
+        const int koff0 = s_chunk[ks * 8 + 2 * lq] * 4;
+        const int koff1 = s_chunk[ks * 8 + 2 * lq + 1] * 4;
         bf16x8 bfr[NJ][S];
 #pragma unroll
         for (int jj = 0; jj < NJ; ++jj)
 #pragma unroll
-            for (int s = 0; s < S; ++s)
-                bfr[jj][s] = *reinterpret_cast<const bf16x8*>(Ps + s * plane + prow[jj] + koff);
+            for (int s = 0; s < S; ++s) {
+                const unsigned short* row = Ps + s * plane + prow[jj];
+                const bf16x4 lo = *reinterpret_cast<const bf16x4*>(row + koff0);
+                const bf16x4 hi = *reinterpret_cast<const bf16x4*>(row + koff1);
+                bfr[jj][s] = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+            }
         bf16x8 afr[NP][S];
 #pragma unroll
         for (int ip = 0; ip < NP; ++ip)
@@ -130,10 +140,14 @@ __device__ inline void rho_split_tiles(int wave, int lane, int ldnp, int nch, co
 #pragma unroll
         for (int ip = 0; ip < NP; ++ip)
 #pragma unroll
-            for (int jj = 0; jj < NJ; ++jj)
+            for (int jj = 0; jj < NJ; ++jj) {
+                f32x4 a32 = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int q = 0; q < SlicePairs<S>::n; ++q)
-                    acc[ip][jj] = mfma_bf16(afr[ip][SlicePairs<S>::a[q]], bfr[jj][SlicePairs<S>::b[q]], acc[ip][jj]);
+                    a32 = mfma_bf16(afr[ip][SlicePairs<S>::a[q]], bfr[jj][SlicePairs<S>::b[q]], a32);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[ip][jj][r] += (double)a32[r];
+            }
     }
     __syncthreads();  // every wave has stopped reading the Phi plane
     const int RX = ldnp + 4;
@@ -143,7 +157,7 @@ __device__ inline void rho_split_tiles(int wave, int lane, int ldnp, int nch, co
         for (int jj = 0; jj < NJ; ++jj)
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-                s_x[(ip * 16 + 4 * lq + r) * RX + (wave + 8 * jj) * 16 + li] = acc[ip][jj][r];
+                s_x[(ip * 16 + 4 * lq + r) * RX + (wave + 8 * jj) * 16 + li] = (float)acc[ip][jj][r];
 }
 
 template <int S, int PB, int NJMAX>
@@ -171,14 +185,15 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ldnp = SB.ldnp, nch = SB.nchunk, nsh = B.nshell;
     const int RA = ldnp + 8;
-    // LDS: plane (S x PB x RA bf16, reused as PB x (ldnp + 4) FP32) | records | chunk [ldnp/8] | shcol | sorder
+    // LDS: plane (S x PB x RA bf16, reused as PB x (ldnp + 4) FP32 and as partial sums) | records | chunk [ldnp/4] | shcol | sorder
     unsigned short* s_a = reinterpret_cast<unsigned short*>(s_dyn);
     const int plane_dbl = (S * PB * RA * 2 + 15) / 16 * 2;  // doubles, 16-byte multiple
-    double* s_rec = s_dyn + plane_dbl;
+    const int region_dbl = max(plane_dbl, 4 * 512);  // also holds the 4 x 512 partial sums
+    double* s_rec = s_dyn + region_dbl;
     int* s_chunk = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
-    int* s_shcol = s_chunk + ldnp / 8;
+    int* s_shcol = s_chunk + ldnp / 4;
     int* s_sord = s_shcol + nsh;
-    for (int i = tid; i < ldnp / 8; i += 512) s_chunk[i] = (i < nch) ? chunks[SB.chunk_begin + i] : 0;
+    for (int i = tid; i < ldnp / 4; i += 512) s_chunk[i] = (i < nch) ? chunks[SB.chunk_begin + i] : 0;
     for (int k = tid; k < nsh; k += 512) {
         s_shcol[k] = sshell_col[B.shell_begin + k];
         s_sord[k] = bsorder[B.shell_begin + k];
@@ -199,8 +214,10 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
     }
     __syncthreads();
 
-    const int p = tid / NSL;
-    const int kk = tid % NSL;
+    // lanes take consecutive points of one shell: the slice writes of a
+    // column land on distinct banks, and a lane group runs one code path
+    const int p = tid & (PB - 1);
+    const int kk = tid / PB;
     const double px = s_pt[p][0], py = s_pt[p][1], pz = s_pt[p][2];
     // 1. Phi slices of the kept AOs (columns of dropped AOs stay zero)
     for (int j = kk; j < nsh; j += NSL) {
@@ -218,7 +235,7 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
     }
     __syncthreads();
 
-    // 2. X = Phi P_bb (FP32), stored over the plane
+    // 2. X = Phi P_bb, stored over the plane (FP32)
     float* s_x = reinterpret_cast<float*>(s_dyn);
     {
         const int nJ = ldnp / 16;
@@ -275,19 +292,22 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
             r3 += x * vz[m];
         }
     }
-    // the NSL lanes of a point are consecutive: butterfly sum in a fixed order
-#pragma unroll
-    for (int o = 1; o < NSL; o <<= 1) {
-        r0 += __shfl_xor(r0, o);
-        r1 += __shfl_xor(r1, o);
-        r2 += __shfl_xor(r2, o);
-        r3 += __shfl_xor(r3, o);
-    }
-    if (kk == 0) {
-        const int64_t slot = B.pt_begin + p0 + p;
-        const double v[4] = {r0, 2.0 * r1, 2.0 * r2, 2.0 * r3};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v[q] : v[q];
+    // partial sums of the NSL shell lanes of each point, added in a fixed order
+    __syncthreads();  // X no longer read: the region holds the partial sums
+    double* s_part = s_dyn;  // [4][NSL][PB]
+    s_part[(0 * NSL + kk) * PB + p] = r0;
+    s_part[(1 * NSL + kk) * PB + p] = r1;
+    s_part[(2 * NSL + kk) * PB + p] = r2;
+    s_part[(3 * NSL + kk) * PB + p] = r3;
+    __syncthreads();
+    if (tid < 4 * PB) {
+        const int q = tid / PB;
+        const int pp = tid & (PB - 1);
+        double sum = 0.0;
+        for (int k2 = 0; k2 < NSL; ++k2) sum += s_part[(q * NSL + k2) * PB + pp];
+        const int64_t slot = B.pt_begin + p0 + pp;
+        const double v = (q == 0) ? sum : 2.0 * sum;
+        rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;
     }
 }
 
@@ -308,7 +328,8 @@ __global__ __launch_bounds__(512) void k_mfull_split(const SKWork* __restrict__ 
                                                      const double* __restrict__ shrec, int RS,
                                                      const int* __restrict__ bshells, const int* __restrict__ bsorder,
                                                      const int* __restrict__ sshell_col, const int* __restrict__ chunks,
-                                                     const double* __restrict__ coef, double* __restrict__ M, int N) {
+                                                     const int* __restrict__ perm, const double* __restrict__ coef,
+                                                     double* __restrict__ M, int N) {
     __shared__ double s_pt[kBlockPts][3];
     __shared__ double s_cf[kBlockPts][4];
     extern __shared__ __attribute__((aligned(16))) double s_dyn[];
@@ -361,7 +382,7 @@ __global__ __launch_bounds__(512) void k_mfull_split(const SKWork* __restrict__ 
         const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
         for (int m = 0; m < nc; ++m) {
             const int c = c0 + m;
-            const int g = chunks[SB.chunk_begin + (c >> 3)] * 8 + (c & 7);
+            const int g = perm[chunks[SB.chunk_begin + (c >> 2)] * 4 + (c & 3)];  // caller AO index
             if (c >= cI0 && c < cI0 + ncI) s_gI[c - cI0] = g;
             if (c >= cJ0 && c < cJ0 + ncJ) s_gJ[c - cJ0] = g;
         }
@@ -496,8 +517,8 @@ int split_points_per_workgroup(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 :
 template <int S>
 static int psplit_one(Context& c) {
     const int64_t n = (int64_t)c.n_aos * c.split_nc;
-    hipLaunchKernelGGL((k_psplit<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.d_P, c.n_aos,
-                       c.split_nc, c.d_Ps);
+    hipLaunchKernelGGL((k_psplit<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.d_P, c.d_perm,
+                       c.n_aos, c.split_nc, c.d_Ps);
     return check("k_psplit");
 }
 
@@ -516,7 +537,7 @@ static int mfull_one(Context& c, const LaunchRange& r) {
     hipLaunchKernelGGL((k_mfull_split<S>), dim3(r.count), dim3(512), r.lds, c.stream,
                        reinterpret_cast<const SKWork*>(c.d_sk_work) + r.offset, c.d_blocks, c.d_sblocks, c.d_pts,
                        c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_sorder, c.d_sshell_col,
-                       c.d_chunks, c.d_coef, c.d_M, c.n_aos);
+                       c.d_chunks, c.d_perm, c.d_coef, c.d_M, c.n_aos);
     return check("k_mfull_split");
 }
 
