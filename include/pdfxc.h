@@ -173,12 +173,13 @@ const char* pdfxc_last_error(void);
  * confirm the lane layout the kernels rely on. */
 int pdfxc_selftest_mfma(double* out16x16);
 
-/* Diagnostics: one v_mfma_f32_16x16x32_bf16 product of the integer operands
- * A[i][k] = ((3 i + 5 k) mod 11) - 5, B[k][j] = ((7 k + 2 j) mod 13) - 6
- * (i, j < 16, k < 32; exact in bf16 and FP32), written as the 16 x 16
- * row-major D the split-precision kernels' operand and accumulator layout
- * assumes. */
-int pdfxc_selftest_mfma_split(float* out16x16);
+/* Diagnostics: the integer operands A[i][k] = ((3 i + 5 k) mod 11) - 5,
+ * B[k][j] = ((7 k + 2 j) mod 13) - 6 (exact in bf16 and FP32) multiplied by
+ * v_mfma_f32_16x16x32_bf16 (k < 32; out[0..255]) and by
+ * v_mfma_f32_16x16x16_bf16 (k < 16; out[256..511]), each written as the
+ * 16 x 16 row-major D the split-precision kernels' operand and accumulator
+ * layouts assume. */
+int pdfxc_selftest_mfma_split(float* out2x16x16);
 
 /* Diagnostics: a plain streaming copy of n doubles between device buffers
  * (reads 8n bytes, writes 8n bytes), the reference point for HBM traffic

@@ -298,7 +298,6 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
                 for (int m = 0; m < 2 * c->shells[k].l + 1; ++m) c->split_perm.push_back(c->shells[k].first_ao + m);
         }
         rc = dalloc(&c->d_Ps, (size_t)S * n_aos * c->split_nc);
-        if (rc == PDFXC_OK) rc = dalloc(&c->d_M, (size_t)n_aos * n_aos);
         if (rc == PDFXC_OK) rc = dalloc(&c->d_perm, (size_t)n_aos);
         if (rc == PDFXC_OK) {
             e = hipMemcpy(c->d_perm, c->split_perm.data(), sizeof(int) * n_aos, hipMemcpyHostToDevice);
@@ -352,7 +351,6 @@ int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
     dfree(c->d_K);
     dfree(c->d_E);
     dfree(c->d_Ps);
-    dfree(c->d_M);
     dfree(c->d_perm);
     if (c->h_pinned) (void)hipHostFree(c->h_pinned);
     for (auto& e : c->ev)
@@ -620,10 +618,10 @@ static int build_blocks(Context& c) {
         kall.insert(kall.end(), kw[cls].begin(), kw[cls].end());
     }
 
-    // split precision: chunked column layout, k_rho_split classes (ldnp <= 256:
-    // 64 points per workgroup, <= 512: 32, <= 1024: 16), M window items
+    // split precision: chunked column layout and k_rho_split classes (ldnp <=
+    // 256: 64 points per workgroup, <= 512: 32, <= 1024: 16); k_kmat_split items
     std::vector<int> srho_work;
-    std::vector<SKWork> skw;
+    std::vector<KWork> skw;
     size_t sk_lds = 0;
     c.sblocks.clear();
     c.chunks.clear();
@@ -687,22 +685,39 @@ static int build_blocks(Context& c) {
             r.count = (int)srho_work.size() - r.offset;
             if (r.count) c.srho_launches.push_back(r);
         }
+        // k_kmat_split items: lower 2 x 2 tile patches of the compact columns,
+        // at most 32 per item (8 consumer waves x 4); two slab buffers if they fit
         for (int b = 0; b < nb; ++b) {
-            const int T = c.sblocks[b].ldnp / 16;
-            for (int I0 = 0; I0 < T; I0 += 16)
-                for (int J0 = 0; J0 < T; J0 += 16) {
-                    SKWork w{b, I0, J0, std::min(16, T - I0), std::min(16, T - J0), 0};
-                    const size_t slab = ((size_t)S * 16 * (w.nI + w.nJ) * 32 * 2 + 15) / 16 * 16;
-                    const size_t lds = slab + sizeof(double) * (size_t)c.blocks[b].nshell * RS +
-                                       sizeof(int) * (size_t)(2 * c.blocks[b].nshell + 16 * (w.nI + w.nJ));
-                    if (lds > (size_t)kMaxDynLds) {
-                        set_error("set_grid: a block with " + std::to_string(c.blocks[b].nshell) +
-                                  " shells needs more LDS than a workgroup has in the split K kernel (raise ao_screen)");
-                        return PDFXC_ERR_UNSUPPORTED;
-                    }
-                    sk_lds = std::max(sk_lds, lds);
-                    skw.push_back(w);
+            const BlockDesc& B = c.blocks[b];
+            const int T = B.ldn / 16;
+            const int Tp = (T + 1) / 2;
+            const int np = Tp * (Tp + 1) / 2;
+            for (int q0 = 0; q0 < np; q0 += 32) {
+                KWork w;
+                w.block = b;
+                w.t0 = q0;
+                w.t1 = std::min(np, q0 + 32);
+                int P = 0;
+                while ((P + 1) * (P + 2) / 2 <= w.t1 - 1) ++P;
+                w.ncols = 32 * (P + 1);
+                int nsh = 0;
+                while (nsh < B.nshell && c.block_shell_col[B.shell_begin + nsh] < w.ncols) ++nsh;
+                w.nsh = nsh;
+                size_t lds = 0;
+                for (int nbuf = 2; nbuf >= 1; --nbuf) {
+                    w.ps = nbuf;
+                    lds = ((size_t)nbuf * 2 * S * w.ncols * 16 * 2 + 15) / 16 * 16 + sizeof(double) * (size_t)nsh * RS +
+                          sizeof(int) * (size_t)(w.ncols + 2 * nsh);
+                    if (lds <= (size_t)kMaxDynLds) break;
                 }
+                if (lds > (size_t)kMaxDynLds) {
+                    set_error("set_grid: a block with " + std::to_string(B.nao) +
+                              " AOs needs more LDS than a workgroup has in the split K kernel (raise ao_screen)");
+                    return PDFXC_ERR_UNSUPPORTED;
+                }
+                sk_lds = std::max(sk_lds, lds);
+                skw.push_back(w);
+            }
         }
         if (!skw.empty()) c.sk_launches.push_back(LaunchRange{0, 0, (int)skw.size(), sk_lds});
     }
@@ -725,7 +740,7 @@ static int build_blocks(Context& c) {
         if (rc == PDFXC_OK) rc = dalloc(&c.d_chunks, c.chunks.size());
         if (rc == PDFXC_OK) rc = dalloc(&c.d_sshell_col, c.sshell_col.size());
         if (rc == PDFXC_OK) rc = dalloc(&c.d_srho_work, srho_work.size());
-        if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&c.d_sk_work), skw.size() * sizeof(SKWork));
+        if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&c.d_sk_work), skw.size() * sizeof(KWork));
     }
     if (rc != PDFXC_OK) return rc;
     auto up = [](void* d, const void* h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
@@ -744,7 +759,7 @@ static int build_blocks(Context& c) {
         PDFXC_HIP(up(c.d_chunks, c.chunks.data(), c.chunks.size() * sizeof(int)));
         PDFXC_HIP(up(c.d_sshell_col, c.sshell_col.data(), c.sshell_col.size() * sizeof(int)));
         PDFXC_HIP(up(c.d_srho_work, srho_work.data(), srho_work.size() * sizeof(int)));
-        PDFXC_HIP(up(c.d_sk_work, skw.data(), skw.size() * sizeof(SKWork)));
+        PDFXC_HIP(up(c.d_sk_work, skw.data(), skw.size() * sizeof(KWork)));
     }
     PDFXC_HIP(hipMemset(c.d_coef, 0, 4 * (size_t)c.n_slots * sizeof(double)));
     PDFXC_HIP(hipMemset(c.d_rho, 0, 4 * (size_t)c.n_slots * sizeof(double)));
@@ -1428,7 +1443,7 @@ int pdfxc_get_stats(pdfxc_ctx* ctx, pdfxc_stats* s) {
     s->n_points = c->n_pts;
     s->n_blocks = (int64_t)c->blocks.size();
     s->n_chunks = (c->opts.precision != PDFXC_FP64)
-                      ? (int64_t)(c->srho_launches.size() + c->sk_launches.size() + 3)
+                      ? (int64_t)(c->srho_launches.size() + c->sk_launches.size() + 2)
                       : (int64_t)(c->rho_launches.size() + c->kmat_launches.size());
     s->flops = c->flops;
     s->mean_nao = c->mean_nao;
@@ -1452,14 +1467,14 @@ int pdfxc_selftest_mfma(double* out16x16) {
     return rc;
 }
 
-int pdfxc_selftest_mfma_split(float* out16x16) {
-    if (!out16x16) return arg_error("pdfxc_selftest_mfma_split: null output");
+int pdfxc_selftest_mfma_split(float* out2x16x16) {
+    if (!out2x16x16) return arg_error("pdfxc_selftest_mfma_split: null output");
     float* d = nullptr;
-    int rc = dalloc(&d, 256);
+    int rc = dalloc(&d, 512);
     if (rc) return rc;
     rc = launch_mfma_split_selftest(d, nullptr);
     if (rc == PDFXC_OK) {
-        hipError_t e = hipMemcpy(out16x16, d, 256 * sizeof(float), hipMemcpyDeviceToHost);
+        hipError_t e = hipMemcpy(out2x16x16, d, 512 * sizeof(float), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(e, "pdfxc_selftest_mfma_split");
     }
     dfree(d);

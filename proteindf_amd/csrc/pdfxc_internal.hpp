@@ -48,12 +48,8 @@ struct SplitBlock {
     int chunk_begin, nchunk, ldnp, pad;
 };
 
-// One k_mfull_split work item: tiles [I0, I0 + nI) x [J0, J0 + nJ) of M_bb.
-struct SKWork {
-    int block, I0, J0, nI, nJ, pad;
-};
-
-// One k_kmat work item: lower 16 x 16 tiles [t0, t1) of a block.
+// One k_kmat work item: lower 16 x 16 tiles [t0, t1) of a block (k_kmat_split:
+// lower 2 x 2 tile patches [t0, t1), ps = slab buffers).
 struct KWork {
     int block, t0, t1, ncols, nsh, ps;
 };
@@ -130,12 +126,11 @@ struct Context {
     std::vector<int> sshell_col;        // per block shell: local column in the chunk layout
     std::vector<LaunchRange> srho_launches, sk_launches;
     unsigned short* d_Ps = nullptr;     // S x N x split_nc bf16 slices of P in the split AO order
-    double* d_M = nullptr;              // dense N x N: M = Phi^T V
     SplitBlock* d_sblocks = nullptr;
     int* d_chunks = nullptr;
     int* d_sshell_col = nullptr;
     int* d_srho_work = nullptr;
-    void* d_sk_work = nullptr;
+    void* d_sk_work = nullptr;          // KWork items of k_kmat_split
 
     int rho_accumulate = 0;             // update mode: the rho kernels add rho[dP] to d_rho
 
@@ -169,7 +164,7 @@ int launch_mfma_selftest(double* d_out, hipStream_t s);
 // split precision (pdfxc_split.hip)
 int split_points_per_workgroup(int cls);
 int launch_split_rho(Context& c);   // P slices + k_rho_split
-int launch_split_kmat(Context& c);  // M = Phi^T V, K += M + M^T
+int launch_split_kmat(Context& c);  // k_kmat_split
 int launch_mfma_split_selftest(float* d_out, hipStream_t s);
 // XC nuclear gradient (pdfxc_grad.hip); d_gacc: functional term [3 n_atoms] then weight term [3 n_atoms]
 int launch_grad_fn(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,

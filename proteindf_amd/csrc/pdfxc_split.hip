@@ -14,9 +14,9 @@
 //   k_psplit      P (dense FP64 N x N) -> S bf16 slice planes [S][N][Nc], split order
 //   k_rho_split   X = Phi_b P_bb for PB points, then rho, grad rho
 //                 (calcRho_GGA_part, DfCalcGridX.cpp:1923-1962)
-//   k_mfull_split M_bb = Phi_b^T V_b over a block's points, tiles of a 16 x 16
-//                 tile window, FP64 atomics into a dense N x N M
-//   k_ksym        K_lower += M + M^T (build_XC_Matrix GGA, :2455-2500)
+//   k_kmat_split  K_bb += Phi_b^T V_b + V_b^T Phi_b on the block's compact AO
+//                 columns, lower 2 x 2 tile patches, FP64 atomics into the
+//                 lower triangle of K (build_XC_Matrix GGA, :2455-2500)
 #include <hip/hip_runtime.h>
 
 #include "pdfxc_device.hpp"
@@ -312,56 +312,152 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
 }
 
 // ---------------------------------------------------------------------------
-// K pass, split precision: M_bb tiles [I0, I0 + nI) x [J0, J0 + nJ) (nI, nJ
-// <= 16) of M = Phi^T V over the block's points in slabs of 32 (one MFMA K
-// step).  All 512 threads evaluate a slab -- phi of the window's row columns,
-// V = (w vrho / 2) phi + c . grad phi of its column columns -- as bf16
-// slices [s][col][32 points] (the 8-point group g of column c stored at
-// group g ^ ((c >> 2) & 3)), then wave w multiplies its 4 x 8 tiles
-// (rows 4 (w >> 1) .., columns 8 (w & 1) ..).  FP64 atomics put the tiles
-// into the dense M; k_ksym folds M + M^T into the lower triangle of K.
+// K pass, split precision: K_bb += Phi^T V + V^T Phi for the lower 2 x 2
+// patches [q0, q1) of 16 x 16 tiles of one block (q = P (P + 1) / 2 + Q
+// covers tile rows 2P, 2P + 1 and columns 2Q, 2Q + 1; a diagonal patch's
+// upper tile is computed and not scattered), on the block's compact AO
+// columns.  Twelve waves: 4 producers evaluate phi and V = (w vrho / 2) phi
+// + c . grad phi in FP64 for a 16-point slab and store their bf16 slices as
+// [s][column][16 points] (the 4-point group g of column c at
+// g ^ ((c >> 3) & 3)); 8 consumers hold up to NPW patches each in FP32
+// accumulators and run v_mfma_f32_16x16x16_bf16 on the slab (one K step),
+// the slice products with a + b < S.  Two slab buffers when they fit in
+// LDS, else one.  FP64 atomics into the dense lower triangle of K.
 // ---------------------------------------------------------------------------
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x16_bf16 (checked by pdfxc_selftest_mfma_split): lane l
+// holds A[l % 16][4 (l / 16) + e], B[4 (l / 16) + e][l % 16], e = 0..3; the
+// result as for the 16x16x32 form.
+__device__ inline f32x4 mfma_bf16_k16(s16x4 a, s16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ inline int kslab_at(int c, int g) { return c * 16 + ((g ^ (c >> 3)) & 3) * 4; }
+
+template <int S, int NPW>
+__device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, int ncols, int lane,
+                                          const unsigned short* s_slab, const int* s_cols,
+                                          double* __restrict__ K, int N) {
+    const int li = lane & 15, lq = lane >> 4;
+    const int BUF = 2 * S * ncols * 16;  // bf16 per buffer: [phi, V][s][col][16]
+    int offI[NPW], offJ[NPW];
+    {
+        int P = 0;
+        while ((P + 1) * (P + 2) / 2 <= pb) ++P;
+        int Q = pb - P * (P + 1) / 2;
+#pragma unroll
+        for (int m = 0; m < NPW; ++m) {
+            offI[m] = P * 32;
+            offJ[m] = Q * 32;
+            if (pb + m + 1 < pe) {
+                if (Q == P) {
+                    ++P;
+                    Q = 0;
+                } else {
+                    ++Q;
+                }
+            }
+        }
+    }
+    f32x4 acc[NPW][4];
+#pragma unroll
+    for (int m = 0; m < NPW; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    __syncthreads();  // the first slab is in
+    for (int sl = 0; sl < nslab; ++sl) {
+        const unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
+        const unsigned short* vv = phi + S * ncols * 16;
+#pragma unroll
+        for (int m = 0; m < NPW; ++m) {
+            s16x4 fI[2][S], vI[2][S], fJ[2][S], vJ[2][S];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const size_t so = (size_t)s * ncols * 16;
+                    fI[h][s] = *reinterpret_cast<const s16x4*>(phi + so + kslab_at(offI[m] + 16 * h + li, lq));
+                    vI[h][s] = *reinterpret_cast<const s16x4*>(vv + so + kslab_at(offI[m] + 16 * h + li, lq));
+                    fJ[h][s] = *reinterpret_cast<const s16x4*>(phi + so + kslab_at(offJ[m] + 16 * h + li, lq));
+                    vJ[h][s] = *reinterpret_cast<const s16x4*>(vv + so + kslab_at(offJ[m] + 16 * h + li, lq));
+                }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int hi = t >> 1, hj = t & 1;
+#pragma unroll
+                for (int q = 0; q < SlicePairs<S>::n; ++q) {
+                    const int sa = SlicePairs<S>::a[q], sb = SlicePairs<S>::b[q];
+                    acc[m][t] = mfma_bf16_k16(fI[hi][sa], vJ[hj][sb], acc[m][t]);
+                    acc[m][t] = mfma_bf16_k16(vI[hi][sa], fJ[hj][sb], acc[m][t]);
+                }
+            }
+        }
+        __syncthreads();                 // slab consumed
+        if (nbuf == 1) __syncthreads();  // single buffer: the next slab is in
+    }
+    // scatter: register r of lane l is tile element (4 (l / 16) + r, l % 16)
+#pragma unroll
+    for (int m = 0; m < NPW; ++m) {
+        if (pb + m < pe) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int lc = offJ[m] + 16 * (t & 1) + li;
+                const int gj = s_cols[lc];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int lr = offI[m] + 16 * (t >> 1) + 4 * lq + r;
+                    const int gi = s_cols[lr];
+                    if (gi >= 0 && gj >= 0 && lr >= lc) atomicAdd(K + (int64_t)gi * N + gj, (double)acc[m][t][r]);
+                }
+            }
+        }
+    }
+}
+
 template <int S>
-__global__ __launch_bounds__(512) void k_mfull_split(const SKWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
-                                                     const SplitBlock* __restrict__ sblocks,
-                                                     const double* __restrict__ pts, int64_t n_slots,
-                                                     const double* __restrict__ shrec, int RS,
-                                                     const int* __restrict__ bshells, const int* __restrict__ bsorder,
-                                                     const int* __restrict__ sshell_col, const int* __restrict__ chunks,
-                                                     const int* __restrict__ perm, const double* __restrict__ coef,
-                                                     double* __restrict__ M, int N) {
+__global__ __launch_bounds__(768) void k_kmat_split(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+                                                    const double* __restrict__ pts, int64_t n_slots,
+                                                    const double* __restrict__ shrec, int RS,
+                                                    const int* __restrict__ bshells, const int* __restrict__ bshell_col,
+                                                    const int* __restrict__ bcols, const int* __restrict__ bsorder,
+                                                    const double* __restrict__ coef, double* __restrict__ K, int N) {
     __shared__ double s_pt[kBlockPts][3];
     __shared__ double s_cf[kBlockPts][4];
     extern __shared__ __attribute__((aligned(16))) double s_dyn[];
-    const SKWork W = work[blockIdx.x];
+    const KWork W = work[blockIdx.x];
     const BlockDesc B = blocks[W.block];
-    const SplitBlock SB = sblocks[W.block];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, li = lane & 15, lq = lane >> 4;
+    const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int nsh = B.nshell;
-    const int ncI = 16 * W.nI, ncJ = 16 * W.nJ;
-    const int cI0 = 16 * W.I0, cJ0 = 16 * W.J0;
-    // LDS: phi slices [S][ncI][32] | V slices [S][ncJ][32] (bf16) | records | shcol | sorder | gI [ncI] | gJ [ncJ]
-    unsigned short* s_phi = reinterpret_cast<unsigned short*>(s_dyn);
-    unsigned short* s_v = s_phi + (size_t)S * ncI * 32;
-    const int slab_dbl = (S * (ncI + ncJ) * 32 * 2 + 15) / 16 * 2;
+    const int ncols = W.ncols;  // multiple of 32
+    const int nsh = W.nsh;
+    const int nbuf = W.ps;      // slab buffers: 1 or 2
+    const int BUF = 2 * S * ncols * 16;
+    // LDS: slab buffers (bf16) | records | cols [ncols] | shcol [nsh] | shid [nsh]
+    unsigned short* s_slab = reinterpret_cast<unsigned short*>(s_dyn);
+    const int slab_dbl = (nbuf * BUF * 2 + 15) / 16 * 2;
     double* s_rec = s_dyn + slab_dbl;
-    int* s_shcol = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
-    int* s_sord = s_shcol + nsh;
-    int* s_gI = s_sord + nsh;
-    int* s_gJ = s_gI + ncI;
-    for (int k = tid; k < nsh; k += 512) {
-        s_shcol[k] = sshell_col[B.shell_begin + k];
-        s_sord[k] = bsorder[B.shell_begin + k];
+    int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
+    int* s_shcol = s_cols + ncols;
+    int* s_shid = s_shcol + nsh;
+    for (int c = tid; c < ncols; c += 768) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
+    if (wave == 8) {  // shells this item needs, in (l, nprim) order
+        int base = 0;
+        for (int i0 = 0; i0 < B.nshell; i0 += 64) {
+            const int i = i0 + lane;
+            const int sh = (i < B.nshell) ? bsorder[B.shell_begin + i] : B.nshell;
+            const bool keep = sh < nsh;
+            const unsigned long long msk = __ballot(keep);
+            const int pos = base + __popcll(msk & ((1ull << lane) - 1ull));
+            if (keep) {
+                s_shid[pos] = bshells[B.shell_begin + sh];
+                s_shcol[pos] = bshell_col[B.shell_begin + sh];
+            }
+            base += __popcll(msk);
+        }
     }
-    for (int c = tid; c < ncI; c += 512) s_gI[c] = -1;
-    for (int c = tid; c < ncJ; c += 512) s_gJ[c] = -1;
-    for (int idx = tid; idx < nsh * RS; idx += 512) {
-        const int k = idx / RS;
-        s_rec[idx] = shrec[(int64_t)bshells[B.shell_begin + k] * RS + (idx - k * RS)];
-    }
-    for (int q = tid; q < kBlockPts; q += 512) {
+    for (int q = tid; q < kBlockPts; q += 768) {
         const int64_t slot = B.pt_begin + q;
         s_pt[q][0] = pts[slot];
         s_pt[q][1] = pts[n_slots + slot];
@@ -372,122 +468,84 @@ __global__ __launch_bounds__(512) void k_mfull_split(const SKWork* __restrict__ 
         s_cf[q][3] = coef[4 * slot + 3];
     }
     {
-        uint4* z = reinterpret_cast<uint4*>(s_dyn);
-        for (int i = tid; i < slab_dbl / 2; i += 512) z[i] = uint4{0u, 0u, 0u, 0u};
+        uint4* z = reinterpret_cast<uint4*>(s_dyn);  // columns past the AOs stay zero
+        for (int i = tid; i < slab_dbl / 2; i += 768) z[i] = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    // global AO of every kept column of the window (dropped AOs stay -1)
-    for (int k = tid; k < nsh; k += 512) {
-        const int c0 = s_shcol[k];
-        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-        for (int m = 0; m < nc; ++m) {
-            const int c = c0 + m;
-            const int g = perm[chunks[SB.chunk_begin + (c >> 2)] * 4 + (c & 3)];  // caller AO index
-            if (c >= cI0 && c < cI0 + ncI) s_gI[c - cI0] = g;
-            if (c >= cJ0 && c < cJ0 + ncJ) s_gJ[c - cJ0] = g;
-        }
+    for (int idx = tid; idx < nsh * RS; idx += 768) {
+        const int k = idx / RS;
+        s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
     }
+    __syncthreads();
 
-    const int rg = wave >> 1, cg = wave & 1;
-    f32x4 acc[4][8];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int nslab = (B.npts + 31) / 32;
-    for (int sl = 0; sl < nslab; ++sl) {
-        // evaluate: 32 lanes = the 32 points of one shell
-        for (int it = tid; it < nsh * 32; it += 512) {
-            const int q = it & 31;
-            const int k = s_sord[it >> 5];
-            const int c0 = s_shcol[k];
-            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-            const bool inI = c0 < cI0 + ncI && c0 + nc > cI0;
-            const bool inJ = c0 < cJ0 + ncJ && c0 + nc > cJ0;
-            if (!inI && !inJ) continue;
-            const int pp = sl * 32 + q;
-            double v[5], vx[5], vy[5], vz[5];
-            rec_values(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], v, vx, vy, vz);
-            const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
-            for (int m = 0; m < nc; ++m) {
-                const int c = c0 + m;
-                if (c >= cI0 && c < cI0 + ncI) {
-                    const int lc = c - cI0;
+    const int nslab = (B.npts + 15) / 16;
+    if (wave >= 8) {
+        const int ptid = tid - 512;
+        auto produce = [&](int sl) {
+            unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
+            unsigned short* vv = phi + S * ncols * 16;
+            for (int it = ptid; it < nsh * 16; it += 256) {
+                const int p = it & 15;
+                const int k = it >> 4;
+                const int pp = sl * 16 + p;
+                // radial sums once, then phi and grad phi per component from the
+                // angular polynomial (keeps the producer within its registers)
+                double r3[3], R0, R1, R2;
+                const int l = rec_radial2(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], r3, R0, R1, R2);
+                const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
+                const int c0 = s_shcol[k];
+                for (int m = 0; m < 2 * l + 1; ++m) {
+                    const int c = c0 + m;
+                    if (c >= ncols) break;
+                    double f, fg[3], fh[6];
+                    angular2(l, m, r3, f, fg, fh);
+                    const double v = f * R0;
+                    const double vx = fg[0] * R0 + f * r3[0] * R1;
+                    const double vy = fg[1] * R0 + f * r3[1] * R1;
+                    const double vz = fg[2] * R0 + f * r3[2] * R1;
+                    const int off = kslab_at(c, p >> 2) + (p & 3);
                     unsigned short h[S];
-                    split_bf16<S>(v[m], h);
-                    const int off = lc * 32 + ((((q >> 3) ^ (lc >> 2)) & 3) << 3) + (q & 7);
+                    split_bf16<S>(v, h);
 #pragma unroll
-                    for (int s = 0; s < S; ++s) s_phi[(size_t)s * ncI * 32 + off] = h[s];
-                }
-                if (c >= cJ0 && c < cJ0 + ncJ) {
-                    const int lc = c - cJ0;
-                    unsigned short h[S];
-                    split_bf16<S>(a * v[m] + cx * vx[m] + cy * vy[m] + cz * vz[m], h);
-                    const int off = lc * 32 + ((((q >> 3) ^ (lc >> 2)) & 3) << 3) + (q & 7);
+                    for (int s = 0; s < S; ++s) phi[(size_t)s * ncols * 16 + off] = h[s];
+                    split_bf16<S>(a * v + cx * vx + cy * vy + cz * vz, h);
 #pragma unroll
-                    for (int s = 0; s < S; ++s) s_v[(size_t)s * ncJ * 32 + off] = h[s];
+                    for (int s = 0; s < S; ++s) vv[(size_t)s * ncols * 16 + off] = h[s];
                 }
             }
-        }
+        };
+        // same barrier sequence as the consumers
+        produce(0);
         __syncthreads();
-        // multiply: A[i][k] = phi at point k of row column i, B[k][j] = V at point k of column j;
-        // row-outer order keeps one row's A slices and one column's B slices live
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int I = 4 * rg + i;
-            if (I < W.nI) {
-                const int lci = I * 16 + li;
-                const int offa = lci * 32 + (((lq ^ (lci >> 2)) & 3) << 3);
-                bf16x8 afr[S];
-#pragma unroll
-                for (int s = 0; s < S; ++s) afr[s] = *reinterpret_cast<const bf16x8*>(s_phi + (size_t)s * ncI * 32 + offa);
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int J = 8 * cg + j;
-                    if (J < W.nJ) {
-                        const int lc = J * 16 + li;
-                        const int off = lc * 32 + (((lq ^ (lc >> 2)) & 3) << 3);
-                        bf16x8 bfr[S];
-#pragma unroll
-                        for (int s = 0; s < S; ++s)
-                            bfr[s] = *reinterpret_cast<const bf16x8*>(s_v + (size_t)s * ncJ * 32 + off);
-#pragma unroll
-                        for (int qq = 0; qq < SlicePairs<S>::n; ++qq)
-                            acc[i][j] = mfma_bf16(afr[SlicePairs<S>::a[qq]], bfr[SlicePairs<S>::b[qq]], acc[i][j]);
-                    }
-                }
+        for (int sl = 0; sl < nslab; ++sl) {
+            if (nbuf == 2) {
+                if (sl + 1 < nslab) produce(sl + 1);
+                __syncthreads();
+            } else {
+                __syncthreads();  // the consumers are done with slab sl
+                if (sl + 1 < nslab) produce(sl + 1);
+                __syncthreads();
             }
         }
-        __syncthreads();
+        return;
     }
-    // scatter: register r of lane l is M[row 4 (l / 16) + r][column l % 16] of the tile
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int I = 4 * rg + i;
-        if (I >= W.nI) continue;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int J = 8 * cg + j;
-            if (J >= W.nJ) continue;
-            const int gj = s_gJ[J * 16 + li];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int gi = s_gI[I * 16 + 4 * lq + r];
-                if (gi >= 0 && gj >= 0) atomicAdd(M + (int64_t)gi * N + gj, (double)acc[i][j][r]);
-            }
-        }
+    const int q8 = (W.t1 - W.t0 + 7) >> 3;
+    const int pstart = W.t0 + wave * q8;
+    const int pend = min(W.t1, pstart + q8);
+    const bool has = pstart < pend;
+    const int pb = has ? pstart : W.t1 - 1;
+    const int pe = has ? pend : pb;
+    switch (has ? pend - pstart : 1) {
+        case 1: kmat_split_consume<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
+        case 2: kmat_split_consume<S, 2>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
+        case 3: kmat_split_consume<S, 3>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
+        default: kmat_split_consume<S, 4>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
     }
 }
 
-// K[i][j] += M[i][j] + M[j][i] for j <= i
-__global__ __launch_bounds__(256) void k_ksym(const double* __restrict__ M, int N, double* __restrict__ K) {
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (int64_t)N * N) return;
-    const int i = (int)(t / N), j = (int)(t % N);
-    if (j <= i) K[t] += M[t] + M[(int64_t)j * N + i];
-}
-
-// layout self test: A(i, k) = ((3 i + 5 k) % 11) - 5, B(k, j) = ((7 k + 2 j) % 13) - 6
+// layout self test: A(i, k) = ((3 i + 5 k) % 11) - 5, B(k, j) = ((7 k + 2 j) % 13) - 6;
+// out[0..255] from v_mfma_f32_16x16x32_bf16 (k < 32), out[256..511] from
+// v_mfma_f32_16x16x16_bf16 (k < 16)
 __global__ void k_mfma_split_selftest(float* out) {
     const int l = threadIdx.x;
     const int li = l & 15, lq = l >> 4;
@@ -501,6 +559,16 @@ __global__ void k_mfma_split_selftest(float* out) {
     const f32x4 d = mfma_bf16(a, b, f32x4{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[(4 * lq + r) * 16 + li] = d[r];
+    s16x4 a4, b4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int k = 4 * lq + e;
+        a4[e] = (short)bf16_bits((__bf16)(float)(((3 * li + 5 * k) % 11) - 5));
+        b4[e] = (short)bf16_bits((__bf16)(float)(((7 * k + 2 * li) % 13) - 6));
+    }
+    const f32x4 d4 = mfma_bf16_k16(a4, b4, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[256 + (4 * lq + r) * 16 + li] = d4[r];
 }
 
 // ---------------------------------------------------------------------------
@@ -533,12 +601,12 @@ static int rho_split_one(Context& c, const LaunchRange& r) {
 }
 
 template <int S>
-static int mfull_one(Context& c, const LaunchRange& r) {
-    hipLaunchKernelGGL((k_mfull_split<S>), dim3(r.count), dim3(512), r.lds, c.stream,
-                       reinterpret_cast<const SKWork*>(c.d_sk_work) + r.offset, c.d_blocks, c.d_sblocks, c.d_pts,
-                       c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_sorder, c.d_sshell_col,
-                       c.d_chunks, c.d_perm, c.d_coef, c.d_M, c.n_aos);
-    return check("k_mfull_split");
+static int kmat_split_one(Context& c, const LaunchRange& r) {
+    hipLaunchKernelGGL((k_kmat_split<S>), dim3(r.count), dim3(768), r.lds, c.stream,
+                       reinterpret_cast<const KWork*>(c.d_sk_work) + r.offset, c.d_blocks, c.d_pts, c.n_slots,
+                       c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col, c.d_block_cols,
+                       c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
+    return check("k_kmat_split");
 }
 
 template <int S>
@@ -565,17 +633,13 @@ int launch_split_rho(Context& c) {
 }
 
 int launch_split_kmat(Context& c) {
-    PDFXC_HIP(hipMemsetAsync(c.d_M, 0, sizeof(double) * (size_t)c.n_aos * c.n_aos, c.stream));
     int rc = PDFXC_OK;
     for (const LaunchRange& r : c.sk_launches) {
         if (rc != PDFXC_OK) break;
         if (r.count <= 0) continue;
-        rc = (c.opts.precision == PDFXC_SPLIT_BF16X3) ? mfull_one<3>(c, r) : mfull_one<2>(c, r);
+        rc = (c.opts.precision == PDFXC_SPLIT_BF16X3) ? kmat_split_one<3>(c, r) : kmat_split_one<2>(c, r);
     }
-    if (rc != PDFXC_OK) return rc;
-    const int64_t n = (int64_t)c.n_aos * c.n_aos;
-    hipLaunchKernelGGL(k_ksym, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.d_M, c.n_aos, c.d_K);
-    return check("k_ksym");
+    return rc;
 }
 
 int launch_mfma_split_selftest(float* d_out, hipStream_t s) {

@@ -127,7 +127,8 @@ def selftest_mfma():
 
 
 def selftest_mfma_split():
-    out = np.zeros((16, 16), dtype=np.float32)
+    """(2, 16, 16): the 16x16x32 and the 16x16x16 bf16 products."""
+    out = np.zeros((2, 16, 16), dtype=np.float32)
     check(lib().pdfxc_selftest_mfma_split(_p(out)))
     return out
 

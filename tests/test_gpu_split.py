@@ -25,13 +25,14 @@ def say(*a):
 
 def test_mfma_split_operand_and_accumulator_layout():
     d = engine.selftest_mfma_split()
-    i = np.arange(16)[:, None]
-    k = np.arange(32)[None, :]
-    A = ((3 * i + 5 * k) % 11) - 5
-    kk = np.arange(32)[:, None]
-    j = np.arange(16)[None, :]
-    Bm = ((7 * kk + 2 * j) % 13) - 6
-    np.testing.assert_array_equal(d, (A @ Bm).astype(np.float32))
+    for n, (K, D) in enumerate(((32, d[0]), (16, d[1]))):
+        i = np.arange(16)[:, None]
+        k = np.arange(K)[None, :]
+        A = ((3 * i + 5 * k) % 11) - 5
+        kk = np.arange(K)[:, None]
+        j = np.arange(16)[None, :]
+        Bm = ((7 * kk + 2 * j) % 13) - 6
+        np.testing.assert_array_equal(D, (A @ Bm).astype(np.float32), err_msg=f"16x16x{K}")
 
 
 def _build(info, grid, PA, precision, ao_screen=0.0):
