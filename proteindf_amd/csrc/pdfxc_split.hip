@@ -316,7 +316,7 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
 // patches [q0, q1) of 16 x 16 tiles of one block (q = P (P + 1) / 2 + Q
 // covers tile rows 2P, 2P + 1 and columns 2Q, 2Q + 1; a diagonal patch's
 // upper tile is computed and not scattered), on the block's compact AO
-// columns.  Twelve waves: 4 producers evaluate phi and V = (w vrho / 2) phi
+// columns.  Sixteen waves: 8 producers evaluate phi and V = (w vrho / 2) phi
 // + c . grad phi in FP64 for a 16-point slab and store their bf16 slices as
 // [s][column][16 points] (the 4-point group g of column c at
 // g ^ ((c >> 3) & 3)); 8 consumers hold up to NPW patches each in FP32
@@ -416,7 +416,7 @@ __device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, i
 }
 
 template <int S>
-__global__ __launch_bounds__(768) void k_kmat_split(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                                     const double* __restrict__ pts, int64_t n_slots,
                                                     const double* __restrict__ shrec, int RS,
                                                     const int* __restrict__ bshells, const int* __restrict__ bshell_col,
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(768) void k_kmat_split(const KWork* __restrict__ wo
     int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
     int* s_shcol = s_cols + ncols;
     int* s_shid = s_shcol + nsh;
-    for (int c = tid; c < ncols; c += 768) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
+    for (int c = tid; c < ncols; c += 1024) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
     if (wave == 8) {  // shells this item needs, in (l, nprim) order
         int base = 0;
         for (int i0 = 0; i0 < B.nshell; i0 += 64) {
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(768) void k_kmat_split(const KWork* __restrict__ wo
             base += __popcll(msk);
         }
     }
-    for (int q = tid; q < kBlockPts; q += 768) {
+    for (int q = tid; q < kBlockPts; q += 1024) {
         const int64_t slot = B.pt_begin + q;
         s_pt[q][0] = pts[slot];
         s_pt[q][1] = pts[n_slots + slot];
@@ -469,10 +469,10 @@ __global__ __launch_bounds__(768) void k_kmat_split(const KWork* __restrict__ wo
     }
     {
         uint4* z = reinterpret_cast<uint4*>(s_dyn);  // columns past the AOs stay zero
-        for (int i = tid; i < slab_dbl / 2; i += 768) z[i] = uint4{0u, 0u, 0u, 0u};
+        for (int i = tid; i < slab_dbl / 2; i += 1024) z[i] = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    for (int idx = tid; idx < nsh * RS; idx += 768) {
+    for (int idx = tid; idx < nsh * RS; idx += 1024) {
         const int k = idx / RS;
         s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
     }
@@ -480,11 +480,11 @@ __global__ __launch_bounds__(768) void k_kmat_split(const KWork* __restrict__ wo
 
     const int nslab = (B.npts + 15) / 16;
     if (wave >= 8) {
-        const int ptid = tid - 512;
+        const int ptid = tid - 512;  // 8 producer waves, two per SIMD
         auto produce = [&](int sl) {
             unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
             unsigned short* vv = phi + S * ncols * 16;
-            for (int it = ptid; it < nsh * 16; it += 256) {
+            for (int it = ptid; it < nsh * 16; it += 512) {
                 const int p = it & 15;
                 const int k = it >> 4;
                 const int pp = sl * 16 + p;
@@ -602,7 +602,7 @@ static int rho_split_one(Context& c, const LaunchRange& r) {
 
 template <int S>
 static int kmat_split_one(Context& c, const LaunchRange& r) {
-    hipLaunchKernelGGL((k_kmat_split<S>), dim3(r.count), dim3(768), r.lds, c.stream,
+    hipLaunchKernelGGL((k_kmat_split<S>), dim3(r.count), dim3(1024), r.lds, c.stream,
                        reinterpret_cast<const KWork*>(c.d_sk_work) + r.offset, c.d_blocks, c.d_pts, c.n_slots,
                        c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col, c.d_block_cols,
                        c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
