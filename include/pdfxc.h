@@ -37,8 +37,20 @@ enum {
     PDFXC_ERR_MEMORY = 5       /* device or host allocation failed */
 };
 
-/* functionals (DfXCFunctional.h:145-171 dispatch; B3LYP first) */
-enum { PDFXC_B3LYP = 1 };
+/* functionals: the XC_TYPE values of DfXCFunctional.h:44, dispatched as
+ * getFxc does (DfXCFunctional.h:140-180).  HF has no grid term: a build
+ * returns E_xc = 0 and leaves K (and rho_out) untouched.  HFS and SVWN are
+ * LDA (the reference's calcRho_LDA / buildVxc(LDA) path, DfCalcGridX.cpp
+ * :1816-1890, :2056-2091, :2350-2368: K = sum w v_rho phi phi); HFB, BLYP
+ * and B3LYP are GGA.  rho_out carries the gradient columns for all of them. */
+enum {
+    PDFXC_HF = 1,    /* no XC term */
+    PDFXC_HFS = 2,   /* Slater (DfFunctional_HFS.cpp) */
+    PDFXC_SVWN = 3,  /* Slater + VWN5 (DfFunctional_SVWN.cpp) */
+    PDFXC_HFB = 4,   /* Becke 88 (DfFunctional_Becke88.cpp) */
+    PDFXC_BLYP = 5,  /* Becke 88 + LYP (DfFunctional_B88LYP.cpp) */
+    PDFXC_B3LYP = 6  /* DfFunctional_B3LYP.cpp */
+};
 
 /* arithmetic of the two contractions X = Phi.P and K = Phi^T.V:
  * FP64 MFMA, or bf16 MFMA with FP32 accumulation on operands split into
@@ -106,7 +118,7 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
  * columns, n_pts each). */
 int pdfxc_get_grid(pdfxc_ctx* ctx, double* x, double* y, double* z, double* w);
 
-/* calcXCIntegForFockAndEnergy, RKS GGA (DfCalcGridX.cpp:2305-2315):
+/* calcXCIntegForFockAndEnergy, RKS (DfCalcGridX.cpp:2280-2289 LDA, :2305-2315 GGA):
  * P_alpha_packed: N(N+1)/2 doubles, row r, column c <= r (RLHD order).
  * K_alpha_packed: same layout, ACCUMULATED into (the caller zeroes it).
  * E_xc: total XC energy.  rho_out (may be NULL): n_pts x 4 (rho_alpha,
@@ -133,7 +145,7 @@ int pdfxc_build_rks_update(pdfxc_ctx* ctx, int functional, const double* dP_alph
 int pdfxc_build_rks_update_device(pdfxc_ctx* ctx, int functional, const double* d_dP, double* dK,
                                   double* dE);
 
-/* XC nuclear gradient, RKS GGA (DfCalcGridX::energyGradient,
+/* XC nuclear gradient, RKS (DfCalcGridX::energyGradient,
  * DfCalcGridX.cpp:1215-1415), on the context's grid with its points moving
  * rigidly with their atoms: point_atom[p] (caller point order) gives each
  * point's atom, or NULL when pdfxc_generate_grid made the grid (its own
@@ -151,7 +163,8 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
 /* Functional kernel over arbitrary closed-shell points, as the grid path
  * calls it (DfFunctional.h:190-192, :221-232: rhoB = rhoA, gAB = gBB = gAA).
  * in: n x 2 (rho_alpha, gamma_alpha_alpha); out: n x 4 (f, df/drhoA,
- * df/dgammaAA, df/dgammaAB).  Host buffers; runs on the current device. */
+ * df/dgammaAA, df/dgammaAB; both zero for HFS and SVWN, all four for HF).
+ * Host buffers; runs on the current device. */
 int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* out);
 
 /* AO values and gradients of all AOs at arbitrary points (getAOs / getDAOs,

@@ -15,8 +15,11 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "liboracle.so"
 MAX_PRIM = 16
 
-SLATER, BECKE88, BECKE88_EXCEPT_LDA, VWN5, VWN3, LYP, B3LYP = range(7)
+SLATER, BECKE88, BECKE88_EXCEPT_LDA, VWN5, VWN3, LYP, B3LYP, HFS, SVWN, BLYP = range(10)
 FUNCTIONAL_IDS = {
+    "DfFunctional_HFS": HFS,
+    "DfFunctional_SVWN": SVWN,
+    "DfFunctional_B88LYP": BLYP,
     "DfFunctional_Slater": SLATER,
     "DfFunctional_Becke88": BECKE88,
     "DfFunctional_Becke88_ExceptLDA": BECKE88_EXCEPT_LDA,

@@ -403,6 +403,25 @@ static void functional_point(int id, const double* x, double* o) {
         case ORACLE_B3LYP:
             b3lyp(ra, rb, gaa, gab, gbb, o);
             break;
+        case ORACLE_HFS: /* DfFunctional_HFS.cpp: Slater alone */
+            o[0] = slater_e(ra, rb);
+            slater_d(ra, rb, &o[1], &o[2]);
+            break;
+        case ORACLE_SVWN: { /* DfFunctional_SVWN.cpp: ex + ec */
+            double xa, xb, ca, cb;
+            slater_d(ra, rb, &xa, &xb);
+            vwn_d(ra, rb, 0, &ca, &cb);
+            o[0] = slater_e(ra, rb) + vwn_e(ra, rb, 0);
+            o[1] = xa + ca;
+            o[2] = xb + cb;
+        } break;
+        case ORACLE_BLYP: { /* DfFunctional_B88LYP.cpp: ex + ec term by term */
+            double x[5], c[5];
+            b88_d(ra, rb, gaa, gbb, 1, &x[0], &x[1], &x[2], &x[3], &x[4]);
+            lyp_d(ra, rb, gaa, gab, gbb, &c[0], &c[1], &c[2], &c[3], &c[4]);
+            o[0] = b88_e(ra, rb, gaa, gbb, 1) + lyp_e(ra, rb, gaa, gab, gbb);
+            for (int k = 0; k < 5; ++k) o[1 + k] = x[k] + c[k];
+        } break;
         default:
             break;
     }

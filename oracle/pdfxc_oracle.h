@@ -38,7 +38,10 @@ enum {
     ORACLE_VWN5 = 3,
     ORACLE_VWN3 = 4,
     ORACLE_LYP = 5,
-    ORACLE_B3LYP = 6
+    ORACLE_B3LYP = 6,
+    ORACLE_HFS = 7,  /* Slater (DfFunctional_HFS.cpp) */
+    ORACLE_SVWN = 8, /* Slater + VWN5 (DfFunctional_SVWN.cpp) */
+    ORACLE_BLYP = 9  /* Becke88 + LYP (DfFunctional_B88LYP.cpp) */
 };
 
 /* in: n x 5 (rhoA, rhoB, gAA, gAB, gBB); out: n x 6
@@ -71,7 +74,8 @@ void oracle_eval_aos(const oracle_aos* aos, int64_t n_pts, const double* xyz,
  * :2455-2500).  grid: n_pts x 4 (x, y, z, w).  PA: dense N x N alpha density
  * (symmetric).  FA: dense N x N, the lower triangle is ACCUMULATED (upper
  * left untouched).  rho_out (optional): n_pts x 4 (rho, drho/dx, dy, dz).
- * Returns E_xc.  functional: ORACLE_B3LYP (GGA ids only). n_threads<=0: all. */
+ * Returns E_xc.  functional: any id (LDA ids give zero gamma derivatives, so
+ * the GGA loop reduces to the LDA one). n_threads<=0: all. */
 double oracle_calc_xc_rks(const oracle_aos* aos, int functional, int64_t n_pts,
                           const double* grid, const double* PA, double* FA,
                           double* rho_out, double density_cutoff,

@@ -3,17 +3,21 @@
 Names, argument meaning and error behaviour follow the reference classes the
 engine replaces, so that callers (and tests) read like the reference's own:
 
-* ``DfFunctional_B3LYP`` -- getFunctional / getDerivativeFunctional for
-  closed-shell points as the grid path calls them (DfFunctional.h:190-192,
-  :221-232), evaluated by the engine's functional kernel.
+* ``DfFunctional_HFS``, ``DfFunctional_SVWN``, ``DfFunctional_Becke88``,
+  ``DfFunctional_B88LYP``, ``DfFunctional_B3LYP`` -- getFunctional /
+  getDerivativeFunctional for closed-shell points as the grid path calls
+  them (DfFunctional.h:104-106, :126-130, :190-192, :221-232), evaluated by
+  the engine's functional kernel.
 * ``DfGenerateGrid`` -- the molecular grid (DfGenerateGrid.cpp:39-250,
   :730-768), generated on the device.
 * ``DfCalcGridX`` -- calcXCIntegForFockAndEnergy(P_A, functional, F_A)
   (DfCalcGridX.cpp:2305-2315): P_A and F_A in packed lower-triangle order,
   F_A accumulated, E_xc returned; getWholeDensity (:2502-2537).
-* ``DfXCFunctional`` -- buildXcMatrix for RKS (DfXCFunctional.cpp:128-152):
-  P_A = P/2, a zeroed F_xc, the B3LYP build, the electron-count check
-  (checkGridAccuracy, :227-248).
+* ``DfXCFunctional`` -- the functional chosen by name as its constructor
+  does (DfXCFunctional.cpp:62-101) and buildXcMatrix for RKS
+  (DfXCFunctional.cpp:128-152, getFxc DfXCFunctional.h:132-190): P_A = P/2,
+  a zeroed F_xc, the functional's build (none for HF), the electron-count
+  check (checkGridAccuracy, :227-248).
 
 Every call goes through the HIP library; nothing here computes on the CPU.
 """
@@ -23,15 +27,47 @@ from . import engine
 from .basis import OrbitalInfo, basis_631gd
 
 
-class DfFunctional_B3LYP:
+class DfFunctional_GGA:
+    """Closed-shell evaluation of one engine functional (functional_id)."""
+    functional_id = None
+
+    def getFunctional(self, rhoA, gammaAA=None):
+        return engine.eval_functional(rhoA, _gamma(rhoA, gammaAA), self.functional_id)[:, 0]
+
+    def getDerivativeFunctional(self, rhoA, gammaAA=None):
+        """(df/drhoA, df/dgammaAA, df/dgammaAB); the gamma terms are zero for LDA."""
+        o = engine.eval_functional(rhoA, _gamma(rhoA, gammaAA), self.functional_id)
+        return o[:, 1], o[:, 2], o[:, 3]
+
+
+def _gamma(rhoA, gammaAA):
+    return np.zeros_like(np.asarray(rhoA, dtype=np.float64)) if gammaAA is None else gammaAA
+
+
+class DfFunctional_HFS(DfFunctional_GGA):
+    functional_id = engine.PDFXC_HFS
+
+
+class DfFunctional_SVWN(DfFunctional_GGA):
+    functional_id = engine.PDFXC_SVWN
+
+
+class DfFunctional_Becke88(DfFunctional_GGA):
+    functional_id = engine.PDFXC_HFB
+
+
+class DfFunctional_B88LYP(DfFunctional_GGA):
+    functional_id = engine.PDFXC_BLYP
+
+
+class DfFunctional_B3LYP(DfFunctional_GGA):
     functional_id = engine.PDFXC_B3LYP
 
-    def getFunctional(self, rhoA, gammaAA):
-        return engine.eval_functional(rhoA, gammaAA, self.functional_id)[:, 0]
 
-    def getDerivativeFunctional(self, rhoA, gammaAA):
-        o = engine.eval_functional(rhoA, gammaAA, self.functional_id)
-        return o[:, 1], o[:, 2], o[:, 3]
+# getFxc's dispatch (DfXCFunctional.h:140-180); HF has no grid functional
+FUNCTIONAL_FOR = {engine.PDFXC_HFS: DfFunctional_HFS, engine.PDFXC_SVWN: DfFunctional_SVWN,
+                  engine.PDFXC_HFB: DfFunctional_Becke88, engine.PDFXC_BLYP: DfFunctional_B88LYP,
+                  engine.PDFXC_B3LYP: DfFunctional_B3LYP}
 
 
 def orbital_info(mol, basis_for=basis_631gd):
@@ -69,8 +105,8 @@ class DfCalcGridX:
         return self.ctx.get_grid()
 
     def calcXCIntegForFockAndEnergy(self, P_A, functional, F_A, keep_rho=False):
-        if not isinstance(functional, DfFunctional_B3LYP):
-            raise NotImplementedError("only B3LYP is implemented")
+        if not isinstance(functional, DfFunctional_GGA) or functional.functional_id is None:
+            raise TypeError("functional must be one of the DfFunctional_* classes")
         E, _, rho = self.ctx.build_rks(P_A, F_A, want_rho=keep_rho,
                                        functional=functional.functional_id)
         if keep_rho:
@@ -84,17 +120,34 @@ class DfCalcGridX:
 
 
 class DfXCFunctional:
-    def __init__(self, calc_grid, n_electrons):
+    """xc_functional: the reference's names ("HF", "HFS", "SVWN", "HFB",
+    "BLYP" or "B88LYP", "B3LYP"); a trailing "~" (XC fitting) is removed as
+    the reference removes it.  An unknown name raises, as getFxc aborts."""
+
+    def __init__(self, calc_grid, n_electrons, xc_functional="B3LYP"):
+        name = xc_functional[:-1] if xc_functional.endswith("~") else xc_functional
+        if name not in engine.XC_TYPES:
+            raise ValueError(f"unknown XC functional: {xc_functional}")
+        self.m_nXCFunctional = engine.XC_TYPES[name]
         self.calc_grid = calc_grid
         self.n_electrons = n_electrons
         self.XC_energy_ = 0.0
         self.electrons_on_grid = None
 
+    def isHybridFunctional(self):
+        return self.m_nXCFunctional in (engine.PDFXC_HF, engine.PDFXC_B3LYP)
+
+    def getFockExchangeCoefficient(self):
+        return {engine.PDFXC_HF: 1.0, engine.PDFXC_B3LYP: 0.2}.get(self.m_nXCFunctional, 0.0)
+
     def buildXcMatrix(self, P_packed):
         n = self.calc_grid.m_nNumOfAOs
-        P_A = 0.5 * np.asarray(P_packed, dtype=np.float64)
         Fxc = np.zeros(n * (n + 1) // 2)
+        if self.m_nXCFunctional == engine.PDFXC_HF:
+            self.XC_energy_ = 0.0
+            return Fxc
+        P_A = 0.5 * np.asarray(P_packed, dtype=np.float64)
         self.XC_energy_ = self.calc_grid.calcXCIntegForFockAndEnergy(
-            P_A, DfFunctional_B3LYP(), Fxc, keep_rho=True)
+            P_A, FUNCTIONAL_FOR[self.m_nXCFunctional](), Fxc, keep_rho=True)
         self.electrons_on_grid = 2.0 * self.calc_grid.getWholeDensity()
         return Fxc

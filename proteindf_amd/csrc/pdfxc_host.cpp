@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "pdfxc_internal.hpp"
+#include "xc_functional.hpp"
 
 namespace pdfxc {
 
@@ -1109,10 +1110,28 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
     return rc;
 }
 
-static int build_device(Context* c, const double* dP, double* dK, double* dE, int accumulate_rho = 0) {
+// Functional ids are the XC_TYPE values of DfXCFunctional.h:44; an unknown
+// one is refused as the reference's getFxc aborts on it (DfXCFunctional.h:182-186).
+static int check_functional(int functional, const char* who) {
+    if (xc_type_valid(functional)) return PDFXC_OK;
+    set_error(std::string(who) + ": unknown functional " + std::to_string(functional) +
+              " (1 HF, 2 HFS, 3 SVWN, 4 HFB, 5 BLYP, 6 B3LYP)");
+    return PDFXC_ERR_UNSUPPORTED;
+}
+
+static int build_device(Context* c, int xc, const double* dP, double* dK, double* dE, int accumulate_rho = 0) {
     if (c->n_pts == 0 || c->blocks.empty()) {
         set_error("build: no grid set");
         return PDFXC_ERR_NO_GRID;
+    }
+    if (xc == kHF) {
+        // HF has no grid term: E_xc = 0 and K untouched (DfXCFunctional.h:141-143)
+        const int rc = launch_zero(dE, 1, c->stream);
+        if (rc) return rc;
+        PDFXC_HIP(hipStreamSynchronize(c->stream));
+        for (double& t : c->last_kernel_ms) t = 0.0;
+        c->last_build_ms = 0.0;
+        return PDFXC_OK;
     }
     double* saveP = c->d_P;
     double* saveK = c->d_K;
@@ -1128,7 +1147,7 @@ static int build_device(Context* c, const double* dP, double* dK, double* dE, in
         for (const LaunchRange& r : c->rho_launches)
             if (rc == PDFXC_OK) rc = launch_rho(*c, r);
     }
-    if (rc == PDFXC_OK) rc = launch_xc_points(*c);
+    if (rc == PDFXC_OK) rc = launch_xc_points(*c, xc);
     PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
     if (split) {
         if (rc == PDFXC_OK) rc = launch_split_kmat(*c);
@@ -1159,57 +1178,52 @@ static int build_device(Context* c, const double* dP, double* dK, double* dE, in
 int pdfxc_build_rks_device(pdfxc_ctx* ctx, int functional, const double* dP, double* dK, double* dE) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return arg_error("pdfxc_build_rks_device: null context");
-    if (functional != PDFXC_B3LYP) {
-        set_error("pdfxc_build_rks_device: only B3LYP is implemented");
-        return PDFXC_ERR_UNSUPPORTED;
-    }
+    if (int rc = check_functional(functional, "pdfxc_build_rks_device")) return rc;
     if (!dP || !dK || !dE) return arg_error("pdfxc_build_rks_device: null buffer");
-    return build_device(c, dP, dK, dE);
+    return build_device(c, functional, dP, dK, dE);
 }
 
 int pdfxc_build_rks_update_device(pdfxc_ctx* ctx, int functional, const double* d_dP, double* dK, double* dE) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return arg_error("pdfxc_build_rks_update_device: null context");
-    if (functional != PDFXC_B3LYP) {
-        set_error("pdfxc_build_rks_update_device: only B3LYP is implemented");
-        return PDFXC_ERR_UNSUPPORTED;
-    }
+    if (int rc = check_functional(functional, "pdfxc_build_rks_update_device")) return rc;
     if (!d_dP || !dK || !dE) return arg_error("pdfxc_build_rks_update_device: null buffer");
-    return build_device(c, d_dP, dK, dE, 1);
+    return build_device(c, functional, d_dP, dK, dE, 1);
 }
 
-static int build_host(Context* c, const double* P_packed, double* K_packed, double* E_xc, double* rho_out,
-                      int accumulate_rho);
+static int build_host(Context* c, int xc, const double* P_packed, double* K_packed, double* E_xc,
+                      double* rho_out, int accumulate_rho);
 
 int pdfxc_build_rks_update(pdfxc_ctx* ctx, int functional, const double* dP_packed, double* K_packed,
                            double* E_xc, double* rho_out) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return arg_error("pdfxc_build_rks_update: null context");
-    if (functional != PDFXC_B3LYP) {
-        set_error("pdfxc_build_rks_update: only B3LYP is implemented");
-        return PDFXC_ERR_UNSUPPORTED;
-    }
+    if (int rc = check_functional(functional, "pdfxc_build_rks_update")) return rc;
     if (!dP_packed || !K_packed || !E_xc) return arg_error("pdfxc_build_rks_update: null buffer");
-    return build_host(c, dP_packed, K_packed, E_xc, rho_out, 1);
+    return build_host(c, functional, dP_packed, K_packed, E_xc, rho_out, 1);
 }
 
 int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_packed, double* K_packed,
                     double* E_xc, double* rho_out) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return arg_error("pdfxc_build_rks: null context");
-    if (functional != PDFXC_B3LYP) {
-        set_error("pdfxc_build_rks: only B3LYP is implemented");
-        return PDFXC_ERR_UNSUPPORTED;
-    }
+    if (int rc = check_functional(functional, "pdfxc_build_rks")) return rc;
     if (!P_packed || !K_packed || !E_xc) return arg_error("pdfxc_build_rks: null buffer");
-    return build_host(c, P_packed, K_packed, E_xc, rho_out, 0);
+    return build_host(c, functional, P_packed, K_packed, E_xc, rho_out, 0);
 }
 
-static int build_host(Context* c, const double* P_packed, double* K_packed, double* E_xc, double* rho_out,
-                      int accumulate_rho) {
+static int build_host(Context* c, int xc, const double* P_packed, double* K_packed, double* E_xc,
+                      double* rho_out, int accumulate_rho) {
     if (c->n_pts == 0 || c->blocks.empty()) {
         set_error("build: no grid set");
         return PDFXC_ERR_NO_GRID;
+    }
+    if (xc == kHF) {
+        // no grid term: E_xc = 0, K_packed and rho_out untouched (DfXCFunctional.h:141-143)
+        *E_xc = 0.0;
+        for (double& t : c->last_kernel_ms) t = 0.0;
+        c->last_build_ms = 0.0;
+        return PDFXC_OK;
     }
     const int N = c->n_aos;
     double* H = c->h_pinned;
@@ -1220,7 +1234,7 @@ static int build_host(Context* c, const double* P_packed, double* K_packed, doub
         }
     PDFXC_HIP(hipMemcpyAsync(c->d_P, H, sizeof(double) * (size_t)N * N, hipMemcpyHostToDevice, c->stream));
     PDFXC_HIP(hipMemsetAsync(c->d_K, 0, sizeof(double) * (size_t)N * N, c->stream));
-    int rc = build_device(c, c->d_P, c->d_K, c->d_E, accumulate_rho);
+    int rc = build_device(c, xc, c->d_P, c->d_K, c->d_E, accumulate_rho);
     if (rc) return rc;
     double E = 0.0;
     PDFXC_HIP(hipMemcpyAsync(H, c->d_K, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost, c->stream));
@@ -1245,10 +1259,7 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
                        double* grad_weight, double* E_xc) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return arg_error("pdfxc_gradient_rks: null context");
-    if (functional != PDFXC_B3LYP) {
-        set_error("pdfxc_gradient_rks: only B3LYP is implemented");
-        return PDFXC_ERR_UNSUPPORTED;
-    }
+    if (int rc = check_functional(functional, "pdfxc_gradient_rks")) return rc;
     if (n_atoms <= 0 || !Z || !xyz || !P_packed || !grad_functional || !grad_weight)
         return arg_error("pdfxc_gradient_rks: bad arguments");
     if (c->n_pts == 0 || c->blocks.empty()) {
@@ -1259,6 +1270,13 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
         return arg_error("pdfxc_gradient_rks: the grid was not made by pdfxc_generate_grid; pass point_atom");
     const int N = c->n_aos;
     const int na = n_atoms;
+    if (functional == kHF) {
+        // no grid term, so no XC gradient
+        std::fill(grad_functional, grad_functional + 3 * na, 0.0);
+        std::fill(grad_weight, grad_weight + 3 * na, 0.0);
+        if (E_xc) *E_xc = 0.0;
+        return PDFXC_OK;
+    }
     // atom of every shell: the atom at its centre
     std::vector<int> shell_atom(c->shells.size(), -1);
     for (size_t k = 0; k < c->shells.size(); ++k) {
@@ -1361,7 +1379,7 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
     // rho, the point terms (with w e_xc per slot), the two gradient terms, E_xc
     for (const LaunchRange& r : c->rho_launches)
         if (rc == PDFXC_OK) rc = launch_rho(*c, r);
-    if (rc == PDFXC_OK) rc = launch_xc_points(*c, d_ewf);
+    if (rc == PDFXC_OK) rc = launch_xc_points(*c, functional, d_ewf);
     for (const LaunchRange& r : gl)
         if (rc == PDFXC_OK) rc = launch_grad_fn(*c, r, d_shat, d_slat, na, d_g);
     if (rc == PDFXC_OK) rc = launch_grad_w(*c, d_at, na, d_slat, d_ewf, d_g);
@@ -1387,12 +1405,13 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
 }
 
 int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* out) {
-    if (functional != PDFXC_B3LYP) {
-        set_error("pdfxc_eval_functional: only B3LYP is implemented");
-        return PDFXC_ERR_UNSUPPORTED;
-    }
+    if (int rc = check_functional(functional, "pdfxc_eval_functional")) return rc;
     if (n < 0 || (n > 0 && (!in || !out))) return arg_error("pdfxc_eval_functional: bad arguments");
     if (n == 0) return PDFXC_OK;
+    if (functional == kHF) {
+        std::fill(out, out + 4 * n, 0.0);
+        return PDFXC_OK;
+    }
     double *di = nullptr, *dout = nullptr;
     int rc = dalloc(&di, 2 * (size_t)n);
     if (rc == PDFXC_OK) rc = dalloc(&dout, 4 * (size_t)n);
@@ -1400,7 +1419,7 @@ int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* o
         hipError_t e = hipMemcpy(di, in, 2 * n * sizeof(double), hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_functional: upload");
     }
-    if (rc == PDFXC_OK) rc = launch_eval_functional(n, di, dout, nullptr);
+    if (rc == PDFXC_OK) rc = launch_eval_functional(functional, n, di, dout, nullptr);
     if (rc == PDFXC_OK) {
         hipError_t e = hipMemcpy(out, dout, 4 * n * sizeof(double), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_functional: download");

@@ -151,11 +151,12 @@ int hip_fail(hipError_t e, const char* what);
 // kernel launchers (pdfxc_kernels.hip)
 int launch_rho(Context& c, const LaunchRange& r);
 int rho_points_per_workgroup(int cls);
-int launch_xc_points(Context& c, double* d_ewf = nullptr);
+int launch_xc_points(Context& c, int xc_type, double* d_ewf = nullptr);
 int launch_kmat(Context& c, const LaunchRange& r);
 int configure_kernels();
 int launch_energy_sum(Context& c, double* dE);
-int launch_eval_functional(int64_t n, const double* d_in, double* d_out, hipStream_t s);
+int launch_zero(double* d, int64_t n, hipStream_t s);
+int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s);
 int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, double* d_out,
                            hipStream_t s);
 int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,

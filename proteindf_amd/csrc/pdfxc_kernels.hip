@@ -299,12 +299,15 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
 }
 
 // ---------------------------------------------------------------------------
-// B3LYP point terms for every slot (buildVxc, DfCalcGridX.cpp:2137-2193, and
-// the GGA coefficients of build_XC_Matrix, :2455-2500): coef = (w vrho / 2,
-// w (2 vgAA + vgAB) grad rho), zero where rho <= the density cut-off or
-// w = 0 (padding slots); per-workgroup sums of w e_xc in a fixed order, and
-// w e_xc per slot when ewf is given.
+// Functional point terms for every slot (buildVxc, DfCalcGridX.cpp:2137-2193
+// for GGA and :2056-2091 for LDA, and the coefficients of build_XC_Matrix,
+// :2353-2368 and :2455-2500): coef = (w vrho / 2, w (2 vgAA + vgAB) grad rho),
+// zero where rho <= the density cut-off or w = 0 (padding slots).  For the
+// LDA functionals vgAA = vgAB = 0, so the gradient part of coef is exactly
+// zero and K = sum_g w vrho phi phi, the LDA matrix.  Per-workgroup sums of
+// w e_xc in a fixed order, and w e_xc per slot when ewf is given.
 // ---------------------------------------------------------------------------
+template <int XC>
 __global__ __launch_bounds__(256) void k_xc_points(int64_t n_slots, const double* __restrict__ pts,
                                                    const double* __restrict__ rho, double* __restrict__ coef,
                                                    double* __restrict__ epart, double* __restrict__ ewf,
@@ -318,7 +321,7 @@ __global__ __launch_bounds__(256) void k_xc_points(int64_t n_slots, const double
         double a = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
         if (w != 0.0 && r > dens_cut) {
             const double gamma = gx * gx + gy * gy + gz * gz;
-            const XcPoint f = b3lyp::eval_rks(r, gamma);
+            const XcPoint f = eval_rks(XC, r, gamma);
             a = 0.5 * (w * f.vr);
             const double c2 = 2.0 * f.vgaa;
             cx = w * (c2 * gx + f.vgab * gx);
@@ -560,11 +563,17 @@ __global__ __launch_bounds__(256) void k_energy_sum(const double* __restrict__ e
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+__global__ __launch_bounds__(64) void k_zero(double* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+template <int XC>
 __global__ __launch_bounds__(256) void k_eval_functional(int64_t n, const double* __restrict__ in,
                                                          double* __restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    const XcPoint f = b3lyp::eval_rks(in[2 * i], in[2 * i + 1]);
+    const XcPoint f = eval_rks(XC, in[2 * i], in[2 * i + 1]);
     out[4 * i + 0] = f.e;
     out[4 * i + 1] = f.vr;
     out[4 * i + 2] = f.vgaa;
@@ -669,10 +678,22 @@ int launch_rho(Context& c, const LaunchRange& r) {
     }
 }
 
-int launch_xc_points(Context& c, double* d_ewf) {
+template <int XC>
+static void xc_points_one(Context& c, double* d_ewf) {
     const int64_t g = (c.n_slots + 255) / 256;
-    hipLaunchKernelGGL(k_xc_points, dim3((unsigned)g), dim3(256), 0, c.stream, c.n_slots, c.d_pts, c.d_rho,
-                       c.d_coef, c.d_eblock, d_ewf, c.opts.density_cutoff);
+    hipLaunchKernelGGL(k_xc_points<XC>, dim3((unsigned)g), dim3(256), 0, c.stream, c.n_slots, c.d_pts,
+                       c.d_rho, c.d_coef, c.d_eblock, d_ewf, c.opts.density_cutoff);
+}
+
+int launch_xc_points(Context& c, int xc_type, double* d_ewf) {
+    switch (xc_type) {
+        case kHFS: xc_points_one<kHFS>(c, d_ewf); break;
+        case kSVWN: xc_points_one<kSVWN>(c, d_ewf); break;
+        case kHFB: xc_points_one<kHFB>(c, d_ewf); break;
+        case kBLYP: xc_points_one<kBLYP>(c, d_ewf); break;
+        case kB3LYP: xc_points_one<kB3LYP>(c, d_ewf); break;
+        default: set_error("k_xc_points: no grid functional " + std::to_string(xc_type)); return PDFXC_ERR_UNSUPPORTED;
+    }
     return check_launch("k_xc_points");
 }
 
@@ -713,10 +734,23 @@ int launch_energy_sum(Context& c, double* dE) {
     return check_launch("k_energy_sum");
 }
 
-int launch_eval_functional(int64_t n, const double* d_in, double* d_out, hipStream_t s) {
+int launch_zero(double* d, int64_t n, hipStream_t s) {
     if (n <= 0) return PDFXC_OK;
-    const int64_t g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_eval_functional, dim3((unsigned)g), dim3(256), 0, s, n, d_in, d_out);
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, d, n);
+    return check_launch("k_zero");
+}
+
+int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s) {
+    if (n <= 0) return PDFXC_OK;
+    const dim3 g((unsigned)((n + 255) / 256));
+    switch (xc_type) {
+        case kHFS: hipLaunchKernelGGL(k_eval_functional<kHFS>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kSVWN: hipLaunchKernelGGL(k_eval_functional<kSVWN>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kHFB: hipLaunchKernelGGL(k_eval_functional<kHFB>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kBLYP: hipLaunchKernelGGL(k_eval_functional<kBLYP>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kB3LYP: hipLaunchKernelGGL(k_eval_functional<kB3LYP>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        default: set_error("k_eval_functional: no grid functional " + std::to_string(xc_type)); return PDFXC_ERR_UNSUPPORTED;
+    }
     return check_launch("k_eval_functional");
 }
 

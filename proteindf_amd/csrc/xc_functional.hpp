@@ -1,22 +1,32 @@
-// B3LYP for closed-shell points, as ProteinDF's grid path evaluates it.
+// The closed-shell functionals of ProteinDF's grid path, on the device.
 //
-// The grid path reaches the functional through the non-virtual closed-shell
-// forwarders of DfFunctional_GGA (DfFunctional.h:190-192, :221-232), i.e. the
-// spin-resolved formulas at rhoB = rhoA, gAB = gBB = gAA.  This device code
-// evaluates those formulas specialised to that point: the spin-polarisation
-// terms of VWN vanish exactly at zeta = 0 (they are multiplied by g(0) = 0 and
-// g'(0) = 0), and the alpha and beta halves of Slater and B88 are equal.
+// The grid path reaches a functional through the non-virtual closed-shell
+// forwarders of DfFunctional_LDA / DfFunctional_GGA (DfFunctional.h:104-106,
+// :126-130, :190-192, :221-232), i.e. the spin-resolved formulas at
+// rhoB = rhoA and gAB = gBB = gAA.  This code evaluates those formulas
+// specialised to that point: the spin-polarisation terms of VWN vanish exactly
+// at zeta = 0 (they are multiplied by g(0) = 0 and g'(0) = 0), and the alpha
+// and beta halves of Slater and B88 are equal.
 //
-//   f = 0.80 Slater + 0.72 B88(non-LDA) + 0.19 VWN3 + 0.81 LYP
-//       (DfFunctional_B3LYP.cpp:25-28, :59-75, :91-141)
+// Components
 //   Slater  DfFunctional_Slater.cpp:25-52, :64-76
 //   B88     DfFunctional_Becke88.cpp:25-31, :40-57, :71-108, :137-163;
-//           DfFunctional_Becke88_ExceptLDA.cpp:26-33; asinh = log(x+sqrt(x^2+1)) (TlMath.h:57-59)
-//   VWN3    DfFunctional_VWN.cpp:76-124, :145-260, :262-346; DfFunctional_VWN3.cpp:23-54
+//           without the LDA part: DfFunctional_Becke88_ExceptLDA.cpp:26-33;
+//           asinh = log(x+sqrt(x^2+1)) (TlMath.h:57-59)
+//   VWN     DfFunctional_VWN.cpp:23-46, :76-124, :145-260, :262-346 (VWN5 sets);
+//           DfFunctional_VWN3.cpp:23-54 (VWN3 para set)
 //   LYP     DfFunctional_LYP.cpp:25-43, :64-197, :288-317, :346-386, :507-594, :652-674
+// Functionals (the XC_TYPE values of DfXCFunctional.h:44; dispatch :140-180)
+//   HF    no grid term (DfXCFunctional.h:141-143)
+//   HFS   Slater                                   (DfFunctional_HFS.cpp)
+//   SVWN  Slater + VWN5                             (DfFunctional_SVWN.cpp)
+//   HFB   B88 with its LDA part                     (DfFunctional_Becke88.cpp)
+//   BLYP  B88 + LYP                                 (DfFunctional_B88LYP.cpp)
+//   B3LYP 0.80 Slater + 0.72 B88(non-LDA) + 0.19 VWN3 + 0.81 LYP
+//         (DfFunctional_B3LYP.cpp:25-28, :59-75, :91-141)
 //
 // Outputs: e = f(rhoA, rhoA, g, g, g); vr = df/drhoA; vgaa = df/dgammaAA;
-// vgab = df/dgammaAB.  All FP64.
+// vgab = df/dgammaAB (zero for the LDA functionals).  All FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,10 +37,54 @@ struct XcPoint {
     double e, vr, vgaa, vgab;
 };
 
-namespace b3lyp {
+// XC_TYPE of DfXCFunctional.h:44
+enum XcType { kHF = 1, kHFS = 2, kSVWN = 3, kHFB = 4, kBLYP = 5, kB3LYP = 6 };
+
+__host__ __device__ constexpr bool xc_type_valid(int t) { return t >= kHF && t <= kB3LYP; }
+
+namespace xcf {
 
 constexpr double kPi = 3.14159265358979323846;
-constexpr double kSlaterCoefs = 0.80, kB88Coef = 0.72, kVwnCoef = 0.19, kLypCoef = 0.81;
+
+// Slater at rhoA = rhoB = ra: e = C (ra^4/3 + ra^4/3), v = C' ra^1/3
+__device__ __host__ inline void slater(double ra13, double ra43, double* e, double* v) {
+    const double c13 = pow(3.0 / (4.0 * kPi), 1.0 / 3.0);
+    const double slater_c = -(9.0 / 4.0) * c13 * (2.0 / 3.0);
+    const double slater_dc = -3.0 * c13 * (2.0 / 3.0);
+    *e = slater_c * (ra43 + ra43);
+    *v = slater_dc * ra13;
+}
+
+// B88 at rhoA = rhoB = ra, gAA = gBB = g; x = sqrt(g)/ra^4/3.  The reference
+// sets vr and vg only where ra and g both exceed 1e-16, also for the LDA part.
+template <bool WITH_LDA>
+__device__ __host__ inline void b88(double ra, double g, double ra13, double ra43, double* e,
+                                    double* v, double* vg) {
+    *e = 0.0;
+    *v = 0.0;
+    *vg = 0.0;
+    if (ra > 1.0e-16) {
+        const double sg = sqrt(g);
+        const double x = sg / ra43;
+        const double bx = 0.0042 * x;
+        const double ash = log(x + sqrt(x * x + 1));
+        const double M = 1.0 + 6.0 * bx * ash;
+        double gx;
+        if (WITH_LDA) {
+            const double gparam = -(3.0 / 2.0) * pow(3.0 / (4.0 * kPi), 1.0 / 3.0);
+            gx = gparam - ((bx * x) / M);
+        } else {
+            gx = -((bx * x) / M);
+        }
+        *e = ra43 * gx + ra43 * gx;
+        if (g > 1.0e-16) {
+            const double invM = 1.0 / M;
+            const double gp = 6.0 * bx * bx * (ash + x / sqrt(x * x + 1.0)) * invM * invM - 2.0 * bx * invM;
+            *v = (4.0 / 3.0) * ra13 * (gx - x * gp);
+            *vg = 0.5 * (1.0 / sg) * gp;
+        }
+    }
+}
 
 // VWN eps_c(x) and its x-derivative for one parameter set (VWN eq. A13, A18)
 __device__ __host__ inline void vwn_eps(double A, double b, double c, double x0, double x,
@@ -53,45 +107,25 @@ __device__ __host__ inline void vwn_eps(double A, double b, double c, double x0,
     *deps = A * ((2.0 / x) + (-x2b * invX) + (-4.0 * b * inv_x2bQ) + d4);
 }
 
-__device__ __host__ inline XcPoint eval_rks(double ra, double g) {
-    XcPoint o;
+// VWN at zeta = 0 (paramagnetic set only): eps_c = eps_para,
+// v = eps + rho (-x / (6 rho)) eps'
+template <bool VWN3>
+__device__ __host__ inline void vwn(double ra, double* e, double* v) {
     const double rho = ra + ra;
-    const double ra13 = pow(ra, 1.0 / 3.0);
-    const double ra43 = pow(ra, 4.0 / 3.0);
-
-    // Slater: e = C (rhoA^4/3 + rhoB^4/3), v = C' rhoA^1/3
-    const double c13 = pow(3.0 / (4.0 * kPi), 1.0 / 3.0);
-    const double slater_c = -(9.0 / 4.0) * c13 * (2.0 / 3.0);
-    const double slater_dc = -3.0 * c13 * (2.0 / 3.0);
-    const double e_s = slater_c * (ra43 + ra43);
-    const double v_s = slater_dc * ra13;
-
-    // B88 without the LDA part; x = sqrt(gamma)/rho^4/3
-    double e_b = 0.0, v_b = 0.0, vg_b = 0.0;
-    if (ra > 1.0e-16) {
-        const double sg = sqrt(g);
-        const double x = sg / ra43;
-        const double bx = 0.0042 * x;
-        const double ash = log(x + sqrt(x * x + 1));
-        const double M = 1.0 + 6.0 * bx * ash;
-        const double gx = -((bx * x) / M);
-        e_b = ra43 * gx + ra43 * gx;
-        if (g > 1.0e-16) {
-            const double invM = 1.0 / M;
-            const double gp = 6.0 * bx * bx * (ash + x / sqrt(x * x + 1.0)) * invM * invM - 2.0 * bx * invM;
-            v_b = (4.0 / 3.0) * ra13 * (gx - x * gp);
-            vg_b = 0.5 * (1.0 / sg) * gp;
-        }
-    }
-
-    // VWN3 at zeta = 0: eps_c = eps_para; v = eps + rho * (-x/(6 rho)) eps'
     const double xv = pow((3.0 / (4.0 * kPi)) * (1.0 / rho), 1.0 / 6.0);
     double ep, epp;
-    vwn_eps(0.0310907, 13.0720, 42.7198, -0.409286, xv, &ep, &epp);
-    const double e_v = rho * ep;
-    const double v_v = (ra > 1.0e-20) ? ep + rho * (-xv / (6.0 * rho) * epp) : 0.0;
+    if (VWN3)
+        vwn_eps(0.0310907, 13.0720, 42.7198, -0.409286, xv, &ep, &epp);
+    else
+        vwn_eps(0.0310907, 3.72744, 12.9352, -0.10498, xv, &ep, &epp);
+    *e = rho * ep;
+    *v = (ra > 1.0e-20) ? ep + rho * (-xv / (6.0 * rho) * epp) : 0.0;
+}
 
-    // LYP with rhoA = rhoB = ra, gAA = gAB = gBB = g
+// LYP with rhoA = rhoB = ra, gAA = gAB = gBB = g
+__device__ __host__ inline void lyp(double ra, double g, double* e, double* v, double* vgaa,
+                                    double* vgab) {
+    const double rho = ra + ra;
     const double A = 0.04918, B = 0.132, C = 0.2533, D = 0.349;
     const double AB = A * B;
     const double lyp_c = pow(2.0, 11.0 / 3.0) * 0.3 * pow((3.0 * kPi * kPi), 2.0 / 3.0);
@@ -106,16 +140,15 @@ __device__ __host__ inline XcPoint eval_rks(double ra, double g) {
     const double half = ra / rho;  // 1/2
     const double gaa = -abw * ((1.0 / 9.0) * rab * (1.0 - 3.0 * delta - (delta - 11.0) * half) - rab);
     const double gab = -abw * ((1.0 / 9.0) * rab * (47.0 - 7.0 * delta) - (4.0 / 3.0) * rho * rho);
-    const double e_l = -4.0 * A / den * rab / rho - lyp_c * AB * omega * rab * (ra83 + ra83) +
-                       (gaa + gab + gaa) * g;
-    double v_l = 0.0, vgaa_l = 0.0, vgab_l = 0.0;
+    *e = -4.0 * A / den * rab / rho - lyp_c * AB * omega * rab * (ra83 + ra83) + (gaa + gab + gaa) * g;
+    *v = 0.0;
+    *vgaa = 0.0;
+    *vgab = 0.0;
     if (ra > 1.0e-20) {
         const double inv = 1.0 / rho;
-        const double r13 = 1.0 / rm13;
         const double rm43 = pow(rho, -4.0 / 3.0);
         const double omegap = -(1.0 / 3.0) * ((1.0 / rho) * (1.0 / pow(rho, 1.0 / 3.0))) * omega *
                               (11.0 * pow(rho, 1.0 / 3.0) - C - (D / den));
-        (void)r13;
         const double dd = 1.0 + D * rm13;
         const double deltap = (1.0 / 3.0) * ((D * D) * pow(rho, -(5.0 / 3.0)) / (dd * dd) + (-1.0 / rho * delta));
         const double wpw = -(1.0 / 3.0) * rm43 * (11.0 * pow(rho, 1.0 / 3.0) - C - D / den);
@@ -132,17 +165,57 @@ __device__ __host__ inline XcPoint eval_rks(double ra, double g) {
         const double a1c = -4.0 * A / den * rab / rho;
         const double a1 = a1c * ((1.0 / 3.0) * D * rm43 / den + 1.0 / ra - inv);
         const double a2 = -lyp_c * AB * (omegap * rab * (ra83 + ra83) + omega * ra * ((11.0 / 3.0) * ra83 + ra83));
-        v_l = a1 + a2 + rA_AA * g + rA_AB * g + rA_BB * g;
-        vgaa_l = gaa;
-        vgab_l = gab;
+        *v = a1 + a2 + rA_AA * g + rA_AB * g + rA_BB * g;
+        *vgaa = gaa;
+        *vgab = gab;
     }
+}
 
-    o.e = kSlaterCoefs * e_s + kB88Coef * e_b + kVwnCoef * e_v + kLypCoef * e_l;
-    o.vr = kSlaterCoefs * v_s + kB88Coef * v_b + kVwnCoef * v_v + kLypCoef * v_l;
-    o.vgaa = kB88Coef * vg_b + kLypCoef * vgaa_l;
-    o.vgab = kB88Coef * 0.0 + kLypCoef * vgab_l;
+}  // namespace xcf
+
+// One closed-shell point of functional `type` (an XcType other than kHF).
+__device__ __host__ inline XcPoint eval_rks(int type, double ra, double g) {
+    XcPoint o{0.0, 0.0, 0.0, 0.0};
+    const double ra13 = pow(ra, 1.0 / 3.0);
+    const double ra43 = pow(ra, 4.0 / 3.0);
+    switch (type) {
+        case kHFS:
+            xcf::slater(ra13, ra43, &o.e, &o.vr);
+            break;
+        case kSVWN: {
+            double es, vs, ev, vv;
+            xcf::slater(ra13, ra43, &es, &vs);
+            xcf::vwn<false>(ra, &ev, &vv);
+            o.e = es + ev;
+            o.vr = vs + vv;
+        } break;
+        case kHFB:
+            xcf::b88<true>(ra, g, ra13, ra43, &o.e, &o.vr, &o.vgaa);
+            break;
+        case kBLYP: {
+            double eb, vb, gb, el, vl, gal, gbl;
+            xcf::b88<true>(ra, g, ra13, ra43, &eb, &vb, &gb);
+            xcf::lyp(ra, g, &el, &vl, &gal, &gbl);
+            o.e = eb + el;
+            o.vr = vb + vl;
+            o.vgaa = gb + gal;
+            o.vgab = 0.0 + gbl;
+        } break;
+        case kB3LYP: {
+            double es, vs, eb, vb, gb, ev, vv, el, vl, gal, gbl;
+            xcf::slater(ra13, ra43, &es, &vs);
+            xcf::b88<false>(ra, g, ra13, ra43, &eb, &vb, &gb);
+            xcf::vwn<true>(ra, &ev, &vv);
+            xcf::lyp(ra, g, &el, &vl, &gal, &gbl);
+            o.e = 0.80 * es + 0.72 * eb + 0.19 * ev + 0.81 * el;
+            o.vr = 0.80 * vs + 0.72 * vb + 0.19 * vv + 0.81 * vl;
+            o.vgaa = 0.72 * gb + 0.81 * gal;
+            o.vgab = 0.72 * 0.0 + 0.81 * gbl;
+        } break;
+        default:
+            break;
+    }
     return o;
 }
 
-}  // namespace b3lyp
 }  // namespace pdfxc

@@ -13,7 +13,15 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libpdfxc.so"
 
 PDFXC_OK = 0
-PDFXC_B3LYP = 1
+# functionals: the XC_TYPE values of DfXCFunctional.h:44
+PDFXC_HF = 1
+PDFXC_HFS = 2
+PDFXC_SVWN = 3
+PDFXC_HFB = 4
+PDFXC_BLYP = 5
+PDFXC_B3LYP = 6
+XC_TYPES = {"HF": PDFXC_HF, "HFS": PDFXC_HFS, "SVWN": PDFXC_SVWN, "HFB": PDFXC_HFB,
+            "BLYP": PDFXC_BLYP, "B88LYP": PDFXC_BLYP, "B3LYP": PDFXC_B3LYP}
 PDFXC_FP64 = 0
 PDFXC_SPLIT_BF16 = 1
 PDFXC_SPLIT_BF16X3 = 2

@@ -70,3 +70,25 @@ def test_precision_modes_are_validated_before_any_device_work():
         with pytest.raises(engine.PdfxcError) as ei:
             engine.XcContext(info, engine.default_opts(precision=bad))
         assert ei.value.code == 1 and "precision" in str(ei.value)
+
+
+def test_functional_ids_follow_the_reference_xc_type():
+    # XC_TYPE of DfXCFunctional.h:44: UNDEFINED_XC, HF, HFS, SVWN, HFB, BLYP, B3LYP
+    assert (engine.PDFXC_HF, engine.PDFXC_HFS, engine.PDFXC_SVWN, engine.PDFXC_HFB,
+            engine.PDFXC_BLYP, engine.PDFXC_B3LYP) == (1, 2, 3, 4, 5, 6)
+    assert engine.XC_TYPES["B88LYP"] == engine.XC_TYPES["BLYP"]
+    # an unknown functional is refused before any device work
+    for bad in (0, 7, -3):
+        with pytest.raises(engine.PdfxcError) as ei:
+            engine.eval_functional([0.1], [0.01], bad)
+        assert ei.value.code == 4 and "unknown functional" in str(ei.value)
+
+
+def test_xc_functional_names_follow_the_reference():
+    from proteindf_amd import calc_grid
+
+    assert calc_grid.DfXCFunctional(None, 0, "B3LYP").getFockExchangeCoefficient() == 0.2
+    assert calc_grid.DfXCFunctional(None, 0, "HF").isHybridFunctional()
+    assert not calc_grid.DfXCFunctional(None, 0, "SVWN~").isHybridFunctional()
+    with pytest.raises(ValueError):
+        calc_grid.DfXCFunctional(None, 0, "PBE")

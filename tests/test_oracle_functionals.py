@@ -67,7 +67,8 @@ FD_POINTS = [
 
 
 @pytest.mark.parametrize("fid", [oracle.SLATER, oracle.BECKE88, oracle.BECKE88_EXCEPT_LDA,
-                                 oracle.VWN5, oracle.VWN3, oracle.LYP, oracle.B3LYP])
+                                 oracle.VWN5, oracle.VWN3, oracle.LYP, oracle.B3LYP,
+                                 oracle.HFS, oracle.SVWN, oracle.BLYP])
 @pytest.mark.parametrize("pt", FD_POINTS)
 def test_oracle_derivatives_match_finite_differences(fid, pt):
     x = np.array([pt])
@@ -109,3 +110,30 @@ def test_closed_shell_vwn3_has_no_spin_polarisation_terms():
     x = np.array([[0.12, 0.12, 0, 0, 0]])
     o = oracle.functional(oracle.VWN3, x)[0]
     assert o[1] == o[2]
+
+
+def test_composite_functionals_are_their_components():
+    # DfFunctional_HFS = Slater, DfFunctional_SVWN = Slater + VWN (VWN5 sets),
+    # DfFunctional_B88LYP = Becke88 (with its LDA part) + LYP, term by term;
+    # each component is pinned by the reference's known answers above
+    x = np.array(FD_POINTS)
+    s = oracle.functional(oracle.SLATER, x)
+    np.testing.assert_array_equal(oracle.functional(oracle.HFS, x), s)
+    np.testing.assert_allclose(oracle.functional(oracle.SVWN, x), s + oracle.functional(oracle.VWN5, x),
+                               rtol=1e-15, atol=0)
+    np.testing.assert_allclose(oracle.functional(oracle.BLYP, x),
+                               oracle.functional(oracle.BECKE88, x) + oracle.functional(oracle.LYP, x),
+                               rtol=1e-15, atol=0)
+
+
+def test_svwn_and_hfs_differ_by_vwn5_not_vwn3():
+    # SVWN's correlation is DfFunctional_VWN (VWN5): at rs ~ 1 the paramagnetic
+    # VWN5 and VWN3 energies per electron differ by more than 1 mEh
+    rho = 3.0 / (4.0 * np.pi)  # rs = 1
+    x = np.array([[rho / 2, rho / 2, 0, 0, 0]])
+    ec = (oracle.functional(oracle.SVWN, x) - oracle.functional(oracle.HFS, x))[0, 0] / rho
+    e5 = oracle.functional(oracle.VWN5, x)[0, 0] / rho
+    e3 = oracle.functional(oracle.VWN3, x)[0, 0] / rho
+    assert abs(ec - e5) < 1e-15 and abs(e5 - e3) > 1e-3
+    # VWN5 fits Ceperley-Alder's unpolarised correlation, about -0.120 Ry = -0.060 Eh at rs = 1
+    assert abs(e5 - (-0.0600)) < 5e-4
