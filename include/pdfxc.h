@@ -155,7 +155,8 @@ int pdfxc_build_rks_update_device(pdfxc_ctx* ctx, int functional, const double* 
  * derivative of the size-adjusted Becke partition weights (the reference's
  * own weight term evaluates to zero, DfGenerateGrid.cpp:2355-2372).  Their
  * sum is the gradient of E_xc.  E_xc (may be NULL) is returned.  FP64
- * contractions whatever the context's precision. */
+ * contractions whatever the context's precision.  The context's stored rho
+ * becomes rho[P_alpha] (an update-mode SCF restarts from a full build). */
 int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z, const double* xyz,
                        const int* point_atom, const double* P_alpha_packed, double* grad_functional,
                        double* grad_weight, double* E_xc);
@@ -166,6 +167,33 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
  * df/dgammaAA, df/dgammaAB; both zero for HFS and SVWN, all four for HF).
  * Host buffers; runs on the current device. */
 int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* out);
+
+/* ProteinDF matrix files (TlMatrixObject::MatrixType, tl_matrix_object.h:50-61):
+ * a header (type, rows, cols) and the doubles.  Readers accept every header
+ * width the reference accepts (type 1/4/8 bytes, counts 4/8 bytes, the first
+ * whose implied size is the file's, tl_matrix_utils.cc:114-200, :334-358);
+ * writers use the reference's own (1-byte type, 32-bit counts). */
+enum { PDFXC_MAT_RSFD = 0, PDFXC_MAT_CSFD = 1, PDFXC_MAT_RLHD = 2 };
+/* Type and shape of a dense matrix file; PDFXC_ERR_UNSUPPORTED if no header fits. */
+int pdfxc_matrix_file_info(const char* path, int* type, int64_t* rows, int64_t* cols);
+/* Symmetric matrix <-> RLHD file, packed lower triangle by rows
+ * (TlDenseSymmetricMatrixObject::load / save, tl_dense_symmetric_matrix_object.cc:245-321).
+ * The load checks the file's dimension against n. */
+int pdfxc_load_symmetric_matrix(const char* path, int64_t n, double* packed);
+int pdfxc_save_symmetric_matrix(const char* path, int64_t n, const double* packed);
+/* General matrix, column-major in memory: loads RSFD or CSFD, saves CSFD
+ * (TlDenseGeneralMatrixObject::load / save, tl_dense_general_matrix_object.cc:178-266). */
+int pdfxc_load_general_matrix(const char* path, int64_t rows, int64_t cols, double* colmajor);
+int pdfxc_save_general_matrix(const char* path, int64_t rows, int64_t cols, const double* colmajor);
+/* grid.mat (DfObject::getGridMatrix / saveGridMatrix; columns DfCalcGridX.h:37-54):
+ * one row per point: x, y, z, w, atom, then rho (LDA RKS, 6 columns) or rho
+ * and grad rho (GGA RKS, 9 columns).  load: sets the context's grid from
+ * columns 0-3, the point atoms (for pdfxc_gradient_rks) from column 4, and,
+ * for 6 or 9 columns, the stored rho an update-mode build adds to.  save:
+ * n_rho_cols 0, 1 or 4 after x, y, z, w, atom (-1 where the context has no
+ * point atoms); rho is the context's current (last built) rho. */
+int pdfxc_load_grid_matrix(pdfxc_ctx* ctx, const char* path, int64_t* n_pts, int* n_cols);
+int pdfxc_save_grid_matrix(pdfxc_ctx* ctx, const char* path, int n_rho_cols);
 
 /* AO values and gradients of all AOs at arbitrary points (getAOs / getDAOs,
  * DfCalcGridX.cpp:655-769).  Outputs n_pts x n_aos row-major. */

@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 SRC = PKG / "csrc"
 OUT = PKG / "_lib" / "libpdfxc.so"
-SOURCES = [SRC / "pdfxc_host.cpp", SRC / "pdfxc_kernels.hip", SRC / "pdfxc_split.hip", SRC / "pdfxc_grad.hip"]
+SOURCES = [SRC / "pdfxc_host.cpp", SRC / "pdfxc_io.cpp", SRC / "pdfxc_kernels.hip", SRC / "pdfxc_split.hip", SRC / "pdfxc_grad.hip"]
 HEADERS = [SRC / "pdfxc_internal.hpp", SRC / "pdfxc_device.hpp", SRC / "xc_functional.hpp", ROOT / "include" / "pdfxc.h"]
 
 

@@ -19,8 +19,15 @@ engine replaces, so that callers (and tests) read like the reference's own:
   a zeroed F_xc, the functional's build (none for HF), the electron-count
   check (checkGridAccuracy, :227-248).
 
+* ``WorkFiles`` -- the fl_Work file names of the path (DfObject.cpp:357-389,
+  :540-562, :657-683): ``~Ppq.rks<n>.mat`` (P in), ``dP.rks<n>.mat`` (update
+  mode), ``Fxc.rks<n>.mat`` (out), ``grid.mat``; read and written by the
+  library's RLHD / CSFD routines.
+
 Every call goes through the HIP library; nothing here computes on the CPU.
 """
+from pathlib import Path
+
 import numpy as np
 
 from . import engine
@@ -86,16 +93,40 @@ class DfGenerateGrid:
                                  self.weight_cutoff)
 
 
+class WorkFiles:
+    """The files the XC step reads and writes in the work directory (fl_Work)."""
+
+    def __init__(self, work_dir="fl_Work"):
+        self.dir = Path(work_dir)
+
+    def p_in(self, iteration):        # getPInMatrixPath (DfObject.cpp:677-679)
+        return self.dir / f"~Ppq.rks{iteration}.mat"
+
+    def diff_density(self, iteration):  # getDiffDensityMatrixPath (:665-667)
+        return self.dir / f"dP.rks{iteration}.mat"
+
+    def fxc(self, iteration):         # getFxcMatrixPath (:712-714)
+        return self.dir / f"Fxc.rks{iteration}.mat"
+
+    def grid_matrix(self):            # getGridMatrixPath: "grid.mat", no iteration in the name
+        return self.dir / "grid.mat"
+
+
 class DfCalcGridX:
+    """grid: an (n, 4) array, or a grid.mat path (its points, atoms and stored rho), or
+    None to generate the grid."""
+
     def __init__(self, mol, grid=None, n_rad=75, n_ang=302, basis_for=basis_631gd,
-                 density_cutoff=1e-16, ao_screen=0.0, info=None):
+                 density_cutoff=1e-16, ao_screen=0.0, info=None, precision=engine.PDFXC_FP64):
         self.mol = mol
         self.info = info if info is not None else orbital_info(mol, basis_for)
         self.m_nNumOfAOs = self.info.n_aos
-        opts = engine.default_opts(density_cutoff=density_cutoff, ao_screen=ao_screen)
+        opts = engine.default_opts(density_cutoff=density_cutoff, ao_screen=ao_screen, precision=precision)
         self.ctx = engine.XcContext(self.info, opts)
         if grid is None:
             DfGenerateGrid(mol, n_rad, n_ang).generate(self.ctx)
+        elif isinstance(grid, (str, Path)):
+            self.ctx.load_grid_matrix(grid)
         else:
             self.ctx.set_grid(grid)
         self._rho = None
@@ -104,14 +135,21 @@ class DfCalcGridX:
     def grid(self):
         return self.ctx.get_grid()
 
-    def calcXCIntegForFockAndEnergy(self, P_A, functional, F_A, keep_rho=False):
+    def calcXCIntegForFockAndEnergy(self, P_A, functional, F_A, keep_rho=False, update=False):
+        """update: P_A is the change of the density and rho accumulates on the
+        stored one (calcRho_*_part with m_bIsUpdateXC, DfCalcGridX.cpp:1866-1869)."""
         if not isinstance(functional, DfFunctional_GGA) or functional.functional_id is None:
             raise TypeError("functional must be one of the DfFunctional_* classes")
-        E, _, rho = self.ctx.build_rks(P_A, F_A, want_rho=keep_rho,
-                                       functional=functional.functional_id)
+        build = self.ctx.build_rks_update if update else self.ctx.build_rks
+        E, _, rho = build(P_A, F_A, want_rho=keep_rho, functional=functional.functional_id)
         if keep_rho:
             self._rho = rho
         return E
+
+    def saveGridMatrix(self, path, functional):
+        """grid.mat with the rho columns the functional's type uses (DfGenerateGrid.cpp:229-239)."""
+        lda = functional.functional_id in (engine.PDFXC_HFS, engine.PDFXC_SVWN)
+        self.ctx.save_grid_matrix(path, 1 if lda else 4)
 
     def getWholeDensity(self):
         if self._rho is None:
@@ -139,6 +177,26 @@ class DfXCFunctional:
 
     def getFockExchangeCoefficient(self):
         return {engine.PDFXC_HF: 1.0, engine.PDFXC_B3LYP: 0.2}.get(self.m_nXCFunctional, 0.0)
+
+    def buildXcMatrixFromFiles(self, work, iteration, update_xc=False):
+        """DfXCFunctional::buildXcMatrix for RKS with its files (DfXCFunctional.cpp:128-152):
+        P_A = P/2 from ~Ppq.rks<n>.mat (dP.rks<n>.mat, rho accumulated, in update mode
+        after the first iteration), Fxc.rks<n>.mat written, grid.mat rewritten with rho."""
+        n = self.calc_grid.m_nNumOfAOs
+        Fxc = np.zeros(n * (n + 1) // 2)
+        if self.m_nXCFunctional == engine.PDFXC_HF:
+            self.XC_energy_ = 0.0
+        else:
+            accumulate = update_xc and iteration > 1
+            src = work.diff_density(iteration) if update_xc else work.p_in(iteration)
+            P_A = 0.5 * engine.load_symmetric_matrix(src)
+            functional = FUNCTIONAL_FOR[self.m_nXCFunctional]()
+            self.XC_energy_ = self.calc_grid.calcXCIntegForFockAndEnergy(
+                P_A, functional, Fxc, keep_rho=True, update=accumulate)
+            self.electrons_on_grid = 2.0 * self.calc_grid.getWholeDensity()
+            self.calc_grid.saveGridMatrix(work.grid_matrix(), functional)
+        engine.save_symmetric_matrix(work.fxc(iteration), Fxc)
+        return Fxc
 
     def buildXcMatrix(self, P_packed):
         n = self.calc_grid.m_nNumOfAOs

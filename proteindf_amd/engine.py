@@ -27,6 +27,7 @@ PDFXC_SPLIT_BF16 = 1
 PDFXC_SPLIT_BF16X3 = 2
 PRECISIONS = {"fp64": PDFXC_FP64, "bf16x2": PDFXC_SPLIT_BF16, "bf16x3": PDFXC_SPLIT_BF16X3}
 ERROR_NAMES = {1: "ARGUMENT", 2: "HIP", 3: "NO_GRID", 4: "UNSUPPORTED", 5: "MEMORY"}
+ERROR_CODES = {v: k for k, v in ERROR_NAMES.items()}
 
 # every symbol declared in include/pdfxc.h
 EXPORTED = [
@@ -35,8 +36,12 @@ EXPORTED = [
     "pdfxc_build_rks_device", "pdfxc_eval_functional", "pdfxc_eval_aos", "pdfxc_get_stats",
     "pdfxc_stream", "pdfxc_last_error", "pdfxc_gradient_rks", "pdfxc_build_rks_update",
     "pdfxc_build_rks_update_device", "pdfxc_selftest_mfma", "pdfxc_selftest_mfma_split",
-    "pdfxc_selftest_copy",
+    "pdfxc_selftest_copy", "pdfxc_matrix_file_info", "pdfxc_load_symmetric_matrix",
+    "pdfxc_save_symmetric_matrix", "pdfxc_load_general_matrix", "pdfxc_save_general_matrix",
+    "pdfxc_load_grid_matrix", "pdfxc_save_grid_matrix",
 ]
+# matrix file types (TlMatrixObject::MatrixType)
+PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
 
 
 class PdfxcError(RuntimeError):
@@ -91,6 +96,14 @@ def lib():
         L.pdfxc_build_rks_update_device.argtypes = [P, i32, P, P, P]
         L.pdfxc_selftest_mfma_split.argtypes = [P]
         L.pdfxc_selftest_copy.argtypes = [P, P, i64]
+        c = ctypes.c_char_p
+        L.pdfxc_matrix_file_info.argtypes = [c, P, P, P]
+        L.pdfxc_load_symmetric_matrix.argtypes = [c, i64, P]
+        L.pdfxc_save_symmetric_matrix.argtypes = [c, i64, P]
+        L.pdfxc_load_general_matrix.argtypes = [c, i64, i64, P]
+        L.pdfxc_save_general_matrix.argtypes = [c, i64, i64, P]
+        L.pdfxc_load_grid_matrix.argtypes = [P, c, P, P]
+        L.pdfxc_save_grid_matrix.argtypes = [P, c, i32]
         _lib = L
     return _lib
 
@@ -126,6 +139,47 @@ def eval_functional(rho_a, gamma_aa, functional=PDFXC_B3LYP):
     out = np.zeros((x.shape[0], 4))
     check(lib().pdfxc_eval_functional(functional, x.shape[0], _p(x), _p(out)))
     return out
+
+
+def _path(path):
+    return str(path).encode()
+
+
+def matrix_file_info(path):
+    """(type, rows, cols) of a ProteinDF dense matrix file."""
+    t, r, c = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+    check(lib().pdfxc_matrix_file_info(_path(path), ctypes.byref(t), ctypes.byref(r), ctypes.byref(c)))
+    return t.value, r.value, c.value
+
+
+def load_symmetric_matrix(path):
+    """RLHD file -> packed lower triangle by rows (N(N+1)/2 doubles)."""
+    _, n, _ = matrix_file_info(path)
+    out = np.zeros(n * (n + 1) // 2)
+    check(lib().pdfxc_load_symmetric_matrix(_path(path), n, _p(out)))
+    return out
+
+
+def save_symmetric_matrix(path, packed):
+    a = np.ascontiguousarray(packed, dtype=np.float64)
+    n = int(round((np.sqrt(8 * a.size + 1) - 1) / 2))
+    if n * (n + 1) // 2 != a.size:
+        raise ValueError("packed symmetric matrix must hold N(N+1)/2 values")
+    check(lib().pdfxc_save_symmetric_matrix(_path(path), n, _p(a)))
+
+
+def load_general_matrix(path):
+    """RSFD or CSFD file -> (rows, cols) array."""
+    _, r, c = matrix_file_info(path)
+    out = np.zeros((c, r))  # column-major buffer
+    check(lib().pdfxc_load_general_matrix(_path(path), r, c, _p(out)))
+    return out.T
+
+
+def save_general_matrix(path, m):
+    m = np.asarray(m, dtype=np.float64)
+    cm = np.ascontiguousarray(m.T)
+    check(lib().pdfxc_save_general_matrix(_path(path), m.shape[0], m.shape[1], _p(cm)))
 
 
 def selftest_mfma():
@@ -188,6 +242,16 @@ class XcContext:
                                         weight_cutoff, ctypes.byref(n)))
         self.n_pts = n.value
         return n.value
+
+    def load_grid_matrix(self, path):
+        """grid.mat -> the context's grid (and point atoms, and stored rho); returns its column count."""
+        n, nc = ctypes.c_int64(), ctypes.c_int()
+        check(lib().pdfxc_load_grid_matrix(self.h, _path(path), ctypes.byref(n), ctypes.byref(nc)))
+        self.n_pts = n.value
+        return nc.value
+
+    def save_grid_matrix(self, path, n_rho_cols=4):
+        check(lib().pdfxc_save_grid_matrix(self.h, _path(path), n_rho_cols))
 
     def get_grid(self):
         cols = [np.zeros(self.n_pts) for _ in range(4)]
