@@ -145,6 +145,27 @@ int pdfxc_build_rks_update(pdfxc_ctx* ctx, int functional, const double* dP_alph
 int pdfxc_build_rks_update_device(pdfxc_ctx* ctx, int functional, const double* d_dP, double* dK,
                                   double* dE);
 
+/* UKS (calcXCIntegForFockAndEnergy(P_A, P_B, ...), DfCalcGridX.cpp:2291-2330;
+ * buildVxc UKS :2093-2135, :2195-2278): PA, PB the alpha and beta densities
+ * (not halved), KA, KB ACCUMULATED, layouts as pdfxc_build_rks.  A point
+ * counts when rhoA or rhoB exceeds the density cut-off.  As the reference's
+ * build_XC_Matrix does, K_A takes df/dgammaAB times grad rhoA (and K_B
+ * times grad rhoB).  rho_out (may be NULL): n_pts x 8 (rhoA, grad rhoA,
+ * rhoB, grad rhoB).  The update forms add rho[dPA], rho[dPB] to the stored
+ * two-spin rho.  ROKS is this build with PA = P_closed/2 + P_open,
+ * PB = P_closed/2 (DfXCFunctional.cpp:184-205). */
+int pdfxc_build_uks(pdfxc_ctx* ctx, int functional, const double* PA_packed, const double* PB_packed,
+                    double* KA_packed, double* KB_packed, double* E_xc, double* rho_out);
+int pdfxc_build_uks_update(pdfxc_ctx* ctx, int functional, const double* dPA_packed, const double* dPB_packed,
+                           double* KA_packed, double* KB_packed, double* E_xc, double* rho_out);
+int pdfxc_build_uks_device(pdfxc_ctx* ctx, int functional, const double* dPA, const double* dPB, double* dKA,
+                           double* dKB, double* dE);
+int pdfxc_build_uks_update_device(pdfxc_ctx* ctx, int functional, const double* d_dPA, const double* d_dPB,
+                                  double* dKA, double* dKB, double* dE);
+/* Spin-resolved functional kernel: in n x 5 (rhoA, rhoB, gAA, gAB, gBB),
+ * out n x 6 (f, df/drhoA, df/drhoB, df/dgAA, df/dgAB, df/dgBB). */
+int pdfxc_eval_functional_uks(int functional, int64_t n, const double* in, double* out);
+
 /* XC nuclear gradient, RKS (DfCalcGridX::energyGradient,
  * DfCalcGridX.cpp:1215-1415), on the context's grid with its points moving
  * rigidly with their atoms: point_atom[p] (caller point order) gives each
@@ -186,12 +207,13 @@ int pdfxc_save_symmetric_matrix(const char* path, int64_t n, const double* packe
 int pdfxc_load_general_matrix(const char* path, int64_t rows, int64_t cols, double* colmajor);
 int pdfxc_save_general_matrix(const char* path, int64_t rows, int64_t cols, const double* colmajor);
 /* grid.mat (DfObject::getGridMatrix / saveGridMatrix; columns DfCalcGridX.h:37-54):
- * one row per point: x, y, z, w, atom, then rho (LDA RKS, 6 columns) or rho
- * and grad rho (GGA RKS, 9 columns).  load: sets the context's grid from
- * columns 0-3, the point atoms (for pdfxc_gradient_rks) from column 4, and,
- * for 6 or 9 columns, the stored rho an update-mode build adds to.  save:
- * n_rho_cols 0, 1 or 4 after x, y, z, w, atom (-1 where the context has no
- * point atoms); rho is the context's current (last built) rho. */
+ * one row per point: x, y, z, w, atom, then per spin rho (LDA: 6 columns
+ * RKS, 7 UKS) or rho and grad rho (GGA: 9 RKS, 13 UKS, alpha first).  load:
+ * sets the context's grid from columns 0-3, the point atoms (for
+ * pdfxc_gradient_rks) from column 4, and, for 6, 7, 9 or 13 columns, the
+ * stored rho an update-mode build adds to.  save: n_rho_cols 0, 1, 2, 4 or 8
+ * after x, y, z, w, atom (-1 where the context has no point atoms); rho is the
+ * context's current (last built) rho. */
 int pdfxc_load_grid_matrix(pdfxc_ctx* ctx, const char* path, int64_t* n_pts, int* n_cols);
 int pdfxc_save_grid_matrix(pdfxc_ctx* ctx, const char* path, int n_rho_cols);
 

@@ -49,6 +49,8 @@ def lib():
         L.oracle_eval_aos.argtypes = [P, i64, P, P, P, P, P]
         L.oracle_calc_xc_rks.argtypes = [P, i32, i64, P, P, P, P, d, i32]
         L.oracle_calc_xc_rks.restype = d
+        L.oracle_calc_xc_uks.argtypes = [P, i32, i64, P, P, P, P, P, P, d, i32, i32]
+        L.oracle_calc_xc_uks.restype = d
         L.oracle_lebedev.argtypes = [i32, P, P]
         L.oracle_lebedev.restype = i32
         L.oracle_radial_em.argtypes = [d, i32, i32, P, P]
@@ -147,6 +149,20 @@ def calc_xc_rks(aos, grid, PA, fid=B3LYP, density_cutoff=1e-16, n_threads=0,
     F = np.tril(F)
     F = F + np.tril(F, -1).T
     return e, F, rho
+
+
+def calc_xc_uks(aos, grid, PA, PB, fid=B3LYP, density_cutoff=1e-16, n_threads=0, want_rho=False,
+                exact_gab=False):
+    """Returns (E_xc, F_alpha, F_beta dense symmetric, rho (n,8) or None)."""
+    grid = np.ascontiguousarray(grid, dtype=np.float64).reshape(-1, 4)
+    PA = np.ascontiguousarray(PA, dtype=np.float64)
+    PB = np.ascontiguousarray(PB, dtype=np.float64)
+    FA, FB = np.zeros_like(PA), np.zeros_like(PB)
+    rho = np.zeros((grid.shape[0], 8)) if want_rho else None
+    e = lib().oracle_calc_xc_uks(aos.ref(), fid, grid.shape[0], _p(grid), _p(PA), _p(PB), _p(FA), _p(FB),
+                                 _p(rho) if want_rho else None, density_cutoff, int(exact_gab), n_threads)
+    sym = lambda F: np.tril(F) + np.tril(F, -1).T
+    return e, sym(FA), sym(FB), rho
 
 
 def eval_ao_hessians(aos, xyz):

@@ -710,6 +710,128 @@ double oracle_calc_xc_rks(const oracle_aos* aos, int functional, int64_t n_pts,
 }
 
 /* ------------------------------------------------------------------------ */
+/* UKS build: calcXCIntegForFockAndEnergy(P_A, P_B, ...) (DfCalcGridX.cpp     */
+/* :2291-2303, :2317-2330) with calcRho_*_part two-spin (:1892-1921,          */
+/* :1964-2016) and buildVxc UKS (:2093-2135 LDA, :2195-2278 GGA): a point     */
+/* counts when rhoA or rhoB exceeds the cut-off; F_A gets build_XC_Matrix     */
+/* (:2455-2500) with (vrhoA, vgAA, vgAB, grad rhoA) and F_B with (vrhoB,      */
+/* vgBB, vgAB, grad rhoB).  build_XC_Matrix multiplies vgAB by the same-spin  */
+/* gradient, so K_A holds vgAB grad rhoA where dE/dP_A has vgAB grad rhoB.    */
+/* exact_gab = 1 (test use only) takes the other-spin gradient instead, the   */
+/* exact derivative, so that finite differences can pin the rest.             */
+/* ------------------------------------------------------------------------ */
+static void add_point_matrix(double* F, int N, const double* phi, const double* gx, const double* gy,
+                             const double* gz, double coef1, double cx, double cy, double cz) {
+    for (int i = 0; i < N; ++i) {
+        const double ai = phi[i];
+        const double ci = cx * gx[i] + cy * gy[i] + cz * gz[i];
+        double* Fi = F + (size_t)i * N;
+        Fi[i] += coef1 * ai * ai + ai * ci * 2.0;
+        for (int j = 0; j < i; ++j) {
+            const double aj = phi[j];
+            double v = coef1 * ai * aj;
+            v += ai * (cx * gx[j] + cy * gy[j] + cz * gz[j]);
+            v += aj * ci;
+            Fi[j] += v;
+        }
+    }
+}
+
+static void rho_at_point(int N, const double* P, const double* phi, const double* gx, const double* gy,
+                         const double* gz, double* r) {
+    double rho = 0.0, grx = 0.0, gry = 0.0, grz = 0.0;
+    for (int i = 0; i < N; ++i) {
+        const double ai = phi[i];
+        const double* Pi = P + (size_t)i * N;
+        for (int j = 0; j < i; ++j) rho += 2.0 * Pi[j] * ai * phi[j];
+        rho += Pi[i] * ai * ai;
+        for (int j = 0; j < N; ++j) {
+            grx += Pi[j] * ai * gx[j];
+            gry += Pi[j] * ai * gy[j];
+            grz += Pi[j] * ai * gz[j];
+        }
+    }
+    r[0] = rho;
+    r[1] = 2.0 * grx;
+    r[2] = 2.0 * gry;
+    r[3] = 2.0 * grz;
+}
+
+double oracle_calc_xc_uks(const oracle_aos* aos, int functional, int64_t n_pts, const double* grid,
+                          const double* PA, const double* PB, double* FA, double* FB, double* rho_out,
+                          double density_cutoff, int exact_gab, int n_threads) {
+    const int N = aos->n_aos;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = (n_threads > 0) ? n_threads : omp_get_max_threads();
+#else
+    (void)n_threads;
+#endif
+    double* Fpriv = (double*)calloc((size_t)2 * nt * N * N, sizeof(double));
+    double energy = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt) reduction(+ : energy)
+#endif
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double* Fa = Fpriv + (size_t)2 * tid * N * N;
+        double* Fb = Fa + (size_t)N * N;
+        double* phi = (double*)malloc(sizeof(double) * 4 * N);
+        double *gx = phi + N, *gy = gx + N, *gz = gy + N;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int64_t g = 0; g < n_pts; ++g) {
+            const double* p = grid + 4 * g;
+            const double w = p[3];
+            eval_point(aos, p, phi, gx, gy, gz);
+            double ra[4], rb[4];
+            rho_at_point(N, PA, phi, gx, gy, gz, ra);
+            rho_at_point(N, PB, phi, gx, gy, gz, rb);
+            if (rho_out) {
+                for (int q = 0; q < 4; ++q) {
+                    rho_out[8 * g + q] = ra[q];
+                    rho_out[8 * g + 4 + q] = rb[q];
+                }
+            }
+            if (ra[0] > density_cutoff || rb[0] > density_cutoff) {
+                const double gaa = ra[1] * ra[1] + ra[2] * ra[2] + ra[3] * ra[3];
+                const double gab = ra[1] * rb[1] + ra[2] * rb[2] + ra[3] * rb[3];
+                const double gbb = rb[1] * rb[1] + rb[2] * rb[2] + rb[3] * rb[3];
+                const double in[5] = {ra[0], rb[0], gaa, gab, gbb};
+                double o[6];
+                functional_point(functional, in, o);
+                const double* oa = exact_gab ? rb : ra; /* gradient multiplying vgAB in F_A */
+                const double* ob = exact_gab ? ra : rb;
+                double c[3], d[3];
+                for (int k = 0; k < 3; ++k) {
+                    c[k] = w * (2.0 * o[3] * ra[1 + k] + o[4] * oa[1 + k]);
+                    d[k] = w * (2.0 * o[5] * rb[1 + k] + o[4] * ob[1 + k]);
+                }
+                add_point_matrix(Fa, N, phi, gx, gy, gz, w * o[1], c[0], c[1], c[2]);
+                add_point_matrix(Fb, N, phi, gx, gy, gz, w * o[2], d[0], d[1], d[2]);
+                energy += w * o[0];
+            }
+        }
+        free(phi);
+    }
+    for (int t = 0; t < nt; ++t) {
+        const double* Fa = Fpriv + (size_t)2 * t * N * N;
+        const double* Fb = Fa + (size_t)N * N;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j <= i; ++j) {
+                FA[(size_t)i * N + j] += Fa[(size_t)i * N + j];
+                FB[(size_t)i * N + j] += Fb[(size_t)i * N + j];
+            }
+    }
+    free(Fpriv);
+    return energy;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Lebedev rules.  Each rule is a list of octahedral orbits (TlLebedevGrid.cpp */
 /* :93-269 lists the six orbit shapes; the per-order parameters :271-, 302    */
 /* at :649-697, 590 at :816-892).  The parameters are the published           */

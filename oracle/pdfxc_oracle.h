@@ -81,6 +81,15 @@ double oracle_calc_xc_rks(const oracle_aos* aos, int functional, int64_t n_pts,
                           double* rho_out, double density_cutoff,
                           int n_threads);
 
+/* UKS build (DfCalcGridX.cpp:2291-2330 with :1892-1921, :1964-2016,
+ * :2093-2135, :2195-2278).  PA, PB: dense alpha / beta densities; FA, FB:
+ * lower triangles ACCUMULATED.  rho_out (optional): n_pts x 8 (rhoA, grad
+ * rhoA, rhoB, grad rhoB).  exact_gab: 0 = the reference (vgAB times the
+ * same-spin gradient), 1 = the exact derivative (test use). */
+double oracle_calc_xc_uks(const oracle_aos* aos, int functional, int64_t n_pts, const double* grid,
+                          const double* PA, const double* PB, double* FA, double* FB, double* rho_out,
+                          double density_cutoff, int exact_gab, int n_threads);
+
 /* Lebedev rule of the given order (6, 38, 86, 194, 302, 590 supported);
  * returns number of points written (0 if unsupported). xyz: n x 3, w: n. */
 int oracle_lebedev(int order, double* xyz, double* w);

@@ -94,19 +94,21 @@ class DfGenerateGrid:
 
 
 class WorkFiles:
-    """The files the XC step reads and writes in the work directory (fl_Work)."""
+    """The files the XC step reads and writes in the work directory (fl_Work).
+    run: the run-type suffix (DfObject.cpp:45-46): "rks", "uks_alpha", "uks_beta",
+    "roks_closed", "roks_open", "roks_alpha", "roks_beta"."""
 
     def __init__(self, work_dir="fl_Work"):
         self.dir = Path(work_dir)
 
-    def p_in(self, iteration):        # getPInMatrixPath (DfObject.cpp:677-679)
-        return self.dir / f"~Ppq.rks{iteration}.mat"
+    def p_in(self, iteration, run="rks"):          # getPInMatrixPath (DfObject.cpp:677-679)
+        return self.dir / f"~Ppq.{run}{iteration}.mat"
 
-    def diff_density(self, iteration):  # getDiffDensityMatrixPath (:665-667)
-        return self.dir / f"dP.rks{iteration}.mat"
+    def diff_density(self, iteration, run="rks"):  # getDiffDensityMatrixPath (:665-667)
+        return self.dir / f"dP.{run}{iteration}.mat"
 
-    def fxc(self, iteration):         # getFxcMatrixPath (:712-714)
-        return self.dir / f"Fxc.rks{iteration}.mat"
+    def fxc(self, iteration, run="rks"):           # getFxcMatrixPath (:712-714)
+        return self.dir / f"Fxc.{run}{iteration}.mat"
 
     def grid_matrix(self):            # getGridMatrixPath: "grid.mat", no iteration in the name
         return self.dir / "grid.mat"
@@ -135,26 +137,44 @@ class DfCalcGridX:
     def grid(self):
         return self.ctx.get_grid()
 
-    def calcXCIntegForFockAndEnergy(self, P_A, functional, F_A, keep_rho=False, update=False):
-        """update: P_A is the change of the density and rho accumulates on the
-        stored one (calcRho_*_part with m_bIsUpdateXC, DfCalcGridX.cpp:1866-1869)."""
+    def calcXCIntegForFockAndEnergy(self, *args, keep_rho=False, update=False):
+        """(P_A, functional, F_A) for RKS or (P_A, P_B, functional, F_A, F_B) for
+        UKS, as the reference's overloads (DfCalcGridX.h:195-232).  update: the
+        densities are changes and rho accumulates on the stored one
+        (calcRho_*_part with m_bIsUpdateXC, DfCalcGridX.cpp:1866-1869)."""
+        if len(args) == 3:
+            P_A, functional, F_A = args
+        elif len(args) == 5:
+            P_A, P_B, functional, F_A, F_B = args
+        else:
+            raise TypeError("calcXCIntegForFockAndEnergy(P_A, functional, F_A) or (P_A, P_B, functional, F_A, F_B)")
         if not isinstance(functional, DfFunctional_GGA) or functional.functional_id is None:
             raise TypeError("functional must be one of the DfFunctional_* classes")
-        build = self.ctx.build_rks_update if update else self.ctx.build_rks
-        E, _, rho = build(P_A, F_A, want_rho=keep_rho, functional=functional.functional_id)
+        fid = functional.functional_id
+        if len(args) == 3:
+            build = self.ctx.build_rks_update if update else self.ctx.build_rks
+            E, _, rho = build(P_A, F_A, want_rho=keep_rho, functional=fid)
+        else:
+            build = self.ctx.build_uks_update if update else self.ctx.build_uks
+            E, _, _, rho = build(P_A, P_B, F_A, F_B, want_rho=keep_rho, functional=fid)
         if keep_rho:
             self._rho = rho
         return E
 
-    def saveGridMatrix(self, path, functional):
+    def saveGridMatrix(self, path, functional, uks=False):
         """grid.mat with the rho columns the functional's type uses (DfGenerateGrid.cpp:229-239)."""
         lda = functional.functional_id in (engine.PDFXC_HFS, engine.PDFXC_SVWN)
-        self.ctx.save_grid_matrix(path, 1 if lda else 4)
+        self.ctx.save_grid_matrix(path, (1 if lda else 4) * (2 if uks else 1))
 
     def getWholeDensity(self):
+        """Integrated rho_alpha (RKS), or (rho_alpha, rho_beta) after a UKS build
+        (DfCalcGridX.cpp:2502-2537)."""
         if self._rho is None:
             raise RuntimeError("no density kept: call calcXCIntegForFockAndEnergy(..., keep_rho=True)")
-        return float((self.grid[:, 3] * self._rho[:, 0]).sum())
+        w = self.grid[:, 3]
+        if self._rho.shape[1] == 8:
+            return float((w * self._rho[:, 0]).sum()), float((w * self._rho[:, 4]).sum())
+        return float((w * self._rho[:, 0]).sum())
 
 
 class DfXCFunctional:
@@ -162,11 +182,14 @@ class DfXCFunctional:
     "BLYP" or "B88LYP", "B3LYP"); a trailing "~" (XC fitting) is removed as
     the reference removes it.  An unknown name raises, as getFxc aborts."""
 
-    def __init__(self, calc_grid, n_electrons, xc_functional="B3LYP"):
+    def __init__(self, calc_grid, n_electrons, xc_functional="B3LYP", method="RKS"):
         name = xc_functional[:-1] if xc_functional.endswith("~") else xc_functional
         if name not in engine.XC_TYPES:
             raise ValueError(f"unknown XC functional: {xc_functional}")
+        if method not in ("RKS", "UKS", "ROKS"):
+            raise ValueError(f"unknown method: {method}")
         self.m_nXCFunctional = engine.XC_TYPES[name]
+        self.method = method
         self.calc_grid = calc_grid
         self.n_electrons = n_electrons
         self.XC_energy_ = 0.0
@@ -179,24 +202,55 @@ class DfXCFunctional:
         return {engine.PDFXC_HF: 1.0, engine.PDFXC_B3LYP: 0.2}.get(self.m_nXCFunctional, 0.0)
 
     def buildXcMatrixFromFiles(self, work, iteration, update_xc=False):
-        """DfXCFunctional::buildXcMatrix for RKS with its files (DfXCFunctional.cpp:128-152):
-        P_A = P/2 from ~Ppq.rks<n>.mat (dP.rks<n>.mat, rho accumulated, in update mode
-        after the first iteration), Fxc.rks<n>.mat written, grid.mat rewritten with rho."""
+        """DfXCFunctional::buildXcMatrix with its files (DfXCFunctional.cpp:128-222).
+        RKS: P_A = P/2 from ~Ppq.rks<n>.mat; UKS: P_A, P_B from ~Ppq.uks_alpha<n>.mat,
+        ~Ppq.uks_beta<n>.mat; ROKS: P_A = P_closed/2 + P_open, P_B = P_closed/2.  In
+        update mode the dP.* files are read and, after the first iteration, rho
+        accumulates.  Fxc.<run><n>.mat are written (rks; uks_alpha / uks_beta;
+        roks_alpha / roks_beta) and grid.mat is rewritten with rho.  Returns F_xc
+        (RKS) or (F_alpha, F_beta)."""
         n = self.calc_grid.m_nNumOfAOs
-        Fxc = np.zeros(n * (n + 1) // 2)
+        load = lambda run: engine.load_symmetric_matrix(
+            work.diff_density(iteration, run) if update_xc else work.p_in(iteration, run))
+        accumulate = update_xc and iteration > 1
+        if self.method == "RKS":
+            Fxc = np.zeros(n * (n + 1) // 2)
+            if self.m_nXCFunctional == engine.PDFXC_HF:
+                self.XC_energy_ = 0.0
+            else:
+                functional = FUNCTIONAL_FOR[self.m_nXCFunctional]()
+                self.XC_energy_ = self.calc_grid.calcXCIntegForFockAndEnergy(
+                    0.5 * load("rks"), functional, Fxc, keep_rho=True, update=accumulate)
+                self.electrons_on_grid = 2.0 * self.calc_grid.getWholeDensity()
+                self.calc_grid.saveGridMatrix(work.grid_matrix(), functional)
+            engine.save_symmetric_matrix(work.fxc(iteration), Fxc)
+            return Fxc
+        if self.method == "UKS":
+            PA, PB = load("uks_alpha"), load("uks_beta")
+            runs = ("uks_alpha", "uks_beta")
+        else:
+            Pc, Po = load("roks_closed"), load("roks_open")
+            PA, PB = 0.5 * Pc + Po, 0.5 * Pc
+            runs = ("roks_alpha", "roks_beta")
+        FA, FB = self.buildXcMatrixUKS(PA, PB, update=accumulate)
+        if self.m_nXCFunctional != engine.PDFXC_HF:
+            self.calc_grid.saveGridMatrix(work.grid_matrix(), FUNCTIONAL_FOR[self.m_nXCFunctional](), uks=True)
+        engine.save_symmetric_matrix(work.fxc(iteration, runs[0]), FA)
+        engine.save_symmetric_matrix(work.fxc(iteration, runs[1]), FB)
+        return FA, FB
+
+    def buildXcMatrixUKS(self, PA_packed, PB_packed, update=False):
+        """getFxc(P_A, P_B, ...) (DfXCFunctional.h:194-240): (F_alpha, F_beta)."""
+        n = self.calc_grid.m_nNumOfAOs
+        FA, FB = np.zeros(n * (n + 1) // 2), np.zeros(n * (n + 1) // 2)
         if self.m_nXCFunctional == engine.PDFXC_HF:
             self.XC_energy_ = 0.0
-        else:
-            accumulate = update_xc and iteration > 1
-            src = work.diff_density(iteration) if update_xc else work.p_in(iteration)
-            P_A = 0.5 * engine.load_symmetric_matrix(src)
-            functional = FUNCTIONAL_FOR[self.m_nXCFunctional]()
-            self.XC_energy_ = self.calc_grid.calcXCIntegForFockAndEnergy(
-                P_A, functional, Fxc, keep_rho=True, update=accumulate)
-            self.electrons_on_grid = 2.0 * self.calc_grid.getWholeDensity()
-            self.calc_grid.saveGridMatrix(work.grid_matrix(), functional)
-        engine.save_symmetric_matrix(work.fxc(iteration), Fxc)
-        return Fxc
+            return FA, FB
+        self.XC_energy_ = self.calc_grid.calcXCIntegForFockAndEnergy(
+            np.asarray(PA_packed, dtype=np.float64), np.asarray(PB_packed, dtype=np.float64),
+            FUNCTIONAL_FOR[self.m_nXCFunctional](), FA, FB, keep_rho=True, update=update)
+        self.electrons_on_grid = sum(self.calc_grid.getWholeDensity())
+        return FA, FB
 
     def buildXcMatrix(self, P_packed):
         n = self.calc_grid.m_nNumOfAOs

@@ -333,6 +333,8 @@ static void free_grid_buffers(Context* c) {
     dfree(c->d_k_work);
     dfree(c->d_coef);
     dfree(c->d_rho);
+    dfree(c->d_coef_b);
+    dfree(c->d_rho_b);
     dfree(c->d_eblock);
     dfree(c->d_sblocks);
     dfree(c->d_chunks);
@@ -350,6 +352,8 @@ int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
     dfree(c->d_shrec);
     dfree(c->d_P);
     dfree(c->d_K);
+    dfree(c->d_P2);
+    dfree(c->d_K2);
     dfree(c->d_E);
     dfree(c->d_Ps);
     dfree(c->d_perm);
@@ -959,6 +963,19 @@ static void orientation(int na, const int* Z, const double* xyz, double O[9]) {
     }
 }
 
+int ensure_beta_buffers(Context* c) {
+    const size_t n = 4 * (size_t)c->n_slots;
+    if (!c->d_rho_b) {
+        if (int rc = dalloc(&c->d_rho_b, n)) return rc;
+        PDFXC_HIP(hipMemset(c->d_rho_b, 0, n * sizeof(double)));
+    }
+    if (!c->d_coef_b) {
+        if (int rc = dalloc(&c->d_coef_b, n)) return rc;
+        PDFXC_HIP(hipMemset(c->d_coef_b, 0, n * sizeof(double)));
+    }
+    return PDFXC_OK;
+}
+
 }  // namespace pdfxc
 
 extern "C" {
@@ -1119,7 +1136,46 @@ static int check_functional(int functional, const char* who) {
     return PDFXC_ERR_UNSUPPORTED;
 }
 
-static int build_device(Context* c, int xc, const double* dP, double* dK, double* dE, int accumulate_rho = 0) {
+// rho pass of one density into rho (the context's buffer, or the beta one)
+static int run_rho(Context* c, const double* dP, double* rho) {
+    double* saveP = c->d_P;
+    double* saveR = c->d_rho;
+    c->d_P = const_cast<double*>(dP);
+    c->d_rho = rho;
+    int rc = PDFXC_OK;
+    if (c->opts.precision != PDFXC_FP64) {
+        rc = launch_split_rho(*c);
+    } else {
+        for (const LaunchRange& r : c->rho_launches)
+            if (rc == PDFXC_OK) rc = launch_rho(*c, r);
+    }
+    c->d_P = saveP;
+    c->d_rho = saveR;
+    return rc;
+}
+
+// K pass of one set of point terms into dK (lower triangle accumulated)
+static int run_kmat(Context* c, double* coef, double* dK) {
+    double* saveC = c->d_coef;
+    double* saveK = c->d_K;
+    c->d_coef = coef;
+    c->d_K = dK;
+    int rc = PDFXC_OK;
+    if (c->opts.precision != PDFXC_FP64) {
+        rc = launch_split_kmat(*c);
+    } else {
+        for (const LaunchRange& r : c->kmat_launches)
+            if (rc == PDFXC_OK) rc = launch_kmat(*c, r);
+    }
+    c->d_coef = saveC;
+    c->d_K = saveK;
+    return rc;
+}
+
+// One RKS (nspin 1) or UKS (nspin 2) build on device buffers: the rho pass of
+// each spin, the point terms, the K pass of each spin, the energy sum.
+static int build_device(Context* c, int xc, int nspin, const double* const* dP, double* const* dK, double* dE,
+                        int accumulate_rho) {
     if (c->n_pts == 0 || c->blocks.empty()) {
         set_error("build: no grid set");
         return PDFXC_ERR_NO_GRID;
@@ -1133,33 +1189,20 @@ static int build_device(Context* c, int xc, const double* dP, double* dK, double
         c->last_build_ms = 0.0;
         return PDFXC_OK;
     }
-    double* saveP = c->d_P;
-    double* saveK = c->d_K;
-    c->d_P = const_cast<double*>(dP);
-    c->d_K = dK;
+    if (nspin == 2)
+        if (int rc = ensure_beta_buffers(c)) return rc;
     int rc = PDFXC_OK;
-    const bool split = c->opts.precision != PDFXC_FP64;
     c->rho_accumulate = accumulate_rho;
     PDFXC_HIP(hipEventRecord(c->ev[0], c->stream));
-    if (split) {
-        rc = launch_split_rho(*c);
-    } else {
-        for (const LaunchRange& r : c->rho_launches)
-            if (rc == PDFXC_OK) rc = launch_rho(*c, r);
-    }
-    if (rc == PDFXC_OK) rc = launch_xc_points(*c, xc);
+    rc = run_rho(c, dP[0], c->d_rho);
+    if (nspin == 2 && rc == PDFXC_OK) rc = run_rho(c, dP[1], c->d_rho_b);
+    if (rc == PDFXC_OK) rc = nspin == 2 ? launch_xc_points_uks(*c, xc) : launch_xc_points(*c, xc);
     PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
-    if (split) {
-        if (rc == PDFXC_OK) rc = launch_split_kmat(*c);
-    } else {
-        for (const LaunchRange& r : c->kmat_launches)
-            if (rc == PDFXC_OK) rc = launch_kmat(*c, r);
-    }
+    if (rc == PDFXC_OK) rc = run_kmat(c, c->d_coef, dK[0]);
+    if (nspin == 2 && rc == PDFXC_OK) rc = run_kmat(c, c->d_coef_b, dK[1]);
     PDFXC_HIP(hipEventRecord(c->ev[2], c->stream));
     if (rc == PDFXC_OK) rc = launch_energy_sum(*c, dE);
     PDFXC_HIP(hipEventRecord(c->ev[3], c->stream));
-    c->d_P = saveP;
-    c->d_K = saveK;
     c->rho_accumulate = 0;
     if (rc) return rc;
     PDFXC_HIP(hipEventSynchronize(c->ev[3]));
@@ -1173,6 +1216,12 @@ static int build_device(Context* c, int xc, const double* dP, double* dK, double
     c->last_kernel_ms[3] = 0.0;
     c->last_build_ms = t1 + t2 + t3;
     return PDFXC_OK;
+}
+
+static int build_device(Context* c, int xc, const double* dP, double* dK, double* dE, int accumulate_rho = 0) {
+    const double* P[1] = {dP};
+    double* K[1] = {dK};
+    return build_device(c, xc, 1, P, K, dE, accumulate_rho);
 }
 
 int pdfxc_build_rks_device(pdfxc_ctx* ctx, int functional, const double* dP, double* dK, double* dE) {
@@ -1193,6 +1242,8 @@ int pdfxc_build_rks_update_device(pdfxc_ctx* ctx, int functional, const double* 
 
 static int build_host(Context* c, int xc, const double* P_packed, double* K_packed, double* E_xc,
                       double* rho_out, int accumulate_rho);
+static int build_device(Context* c, int xc, int nspin, const double* const* dP, double* const* dK, double* dE,
+                        int accumulate_rho);
 
 int pdfxc_build_rks_update(pdfxc_ctx* ctx, int functional, const double* dP_packed, double* K_packed,
                            double* E_xc, double* rho_out) {
@@ -1212,46 +1263,142 @@ int pdfxc_build_rks(pdfxc_ctx* ctx, int functional, const double* P_packed, doub
     return build_host(c, functional, P_packed, K_packed, E_xc, rho_out, 0);
 }
 
-static int build_host(Context* c, int xc, const double* P_packed, double* K_packed, double* E_xc,
-                      double* rho_out, int accumulate_rho) {
+static int build_host(Context* c, int xc, int nspin, const double* const* P_packed, double* const* K_packed,
+                      double* E_xc, double* rho_out, int accumulate_rho) {
     if (c->n_pts == 0 || c->blocks.empty()) {
         set_error("build: no grid set");
         return PDFXC_ERR_NO_GRID;
     }
     if (xc == kHF) {
-        // no grid term: E_xc = 0, K_packed and rho_out untouched (DfXCFunctional.h:141-143)
+        // no grid term: E_xc = 0, K and rho_out untouched (DfXCFunctional.h:141-143)
         *E_xc = 0.0;
         for (double& t : c->last_kernel_ms) t = 0.0;
         c->last_build_ms = 0.0;
         return PDFXC_OK;
     }
     const int N = c->n_aos;
+    const size_t NN = (size_t)N * N;
+    if (nspin == 2 && !c->d_P2) {
+        if (int rc = dalloc(&c->d_P2, NN)) return rc;
+        if (int rc = dalloc(&c->d_K2, NN)) return rc;
+    }
+    double* dP[2] = {c->d_P, c->d_P2};
+    double* dK[2] = {c->d_K, c->d_K2};
     double* H = c->h_pinned;
-    for (int r = 0, k = 0; r < N; ++r)
-        for (int q = 0; q <= r; ++q, ++k) {
-            H[(size_t)r * N + q] = P_packed[k];
-            H[(size_t)q * N + r] = P_packed[k];
-        }
-    PDFXC_HIP(hipMemcpyAsync(c->d_P, H, sizeof(double) * (size_t)N * N, hipMemcpyHostToDevice, c->stream));
-    PDFXC_HIP(hipMemsetAsync(c->d_K, 0, sizeof(double) * (size_t)N * N, c->stream));
-    int rc = build_device(c, xc, c->d_P, c->d_K, c->d_E, accumulate_rho);
+    for (int sp = 0; sp < nspin; ++sp) {
+        for (int r = 0, k = 0; r < N; ++r)
+            for (int q = 0; q <= r; ++q, ++k) {
+                H[(size_t)r * N + q] = P_packed[sp][k];
+                H[(size_t)q * N + r] = P_packed[sp][k];
+            }
+        PDFXC_HIP(hipMemcpyAsync(dP[sp], H, sizeof(double) * NN, hipMemcpyHostToDevice, c->stream));
+        PDFXC_HIP(hipMemsetAsync(dK[sp], 0, sizeof(double) * NN, c->stream));
+        PDFXC_HIP(hipStreamSynchronize(c->stream));  // H is reused
+    }
+    int rc = build_device(c, xc, nspin, dP, dK, c->d_E, accumulate_rho);
     if (rc) return rc;
     double E = 0.0;
-    PDFXC_HIP(hipMemcpyAsync(H, c->d_K, sizeof(double) * (size_t)N * N, hipMemcpyDeviceToHost, c->stream));
     PDFXC_HIP(hipMemcpyAsync(&E, c->d_E, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    PDFXC_HIP(hipStreamSynchronize(c->stream));
-    for (int r = 0, k = 0; r < N; ++r)
-        for (int q = 0; q <= r; ++q, ++k) K_packed[k] += H[(size_t)r * N + q];
+    for (int sp = 0; sp < nspin; ++sp) {
+        PDFXC_HIP(hipMemcpyAsync(H, dK[sp], sizeof(double) * NN, hipMemcpyDeviceToHost, c->stream));
+        PDFXC_HIP(hipStreamSynchronize(c->stream));
+        for (int r = 0, k = 0; r < N; ++r)
+            for (int q = 0; q <= r; ++q, ++k) K_packed[sp][k] += H[(size_t)r * N + q];
+    }
     *E_xc = E;
     if (rho_out) {
+        const int stride = 4 * nspin;
         std::vector<double> rs(4 * (size_t)c->n_slots);
-        PDFXC_HIP(hipMemcpy(rs.data(), c->d_rho, rs.size() * sizeof(double), hipMemcpyDeviceToHost));
-        for (int64_t p = 0; p < c->n_pts; ++p) {
-            const int64_t s = c->slot_of_point[p];
-            for (int q = 0; q < 4; ++q) rho_out[4 * p + q] = rs[4 * s + q];
+        for (int sp = 0; sp < nspin; ++sp) {
+            PDFXC_HIP(hipMemcpy(rs.data(), sp ? c->d_rho_b : c->d_rho, rs.size() * sizeof(double),
+                                hipMemcpyDeviceToHost));
+            for (int64_t p = 0; p < c->n_pts; ++p) {
+                const int64_t sl = c->slot_of_point[p];
+                for (int q = 0; q < 4; ++q) rho_out[stride * p + 4 * sp + q] = rs[4 * sl + q];
+            }
         }
     }
     return PDFXC_OK;
+}
+
+static int build_host(Context* c, int xc, const double* P_packed, double* K_packed, double* E_xc,
+                      double* rho_out, int accumulate_rho) {
+    const double* P[1] = {P_packed};
+    double* K[1] = {K_packed};
+    return build_host(c, xc, 1, P, K, E_xc, rho_out, accumulate_rho);
+}
+
+// UKS (calcXCIntegForFockAndEnergy(P_A, P_B, ...), DfCalcGridX.cpp:2291-2330)
+static int uks_args(Context* c, int functional, const void* a, const void* b, const void* ka, const void* kb,
+                    const void* e, const char* who) {
+    if (!c) return arg_error(std::string(who) + ": null context");
+    if (int rc = check_functional(functional, who)) return rc;
+    if (!a || !b || !ka || !kb || !e) return arg_error(std::string(who) + ": null buffer");
+    return PDFXC_OK;
+}
+
+int pdfxc_build_uks(pdfxc_ctx* ctx, int functional, const double* PA_packed, const double* PB_packed,
+                    double* KA_packed, double* KB_packed, double* E_xc, double* rho_out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (int rc = uks_args(c, functional, PA_packed, PB_packed, KA_packed, KB_packed, E_xc, "pdfxc_build_uks"))
+        return rc;
+    const double* P[2] = {PA_packed, PB_packed};
+    double* K[2] = {KA_packed, KB_packed};
+    return build_host(c, functional, 2, P, K, E_xc, rho_out, 0);
+}
+
+int pdfxc_build_uks_update(pdfxc_ctx* ctx, int functional, const double* dPA_packed, const double* dPB_packed,
+                           double* KA_packed, double* KB_packed, double* E_xc, double* rho_out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (int rc = uks_args(c, functional, dPA_packed, dPB_packed, KA_packed, KB_packed, E_xc,
+                          "pdfxc_build_uks_update"))
+        return rc;
+    const double* P[2] = {dPA_packed, dPB_packed};
+    double* K[2] = {KA_packed, KB_packed};
+    return build_host(c, functional, 2, P, K, E_xc, rho_out, 1);
+}
+
+int pdfxc_build_uks_device(pdfxc_ctx* ctx, int functional, const double* dPA, const double* dPB, double* dKA,
+                           double* dKB, double* dE) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (int rc = uks_args(c, functional, dPA, dPB, dKA, dKB, dE, "pdfxc_build_uks_device")) return rc;
+    const double* P[2] = {dPA, dPB};
+    double* K[2] = {dKA, dKB};
+    return build_device(c, functional, 2, P, K, dE, 0);
+}
+
+int pdfxc_build_uks_update_device(pdfxc_ctx* ctx, int functional, const double* d_dPA, const double* d_dPB,
+                                  double* dKA, double* dKB, double* dE) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (int rc = uks_args(c, functional, d_dPA, d_dPB, dKA, dKB, dE, "pdfxc_build_uks_update_device")) return rc;
+    const double* P[2] = {d_dPA, d_dPB};
+    double* K[2] = {dKA, dKB};
+    return build_device(c, functional, 2, P, K, dE, 1);
+}
+
+int pdfxc_eval_functional_uks(int functional, int64_t n, const double* in, double* out) {
+    if (int rc = check_functional(functional, "pdfxc_eval_functional_uks")) return rc;
+    if (n < 0 || (n > 0 && (!in || !out))) return arg_error("pdfxc_eval_functional_uks: bad arguments");
+    if (n == 0) return PDFXC_OK;
+    if (functional == kHF) {
+        std::fill(out, out + 6 * n, 0.0);
+        return PDFXC_OK;
+    }
+    double *di = nullptr, *dout = nullptr;
+    int rc = dalloc(&di, 5 * (size_t)n);
+    if (rc == PDFXC_OK) rc = dalloc(&dout, 6 * (size_t)n);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipMemcpy(di, in, 5 * n * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_functional_uks: upload");
+    }
+    if (rc == PDFXC_OK) rc = launch_eval_functional_uks(functional, n, di, dout, nullptr);
+    if (rc == PDFXC_OK) {
+        hipError_t e = hipMemcpy(out, dout, 6 * n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_eval_functional_uks: download");
+    }
+    dfree(di);
+    dfree(dout);
+    return rc;
 }
 
 int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z, const double* xyz,

@@ -111,9 +111,13 @@ struct Context {
     void* d_k_work = nullptr;     // KWork items grouped by k_kmat class
     double* d_coef = nullptr;     // per slot: (w vrho / 2, cx, cy, cz)
     double* d_rho = nullptr;      // per slot: rho, grad rho (4)
+    double* d_rho_b = nullptr;    // UKS: beta rho, grad rho per slot (allocated by the first UKS build)
+    double* d_coef_b = nullptr;   // UKS: beta point terms
     double* d_eblock = nullptr;   // per 256 slots: energy (k_xc_points partial sums)
     double* d_P = nullptr;        // dense N x N
     double* d_K = nullptr;        // dense N x N
+    double* d_P2 = nullptr;       // UKS host entry points: beta P and K (allocated on first use)
+    double* d_K2 = nullptr;
     double* d_E = nullptr;        // 1 double
     double* h_pinned = nullptr;   // packed staging (pinned)
 
@@ -156,6 +160,9 @@ int launch_kmat(Context& c, const LaunchRange& r);
 int configure_kernels();
 int launch_energy_sum(Context& c, double* dE);
 int launch_zero(double* d, int64_t n, hipStream_t s);
+int launch_xc_points_uks(Context& c, int xc_type);
+int ensure_beta_buffers(Context* c);  // UKS beta rho / point-term buffers, zeroed when first made
+int launch_eval_functional_uks(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s);
 int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s);
 int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, double* d_out,
                            hipStream_t s);

@@ -118,21 +118,21 @@ static int write_matrix(const char* path, int type, int64_t rows, int64_t cols, 
 }
 
 // caller-order rho (n_pts x 4) <-> the context's slots
-static int download_rho(Context* c, double* rho) {
+static int download_rho(Context* c, double* rho, const double* d_src) {
     std::vector<double> rs(4 * (size_t)c->n_slots);
     PDFXC_HIP(hipStreamSynchronize(c->stream));
-    PDFXC_HIP(hipMemcpy(rs.data(), c->d_rho, rs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    PDFXC_HIP(hipMemcpy(rs.data(), d_src, rs.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int64_t p = 0; p < c->n_pts; ++p)
         for (int q = 0; q < 4; ++q) rho[4 * p + q] = rs[4 * c->slot_of_point[p] + q];
     return PDFXC_OK;
 }
 
-static int upload_rho(Context* c, const double* rho) {
+static int upload_rho(Context* c, const double* rho, double* d_dst) {
     std::vector<double> rs(4 * (size_t)c->n_slots, 0.0);
     for (int64_t p = 0; p < c->n_pts; ++p)
         for (int q = 0; q < 4; ++q) rs[4 * c->slot_of_point[p] + q] = rho[4 * p + q];
     PDFXC_HIP(hipStreamSynchronize(c->stream));
-    PDFXC_HIP(hipMemcpy(c->d_rho, rs.data(), rs.size() * sizeof(double), hipMemcpyHostToDevice));
+    PDFXC_HIP(hipMemcpy(d_dst, rs.data(), rs.size() * sizeof(double), hipMemcpyHostToDevice));
     return PDFXC_OK;
 }
 
@@ -236,11 +236,17 @@ int pdfxc_load_grid_matrix(pdfxc_ctx* ctx, const char* path, int64_t* n_pts_out,
         }
         c->gen_atom = std::move(atom);
     }
-    if (nc == 6 || nc == 9) {  // RKS rho columns: continue an update-mode SCF from the file
+    // rho columns (6: LDA RKS, 7: LDA UKS, 9: GGA RKS, 13: GGA UKS): continue an
+    // update-mode SCF from the file
+    const int per_spin = (nc == 6 || nc == 7) ? 1 : (nc == 9 || nc == 13) ? 4 : 0;
+    const int nspin = (nc == 7 || nc == 13) ? 2 : 1;
+    for (int sp = 0; per_spin > 0 && sp < nspin; ++sp) {
         std::vector<double> rho(4 * (size_t)n, 0.0);
         for (int64_t p = 0; p < n; ++p)
-            for (int q = 0; q < (nc == 9 ? 4 : 1); ++q) rho[4 * p + q] = col[(5 + q) * n + p];
-        if (int rc = upload_rho(c, rho.data())) return rc;
+            for (int q = 0; q < per_spin; ++q) rho[4 * p + q] = col[(5 + per_spin * sp + q) * n + p];
+        if (sp == 1)
+            if (int rc = ensure_beta_buffers(c)) return rc;
+        if (int rc = upload_rho(c, rho.data(), sp ? c->d_rho_b : c->d_rho)) return rc;
     }
     if (n_pts_out) *n_pts_out = n;
     if (n_cols_out) *n_cols_out = (int)nc;
@@ -250,8 +256,12 @@ int pdfxc_load_grid_matrix(pdfxc_ctx* ctx, const char* path, int64_t* n_pts_out,
 int pdfxc_save_grid_matrix(pdfxc_ctx* ctx, const char* path, int n_rho_cols) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return io_error("pdfxc_save_grid_matrix: null context");
-    if (n_rho_cols != 0 && n_rho_cols != 1 && n_rho_cols != 4)
-        return io_error("pdfxc_save_grid_matrix: n_rho_cols must be 0, 1 (LDA) or 4 (GGA)");
+    if (n_rho_cols != 0 && n_rho_cols != 1 && n_rho_cols != 2 && n_rho_cols != 4 && n_rho_cols != 8)
+        return io_error("pdfxc_save_grid_matrix: n_rho_cols must be 0, 1 / 2 (LDA RKS / UKS) or 4 / 8 (GGA RKS / UKS)");
+    if ((n_rho_cols == 2 || n_rho_cols == 8) && !c->d_rho_b) {
+        set_error("pdfxc_save_grid_matrix: no UKS build has run on this grid");
+        return PDFXC_ERR_ARGUMENT;
+    }
     if (c->n_pts == 0) {
         set_error("pdfxc_save_grid_matrix: no grid");
         return PDFXC_ERR_NO_GRID;
@@ -263,11 +273,13 @@ int pdfxc_save_grid_matrix(pdfxc_ctx* ctx, const char* path, int n_rho_cols) {
     std::copy(c->gz.begin(), c->gz.end(), m.begin() + 2 * n);
     std::copy(c->gw.begin(), c->gw.end(), m.begin() + 3 * n);
     for (int64_t p = 0; p < n; ++p) m[4 * n + p] = c->gen_atom.size() == (size_t)n ? c->gen_atom[p] : -1.0;
-    if (n_rho_cols > 0) {
+    const int nspin = (n_rho_cols == 2 || n_rho_cols == 8) ? 2 : 1;
+    const int per_spin = n_rho_cols / nspin;
+    for (int sp = 0; per_spin > 0 && sp < nspin; ++sp) {
         std::vector<double> rho(4 * (size_t)n);
-        if (int rc = download_rho(c, rho.data())) return rc;
+        if (int rc = download_rho(c, rho.data(), sp ? c->d_rho_b : c->d_rho)) return rc;
         for (int64_t p = 0; p < n; ++p)
-            for (int q = 0; q < n_rho_cols; ++q) m[(5 + q) * n + p] = rho[4 * p + q];
+            for (int q = 0; q < per_spin; ++q) m[(5 + per_spin * sp + q) * n + p] = rho[4 * p + q];
     }
     return pdfxc_save_general_matrix(path, n, nc, m.data());
 }

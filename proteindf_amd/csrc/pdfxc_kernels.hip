@@ -344,6 +344,55 @@ __global__ __launch_bounds__(256) void k_xc_points(int64_t n_slots, const double
     if (threadIdx.x == 0) epart[blockIdx.x] = s_e[0];
 }
 
+// UKS point terms (buildVxc UKS, DfCalcGridX.cpp:2093-2135 LDA, :2195-2278
+// GGA): a slot counts when rhoA or rhoB exceeds the cut-off; coef_a =
+// (w vrhoA / 2, w (2 vgAA + vgAB) grad rhoA), coef_b = (w vrhoB / 2,
+// w (2 vgBB + vgAB) grad rhoB) -- build_XC_Matrix multiplies vgAB by the
+// same-spin gradient (:2455-2470).
+template <int XC>
+__global__ __launch_bounds__(256) void k_xc_points_uks(int64_t n_slots, const double* __restrict__ pts,
+                                                       const double* __restrict__ rho_a,
+                                                       const double* __restrict__ rho_b,
+                                                       double* __restrict__ coef_a, double* __restrict__ coef_b,
+                                                       double* __restrict__ epart, double dens_cut) {
+    __shared__ double s_e[256];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double e = 0.0;
+    if (i < n_slots) {
+        const double w = pts[3 * n_slots + i];
+        const double ra = rho_a[4 * i], ax = rho_a[4 * i + 1], ay = rho_a[4 * i + 2], az = rho_a[4 * i + 3];
+        const double rb = rho_b[4 * i], bx = rho_b[4 * i + 1], by = rho_b[4 * i + 2], bz = rho_b[4 * i + 3];
+        double ca[4] = {0.0, 0.0, 0.0, 0.0}, cb[4] = {0.0, 0.0, 0.0, 0.0};
+        if (w != 0.0 && (ra > dens_cut || rb > dens_cut)) {
+            const double gaa = ax * ax + ay * ay + az * az;
+            const double gab = ax * bx + ay * by + az * bz;
+            const double gbb = bx * bx + by * by + bz * bz;
+            const XcPointU f = eval_uks(XC, ra, rb, gaa, gab, gbb);
+            const double fa = 2.0 * f.vgaa + f.vgab, fb = 2.0 * f.vgbb + f.vgab;
+            ca[0] = 0.5 * (w * f.va);
+            ca[1] = w * fa * ax;
+            ca[2] = w * fa * ay;
+            ca[3] = w * fa * az;
+            cb[0] = 0.5 * (w * f.vb);
+            cb[1] = w * fb * bx;
+            cb[2] = w * fb * by;
+            cb[3] = w * fb * bz;
+            e = w * f.e;
+        }
+        for (int q = 0; q < 4; ++q) {
+            coef_a[4 * i + q] = ca[q];
+            coef_b[4 * i + q] = cb[q];
+        }
+    }
+    s_e[threadIdx.x] = e;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) s_e[threadIdx.x] += s_e[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) epart[blockIdx.x] = s_e[0];
+}
+
 // ---------------------------------------------------------------------------
 // K pass consumer: NT accumulator tiles of K_bb (a contiguous run [tb, te)
 // of lower tiles, t = I (I + 1) / 2 + J, rounded up to a multiple of 4 with
@@ -580,6 +629,22 @@ __global__ __launch_bounds__(256) void k_eval_functional(int64_t n, const double
     out[4 * i + 3] = f.vgab;
 }
 
+template <int XC>
+__global__ __launch_bounds__(256) void k_eval_functional_uks(int64_t n, const double* __restrict__ in,
+                                                             double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double* x = in + 5 * i;
+    const XcPointU f = eval_uks(XC, x[0], x[1], x[2], x[3], x[4]);
+    double* o = out + 6 * i;
+    o[0] = f.e;
+    o[1] = f.va;
+    o[2] = f.vb;
+    o[3] = f.vgaa;
+    o[4] = f.vgab;
+    o[5] = f.vgbb;
+}
+
 // all AOs at arbitrary points, one thread per (point, shell): out = 4 planes n x N
 __global__ __launch_bounds__(256) void k_eval_aos_points(int64_t n, const double* __restrict__ xyz,
                                                          const DeviceShell* __restrict__ shells,
@@ -697,6 +762,25 @@ int launch_xc_points(Context& c, int xc_type, double* d_ewf) {
     return check_launch("k_xc_points");
 }
 
+template <int XC>
+static void xc_points_uks_one(Context& c) {
+    const int64_t g = (c.n_slots + 255) / 256;
+    hipLaunchKernelGGL(k_xc_points_uks<XC>, dim3((unsigned)g), dim3(256), 0, c.stream, c.n_slots, c.d_pts,
+                       c.d_rho, c.d_rho_b, c.d_coef, c.d_coef_b, c.d_eblock, c.opts.density_cutoff);
+}
+
+int launch_xc_points_uks(Context& c, int xc_type) {
+    switch (xc_type) {
+        case kHFS: xc_points_uks_one<kHFS>(c); break;
+        case kSVWN: xc_points_uks_one<kSVWN>(c); break;
+        case kHFB: xc_points_uks_one<kHFB>(c); break;
+        case kBLYP: xc_points_uks_one<kBLYP>(c); break;
+        case kB3LYP: xc_points_uks_one<kB3LYP>(c); break;
+        default: set_error("k_xc_points_uks: no grid functional " + std::to_string(xc_type)); return PDFXC_ERR_UNSUPPORTED;
+    }
+    return check_launch("k_xc_points_uks");
+}
+
 static int kmat_one(Context& c, const LaunchRange& r) {
     hipLaunchKernelGGL(k_kmat, dim3(r.count), dim3(768), r.lds, c.stream,
                        reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
@@ -752,6 +836,20 @@ int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d
         default: set_error("k_eval_functional: no grid functional " + std::to_string(xc_type)); return PDFXC_ERR_UNSUPPORTED;
     }
     return check_launch("k_eval_functional");
+}
+
+int launch_eval_functional_uks(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s) {
+    if (n <= 0) return PDFXC_OK;
+    const dim3 g((unsigned)((n + 255) / 256));
+    switch (xc_type) {
+        case kHFS: hipLaunchKernelGGL(k_eval_functional_uks<kHFS>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kSVWN: hipLaunchKernelGGL(k_eval_functional_uks<kSVWN>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kHFB: hipLaunchKernelGGL(k_eval_functional_uks<kHFB>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kBLYP: hipLaunchKernelGGL(k_eval_functional_uks<kBLYP>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        case kB3LYP: hipLaunchKernelGGL(k_eval_functional_uks<kB3LYP>, g, dim3(256), 0, s, n, d_in, d_out); break;
+        default: set_error("k_eval_functional_uks: no grid functional " + std::to_string(xc_type)); return PDFXC_ERR_UNSUPPORTED;
+    }
+    return check_launch("k_eval_functional_uks");
 }
 
 int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, double* d_out,

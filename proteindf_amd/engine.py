@@ -38,7 +38,8 @@ EXPORTED = [
     "pdfxc_build_rks_update_device", "pdfxc_selftest_mfma", "pdfxc_selftest_mfma_split",
     "pdfxc_selftest_copy", "pdfxc_matrix_file_info", "pdfxc_load_symmetric_matrix",
     "pdfxc_save_symmetric_matrix", "pdfxc_load_general_matrix", "pdfxc_save_general_matrix",
-    "pdfxc_load_grid_matrix", "pdfxc_save_grid_matrix",
+    "pdfxc_load_grid_matrix", "pdfxc_save_grid_matrix", "pdfxc_build_uks", "pdfxc_build_uks_update",
+    "pdfxc_build_uks_device", "pdfxc_build_uks_update_device", "pdfxc_eval_functional_uks",
 ]
 # matrix file types (TlMatrixObject::MatrixType)
 PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
@@ -104,6 +105,11 @@ def lib():
         L.pdfxc_save_general_matrix.argtypes = [c, i64, i64, P]
         L.pdfxc_load_grid_matrix.argtypes = [P, c, P, P]
         L.pdfxc_save_grid_matrix.argtypes = [P, c, i32]
+        L.pdfxc_build_uks.argtypes = [P, i32, P, P, P, P, P, P]
+        L.pdfxc_build_uks_update.argtypes = [P, i32, P, P, P, P, P, P]
+        L.pdfxc_build_uks_device.argtypes = [P, i32, P, P, P, P, P]
+        L.pdfxc_build_uks_update_device.argtypes = [P, i32, P, P, P, P, P]
+        L.pdfxc_eval_functional_uks.argtypes = [i32, i64, P, P]
         _lib = L
     return _lib
 
@@ -138,6 +144,15 @@ def eval_functional(rho_a, gamma_aa, functional=PDFXC_B3LYP):
     x = np.ascontiguousarray(np.stack([np.ravel(rho_a), np.ravel(gamma_aa)], 1), dtype=np.float64)
     out = np.zeros((x.shape[0], 4))
     check(lib().pdfxc_eval_functional(functional, x.shape[0], _p(x), _p(out)))
+    return out
+
+
+def eval_functional_uks(inputs, functional=PDFXC_B3LYP):
+    """Spin-resolved functional kernel: inputs (n, 5) rhoA, rhoB, gAA, gAB, gBB ->
+    (n, 6) f, df/drhoA, df/drhoB, df/dgAA, df/dgAB, df/dgBB."""
+    x = np.ascontiguousarray(np.asarray(inputs, dtype=np.float64).reshape(-1, 5))
+    out = np.zeros((x.shape[0], 6))
+    check(lib().pdfxc_eval_functional_uks(functional, x.shape[0], _p(x), _p(out)))
     return out
 
 
@@ -296,6 +311,32 @@ class XcContext:
         rho = np.zeros((self.n_pts, 4)) if want_rho else None
         check(lib().pdfxc_build_rks_update(self.h, functional, _p(P), _p(K), ctypes.byref(E), _p(rho)))
         return E.value, K, rho
+
+    def _uks(self, fn, PA, PB, KA, KB, want_rho, functional):
+        n = self.n_aos
+        PA = np.ascontiguousarray(PA, dtype=np.float64)
+        PB = np.ascontiguousarray(PB, dtype=np.float64)
+        if PA.shape != (n * (n + 1) // 2,) or PB.shape != PA.shape:
+            raise ValueError("PA and PB must hold N(N+1)/2 values")
+        KA = np.zeros_like(PA) if KA is None else KA
+        KB = np.zeros_like(PB) if KB is None else KB
+        for K in (KA, KB):
+            if K.dtype != np.float64 or not K.flags.c_contiguous or K.shape != PA.shape:
+                raise ValueError("K must be a contiguous float64 array of N(N+1)/2 values")
+        E = ctypes.c_double()
+        rho = np.zeros((self.n_pts, 8)) if want_rho else None
+        check(fn(self.h, functional, _p(PA), _p(PB), _p(KA), _p(KB), ctypes.byref(E), _p(rho)))
+        return E.value, KA, KB, rho
+
+    def build_uks(self, PA_packed, PB_packed, KA_packed=None, KB_packed=None, want_rho=False,
+                  functional=PDFXC_B3LYP):
+        """UKS build: returns (E_xc, K_alpha, K_beta, rho (n, 8) or None)."""
+        return self._uks(lib().pdfxc_build_uks, PA_packed, PB_packed, KA_packed, KB_packed, want_rho, functional)
+
+    def build_uks_update(self, dPA_packed, dPB_packed, KA_packed=None, KB_packed=None, want_rho=False,
+                         functional=PDFXC_B3LYP):
+        return self._uks(lib().pdfxc_build_uks_update, dPA_packed, dPB_packed, KA_packed, KB_packed, want_rho,
+                         functional)
 
     def build_rks_device(self, dP_ptr, dK_ptr, dE_ptr, functional=PDFXC_B3LYP):
         check(lib().pdfxc_build_rks_device(self.h, functional, ctypes.c_void_p(dP_ptr),
