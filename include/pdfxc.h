@@ -217,6 +217,14 @@ int pdfxc_save_general_matrix(const char* path, int64_t rows, int64_t cols, cons
 int pdfxc_load_grid_matrix(pdfxc_ctx* ctx, const char* path, int64_t* n_pts, int* n_cols);
 int pdfxc_save_grid_matrix(pdfxc_ctx* ctx, const char* path, int n_rho_cols);
 
+/* Density of P at arbitrary points: DfCalcGridX::gridDensity
+ * (DfCalcGridX.cpp:1038-1044) as TlDensField::makeDensFld uses it
+ * (src/tools/TlDensField.cpp:35-53): rho = phi^T P phi with P as given (pass
+ * the total density for the total rho).  xyz: n x 3; rho: n; grad_rho
+ * (may be NULL): n x 3.  The context's grid and stored rho are not changed. */
+int pdfxc_density_at_points(pdfxc_ctx* ctx, const double* P_packed, int64_t n, const double* xyz, double* rho,
+                            double* grad_rho);
+
 /* AO values and gradients of all AOs at arbitrary points (getAOs / getDAOs,
  * DfCalcGridX.cpp:655-769).  Outputs n_pts x n_aos row-major. */
 int pdfxc_eval_aos(pdfxc_ctx* ctx, int64_t n_pts, const double* xyz, double* phi,

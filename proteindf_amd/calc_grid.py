@@ -177,6 +177,17 @@ class DfCalcGridX:
         return float((w * self._rho[:, 0]).sum())
 
 
+class TlDensField:
+    """makeDensFld (src/tools/TlDensField.cpp:35-53): the density of P at the
+    given points, through the engine's gridDensity."""
+
+    def __init__(self, calc_grid):
+        self.calc_grid = calc_grid
+
+    def makeDensFld(self, P_packed, grids):
+        return self.calc_grid.ctx.density_at_points(P_packed, grids)
+
+
 class DfXCFunctional:
     """xc_functional: the reference's names ("HF", "HFS", "SVWN", "HFB",
     "BLYP" or "B88LYP", "B3LYP"); a trailing "~" (XC fitting) is removed as

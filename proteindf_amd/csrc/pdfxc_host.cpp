@@ -1551,6 +1551,73 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
     return PDFXC_OK;
 }
 
+// Density (and its gradient) of P at arbitrary points: DfCalcGridX::gridDensity
+// (DfCalcGridX.cpp:1038-1044) as TlDensField::makeDensFld calls it
+// (src/tools/TlDensField.cpp:35-53).  The points go through the same block
+// pipeline as a grid, in a context of their own, so the caller's grid and
+// stored rho are untouched.
+int pdfxc_density_at_points(pdfxc_ctx* ctx, const double* P_packed, int64_t n, const double* xyz, double* rho,
+                            double* grad_rho) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_density_at_points: null context");
+    if (n < 0 || (n > 0 && (!P_packed || !xyz || !rho))) return arg_error("pdfxc_density_at_points: bad arguments");
+    if (n == 0) return PDFXC_OK;
+    const int N = c->n_aos;
+    std::vector<double> center(3 * (size_t)N), exps((size_t)N * kMaxPrim, 0.0), coefs((size_t)N * kMaxPrim, 0.0);
+    std::vector<int> type(N), nprim(N);
+    static const int first_type[3] = {0, 1, 4};
+    for (const Shell& sh : c->shells)
+        for (int m = 0; m < 2 * sh.l + 1; ++m) {
+            const int a = sh.first_ao + m;
+            type[a] = first_type[sh.l] + m;
+            nprim[a] = sh.nprim;
+            for (int d = 0; d < 3; ++d) center[3 * a + d] = sh.center[d];
+            for (int k = 0; k < sh.nprim; ++k) {
+                exps[(size_t)a * kMaxPrim + k] = sh.exps[k];
+                coefs[(size_t)a * kMaxPrim + k] = sh.coefs[k];
+            }
+        }
+    pdfxc_opts o = c->opts;
+    o.precision = PDFXC_FP64;
+    pdfxc_ctx* th = nullptr;
+    int rc = pdfxc_ctx_create(&th, N, center.data(), type.data(), nprim.data(), exps.data(), coefs.data(), kMaxPrim,
+                              &o);
+    if (rc) return rc;
+    Context* t = reinterpret_cast<Context*>(th);
+    std::vector<double> X(n), Y(n), Zc(n), W(n, 1.0);
+    for (int64_t i = 0; i < n; ++i) {
+        X[i] = xyz[3 * i];
+        Y[i] = xyz[3 * i + 1];
+        Zc[i] = xyz[3 * i + 2];
+    }
+    rc = pdfxc_set_grid(th, n, X.data(), Y.data(), Zc.data(), W.data());
+    if (rc == PDFXC_OK) {
+        double* H = t->h_pinned;
+        for (int r = 0, k = 0; r < N; ++r)
+            for (int q = 0; q <= r; ++q, ++k) {
+                H[(size_t)r * N + q] = P_packed[k];
+                H[(size_t)q * N + r] = P_packed[k];
+            }
+        hipError_t e = hipMemcpyAsync(t->d_P, H, sizeof(double) * (size_t)N * N, hipMemcpyHostToDevice, t->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_density_at_points: P upload");
+    }
+    if (rc == PDFXC_OK) rc = run_rho(t, t->d_P, t->d_rho);
+    if (rc == PDFXC_OK) {
+        std::vector<double> rs(4 * (size_t)t->n_slots);
+        hipError_t e = hipStreamSynchronize(t->stream);
+        if (e == hipSuccess) e = hipMemcpy(rs.data(), t->d_rho, rs.size() * sizeof(double), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_density_at_points: download");
+        for (int64_t i = 0; rc == PDFXC_OK && i < n; ++i) {
+            const int64_t sl = t->slot_of_point[i];
+            rho[i] = rs[4 * sl];
+            if (grad_rho)
+                for (int d = 0; d < 3; ++d) grad_rho[3 * i + d] = rs[4 * sl + 1 + d];
+        }
+    }
+    pdfxc_ctx_destroy(th);
+    return rc;
+}
+
 int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* out) {
     if (int rc = check_functional(functional, "pdfxc_eval_functional")) return rc;
     if (n < 0 || (n > 0 && (!in || !out))) return arg_error("pdfxc_eval_functional: bad arguments");

@@ -40,6 +40,7 @@ EXPORTED = [
     "pdfxc_save_symmetric_matrix", "pdfxc_load_general_matrix", "pdfxc_save_general_matrix",
     "pdfxc_load_grid_matrix", "pdfxc_save_grid_matrix", "pdfxc_build_uks", "pdfxc_build_uks_update",
     "pdfxc_build_uks_device", "pdfxc_build_uks_update_device", "pdfxc_eval_functional_uks",
+    "pdfxc_density_at_points",
 ]
 # matrix file types (TlMatrixObject::MatrixType)
 PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
@@ -110,6 +111,7 @@ def lib():
         L.pdfxc_build_uks_device.argtypes = [P, i32, P, P, P, P, P]
         L.pdfxc_build_uks_update_device.argtypes = [P, i32, P, P, P, P, P]
         L.pdfxc_eval_functional_uks.argtypes = [i32, i64, P, P]
+        L.pdfxc_density_at_points.argtypes = [P, P, i64, P, P, P]
         _lib = L
     return _lib
 
@@ -267,6 +269,17 @@ class XcContext:
 
     def save_grid_matrix(self, path, n_rho_cols=4):
         check(lib().pdfxc_save_grid_matrix(self.h, _path(path), n_rho_cols))
+
+    def density_at_points(self, P_packed, xyz, want_grad=False):
+        """rho = phi^T P phi at arbitrary points (gridDensity); returns rho (n,) or (rho, grad (n, 3))."""
+        P = np.ascontiguousarray(P_packed, dtype=np.float64)
+        if P.shape != (self.n_aos * (self.n_aos + 1) // 2,):
+            raise ValueError("P_packed must hold N(N+1)/2 values")
+        x = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(-1, 3))
+        rho = np.zeros(x.shape[0])
+        grad = np.zeros((x.shape[0], 3)) if want_grad else None
+        check(lib().pdfxc_density_at_points(self.h, _p(P), x.shape[0], _p(x), _p(rho), _p(grad)))
+        return (rho, grad) if want_grad else rho
 
     def get_grid(self):
         cols = [np.zeros(self.n_pts) for _ in range(4)]

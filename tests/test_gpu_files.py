@@ -103,3 +103,22 @@ def test_xc_step_through_work_files(tmp_path, xc):
     assert np.abs(guess.unpack_lower(F_update, n) - K_ref).max() <= 1e-10
     assert np.abs(guess.unpack_lower(F_full, n) - K_ref).max() <= 1e-10
     assert abs(xcf.electrons_on_grid - 2.0 * (grid[:, 3] * rho_ref[:, 0]).sum()) <= 1e-9
+
+
+def test_density_field_matches_oracle():
+    # TlDensField / gridDensity: rho of P at arbitrary points, the caller's grid untouched
+    mol, info, P1, P2 = _case(10)
+    cg = calc_grid.DfCalcGridX(mol, n_rad=30, n_ang=86, info=info)
+    grid0 = cg.grid
+    rng = np.random.default_rng(4)
+    pts = mol.xyz[rng.integers(0, mol.n_atoms, 3000)] + rng.normal(0, 1.2, size=(3000, 3))
+    pts[:30] = mol.xyz.mean(0) + rng.normal(0, 12.0, size=(30, 3))
+    say("density field", len(pts), "points")
+    rho = calc_grid.TlDensField(cg).makeDensFld(guess.pack_lower(P1), pts)
+    rho2, grad = cg.ctx.density_at_points(guess.pack_lower(P1), pts, want_grad=True)
+    np.testing.assert_array_equal(rho, rho2)
+    ref = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), np.c_[pts, np.ones(len(pts))], P1, want_rho=True)[2]
+    np.testing.assert_allclose(rho, ref[:, 0], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(grad, ref[:, 1:], rtol=1e-11, atol=1e-14)
+    np.testing.assert_array_equal(cg.grid, grid0)
+    cg.ctx.close()
