@@ -78,6 +78,8 @@ typedef struct pdfxc_stats {
     int max_nao;             /* max N_b */
     double last_build_ms;    /* device time of the last build (HIP events) */
     double last_kernel_ms[4];/* per kernel: k_rho (rho + functional), k_kmat (K_xc), energy sum, unused */
+    int64_t n_k_fp64_blocks; /* split precision: blocks whose K pass runs in FP64 (too wide for the
+                                bf16 K kernel's LDS) */
 } pdfxc_stats;
 
 typedef struct pdfxc_ctx pdfxc_ctx;

@@ -633,6 +633,8 @@ static int build_blocks(Context& c) {
     c.sshell_col.clear();
     c.srho_launches.clear();
     c.sk_launches.clear();
+    c.k_fp64_launches.clear();
+    c.n_k_fp64_blocks = 0;
     if (c.opts.precision != PDFXC_FP64) {
         const int S = (c.opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
         c.sblocks.resize(nb);
@@ -692,8 +694,13 @@ static int build_blocks(Context& c) {
         }
         // k_kmat_split items: lower 2 x 2 tile patches of the compact columns,
         // at most 32 per item (8 consumer waves x 4); two slab buffers if they fit
+        size_t k_lds_max = 0;
+        if (int rc = split_kmat_max_dyn_lds(S, &k_lds_max)) return rc;
+        std::vector<char> k_fp64_block(nb, 0);
         for (int b = 0; b < nb; ++b) {
             const BlockDesc& B = c.blocks[b];
+            const size_t first_item = skw.size();
+            bool wide = false;
             const int T = B.ldn / 16;
             const int Tp = (T + 1) / 2;
             const int np = Tp * (Tp + 1) / 2;
@@ -713,18 +720,33 @@ static int build_blocks(Context& c) {
                     w.ps = nbuf;
                     lds = ((size_t)nbuf * 2 * S * w.ncols * 16 * 2 + 15) / 16 * 16 + sizeof(double) * (size_t)nsh * RS +
                           sizeof(int) * (size_t)(w.ncols + 2 * nsh);
-                    if (lds <= (size_t)kMaxDynLds) break;
+                    if (lds <= k_lds_max) break;
                 }
-                if (lds > (size_t)kMaxDynLds) {
-                    set_error("set_grid: a block with " + std::to_string(B.nao) +
-                              " AOs needs more LDS than a workgroup has in the split K kernel (raise ao_screen)");
-                    return PDFXC_ERR_UNSUPPORTED;
+                if (lds > k_lds_max) {
+                    wide = true;
+                    break;
                 }
                 sk_lds = std::max(sk_lds, lds);
                 skw.push_back(w);
             }
+            if (wide) {
+                // the 16-point bf16 slabs of this block's widest item exceed a
+                // workgroup's LDS: its K pass runs in FP64 (k_kmat, whose slab
+                // depth adapts), which is at least as accurate
+                skw.resize(first_item);
+                k_fp64_block[b] = 1;
+                ++c.n_k_fp64_blocks;
+            }
         }
         if (!skw.empty()) c.sk_launches.push_back(LaunchRange{0, 0, (int)skw.size(), sk_lds});
+        // FP64 K items of those blocks, appended to the FP64 work list
+        for (int cls = 0; cls < 2; ++cls) {
+            LaunchRange r{cls, (int)kall.size(), 0, klds[cls]};
+            for (const KWork& w : kw[cls])
+                if (k_fp64_block[w.block]) kall.push_back(w);
+            r.count = (int)kall.size() - r.offset;
+            if (r.count) c.k_fp64_launches.push_back(r);
+        }
     }
 
     // device buffers
@@ -1163,6 +1185,8 @@ static int run_kmat(Context* c, double* coef, double* dK) {
     int rc = PDFXC_OK;
     if (c->opts.precision != PDFXC_FP64) {
         rc = launch_split_kmat(*c);
+        for (const LaunchRange& r : c->k_fp64_launches)
+            if (rc == PDFXC_OK) rc = launch_kmat(*c, r);
     } else {
         for (const LaunchRange& r : c->kmat_launches)
             if (rc == PDFXC_OK) rc = launch_kmat(*c, r);
@@ -1676,13 +1700,14 @@ int pdfxc_get_stats(pdfxc_ctx* ctx, pdfxc_stats* s) {
     s->n_points = c->n_pts;
     s->n_blocks = (int64_t)c->blocks.size();
     s->n_chunks = (c->opts.precision != PDFXC_FP64)
-                      ? (int64_t)(c->srho_launches.size() + c->sk_launches.size() + 2)
+                      ? (int64_t)(c->srho_launches.size() + c->sk_launches.size() + c->k_fp64_launches.size() + 2)
                       : (int64_t)(c->rho_launches.size() + c->kmat_launches.size());
     s->flops = c->flops;
     s->mean_nao = c->mean_nao;
     s->max_nao = c->max_nao;
     s->last_build_ms = c->last_build_ms;
     for (int k = 0; k < 4; ++k) s->last_kernel_ms[k] = c->last_kernel_ms[k];
+    s->n_k_fp64_blocks = c->n_k_fp64_blocks;
     return PDFXC_OK;
 }
 

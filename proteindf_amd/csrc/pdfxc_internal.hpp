@@ -135,6 +135,8 @@ struct Context {
     int* d_sshell_col = nullptr;
     int* d_srho_work = nullptr;
     void* d_sk_work = nullptr;          // KWork items of k_kmat_split
+    std::vector<LaunchRange> k_fp64_launches;  // k_kmat items of blocks too wide for k_kmat_split
+    int64_t n_k_fp64_blocks = 0;
 
     int rho_accumulate = 0;             // update mode: the rho kernels add rho[dP] to d_rho
 
@@ -173,6 +175,7 @@ int launch_mfma_selftest(double* d_out, hipStream_t s);
 int split_points_per_workgroup(int cls);
 int launch_split_rho(Context& c);   // P slices + k_rho_split
 int launch_split_kmat(Context& c);  // k_kmat_split
+int split_kmat_max_dyn_lds(int S, size_t* out);
 int launch_mfma_split_selftest(float* d_out, hipStream_t s);
 // XC nuclear gradient (pdfxc_grad.hip); d_gacc: functional term [3 n_atoms] then weight term [3 n_atoms]
 int launch_grad_fn(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,

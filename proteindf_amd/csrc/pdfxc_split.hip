@@ -642,6 +642,17 @@ int launch_split_kmat(Context& c) {
     return rc;
 }
 
+// dynamic LDS a k_kmat_split<S> workgroup may take: the 160 KiB of a gfx950
+// workgroup less the kernel's static LDS, read from the code object
+int split_kmat_max_dyn_lds(int S, size_t* out) {
+    hipFuncAttributes a;
+    const hipError_t e = (S == 3) ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<3>))
+                                  : hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<2>));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(k_kmat_split)");
+    *out = (size_t)160 * 1024 - a.sharedSizeBytes;
+    return PDFXC_OK;
+}
+
 int launch_mfma_split_selftest(float* d_out, hipStream_t s) {
     hipLaunchKernelGGL(k_mfma_split_selftest, dim3(1), dim3(64), 0, s, d_out);
     return check("k_mfma_split_selftest");

@@ -62,7 +62,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("n_points", ctypes.c_int64), ("n_blocks", ctypes.c_int64),
                 ("n_chunks", ctypes.c_int64), ("flops", ctypes.c_double),
                 ("mean_nao", ctypes.c_double), ("max_nao", ctypes.c_int),
-                ("last_build_ms", ctypes.c_double), ("last_kernel_ms", ctypes.c_double * 4)]
+                ("last_build_ms", ctypes.c_double), ("last_kernel_ms", ctypes.c_double * 4),
+                ("n_k_fp64_blocks", ctypes.c_int64)]
 
 
 _lib = None

@@ -99,3 +99,48 @@ def test_split_zero_density_and_accumulate():
     ctx.close()
     assert E1 == E2
     np.testing.assert_allclose(K2 - 0.25, K1, rtol=0, atol=1e-14)
+
+
+def _ball_grid(center, radius, n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    pts = center + v * radius * rng.uniform(size=(n, 1)) ** (1 / 3)
+    return np.c_[pts, rng.uniform(0.001, 0.05, size=n)]
+
+
+def test_wide_blocks_take_the_fp64_k_pass():
+    # 128-point blocks spread over a ball in the middle of (Gly)30 keep 600-700
+    # AOs at 1e-12 screening; the 16-point bf16 slabs of hi+mid+lo for that
+    # many columns exceed a workgroup's LDS, so those blocks' K pass runs in
+    # FP64 (the FP64 kernels' slab depth adapts).  The ball radius is the
+    # smallest of 6, 7, 8 bohr whose blocks are that wide (set-up only).
+    mol = molecules.polyglycine_helix(30, seed=2)
+    info = calc_grid.orbital_info(mol)
+    PA = 0.5 * guess.density_guess(mol, info, seed=1, eps=0.05)
+    grid = None
+    for radius in (6.0, 7.0, 8.0):
+        g = _ball_grid(mol.xyz.mean(0), radius, 512, 5)
+        ctx = engine.XcContext(info, engine.default_opts(precision=engine.PDFXC_SPLIT_BF16X3, ao_screen=1e-12))
+        ctx.set_grid(g)
+        st = ctx.stats()
+        ctx.close()
+        say(f"ball {radius}: max N_b {st.max_nao}, FP64-K blocks {st.n_k_fp64_blocks}")
+        if st.n_k_fp64_blocks > 0:
+            grid = g
+            break
+    assert grid is not None
+    out = {}
+    for prec in (engine.PDFXC_FP64, engine.PDFXC_SPLIT_BF16X3):
+        ctx = engine.XcContext(info, engine.default_opts(precision=prec, ao_screen=1e-12))
+        ctx.set_grid(grid)
+        st = ctx.stats()
+        E, K, _ = ctx.build_rks(guess.pack_lower(PA))
+        say(f"  precision {prec}: max N_b {st.max_nao}, FP64-K blocks {st.n_k_fp64_blocks}, E {E}")
+        out[prec] = (E, K, st)
+        ctx.close()
+    assert out[engine.PDFXC_FP64][2].n_k_fp64_blocks == 0
+    dE = abs(out[engine.PDFXC_SPLIT_BF16X3][0] - out[engine.PDFXC_FP64][0])
+    dK = np.abs(out[engine.PDFXC_SPLIT_BF16X3][1] - out[engine.PDFXC_FP64][1]).max()
+    say(f"  |dE| {dE:.3e} max|dK| {dK:.3e}")
+    assert dE <= BOUNDS[engine.PDFXC_SPLIT_BF16X3][0] and dK <= BOUNDS[engine.PDFXC_SPLIT_BF16X3][1]
