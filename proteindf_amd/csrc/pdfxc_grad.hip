@@ -164,17 +164,17 @@ __global__ __launch_bounds__(512) void k_grad_fn(const int* __restrict__ work, c
     {
         const int nI = (T > wave) ? (T - wave + 7) / 8 : 0;
         switch (nI) {
-            case 1: grad_tiles<1>(wave, lane, T, s_cols, s_phi, s_w, P, N); break;
+            case 1: grad_tiles<1>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp); break;
             case 2:
-                if constexpr (NIMAX >= 2) grad_tiles<2>(wave, lane, T, s_cols, s_phi, s_w, P, N);
+                if constexpr (NIMAX >= 2) grad_tiles<2>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= 8 NIMAX
                 break;
             case 3:
-                if constexpr (NIMAX >= 3) grad_tiles<3>(wave, lane, T, s_cols, s_phi, s_w, P, N);
+                if constexpr (NIMAX >= 3) grad_tiles<3>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp);
                 else __syncthreads();
                 break;
             case 4:
-                if constexpr (NIMAX >= 4) grad_tiles<4>(wave, lane, T, s_cols, s_phi, s_w, P, N);
+                if constexpr (NIMAX >= 4) grad_tiles<4>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp);
                 else __syncthreads();
                 break;
             default: __syncthreads(); break;

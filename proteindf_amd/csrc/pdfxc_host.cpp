@@ -508,6 +508,9 @@ static int build_blocks(Context& c) {
         B.col_begin = (int)c.block_cols.size();
         B.nao = t.nao;
         B.ldn = std::max(kColPad, (t.nao + kColPad - 1) / kColPad * kColPad);
+        B.ldp = c.n_aos;  // one molecule: P and K start at 0
+        B.poff = 0;
+        B.psoff = 0;
         if (B.ldn > 1024) {
             set_error("set_grid: a block keeps " + std::to_string(B.nao) +
                       " AOs; more than 1024 per block is not supported (raise ao_screen)");

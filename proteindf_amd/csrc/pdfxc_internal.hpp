@@ -38,7 +38,9 @@ struct BlockDesc {
     int nao;          // screened AO columns N_b
     int ldn;          // N_b padded to a multiple of kColPad
     int col_begin;    // offset into the block-AO-index list
-    int pad;
+    int ldp;          // AO count of the block's molecule: leading dimension of its P and K
+    int64_t poff;     // offset of the molecule's dense P and K in the P / K buffers
+    int64_t psoff;    // offset of the molecule's P slices (split precision)
 };
 
 // Split-precision column layout of a block: nchunk 4-aligned runs of AO

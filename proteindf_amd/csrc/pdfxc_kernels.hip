@@ -187,6 +187,8 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ldn = B.ldn, nao = B.nao, nsh = B.nshell;
     const int T = ldn >> 4;
+    const double* Pb = P + B.poff;  // the block's molecule
+    const int ldp = B.ldp;
     // LDS: plane [max(ldn PB, 4 x 512)] | records [nsh][RS] | cols [ldn] | shcol [nsh] | sorder [nsh]
     double* s_phi = s_dyn;
     double* s_rec = s_phi + max(ldn * PB, 4 * 512);
@@ -230,33 +232,33 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
         const int wI = wave % WI, wP = wave / WI;
         const int nI = (T > wI) ? (T - wI + WI - 1) / WI : 0;
         switch (nI) {
-            case 1: rho_tiles<PB, RT, 1, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N); break;
+            case 1: rho_tiles<PB, RT, 1, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp); break;
             case 2:
-                if constexpr (NIMAX >= 2) rho_tiles<PB, RT, 2, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                if constexpr (NIMAX >= 2) rho_tiles<PB, RT, 2, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
                 break;
             case 3:
-                if constexpr (NIMAX >= 3) rho_tiles<PB, RT, 3, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                if constexpr (NIMAX >= 3) rho_tiles<PB, RT, 3, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
                 break;
             case 4:
-                if constexpr (NIMAX >= 4) rho_tiles<PB, RT, 4, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                if constexpr (NIMAX >= 4) rho_tiles<PB, RT, 4, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
                 break;
             case 5:
-                if constexpr (NIMAX >= 5) rho_tiles<PB, RT, 5, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                if constexpr (NIMAX >= 5) rho_tiles<PB, RT, 5, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
                 break;
             case 6:
-                if constexpr (NIMAX >= 6) rho_tiles<PB, RT, 6, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                if constexpr (NIMAX >= 6) rho_tiles<PB, RT, 6, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
                 break;
             case 7:
-                if constexpr (NIMAX >= 7) rho_tiles<PB, RT, 7, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                if constexpr (NIMAX >= 7) rho_tiles<PB, RT, 7, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
                 break;
             case 8:
-                if constexpr (NIMAX >= 8) rho_tiles<PB, RT, 8, G>(wI, WI, wP, T, lane, s_cols, s_phi, P, N);
+                if constexpr (NIMAX >= 8) rho_tiles<PB, RT, 8, G>(wI, WI, wP, T, lane, s_cols, s_phi, Pb, ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= NIMAX WI
                 break;
             default: __syncthreads(); break;  // no rows: only the barrier before the store
@@ -591,9 +593,9 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
     const int te = has ? tend : tb;          // and scatters nothing
     const int ngrp = has ? (tend - tstart + 3) >> 2 : 1;
     switch (ngrp) {
-        case 1: kmat_consume<4>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K, N); break;
-        case 2: kmat_consume<8>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K, N); break;
-        default: kmat_consume<12>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K, N); break;
+        case 1: kmat_consume<4>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K + B.poff, B.ldp); break;
+        case 2: kmat_consume<8>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K + B.poff, B.ldp); break;
+        default: kmat_consume<12>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K + B.poff, B.ldp); break;
     }
 }
 

@@ -185,6 +185,8 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int ldnp = SB.ldnp, nch = SB.nchunk, nsh = B.nshell;
     const int RA = ldnp + 8;
+    const unsigned short* Pbs = Ps + B.psoff;  // the block's molecule: N = ldp, Nc = ldp rounded up to 8
+    const int ldp = B.ldp, ldpc = (B.ldp + 7) / 8 * 8;
     // LDS: plane (S x PB x RA bf16, reused as PB x (ldnp + 4) FP32 and as partial sums) | records | chunk [ldnp/4] | shcol | sorder
     unsigned short* s_a = reinterpret_cast<unsigned short*>(s_dyn);
     const int plane_dbl = (S * PB * RA * 2 + 15) / 16 * 2;  // doubles, 16-byte multiple
@@ -241,33 +243,33 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
         const int nJ = ldnp / 16;
         const int nJw = (nJ > wave) ? (nJ - wave + 7) / 8 : 0;
         switch (nJw) {
-            case 1: rho_split_tiles<S, PB, 1>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc); break;
+            case 1: rho_split_tiles<S, PB, 1>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc); break;
             case 2:
-                if constexpr (NJMAX >= 2) rho_split_tiles<S, PB, 2>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc);
+                if constexpr (NJMAX >= 2) rho_split_tiles<S, PB, 2>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
                 else __syncthreads();  // unreachable: the host keeps ldnp <= 128 NJMAX
                 break;
             case 3:
-                if constexpr (NJMAX >= 3) rho_split_tiles<S, PB, 3>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc);
+                if constexpr (NJMAX >= 3) rho_split_tiles<S, PB, 3>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
                 else __syncthreads();
                 break;
             case 4:
-                if constexpr (NJMAX >= 4) rho_split_tiles<S, PB, 4>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc);
+                if constexpr (NJMAX >= 4) rho_split_tiles<S, PB, 4>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
                 else __syncthreads();
                 break;
             case 5:
-                if constexpr (NJMAX >= 5) rho_split_tiles<S, PB, 5>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc);
+                if constexpr (NJMAX >= 5) rho_split_tiles<S, PB, 5>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
                 else __syncthreads();
                 break;
             case 6:
-                if constexpr (NJMAX >= 6) rho_split_tiles<S, PB, 6>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc);
+                if constexpr (NJMAX >= 6) rho_split_tiles<S, PB, 6>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
                 else __syncthreads();
                 break;
             case 7:
-                if constexpr (NJMAX >= 7) rho_split_tiles<S, PB, 7>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc);
+                if constexpr (NJMAX >= 7) rho_split_tiles<S, PB, 7>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
                 else __syncthreads();
                 break;
             case 8:
-                if constexpr (NJMAX >= 8) rho_split_tiles<S, PB, 8>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Ps, N, Nc);
+                if constexpr (NJMAX >= 8) rho_split_tiles<S, PB, 8>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
                 else __syncthreads();
                 break;
             default: __syncthreads(); break;  // no columns: only the barrier before the store
@@ -536,10 +538,10 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
     const int pb = has ? pstart : W.t1 - 1;
     const int pe = has ? pend : pb;
     switch (has ? pend - pstart : 1) {
-        case 1: kmat_split_consume<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
-        case 2: kmat_split_consume<S, 2>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
-        case 3: kmat_split_consume<S, 3>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
-        default: kmat_split_consume<S, 4>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K, N); break;
+        case 1: kmat_split_consume<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
+        case 2: kmat_split_consume<S, 2>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
+        case 3: kmat_split_consume<S, 3>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
+        default: kmat_split_consume<S, 4>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
     }
 }
 
