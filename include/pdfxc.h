@@ -83,6 +83,7 @@ typedef struct pdfxc_stats {
 } pdfxc_stats;
 
 typedef struct pdfxc_ctx pdfxc_ctx;
+typedef struct pdfxc_batch pdfxc_batch;
 
 /* defaults: density_cutoff 1e-16, ao_screen 0, FP64, 128 points per block */
 void pdfxc_default_opts(pdfxc_opts* opts);
@@ -226,6 +227,24 @@ int pdfxc_save_grid_matrix(pdfxc_ctx* ctx, const char* path, int n_rho_cols);
  * (may be NULL): n x 3.  The context's grid and stored rho are not changed. */
 int pdfxc_density_at_points(pdfxc_ctx* ctx, const double* P_packed, int64_t n, const double* xyz, double* rho,
                             double* grad_rho);
+
+/* Batches: the grid blocks of several molecules (contexts with their grids
+ * set) in one set of launches, captured in a HIP graph on the first build and
+ * replayed while the buffers stay the same.  No counterpart in the reference,
+ * which builds one molecule per call; each molecule's result is the one its
+ * own context gives (E_xc bit for bit, K up to the order of its atomic sums).
+ * The contexts must share precision and density cut-off and must outlive
+ * nothing: the batch copies what it needs.
+ * Buffers: the molecules' dense N_i x N_i P (and K, accumulated) one after
+ * another, molecule i at dense_off[i] (pdfxc_batch_offsets); E: one E_xc per
+ * molecule.  The host form takes the molecules' packed lower triangles one
+ * after another (packed_off[i]). */
+int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs);
+int pdfxc_batch_destroy(pdfxc_batch* batch);
+int pdfxc_batch_offsets(pdfxc_batch* batch, int64_t* dense_off, int64_t* packed_off); /* n_mol + 1 each */
+int pdfxc_batch_build_rks_device(pdfxc_batch* batch, int functional, const double* dP, double* dK, double* dE);
+int pdfxc_batch_build_rks(pdfxc_batch* batch, int functional, const double* P_packed, double* K_packed, double* E);
+int pdfxc_batch_stats(pdfxc_batch* batch, pdfxc_stats* stats);
 
 /* AO values and gradients of all AOs at arbitrary points (getAOs / getDAOs,
  * DfCalcGridX.cpp:655-769).  Outputs n_pts x n_aos row-major. */

@@ -794,6 +794,11 @@ static int build_blocks(Context& c) {
     PDFXC_HIP(hipMemset(c.d_coef, 0, 4 * (size_t)c.n_slots * sizeof(double)));
     PDFXC_HIP(hipMemset(c.d_rho, 0, 4 * (size_t)c.n_slots * sizeof(double)));
     PDFXC_HIP(hipMemset(c.d_eblock, 0, (size_t)(c.n_slots + 255) / 256 * sizeof(double)));
+    // host copies of the work lists (a batch concatenates them)
+    c.h_rho_work = std::move(rho_work);
+    c.h_k_work = std::move(kall);
+    c.h_srho_work = std::move(srho_work);
+    c.h_sk_work = std::move(skw);
     return PDFXC_OK;
 }
 
@@ -1747,6 +1752,418 @@ int pdfxc_selftest_copy(const double* d_src, double* d_dst, int64_t n) {
     int rc = launch_copy(d_src, d_dst, n, nullptr);
     if (rc == PDFXC_OK) PDFXC_HIP(hipDeviceSynchronize());
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: several molecules' grid blocks in one set of launches.  Each
+// molecule keeps its own AO indexing; its blocks carry the offset of its
+// dense P and K in the concatenated buffers (BlockDesc::poff, ldp) and of its
+// P slices (psoff), its slots start at a multiple of 256 so that the
+// per-256-slot energy partials of one molecule are a contiguous range.  The
+// launch sequence (P slicing per molecule, the rho classes, the point terms,
+// the K classes, one energy sum per molecule) is captured once in a HIP graph
+// and replayed.
+// ---------------------------------------------------------------------------
+namespace pdfxc {
+
+struct Batch {
+    Context m;                     // merged block lists, work lists and device buffers
+    int n_mol = 0;
+    std::vector<int> mol_n;        // AOs per molecule
+    std::vector<int64_t> doff;     // dense P / K offsets (n_mol + 1)
+    std::vector<int64_t> poff;     // packed offsets (n_mol + 1)
+    std::vector<int64_t> mol_pts;  // points per molecule
+    int* d_ebounds = nullptr;      // energy partial ranges (n_mol + 1)
+    double* d_Pall = nullptr;      // host entry point: dense P and K of all molecules
+    double* d_Kall = nullptr;
+    double* d_Eall = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* key[3] = {nullptr, nullptr, nullptr};
+    int key_xc = 0;
+};
+
+static void batch_free(Batch* b) {
+    if (b->m.stream) (void)hipStreamSynchronize(b->m.stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    free_grid_buffers(&b->m);
+    dfree(b->m.d_shrec);
+    dfree(b->m.d_Ps);
+    dfree(b->m.d_perm);
+    dfree(b->d_ebounds);
+    dfree(b->d_Pall);
+    dfree(b->d_Kall);
+    dfree(b->d_Eall);
+    for (auto& e : b->m.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->m.stream) (void)hipStreamDestroy(b->m.stream);
+    delete b;
+}
+
+// the build sequence on the batch's stream (no synchronisation: it is captured)
+static int batch_sequence(Batch* b, int xc, const double* dP, double* dK, double* dE) {
+    Context& c = b->m;
+    int rc = run_rho(&c, dP, c.d_rho);
+    if (rc == PDFXC_OK) rc = launch_xc_points(c, xc);
+    if (rc == PDFXC_OK) rc = run_kmat(&c, c.d_coef, dK);
+    if (rc == PDFXC_OK) rc = launch_energy_sums(c, b->d_ebounds, b->n_mol, dE);
+    return rc;
+}
+
+}  // namespace pdfxc
+
+extern "C" {
+
+int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
+    if (!out || n_mol <= 0 || !ctxs) return arg_error("pdfxc_batch_create: bad arguments");
+    *out = nullptr;
+    for (int i = 0; i < n_mol; ++i) {
+        const Context* c = reinterpret_cast<const Context*>(ctxs[i]);
+        if (!c) return arg_error("pdfxc_batch_create: null context " + std::to_string(i));
+        if (c->n_pts == 0 || c->blocks.empty()) {
+            set_error("pdfxc_batch_create: context " + std::to_string(i) + " has no grid");
+            return PDFXC_ERR_NO_GRID;
+        }
+        const Context* c0 = reinterpret_cast<const Context*>(ctxs[0]);
+        if (c->opts.precision != c0->opts.precision || c->opts.density_cutoff != c0->opts.density_cutoff)
+            return arg_error("pdfxc_batch_create: contexts differ in precision or density cut-off");
+    }
+    Batch* b = new Batch();
+    Context& m = b->m;
+    const Context* c0 = reinterpret_cast<const Context*>(ctxs[0]);
+    m.opts = c0->opts;
+    b->n_mol = n_mol;
+    const bool split = m.opts.precision != PDFXC_FP64;
+    const int S = (m.opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
+    int RS = 0;
+    for (int i = 0; i < n_mol; ++i) RS = std::max(RS, reinterpret_cast<const Context*>(ctxs[i])->rec_stride);
+    m.rec_stride = RS;
+    std::vector<double> rec;
+    std::vector<double> pts;
+    std::vector<int> perm;
+    std::vector<int> ebounds{0};
+    std::vector<int> kw_cls;  // class of every FP64 KWork in m.h_k_work order (built per class below)
+    b->doff.assign(1, 0);
+    b->poff.assign(1, 0);
+    int64_t psoff = 0;
+    int shell_base = 0, nblocks = 0;
+    int64_t slot_base = 0;
+    std::vector<int> block_base(n_mol), slot_first(n_mol);
+    for (int i = 0; i < n_mol; ++i) {
+        const Context& c = *reinterpret_cast<const Context*>(ctxs[i]);
+        const int N = c.n_aos;
+        b->mol_n.push_back(N);
+        b->mol_pts.push_back(c.n_pts);
+        m.n_aos = std::max(m.n_aos, N);
+        // shell records with the common stride
+        for (const Shell& sh : c.shells) {
+            const size_t o = rec.size();
+            rec.resize(o + RS, 0.0);
+            rec[o] = sh.center[0];
+            rec[o + 1] = sh.center[1];
+            rec[o + 2] = sh.center[2];
+            rec[o + 3] = (double)(sh.l + 4 * sh.nprim);
+            for (int q = 0; q < sh.nprim; ++q) {
+                rec[o + 4 + 2 * q] = sh.exps[q];
+                rec[o + 5 + 2 * q] = sh.coefs[q];
+            }
+        }
+        // blocks, their lists and slots (the molecule's slots from a multiple of 256)
+        block_base[i] = nblocks;
+        slot_first[i] = (int)slot_base;
+        const int sh_list = (int)m.block_shells.size(), col_list = (int)m.block_cols.size();
+        const int chunk_list = (int)m.chunks.size();
+        for (size_t k = 0; k < c.blocks.size(); ++k) {
+            BlockDesc B = c.blocks[k];
+            B.pt_begin += (int)slot_base;
+            B.shell_begin += sh_list;
+            B.col_begin += col_list;
+            B.ldp = N;
+            B.poff = b->doff.back();
+            B.psoff = psoff;
+            m.blocks.push_back(B);
+            if (split) {
+                SplitBlock SB = c.sblocks[k];
+                SB.chunk_begin += chunk_list;
+                m.sblocks.push_back(SB);
+            }
+        }
+        for (int s : c.block_shells) m.block_shells.push_back(s + shell_base);
+        m.block_shell_col.insert(m.block_shell_col.end(), c.block_shell_col.begin(), c.block_shell_col.end());
+        m.block_sorder.insert(m.block_sorder.end(), c.block_sorder.begin(), c.block_sorder.end());
+        m.block_cols.insert(m.block_cols.end(), c.block_cols.begin(), c.block_cols.end());
+        if (split) {
+            m.chunks.insert(m.chunks.end(), c.chunks.begin(), c.chunks.end());
+            m.sshell_col.insert(m.sshell_col.end(), c.sshell_col.begin(), c.sshell_col.end());
+            m.psplit_jobs.push_back(Context::PsplitJob{b->doff.back(), psoff, (int)perm.size(), N});
+            perm.insert(perm.end(), c.split_perm.begin(), c.split_perm.end());
+            psoff += (int64_t)S * N * ((N + 7) / 8 * 8);
+        }
+        pts.resize(4 * (size_t)(slot_base + c.n_slots), 0.0);
+        for (int64_t p = 0; p < c.n_pts; ++p) {
+            const int64_t sl = slot_base + c.slot_of_point[p];
+            const int64_t ns = slot_base + c.n_slots;  // provisional column stride, fixed below
+            (void)ns;
+            pts[4 * sl + 0] = c.gx[p];
+            pts[4 * sl + 1] = c.gy[p];
+            pts[4 * sl + 2] = c.gz[p];
+            pts[4 * sl + 3] = c.gw[p];
+        }
+        slot_base += c.n_slots;
+        slot_base = (slot_base + 255) / 256 * 256;  // zero-weight padding slots
+        ebounds.push_back((int)(slot_base / 256));
+        nblocks += (int)c.blocks.size();
+        shell_base += (int)c.shells.size();
+        b->doff.push_back(b->doff.back() + (int64_t)N * N);
+        b->poff.push_back(b->poff.back() + (int64_t)N * (N + 1) / 2);
+        m.n_pts += c.n_pts;
+        m.flops += c.flops;
+        m.max_nao = std::max(m.max_nao, c.max_nao);
+    }
+    m.n_slots = slot_base;
+    pts.resize(4 * (size_t)m.n_slots, 0.0);
+    // SoA layout x | y | z | w as the kernels read it
+    std::vector<double> soa(4 * (size_t)m.n_slots);
+    for (int64_t sl = 0; sl < m.n_slots; ++sl)
+        for (int q = 0; q < 4; ++q) soa[q * m.n_slots + sl] = pts[4 * sl + q];
+    pts.clear();
+    pts.shrink_to_fit();
+    {
+        double sum = 0.0;
+        for (int i = 0; i < n_mol; ++i) {
+            const Context& c = *reinterpret_cast<const Context*>(ctxs[i]);
+            sum += c.mean_nao * (double)c.blocks.size();
+        }
+        m.mean_nao = sum / (double)nblocks;
+    }
+    // work lists: every launch class over all molecules
+    auto merge_int = [&](auto get_ranges, auto get_list, std::vector<int>& out_list, std::vector<LaunchRange>& out_r) {
+        for (int cls = 0; cls < 8; ++cls) {
+            LaunchRange r{cls, (int)out_list.size(), 0, 0};
+            for (int i = 0; i < n_mol; ++i) {
+                const Context& c = *reinterpret_cast<const Context*>(ctxs[i]);
+                for (const LaunchRange& q : get_ranges(c)) {
+                    if (q.cls != cls) continue;
+                    const std::vector<int>& L = get_list(c);
+                    for (int k = 0; k < q.count; ++k) out_list.push_back(L[q.offset + k] + block_base[i]);
+                    r.lds = std::max(r.lds, q.lds);
+                }
+            }
+            r.count = (int)out_list.size() - r.offset;
+            if (r.count) out_r.push_back(r);
+        }
+    };
+    auto merge_kw = [&](auto get_ranges, auto get_list, std::vector<KWork>& out_list, std::vector<LaunchRange>& out_r) {
+        for (int cls = 0; cls < 8; ++cls) {
+            LaunchRange r{cls, (int)out_list.size(), 0, 0};
+            for (int i = 0; i < n_mol; ++i) {
+                const Context& c = *reinterpret_cast<const Context*>(ctxs[i]);
+                for (const LaunchRange& q : get_ranges(c)) {
+                    if (q.cls != cls) continue;
+                    const std::vector<KWork>& L = get_list(c);
+                    for (int k = 0; k < q.count; ++k) {
+                        KWork w = L[q.offset + k];
+                        w.block += block_base[i];
+                        out_list.push_back(w);
+                    }
+                    r.lds = std::max(r.lds, q.lds);
+                }
+            }
+            r.count = (int)out_list.size() - r.offset;
+            if (r.count) out_r.push_back(r);
+        }
+    };
+    merge_int([](const Context& c) -> const std::vector<LaunchRange>& { return c.rho_launches; },
+              [](const Context& c) -> const std::vector<int>& { return c.h_rho_work; }, m.h_rho_work, m.rho_launches);
+    merge_kw([](const Context& c) -> const std::vector<LaunchRange>& { return c.kmat_launches; },
+             [](const Context& c) -> const std::vector<KWork>& { return c.h_k_work; }, m.h_k_work, m.kmat_launches);
+    if (split) {
+        merge_kw([](const Context& c) -> const std::vector<LaunchRange>& { return c.k_fp64_launches; },
+                 [](const Context& c) -> const std::vector<KWork>& { return c.h_k_work; }, m.h_k_work,
+                 m.k_fp64_launches);
+        merge_int([](const Context& c) -> const std::vector<LaunchRange>& { return c.srho_launches; },
+                  [](const Context& c) -> const std::vector<int>& { return c.h_srho_work; }, m.h_srho_work,
+                  m.srho_launches);
+        merge_kw([](const Context& c) -> const std::vector<LaunchRange>& { return c.sk_launches; },
+                 [](const Context& c) -> const std::vector<KWork>& { return c.h_sk_work; }, m.h_sk_work,
+                 m.sk_launches);
+        for (int i = 0; i < n_mol; ++i) m.n_k_fp64_blocks += reinterpret_cast<const Context*>(ctxs[i])->n_k_fp64_blocks;
+    }
+    (void)kw_cls;
+    (void)slot_first;
+    // device side
+    hipError_t e = hipGetDevice(&m.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking);
+    for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&m.ev[k]);
+    int rc = (e == hipSuccess) ? PDFXC_OK : hip_fail(e, "pdfxc_batch_create: stream/event creation");
+    auto up = [&](auto* d, const auto& v) {
+        if (rc != PDFXC_OK || v.empty()) return;
+        const hipError_t ee = hipMemcpy(d, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+        if (ee != hipSuccess) rc = hip_fail(ee, "pdfxc_batch_create: upload");
+    };
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_shrec, rec.size());
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_pts, soa.size());
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_blocks, m.blocks.size());
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_block_shells, std::max<size_t>(1, m.block_shells.size()));
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_block_cols, std::max<size_t>(1, m.block_cols.size()));
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_block_shell_col, std::max<size_t>(1, m.block_shell_col.size()));
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_block_sorder, std::max<size_t>(1, m.block_sorder.size()));
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_rho_work, std::max<size_t>(1, m.h_rho_work.size()));
+    if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&m.d_k_work), std::max<size_t>(1, m.h_k_work.size()) * sizeof(KWork));
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_coef, 4 * (size_t)m.n_slots);
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_rho, 4 * (size_t)m.n_slots);
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_eblock, (size_t)m.n_slots / 256);
+    if (rc == PDFXC_OK) rc = dalloc(&b->d_ebounds, ebounds.size());
+    if (rc == PDFXC_OK && split) {
+        rc = dalloc(&m.d_sblocks, m.sblocks.size());
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_chunks, std::max<size_t>(1, m.chunks.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_sshell_col, std::max<size_t>(1, m.sshell_col.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_srho_work, std::max<size_t>(1, m.h_srho_work.size()));
+        if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&m.d_sk_work), std::max<size_t>(1, m.h_sk_work.size()) * sizeof(KWork));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_perm, perm.size());
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_Ps, (size_t)psoff);
+    }
+    up(m.d_shrec, rec);
+    up(m.d_pts, soa);
+    up(m.d_blocks, m.blocks);
+    up(m.d_block_shells, m.block_shells);
+    up(m.d_block_cols, m.block_cols);
+    up(m.d_block_shell_col, m.block_shell_col);
+    up(m.d_block_sorder, m.block_sorder);
+    up(m.d_rho_work, m.h_rho_work);
+    up(reinterpret_cast<KWork*>(m.d_k_work), m.h_k_work);
+    up(b->d_ebounds, ebounds);
+    if (split) {
+        up(m.d_sblocks, m.sblocks);
+        up(m.d_chunks, m.chunks);
+        up(m.d_sshell_col, m.sshell_col);
+        up(m.d_srho_work, m.h_srho_work);
+        up(reinterpret_cast<KWork*>(m.d_sk_work), m.h_sk_work);
+        up(m.d_perm, perm);
+    }
+    if (rc == PDFXC_OK) {
+        e = hipMemset(m.d_rho, 0, 4 * (size_t)m.n_slots * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(m.d_coef, 0, 4 * (size_t)m.n_slots * sizeof(double));
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_batch_create: clear");
+    }
+    if (rc != PDFXC_OK) {
+        batch_free(b);
+        return rc;
+    }
+    *out = reinterpret_cast<pdfxc_batch*>(b);
+    return PDFXC_OK;
+}
+
+int pdfxc_batch_destroy(pdfxc_batch* batch) {
+    if (batch) batch_free(reinterpret_cast<Batch*>(batch));
+    return PDFXC_OK;
+}
+
+int pdfxc_batch_offsets(pdfxc_batch* batch, int64_t* dense_off, int64_t* packed_off) {
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return arg_error("pdfxc_batch_offsets: null batch");
+    if (dense_off) std::copy(b->doff.begin(), b->doff.end(), dense_off);
+    if (packed_off) std::copy(b->poff.begin(), b->poff.end(), packed_off);
+    return PDFXC_OK;
+}
+
+int pdfxc_batch_build_rks_device(pdfxc_batch* batch, int functional, const double* dP, double* dK, double* dE) {
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return arg_error("pdfxc_batch_build_rks_device: null batch");
+    if (int rc = check_functional(functional, "pdfxc_batch_build_rks_device")) return rc;
+    if (!dP || !dK || !dE) return arg_error("pdfxc_batch_build_rks_device: null buffer");
+    Context& c = b->m;
+    if (functional == kHF) {
+        if (int rc = launch_zero(dE, b->n_mol, c.stream)) return rc;
+        PDFXC_HIP(hipStreamSynchronize(c.stream));
+        return PDFXC_OK;
+    }
+    if (!b->exec || b->key[0] != dP || b->key[1] != dK || b->key[2] != dE || b->key_xc != functional) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        PDFXC_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_sequence(b, functional, dP, dK, dE);
+        const hipError_t ec = hipStreamEndCapture(c.stream, &b->graph);
+        if (rc) return rc;
+        if (ec != hipSuccess) return hip_fail(ec, "pdfxc_batch_build_rks_device: capture");
+        PDFXC_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->key[0] = dP;
+        b->key[1] = dK;
+        b->key[2] = dE;
+        b->key_xc = functional;
+    }
+    PDFXC_HIP(hipEventRecord(c.ev[0], c.stream));
+    PDFXC_HIP(hipGraphLaunch(b->exec, c.stream));
+    PDFXC_HIP(hipEventRecord(c.ev[1], c.stream));
+    PDFXC_HIP(hipEventSynchronize(c.ev[1]));
+    float t = 0.f;
+    PDFXC_HIP(hipEventElapsedTime(&t, c.ev[0], c.ev[1]));
+    c.last_build_ms = t;
+    return PDFXC_OK;
+}
+
+int pdfxc_batch_build_rks(pdfxc_batch* batch, int functional, const double* P_packed, double* K_packed, double* E) {
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return arg_error("pdfxc_batch_build_rks: null batch");
+    if (int rc = check_functional(functional, "pdfxc_batch_build_rks")) return rc;
+    if (!P_packed || !K_packed || !E) return arg_error("pdfxc_batch_build_rks: null buffer");
+    Context& c = b->m;
+    const int64_t total = b->doff.back();
+    if (!b->d_Pall) {
+        if (int rc = dalloc(&b->d_Pall, (size_t)total)) return rc;
+        if (int rc = dalloc(&b->d_Kall, (size_t)total)) return rc;
+        if (int rc = dalloc(&b->d_Eall, (size_t)b->n_mol)) return rc;
+    }
+    std::vector<double> H;
+    for (int i = 0; i < b->n_mol; ++i) {
+        const int N = b->mol_n[i];
+        H.assign((size_t)N * N, 0.0);
+        const double* pp = P_packed + b->poff[i];
+        for (int r = 0, k = 0; r < N; ++r)
+            for (int q = 0; q <= r; ++q, ++k) H[(size_t)r * N + q] = H[(size_t)q * N + r] = pp[k];
+        PDFXC_HIP(hipMemcpy(b->d_Pall + b->doff[i], H.data(), H.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    PDFXC_HIP(hipMemsetAsync(b->d_Kall, 0, (size_t)total * sizeof(double), c.stream));
+    PDFXC_HIP(hipStreamSynchronize(c.stream));
+    if (int rc = pdfxc_batch_build_rks_device(batch, functional, b->d_Pall, b->d_Kall, b->d_Eall)) return rc;
+    if (functional == kHF) {
+        std::fill(E, E + b->n_mol, 0.0);
+        return PDFXC_OK;
+    }
+    PDFXC_HIP(hipMemcpy(E, b->d_Eall, sizeof(double) * b->n_mol, hipMemcpyDeviceToHost));
+    for (int i = 0; i < b->n_mol; ++i) {
+        const int N = b->mol_n[i];
+        H.resize((size_t)N * N);
+        PDFXC_HIP(hipMemcpy(H.data(), b->d_Kall + b->doff[i], H.size() * sizeof(double), hipMemcpyDeviceToHost));
+        double* kp = K_packed + b->poff[i];
+        for (int r = 0, k = 0; r < N; ++r)
+            for (int q = 0; q <= r; ++q, ++k) kp[k] += H[(size_t)r * N + q];
+    }
+    return PDFXC_OK;
+}
+
+int pdfxc_batch_stats(pdfxc_batch* batch, pdfxc_stats* s) {
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b || !s) return arg_error("pdfxc_batch_stats: bad arguments");
+    const Context& c = b->m;
+    std::memset(s, 0, sizeof(*s));
+    s->n_points = c.n_pts;
+    s->n_blocks = (int64_t)c.blocks.size();
+    s->n_chunks = (int64_t)((c.opts.precision != PDFXC_FP64)
+                                ? c.psplit_jobs.size() + c.srho_launches.size() + c.sk_launches.size() +
+                                      c.k_fp64_launches.size() + 2
+                                : c.rho_launches.size() + c.kmat_launches.size() + 2);
+    s->flops = c.flops;
+    s->mean_nao = c.mean_nao;
+    s->max_nao = c.max_nao;
+    s->last_build_ms = c.last_build_ms;
+    s->n_k_fp64_blocks = c.n_k_fp64_blocks;
+    return PDFXC_OK;
 }
 
 }  // extern "C"

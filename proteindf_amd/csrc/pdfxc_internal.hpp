@@ -142,6 +142,17 @@ struct Context {
 
     int rho_accumulate = 0;             // update mode: the rho kernels add rho[dP] to d_rho
 
+    // host copies of the device work lists (d_rho_work, d_k_work, d_srho_work, d_sk_work)
+    std::vector<int> h_rho_work, h_srho_work;
+    std::vector<KWork> h_k_work, h_sk_work;
+
+    // batches: P slices made per molecule (empty for one molecule)
+    struct PsplitJob {
+        int64_t poff, psoff;
+        int perm_off, n;
+    };
+    std::vector<PsplitJob> psplit_jobs;
+
     double last_build_ms = 0.0;
     double last_kernel_ms[4] = {0, 0, 0, 0};
 };
@@ -164,6 +175,7 @@ int launch_kmat(Context& c, const LaunchRange& r);
 int configure_kernels();
 int launch_energy_sum(Context& c, double* dE);
 int launch_zero(double* d, int64_t n, hipStream_t s);
+int launch_energy_sums(Context& c, const int* d_bounds, int n_mol, double* dE);  // one E per molecule
 int launch_xc_points_uks(Context& c, int xc_type);
 int ensure_beta_buffers(Context* c);  // UKS beta rho / point-term buffers, zeroed when first made
 int launch_eval_functional_uks(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s);

@@ -600,18 +600,21 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
 }
 
 // deterministic sum of the per-block energies (one workgroup)
-__global__ __launch_bounds__(256) void k_energy_sum(const double* __restrict__ eb, int n,
-                                                    double* __restrict__ out) {
+// E_xc of molecule blockIdx.x: the partial sums [bounds[m], bounds[m + 1]) in a
+// fixed order (one molecule: bounds = {0, n})
+__global__ __launch_bounds__(256) void k_energy_sum(const double* __restrict__ eb, const int* __restrict__ bounds,
+                                                    int n, double* __restrict__ out) {
     __shared__ double s[256];
+    const int lo = bounds ? bounds[blockIdx.x] : 0, hi = bounds ? bounds[blockIdx.x + 1] : n;
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) v += eb[i];
+    for (int i = lo + threadIdx.x; i < hi; i += 256) v += eb[i];
     s[threadIdx.x] = v;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if (threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = s[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
 __global__ __launch_bounds__(64) void k_zero(double* __restrict__ p, int64_t n) {
@@ -815,7 +818,7 @@ int configure_kernels() {
 }
 
 int launch_energy_sum(Context& c, double* dE) {
-    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(256), 0, c.stream, c.d_eblock,
+    hipLaunchKernelGGL(k_energy_sum, dim3(1), dim3(256), 0, c.stream, c.d_eblock, (const int*)nullptr,
                        (int)((c.n_slots + 255) / 256), dE);
     return check_launch("k_energy_sum");
 }
@@ -824,6 +827,11 @@ int launch_zero(double* d, int64_t n, hipStream_t s) {
     if (n <= 0) return PDFXC_OK;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, d, n);
     return check_launch("k_zero");
+}
+
+int launch_energy_sums(Context& c, const int* d_bounds, int n_mol, double* dE) {
+    hipLaunchKernelGGL(k_energy_sum, dim3(n_mol), dim3(256), 0, c.stream, c.d_eblock, d_bounds, 0, dE);
+    return check_launch("k_energy_sum");
 }
 
 int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s) {

@@ -586,10 +586,20 @@ int split_points_per_workgroup(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 :
 
 template <int S>
 static int psplit_one(Context& c) {
-    const int64_t n = (int64_t)c.n_aos * c.split_nc;
-    hipLaunchKernelGGL((k_psplit<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.d_P, c.d_perm,
-                       c.n_aos, c.split_nc, c.d_Ps);
-    return check("k_psplit");
+    if (c.psplit_jobs.empty()) {
+        const int64_t n = (int64_t)c.n_aos * c.split_nc;
+        hipLaunchKernelGGL((k_psplit<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream, c.d_P,
+                           c.d_perm, c.n_aos, c.split_nc, c.d_Ps);
+        return check("k_psplit");
+    }
+    for (const Context::PsplitJob& j : c.psplit_jobs) {  // a batch: each molecule's P into its slices
+        const int nc = (j.n + 7) / 8 * 8;
+        const int64_t n = (int64_t)j.n * nc;
+        hipLaunchKernelGGL((k_psplit<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c.stream,
+                           c.d_P + j.poff, c.d_perm + j.perm_off, j.n, nc, c.d_Ps + j.psoff);
+        if (int rc = check("k_psplit")) return rc;
+    }
+    return PDFXC_OK;
 }
 
 template <int S, int PB, int NJMAX>

@@ -40,7 +40,8 @@ EXPORTED = [
     "pdfxc_save_symmetric_matrix", "pdfxc_load_general_matrix", "pdfxc_save_general_matrix",
     "pdfxc_load_grid_matrix", "pdfxc_save_grid_matrix", "pdfxc_build_uks", "pdfxc_build_uks_update",
     "pdfxc_build_uks_device", "pdfxc_build_uks_update_device", "pdfxc_eval_functional_uks",
-    "pdfxc_density_at_points",
+    "pdfxc_density_at_points", "pdfxc_batch_create", "pdfxc_batch_destroy", "pdfxc_batch_offsets",
+    "pdfxc_batch_build_rks_device", "pdfxc_batch_build_rks", "pdfxc_batch_stats",
 ]
 # matrix file types (TlMatrixObject::MatrixType)
 PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
@@ -113,6 +114,12 @@ def lib():
         L.pdfxc_build_uks_update_device.argtypes = [P, i32, P, P, P, P, P]
         L.pdfxc_eval_functional_uks.argtypes = [i32, i64, P, P]
         L.pdfxc_density_at_points.argtypes = [P, P, i64, P, P, P]
+        L.pdfxc_batch_create.argtypes = [P, i32, P]
+        L.pdfxc_batch_destroy.argtypes = [P]
+        L.pdfxc_batch_offsets.argtypes = [P, P, P]
+        L.pdfxc_batch_build_rks_device.argtypes = [P, i32, P, P, P]
+        L.pdfxc_batch_build_rks.argtypes = [P, i32, P, P, P]
+        L.pdfxc_batch_stats.argtypes = [P, P]
         _lib = L
     return _lib
 
@@ -370,3 +377,49 @@ class XcContext:
 
     def stream(self):
         return lib().pdfxc_stream(self.h)
+
+
+class XcBatch:
+    """Several contexts' grids built together (pdfxc_batch_*): one set of
+    launches, captured in a HIP graph and replayed."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = ctypes.c_void_p()
+        check(lib().pdfxc_batch_create(ctypes.byref(h), len(self.contexts), arr))
+        self.h = h
+        n = len(self.contexts)
+        self.dense_off = np.zeros(n + 1, dtype=np.int64)
+        self.packed_off = np.zeros(n + 1, dtype=np.int64)
+        check(lib().pdfxc_batch_offsets(self.h, _p(self.dense_off), _p(self.packed_off)))
+
+    def close(self):
+        if self.h:
+            lib().pdfxc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_rks(self, P_alpha_packed_list, functional=PDFXC_B3LYP):
+        """Returns (E per molecule, list of packed K)."""
+        P = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64) for p in P_alpha_packed_list]))
+        if P.shape != (self.packed_off[-1],):
+            raise ValueError("P list does not match the molecules' sizes")
+        K = np.zeros_like(P)
+        E = np.zeros(len(self.contexts))
+        check(lib().pdfxc_batch_build_rks(self.h, functional, _p(P), _p(K), _p(E)))
+        return E, [K[self.packed_off[i]:self.packed_off[i + 1]] for i in range(len(self.contexts))]
+
+    def build_rks_device(self, dP_ptr, dK_ptr, dE_ptr, functional=PDFXC_B3LYP):
+        check(lib().pdfxc_batch_build_rks_device(self.h, functional, ctypes.c_void_p(dP_ptr),
+                                                 ctypes.c_void_p(dK_ptr), ctypes.c_void_p(dE_ptr)))
+
+    def stats(self):
+        s = Stats()
+        check(lib().pdfxc_batch_stats(self.h, ctypes.byref(s)))
+        return s
