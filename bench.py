@@ -18,7 +18,7 @@ dominant kernel, timed with HIP events on the engine's stream.  The CPU
 baseline runs the C oracle (the reference algorithm: all AOs at every point)
 on a bounded sample of grid points and scales to the whole grid.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload gly10|h2o8|gly20|gly30]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload gly10|h2o8|gly20|gly30|h2o_batch]
                     [--precision fp64|bf16x2|bf16x3] [--modes fp64,bf16x2,bf16x3]
 """
 import argparse
@@ -150,12 +150,108 @@ def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup):
     return dt, st, E[0], K
 
 
+def bench_batch(args):
+    """64 water clusters (H2O)_n, n ~ U{4..64} from one stream (seed 3), 6-31G(d),
+    B3LYP, 75 x 302: every cluster's blocks in one graph-captured batch vs one
+    build per cluster.  Reports aggregate points/s and per-cluster latency."""
+    mols = molecules.water_cluster_batch(64, 4, 64, seed=3)
+    prec = MODES[args.precision]
+    hip = Hip()
+    ctxs, infos, Ps, singles = [], [], [], []
+    t0 = time.perf_counter()
+    for k, mol in enumerate(mols):
+        if k % 8 == 0:
+            print(f"[bench] set-up: cluster {k} of {len(mols)}, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+        info = calc_grid.orbital_info(mol)
+        ctx = engine.XcContext(info, engine.default_opts(ao_screen=1e-12, precision=prec))
+        ctx.generate_grid(mol.Z, mol.xyz, 75, 302)
+        PA = np.ascontiguousarray(0.5 * guess.density_guess(mol, info, seed=0, eps=0.05))
+        dP, dK, dE = hip.malloc(PA.nbytes), hip.malloc(PA.nbytes), hip.malloc(8)
+        hip.h2d(dP, PA)
+        hip.memset(dK, PA.nbytes)
+        ctxs.append(ctx)
+        infos.append(info)
+        Ps.append(PA)
+        singles.append((dP, dK, dE))
+    t_setup = time.perf_counter() - t0
+    n_pts = sum(c.n_pts for c in ctxs)
+    batch = engine.XcBatch(ctxs)
+    st = batch.stats()
+    total = int(batch.dense_off[-1])
+    bP, bK, bE = hip.malloc(8 * total), hip.malloc(8 * total), hip.malloc(8 * len(mols))
+    for i, PA in enumerate(Ps):
+        hip.h2d(bP + 8 * int(batch.dense_off[i]), PA)
+    hip.memset(bK, 8 * total)
+    print(f"[bench] 64 clusters: {n_pts} points, {st.n_blocks} blocks, N {min(i.n_aos for i in infos)}.."
+          f"{max(i.n_aos for i in infos)}, set-up {t_setup:.1f} s", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        batch.build_rks_device(bP, bK, bE)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.build_rks_device(bP, bK, bE)
+    hip.sync()
+    t_batch = (time.perf_counter() - t0) / max(args.steps, 1)
+    # one build per cluster, each timed on its own
+    lat = []
+    for _ in range(args.warmup):
+        for ctx, (dP, dK, dE) in zip(ctxs, singles):
+            ctx.build_rks_device(dP, dK, dE)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for ctx, (dP, dK, dE) in zip(ctxs, singles):
+            t1 = time.perf_counter()
+            ctx.build_rks_device(dP, dK, dE)  # synchronises before returning
+            lat.append(time.perf_counter() - t1)
+    t_single = (time.perf_counter() - t0) / max(args.steps, 1)
+    # the batch's energies are the single builds' energies
+    Eb = np.zeros(len(mols))
+    hip.d2h(Eb, bE)
+    Es = np.zeros(len(mols))
+    for i, (dP, dK, dE) in enumerate(singles):
+        e = np.zeros(1)
+        hip.d2h(e, dE)
+        Es[i] = e[0]
+    if not np.array_equal(Eb, Es):
+        raise RuntimeError(f"batched and single energies differ: max {np.abs(Eb - Es).max()}")
+    lat_ms = np.array(lat) * 1e3
+    label = {"fp64": "FP64", "bf16x2": "split-BF16 hi+lo", "bf16x3": "split-BF16 hi+mid+lo"}[args.precision]
+    line = {
+        "metric": f"batched K_xc+E_xc build points/s, 64 water clusters, {label} GPU",
+        "value": n_pts / t_batch,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_batch * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "bf16 slices, f32 accumulate, f64 AO/functional/sums",
+        "data": "synthetic (64 (H2O)_n clusters, n ~ U{4..64}, seed 3; deterministic density guess)",
+        "config": {"workload": "64 (H2O)_n clusters, 6-31G(d), B3LYP, 75x302 Becke, one graph-captured batch",
+                   "precision": args.precision, "n_points": n_pts, "n_blocks": int(st.n_blocks),
+                   "algorithmic_gflop": st.flops / 1e9, "ao_screen": 1e-12},
+        "batch": {"ms": t_batch * 1e3, "points_per_s": n_pts / t_batch, "gflops": st.flops / t_batch / 1e9,
+                  "cluster_latency_ms": t_batch * 1e3},
+        "one_launch_per_cluster": {"ms": t_single * 1e3, "points_per_s": n_pts / t_single,
+                                   "cluster_latency_ms_p50": float(np.percentile(lat_ms, 50)),
+                                   "cluster_latency_ms_p99": float(np.percentile(lat_ms, 99))},
+        "speedup_batch_vs_single": t_single / t_batch,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="gly10", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="gly10", choices=sorted(WORKLOADS) + ["h2o_batch"])
     ap.add_argument("--precision", default="bf16x3", choices=sorted(MODES),
                     help="mode whose throughput is the reported value (the others are measured beside it)")
     ap.add_argument("--modes", default="fp64,bf16x2,bf16x3", help="comma list of modes to measure")
@@ -175,6 +271,9 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU per process; the build does not "
                                    "shard across GPUs (use --gpus 1)"}), flush=True)
         sys.exit(2)
+    if args.workload == "h2o_batch":
+        bench_batch(args)
+        return
     modes = [m for m in args.modes.split(",") if m]
     if "fp64" not in modes:
         modes.insert(0, "fp64")  # the split errors are stated against the FP64 build
