@@ -37,16 +37,22 @@ __device__ inline f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
 
 __device__ inline unsigned short bf16_bits(__bf16 h) { return __builtin_bit_cast(unsigned short, h); }
 
-// slices of x: s[0] = bf16(x), s[i] = bf16(x - s[0] - ... - s[i-1]) with the
-// residual kept in FP64
+// slices of x: s[0] = bf16(x), s[1] = bf16 of what s[0] leaves, and the last
+// slice from the exact FP64 residual x - s[0] - ... .  The middle slice of
+// S = 3 comes from f - s[0] in FP32 (f = float(x); exact, as s[0] keeps the top
+// 8 of f's 24 bits), which saves the FP64 round trips of a slice.
 template <int S>
 __device__ inline void split_bf16(double x, unsigned short* out) {
-    double r = x;
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-        const __bf16 h = (__bf16)(float)r;
-        out[i] = bf16_bits(h);
-        r -= (double)(float)h;
+    const float f = (float)x;
+    const __bf16 h0 = (__bf16)f;
+    out[0] = bf16_bits(h0);
+    const double r1 = x - (double)(float)h0;  // exact
+    if constexpr (S == 2) {
+        out[1] = bf16_bits((__bf16)(float)r1);
+    } else {
+        const __bf16 h1 = (__bf16)(f - (float)h0);
+        out[1] = bf16_bits(h1);
+        out[2] = bf16_bits((__bf16)(float)(r1 - (double)(float)h1));  // exact residual, rounded
     }
 }
 
