@@ -681,7 +681,7 @@ static int build_blocks(Context& c) {
                 const SplitBlock& SB = c.sblocks[b];
                 if (SB.ldnp <= lo_lim || SB.ldnp > lim) continue;
                 const size_t plane = std::max(((size_t)S * pb * (SB.ldnp + 8) * 2 + 15) / 16 * 16,
-                                              (size_t)4 * 512 * sizeof(double));
+                                              (size_t)8 * 512 * sizeof(double));
                 const size_t lds = plane + sizeof(double) * (size_t)c.blocks[b].nshell * RS +
                                    sizeof(int) * (size_t)(SB.ldnp / 4 + 2 * c.blocks[b].nshell);
                 if (lds > (size_t)kMaxDynLds) {

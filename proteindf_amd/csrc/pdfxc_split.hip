@@ -96,24 +96,19 @@ __global__ __launch_bounds__(256) void k_psplit(const double* __restrict__ P, co
 // slices; X = Phi P_bb accumulates in FP32 with wave w owning the AO column
 // tiles J = w, w + 8, ... and all PB / 16 point tiles; the B operand (P_bb
 // rows = P rows by symmetry) is loaded per lane as the 8 AOs of two chunks.
-// X is then stored over the plane (FP32, row stride ldnp + 4) and dotted
-// with phi, grad phi in FP64.
+// X is then stored over the plane in FP64 for half of the points at a time
+// (row stride ldnp + 1) and dotted with phi, grad phi in FP64.
 // ---------------------------------------------------------------------------
-template <int S, int PB, int NJ>
+template <int S, int PB, int NJ, int NJMAX>
 __device__ inline void rho_split_tiles(int wave, int lane, int ldnp, int nch, const int* s_chunk,
-                                       const unsigned short* s_a, float* s_x,
-                                       const unsigned short* __restrict__ Ps, int N, int Nc) {
+                                       const unsigned short* s_a, const unsigned short* __restrict__ Ps, int N,
+                                       int Nc, v4d (&acc)[PB / 16][NJMAX]) {
     constexpr int NP = PB / 16;
     const int li = lane & 15, lq = lane >> 4;
     const int RA = ldnp + 8;
     const int64_t plane = (int64_t)N * Nc;
     // each K step's slice products accumulate in FP32 from zero and are then
-    // added to FP64 accumulators, so FP32 rounding acts on one step's sum only
-    v4d acc[NP][NJ];
-#pragma unroll
-    for (int ip = 0; ip < NP; ++ip)
-#pragma unroll
-        for (int jj = 0; jj < NJ; ++jj) acc[ip][jj] = v4d{0.0, 0.0, 0.0, 0.0};
+    // added to the FP64 accumulators, so FP32 rounding acts on one step's sum only
     int64_t prow[NJ];
 #pragma unroll
     for (int jj = 0; jj < NJ; ++jj) {
@@ -147,23 +142,18 @@ __device__ inline void rho_split_tiles(int wave, int lane, int ldnp, int nch, co
         for (int ip = 0; ip < NP; ++ip)
 #pragma unroll
             for (int jj = 0; jj < NJ; ++jj) {
-                f32x4 a32 = f32x4{0.f, 0.f, 0.f, 0.f};
+                // hi x hi in an FP32 sum of its own: the corrections (at most
+                // 2^-8 of it) do not add their roundings to the leading term
+                const f32x4 z = f32x4{0.f, 0.f, 0.f, 0.f};
+                const f32x4 a32 = mfma_bf16(afr[ip][0], bfr[jj][0], z);
+                f32x4 c32 = z;
 #pragma unroll
-                for (int q = 0; q < SlicePairs<S>::n; ++q)
-                    a32 = mfma_bf16(afr[ip][SlicePairs<S>::a[q]], bfr[jj][SlicePairs<S>::b[q]], a32);
+                for (int q = 1; q < SlicePairs<S>::n; ++q)
+                    c32 = mfma_bf16(afr[ip][SlicePairs<S>::a[q]], bfr[jj][SlicePairs<S>::b[q]], c32);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[ip][jj][r] += (double)a32[r];
+                for (int r = 0; r < 4; ++r) acc[ip][jj][r] += (double)a32[r] + (double)c32[r];
             }
     }
-    __syncthreads();  // every wave has stopped reading the Phi plane
-    const int RX = ldnp + 4;
-#pragma unroll
-    for (int ip = 0; ip < NP; ++ip)
-#pragma unroll
-        for (int jj = 0; jj < NJ; ++jj)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                s_x[(ip * 16 + 4 * lq + r) * RX + (wave + 8 * jj) * 16 + li] = (float)acc[ip][jj][r];
 }
 
 template <int S, int PB, int NJMAX>
@@ -196,7 +186,7 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
     // LDS: plane (S x PB x RA bf16, reused as PB x (ldnp + 4) FP32 and as partial sums) | records | chunk [ldnp/4] | shcol | sorder
     unsigned short* s_a = reinterpret_cast<unsigned short*>(s_dyn);
     const int plane_dbl = (S * PB * RA * 2 + 15) / 16 * 2;  // doubles, 16-byte multiple
-    const int region_dbl = max(plane_dbl, 4 * 512);  // also holds the 4 x 512 partial sums
+    const int region_dbl = max(plane_dbl, 8 * 512);  // also holds the 4 x 1024 partial sums
     double* s_rec = s_dyn + region_dbl;
     int* s_chunk = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
     int* s_shcol = s_chunk + ldnp / 4;
@@ -243,76 +233,105 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
     }
     __syncthreads();
 
-    // 2. X = Phi P_bb, stored over the plane (FP32)
-    float* s_x = reinterpret_cast<float*>(s_dyn);
+    // 2. X = Phi P_bb; 3. rho and grad rho (FP64): X (FP64, stored over the
+    // plane for half of the points at a time) dotted with phi, grad phi.
+    // Per round, lanes take point pl of the half and shell lanes kk2.
+    double* s_xd = s_dyn;
+    constexpr int PH = PB / 2;
+    constexpr int NSL2 = 512 / PH;
+    const int pl = tid & (PH - 1), kk2 = tid / PH;
+    double part[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    auto dots = [&](int round) {
+        const int RXD = ldnp + 1;
+        const int pt = pl + round * PH;
+        const double qx = s_pt[pt][0], qy = s_pt[pt][1], qz = s_pt[pt][2];
+        double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+        for (int j = kk2; j < nsh; j += NSL2) {
+            const int k = s_sord[j];
+            double v[5], vx[5], vy[5], vz[5];
+            rec_values(s_rec + k * RS, qx, qy, qz, v, vx, vy, vz);
+            const int c0 = s_shcol[k];
+            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
+            for (int m = 0; m < nc; ++m) {
+                const double x = s_xd[pl * RXD + c0 + m];
+                r0 += x * v[m];
+                r1 += x * vx[m];
+                r2 += x * vy[m];
+                r3 += x * vz[m];
+            }
+        }
+        part[round][0] = r0;
+        part[round][1] = r1;
+        part[round][2] = r2;
+        part[round][3] = r3;
+    };
+    constexpr int NP = PB / 16;
+    v4d acc[NP][NJMAX];
+#pragma unroll
+    for (int ip = 0; ip < NP; ++ip)
+#pragma unroll
+        for (int jj = 0; jj < NJMAX; ++jj) acc[ip][jj] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int nJ = ldnp / 16;
+    const int nJw = (nJ > wave) ? (nJ - wave + 7) / 8 : 0;  // the host keeps ldnp <= 128 NJMAX
+    switch (nJw) {
+        case 1: rho_split_tiles<S, PB, 1, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc); break;
+        case 2:
+            if constexpr (NJMAX >= 2) rho_split_tiles<S, PB, 2, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc);
+            break;
+        case 3:
+            if constexpr (NJMAX >= 3) rho_split_tiles<S, PB, 3, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc);
+            break;
+        case 4:
+            if constexpr (NJMAX >= 4) rho_split_tiles<S, PB, 4, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc);
+            break;
+        case 5:
+            if constexpr (NJMAX >= 5) rho_split_tiles<S, PB, 5, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc);
+            break;
+        case 6:
+            if constexpr (NJMAX >= 6) rho_split_tiles<S, PB, 6, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc);
+            break;
+        case 7:
+            if constexpr (NJMAX >= 7) rho_split_tiles<S, PB, 7, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc);
+            break;
+        case 8:
+            if constexpr (NJMAX >= 8) rho_split_tiles<S, PB, 8, NJMAX>(wave, lane, ldnp, nch, s_chunk, s_a, Pbs, ldp, ldpc, acc);
+            break;
+        default: break;  // no column tiles
+    }
+    __syncthreads();  // every wave has stopped reading the Phi plane
     {
-        const int nJ = ldnp / 16;
-        const int nJw = (nJ > wave) ? (nJ - wave + 7) / 8 : 0;
-        switch (nJw) {
-            case 1: rho_split_tiles<S, PB, 1>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc); break;
-            case 2:
-                if constexpr (NJMAX >= 2) rho_split_tiles<S, PB, 2>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
-                else __syncthreads();  // unreachable: the host keeps ldnp <= 128 NJMAX
-                break;
-            case 3:
-                if constexpr (NJMAX >= 3) rho_split_tiles<S, PB, 3>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
-                else __syncthreads();
-                break;
-            case 4:
-                if constexpr (NJMAX >= 4) rho_split_tiles<S, PB, 4>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
-                else __syncthreads();
-                break;
-            case 5:
-                if constexpr (NJMAX >= 5) rho_split_tiles<S, PB, 5>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
-                else __syncthreads();
-                break;
-            case 6:
-                if constexpr (NJMAX >= 6) rho_split_tiles<S, PB, 6>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
-                else __syncthreads();
-                break;
-            case 7:
-                if constexpr (NJMAX >= 7) rho_split_tiles<S, PB, 7>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
-                else __syncthreads();
-                break;
-            case 8:
-                if constexpr (NJMAX >= 8) rho_split_tiles<S, PB, 8>(wave, lane, ldnp, nch, s_chunk, s_a, s_x, Pbs, ldp, ldpc);
-                else __syncthreads();
-                break;
-            default: __syncthreads(); break;  // no columns: only the barrier before the store
+        const int li = lane & 15, lq = lane >> 4;
+        const int RXD = ldnp + 1;
+#pragma unroll
+        for (int round = 0; round < 2; ++round) {
+#pragma unroll
+            for (int ip = 0; ip < NP; ++ip)
+#pragma unroll
+                for (int jj = 0; jj < NJMAX; ++jj)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = ip * 16 + 4 * lq + r;
+                        if (jj < nJw && (row >= PH) == (round == 1))
+                            s_xd[(row - round * PH) * RXD + (wave + 8 * jj) * 16 + li] = acc[ip][jj][r];
+                    }
+            __syncthreads();  // this round's X is stored
+            dots(round);
+            __syncthreads();  // X no longer read
         }
     }
-    __syncthreads();
-
-    // 3. rho and grad rho (FP64): phi, grad phi dotted with X
-    const int RX = ldnp + 4;
-    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    for (int j = kk; j < nsh; j += NSL) {
-        const int k = s_sord[j];
-        double v[5], vx[5], vy[5], vz[5];
-        rec_values(s_rec + k * RS, px, py, pz, v, vx, vy, vz);
-        const int c0 = s_shcol[k];
-        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-        for (int m = 0; m < nc; ++m) {
-            const double x = (double)s_x[p * RX + c0 + m];
-            r0 += x * v[m];
-            r1 += x * vx[m];
-            r2 += x * vy[m];
-            r3 += x * vz[m];
-        }
-    }
-    // partial sums of the NSL shell lanes of each point, added in a fixed order
-    __syncthreads();  // X no longer read: the region holds the partial sums
-    double* s_part = s_dyn;  // [4][NSL][PB]
-    s_part[(0 * NSL + kk) * PB + p] = r0;
-    s_part[(1 * NSL + kk) * PB + p] = r1;
-    s_part[(2 * NSL + kk) * PB + p] = r2;
-    s_part[(3 * NSL + kk) * PB + p] = r3;
+    // partial sums of the NSL2 shell lanes of each point, added in a fixed
+    // order (the region is free: the last round's barrier has passed)
+    double* s_part = s_dyn;  // [4][NSL2][PB]
+#pragma unroll
+    for (int round = 0; round < 2; ++round)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s_part[(q * NSL2 + kk2) * PB + pl + round * PH] = part[round][q];
     __syncthreads();
     if (tid < 4 * PB) {
         const int q = tid / PB;
         const int pp = tid & (PB - 1);
         double sum = 0.0;
-        for (int k2 = 0; k2 < NSL; ++k2) sum += s_part[(q * NSL + k2) * PB + pp];
+        for (int k2 = 0; k2 < NSL2; ++k2) sum += s_part[(q * NSL2 + k2) * PB + pp];
         const int64_t slot = B.pt_begin + p0 + pp;
         const double v = (q == 0) ? sum : 2.0 * sum;
         rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;
