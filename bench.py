@@ -113,14 +113,16 @@ PEAK_TFLOPS = {"fp64": 78.6, "bf16x2": 2500.0, "bf16x3": 2500.0}
 E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 build
 
 
-def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup):
-    """Build a context with the given precision on the same grid, time steps
-    builds on HBM-resident P/K, return (ms per build, stats, E, K dense)."""
+def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True):
+    """Build a context with the given precision on the same grid, let the
+    autotuner pick its launch shapes (set-up, untimed), time steps builds on
+    HBM-resident P/K, return (ms per build, stats, E, K dense, tuning)."""
     N = info.n_aos
     ctx = engine.XcContext(info, engine.default_opts(ao_screen=ao_screen, precision=precision))
     ctx.set_grid(grid)
     dP, dK, dE = hip.malloc(PA.nbytes), hip.malloc(PA.nbytes), hip.malloc(8)
     hip.h2d(dP, PA)
+    tune = ctx.autotune(dP, dK, dE, reps=2).as_dict() if autotune else None
     hip.memset(dK, PA.nbytes)
     for _ in range(warmup):
         ctx.build_rks_device(dP, dK, dE)
@@ -147,7 +149,19 @@ def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup):
     ctx.close()
     for p in (dP, dK, dE):
         hip.free(p)
-    return dt, st, E[0], K
+    return dt, st, E[0], K, tune
+
+
+def _tally(tunes):
+    """How many contexts the autotuner put on each launch shape."""
+    if not tunes:
+        return None
+    out = {}
+    for t in tunes:
+        key = f"rho_shift={t['rho_shift']} k_tiles={t['k_tiles']} split_rho_shift={t['split_rho_shift']} " \
+              f"split_k_patches={t['split_k_patches']}"
+        out[key] = out.get(key, 0) + 1
+    return out
 
 
 def bench_batch(args):
@@ -157,7 +171,7 @@ def bench_batch(args):
     mols = molecules.water_cluster_batch(64, 4, 64, seed=3)
     prec = MODES[args.precision]
     hip = Hip()
-    ctxs, infos, Ps, singles = [], [], [], []
+    ctxs, infos, Ps, singles, tunes = [], [], [], [], []
     t0 = time.perf_counter()
     for k, mol in enumerate(mols):
         if k % 8 == 0:
@@ -169,6 +183,8 @@ def bench_batch(args):
         PA = np.ascontiguousarray(0.5 * guess.density_guess(mol, info, seed=0, eps=0.05))
         dP, dK, dE = hip.malloc(PA.nbytes), hip.malloc(PA.nbytes), hip.malloc(8)
         hip.h2d(dP, PA)
+        if not args.no_autotune:
+            tunes.append(ctx.autotune(dP, dK, dE, reps=2).as_dict())
         hip.memset(dK, PA.nbytes)
         ctxs.append(ctx)
         infos.append(info)
@@ -240,6 +256,7 @@ def bench_batch(args):
                                    "cluster_latency_ms_p50": float(np.percentile(lat_ms, 50)),
                                    "cluster_latency_ms_p99": float(np.percentile(lat_ms, 99))},
         "speedup_batch_vs_single": t_single / t_batch,
+        "tuning_chosen": _tally(tunes),
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -256,6 +273,7 @@ def main():
                     help="mode whose throughput is the reported value (the others are measured beside it)")
     ap.add_argument("--modes", default="fp64,bf16x2,bf16x3", help="comma list of modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -298,15 +316,16 @@ def main():
     hip = Hip()
     res = {}
     for m in modes:
-        dt, st, E, K = time_mode(hip, info, grid, PA, MODES[m], wl["ao_screen"], args.steps, args.warmup)
-        res[m] = dict(dt=dt, st=st, E=E, K=K, k_ms=list(st.last_kernel_ms))
+        dt, st, E, K, tune = time_mode(hip, info, grid, PA, MODES[m], wl["ao_screen"], args.steps, args.warmup,
+                                       not args.no_autotune)
+        res[m] = dict(dt=dt, st=st, E=E, K=K, k_ms=list(st.last_kernel_ms), tune=tune)
         print(f"[bench] {m:7s} E_xc {E:.10f}  build {dt * 1e3:.2f} ms  kernels ms: "
               + ", ".join(f"{t:.2f}" for t in res[m]["k_ms"][:3]), file=sys.stderr, flush=True)
     ref = res["fp64"]
     summary = {}
     for m in modes:
         r = res[m]
-        e = {"ms": r["dt"] * 1e3, "gflops": r["st"].flops / r["dt"] / 1e9}
+        e = {"ms": r["dt"] * 1e3, "gflops": r["st"].flops / r["dt"] / 1e9, "tuning": r["tune"]}
         if m != "fp64":
             e["abs_dE"] = abs(r["E"] - ref["E"])
             e["max_abs_dK"] = float(np.abs(r["K"] - ref["K"]).max())

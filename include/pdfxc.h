@@ -82,6 +82,20 @@ typedef struct pdfxc_stats {
                                 bf16 K kernel's LDS) */
 } pdfxc_stats;
 
+/* Launch shapes of the build kernels.  rho_shift (0..3) / split_rho_shift
+ * (0..2): move every block that many classes toward fewer points per
+ * workgroup (rho pass); k_tiles (8..96): FP64 K pass work item size in lower
+ * 16 x 16 tiles for blocks of more than 32 tiles; split_k_patches (4..32):
+ * split K pass work item size in 2 x 2 tile patches.  Defaults 0, 96, 0, 32.
+ * ms: the build time measured for it (pdfxc_autotune), else 0. */
+typedef struct pdfxc_tuning {
+    int rho_shift;
+    int k_tiles;
+    int split_rho_shift;
+    int split_k_patches;
+    double ms;
+} pdfxc_tuning;
+
 typedef struct pdfxc_ctx pdfxc_ctx;
 typedef struct pdfxc_batch pdfxc_batch;
 
@@ -219,6 +233,16 @@ int pdfxc_save_general_matrix(const char* path, int64_t rows, int64_t cols, cons
  * context's current (last built) rho. */
 int pdfxc_load_grid_matrix(pdfxc_ctx* ctx, const char* path, int64_t* n_pts, int* n_cols);
 int pdfxc_save_grid_matrix(pdfxc_ctx* ctx, const char* path, int n_rho_cols);
+
+/* Set the launch shapes and rebuild the context's work lists (the grid and its
+ * blocks are kept; the stored rho of update mode is cleared). */
+int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* tuning);
+/* Autotuner: time `reps` builds of functional on device buffers dP / dK / dE
+ * (as pdfxc_build_rks_device) for each launch shape of a fixed candidate list
+ * for the context's precision, keep the fastest and return it (with its time)
+ * in chosen.  dK is accumulated into by every trial build. */
+int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK, double* dE, int reps,
+                   pdfxc_tuning* chosen);
 
 /* Density of P at arbitrary points: DfCalcGridX::gridDensity
  * (DfCalcGridX.cpp:1038-1044) as TlDensField::makeDensFld uses it

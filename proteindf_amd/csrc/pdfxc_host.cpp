@@ -552,19 +552,23 @@ static int build_blocks(Context& c) {
     c.mean_nao = nb ? sum_nao / nb : 0.0;
 
     // k_rho work lists by template class: ldn <= 128 (64 points per
-    // workgroup), <= 256 (32), <= 512, <= 1024 (16)
+    // workgroup), <= 256 (32), <= 512, <= 1024 (16); the tuning may move every
+    // block up by rho_shift classes (fewer points per workgroup)
     std::vector<int> rho_work;
     c.rho_launches.clear();
     const int RS = c.rec_stride;
+    auto rho_class = [&](int ldn) {
+        int base = 0;
+        while ((128 << base) < ldn) ++base;
+        return std::min(3, base + c.tune.rho_shift);
+    };
     for (int cls = 0; cls < 4; ++cls) {
-        const int lim = 128 << cls;
-        const int lo_lim = cls ? (64 << cls) : 0;
         const size_t pb = (size_t)rho_points_per_workgroup(cls);
         LaunchRange r{cls, (int)rho_work.size(), 0, 0};
         size_t maxdyn = 0;
         for (int b = 0; b < nb; ++b) {
             const BlockDesc& B = c.blocks[b];
-            if (B.ldn > lo_lim && B.ldn <= lim) {
+            if (rho_class(B.ldn) == cls) {
                 const size_t lds = sizeof(double) * (std::max<size_t>((size_t)B.ldn * pb, 4 * 512) +
                                                      (size_t)B.nshell * RS) +
                                    sizeof(int) * (size_t)(B.ldn + 2 * B.nshell);
@@ -590,7 +594,8 @@ static int build_blocks(Context& c) {
         const int T = B.ldn / 16;
         const int nt = T * (T + 1) / 2;
         const int cls = (nt <= 32) ? 0 : 1;
-        const int per = (cls == 0) ? 32 : 96;  // at most 8 x 12 tiles (the kernel's consumer limit)
+        // at most 8 x 12 tiles (the kernel's consumer limit); the tuning may cut items smaller
+        const int per = (cls == 0) ? 32 : c.tune.k_tiles;
         for (int t0 = 0; t0 < nt; t0 += per) {
             KWork w;
             w.block = b;
@@ -672,14 +677,17 @@ static int build_blocks(Context& c) {
                 return PDFXC_ERR_UNSUPPORTED;
             }
         }
+        auto srho_class = [&](int ldnp) {
+            int base = 0;
+            while ((256 << base) < ldnp) ++base;
+            return std::min(2, base + c.tune.split_rho_shift);
+        };
         for (int cls = 0; cls < 3; ++cls) {
-            const int lim = 256 << cls;
-            const int lo_lim = cls ? (128 << cls) : 0;
             const size_t pb = (size_t)split_points_per_workgroup(cls);
             LaunchRange r{cls, (int)srho_work.size(), 0, 0};
             for (int b = 0; b < nb; ++b) {
                 const SplitBlock& SB = c.sblocks[b];
-                if (SB.ldnp <= lo_lim || SB.ldnp > lim) continue;
+                if (srho_class(SB.ldnp) != cls) continue;
                 const size_t plane = std::max(((size_t)S * pb * (SB.ldnp + 8) * 2 + 15) / 16 * 16,
                                               (size_t)8 * 512 * sizeof(double));
                 const size_t lds = plane + sizeof(double) * (size_t)c.blocks[b].nshell * RS +
@@ -707,11 +715,12 @@ static int build_blocks(Context& c) {
             const int T = B.ldn / 16;
             const int Tp = (T + 1) / 2;
             const int np = Tp * (Tp + 1) / 2;
-            for (int q0 = 0; q0 < np; q0 += 32) {
+            const int per = c.tune.split_k_patches;  // at most 32 (8 consumer waves x 4)
+            for (int q0 = 0; q0 < np; q0 += per) {
                 KWork w;
                 w.block = b;
                 w.t0 = q0;
-                w.t1 = std::min(np, q0 + 32);
+                w.t1 = std::min(np, q0 + per);
                 int P = 0;
                 while ((P + 1) * (P + 2) / 2 <= w.t1 - 1) ++P;
                 w.ncols = 32 * (P + 1);
@@ -1580,6 +1589,54 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
     std::copy(g.begin(), g.begin() + 3 * na, grad_functional);
     std::copy(g.begin() + 3 * na, g.end(), grad_weight);
     if (E_xc) *E_xc = E;
+    return PDFXC_OK;
+}
+
+int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* t) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c || !t) return arg_error("pdfxc_set_tuning: bad arguments");
+    if (t->rho_shift < 0 || t->rho_shift > 3 || t->split_rho_shift < 0 || t->split_rho_shift > 2 ||
+        t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32)
+        return arg_error("pdfxc_set_tuning: shape out of range");
+    c->tune = *t;
+    if (c->n_pts == 0) return PDFXC_OK;
+    PDFXC_HIP(hipStreamSynchronize(c->stream));
+    return build_blocks(*c);
+}
+
+int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK, double* dE, int reps,
+                   pdfxc_tuning* chosen) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_autotune: null context");
+    if (int rc = check_functional(functional, "pdfxc_autotune")) return rc;
+    if (!dP || !dK || !dE || reps <= 0) return arg_error("pdfxc_autotune: bad arguments");
+    if (c->n_pts == 0 || c->blocks.empty()) {
+        set_error("pdfxc_autotune: no grid set");
+        return PDFXC_ERR_NO_GRID;
+    }
+    std::vector<pdfxc_tuning> cand;
+    if (c->opts.precision == PDFXC_FP64) {
+        for (int rs : {0, 1})
+            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 32, 0.0});
+    } else {
+        for (int rs : {0, 1})
+            for (int kp : {32, 16}) cand.push_back(pdfxc_tuning{0, 96, rs, kp, 0.0});
+    }
+    pdfxc_tuning best = cand[0];
+    best.ms = 1e30;
+    for (pdfxc_tuning t : cand) {
+        if (int rc = pdfxc_set_tuning(ctx, &t)) return rc;
+        if (int rc = build_device(c, functional, dP, dK, dE)) return rc;  // warm-up
+        double sum = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            if (int rc = build_device(c, functional, dP, dK, dE)) return rc;
+            sum += c->last_build_ms;
+        }
+        t.ms = sum / reps;
+        if (t.ms < best.ms) best = t;
+    }
+    if (int rc = pdfxc_set_tuning(ctx, &best)) return rc;
+    if (chosen) *chosen = best;
     return PDFXC_OK;
 }
 

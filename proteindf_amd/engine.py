@@ -41,7 +41,8 @@ EXPORTED = [
     "pdfxc_load_grid_matrix", "pdfxc_save_grid_matrix", "pdfxc_build_uks", "pdfxc_build_uks_update",
     "pdfxc_build_uks_device", "pdfxc_build_uks_update_device", "pdfxc_eval_functional_uks",
     "pdfxc_density_at_points", "pdfxc_batch_create", "pdfxc_batch_destroy", "pdfxc_batch_offsets",
-    "pdfxc_batch_build_rks_device", "pdfxc_batch_build_rks", "pdfxc_batch_stats",
+    "pdfxc_batch_build_rks_device", "pdfxc_batch_build_rks", "pdfxc_batch_stats", "pdfxc_set_tuning",
+    "pdfxc_autotune",
 ]
 # matrix file types (TlMatrixObject::MatrixType)
 PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
@@ -57,6 +58,14 @@ class Opts(ctypes.Structure):
     _fields_ = [("density_cutoff", ctypes.c_double), ("ao_screen", ctypes.c_double),
                 ("precision", ctypes.c_int), ("block_points", ctypes.c_int),
                 ("scratch_bytes", ctypes.c_int64)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("rho_shift", ctypes.c_int), ("k_tiles", ctypes.c_int), ("split_rho_shift", ctypes.c_int),
+                ("split_k_patches", ctypes.c_int), ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class Stats(ctypes.Structure):
@@ -120,6 +129,8 @@ def lib():
         L.pdfxc_batch_build_rks_device.argtypes = [P, i32, P, P, P]
         L.pdfxc_batch_build_rks.argtypes = [P, i32, P, P, P]
         L.pdfxc_batch_stats.argtypes = [P, P]
+        L.pdfxc_set_tuning.argtypes = [P, P]
+        L.pdfxc_autotune.argtypes = [P, i32, P, P, P, i32, P]
         _lib = L
     return _lib
 
@@ -277,6 +288,17 @@ class XcContext:
 
     def save_grid_matrix(self, path, n_rho_cols=4):
         check(lib().pdfxc_save_grid_matrix(self.h, _path(path), n_rho_cols))
+
+    def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=32):
+        t = Tuning(rho_shift, k_tiles, split_rho_shift, split_k_patches, 0.0)
+        check(lib().pdfxc_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, dP_ptr, dK_ptr, dE_ptr, reps=3, functional=PDFXC_B3LYP):
+        """Time the candidate launch shapes on device buffers, keep the fastest; returns it."""
+        t = Tuning()
+        check(lib().pdfxc_autotune(self.h, functional, ctypes.c_void_p(dP_ptr), ctypes.c_void_p(dK_ptr),
+                                   ctypes.c_void_p(dE_ptr), reps, ctypes.byref(t)))
+        return t
 
     def density_at_points(self, P_packed, xyz, want_grad=False):
         """rho = phi^T P phi at arbitrary points (gridDensity); returns rho (n,) or (rho, grad (n, 3))."""

@@ -1,0 +1,74 @@
+"""GPU checks of the launch-shape tuning: every shape gives the same build
+(E to 1e-12, K to 1e-12: only summation orders change) in FP64 and split
+precision, the autotuner returns one of its candidates with its measured
+time and leaves the context on it, and out-of-range shapes are refused."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from proteindf_amd import calc_grid, engine, guess, molecules
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def _case():
+    mol = molecules.water_cluster(4, 21)
+    info = calc_grid.orbital_info(mol)
+    PA = 0.5 * guess.density_guess(mol, info, seed=21, eps=0.05)
+    grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 40, 194)
+    return mol, info, PA, grid
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_every_shape_gives_the_same_build(prec):
+    mol, info, PA, grid = _case()
+    ctx = engine.XcContext(info, engine.default_opts(precision=engine.PRECISIONS[prec]))
+    ctx.set_grid(grid)
+    P = guess.pack_lower(PA)
+    E0, K0, _ = ctx.build_rks(P)
+    shapes = [dict(rho_shift=1), dict(rho_shift=3, k_tiles=8), dict(k_tiles=24), dict(split_rho_shift=1),
+              dict(split_rho_shift=2, split_k_patches=4), dict(split_k_patches=16)]
+    for sh in shapes:
+        ctx.set_tuning(**sh)
+        E, K, _ = ctx.build_rks(P)
+        say(prec, sh, "dE", abs(E - E0), "max dK", np.abs(K - K0).max())
+        assert abs(E - E0) <= 1e-12 * abs(E0)
+        np.testing.assert_allclose(K, K0, rtol=0, atol=1e-12)
+    with pytest.raises(engine.PdfxcError) as ei:
+        ctx.set_tuning(k_tiles=200)
+    assert ei.value.code == engine.ERROR_CODES["ARGUMENT"]
+    ctx.close()
+
+
+def test_autotune_keeps_a_candidate_and_builds_correctly():
+    mol, info, PA, grid = _case()
+    ctx = engine.XcContext(info)
+    ctx.set_grid(grid)
+    n = info.n_aos
+    Pd = np.ascontiguousarray(PA)
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    ptr = []
+    for nbytes in (Pd.nbytes, Pd.nbytes, 8):
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), nbytes) == 0
+        ptr.append(p.value)
+    assert hip.hipMemcpy(ptr[0], Pd.ctypes.data, Pd.nbytes, 1) == 0
+    t = ctx.autotune(*ptr, reps=2)
+    say("autotune chose", t.as_dict())
+    assert t.ms > 0 and t.rho_shift in (0, 1) and t.k_tiles in (96, 48, 24)
+    for p in ptr:
+        hip.hipFree(ctypes.c_void_p(p))
+    E, K, _ = ctx.build_rks(guess.pack_lower(PA))
+    ctx.close()
+    e_ref, K_ref, _ = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), grid, PA)
+    assert abs(E - e_ref) <= 1e-10
+    assert np.abs(guess.unpack_lower(K, n) - K_ref).max() <= 1e-10
