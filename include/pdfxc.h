@@ -127,10 +127,49 @@ int pdfxc_set_grid(pdfxc_ctx* ctx, int64_t n_pts, const double* x, const double*
 /* Generate the molecular grid on the device: grid-type USER, Euler-Maclaurin
  * radial rule, no pruning, Becke partitioning with atomic size adjustments,
  * |w| > weight_cutoff (DfGenerateGrid.cpp:39-250, :730-768, :934-1091,
- * :1597-1610, :1715-1876).  Atoms with Z = 0 are dummies.  Sets the grid of
- * the context and returns the point count. */
+ * :1597-1610, :1715-1876): pdfxc_generate_grid_ex with those options, except
+ * that n_ang must be a supported Lebedev order.  Atoms with Z = 0 are
+ * dummies.  Sets the grid of the context and returns the point count. */
 int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
                         int n_rad, int n_ang, double weight_cutoff, int64_t* n_pts);
+/* Molecular grid options, as DfGenerateGrid's constructor resolves the
+ * pdfparam keys (DfGenerateGrid.cpp:39-170, setCellPara :282-430). */
+enum { PDFXC_RADIAL_EM = 0, PDFXC_RADIAL_GC = 1 };                       /* Euler-Maclaurin, Gauss-Chebyshev */
+enum { PDFXC_GC_BECKE = 0, PDFXC_GC_TA = 1, PDFXC_GC_KK = 2 };           /* GC radial mappings */
+enum { PDFXC_PRUNE_NONE = 0, PDFXC_PRUNE_MHL = 1, PDFXC_PRUNE_SG1 = 2 }; /* angular pruning */
+enum { PDFXC_PARTITION_BECKE = 0, PDFXC_PARTITION_SS = 1 };              /* Becke, Stratmann-Scuseria */
+enum { PDFXC_RADII_BRAGG = 0, PDFXC_RADII_SG1 = 1 };                     /* radial scale R per element */
+typedef struct pdfxc_grid_opts {
+    int n_rad;          /* radial shells (nrgrid) */
+    int n_ang;          /* Lebedev order before pruning (nOgrid), floored to a supported one */
+    int radial;         /* PDFXC_RADIAL_* */
+    int gc_mapping;     /* PDFXC_GC_*, for PDFXC_RADIAL_GC */
+    int pruning;        /* PDFXC_PRUNE_*: MHL n = min(int(5 r / R_Bragg * n_ang), n_ang); SG-1 regions */
+    int partitioning;   /* PDFXC_PARTITION_* */
+    int size_adjust;    /* Becke atomic size adjustments (isAtomicSizeAdjustments_) */
+    int radii;          /* PDFXC_RADII_*: Bragg-Slater table, or the SG-1 list */
+    double weight_cutoff; /* points with |w| <= weight_cutoff are dropped (1e-16) */
+} pdfxc_grid_opts;
+
+/* Options of a grid type ("sg-1", "coarse", "medium", "medium-fine", "fine",
+ * "ultra-fine", "user"; case-insensitive, '_' for '-'), with the defaults the
+ * constructor gives before the grid/ keys override them:
+ * presets: EM, MHL pruning, SS partitioning, size adjustments, Bragg-Slater R;
+ * sg-1: 50 x 194, EM, SG-1 pruning and radii, SS, no size adjustments;
+ * user: 75 x 302, EM, MHL pruning, SS, size adjustments.  Unknown type:
+ * PDFXC_ERR_UNSUPPORTED (the reference aborts). */
+int pdfxc_grid_opts_preset(const char* grid_type, pdfxc_grid_opts* out);
+
+/* Generate the molecular grid for any option set (generateGrid /
+ * generateGrid_atom, DfGenerateGrid.cpp:730-768, :934-1091): per atom and
+ * radial shell (r_i > 30 dropped) the pruned Lebedev rule in the standard
+ * orientation, times the Becke or SS partition weight of the point's atom;
+ * every atom, a dummy (Z = 0) included, has a partition cell as in the
+ * reference (Becke skips dummy neighbours, SS does not).  Sets the grid of
+ * the context and returns the point count. */
+int pdfxc_generate_grid_ex(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
+                           const pdfxc_grid_opts* opts, int64_t* n_pts);
+
 /* Copy the context's grid back in the caller's point order (x, y, z, w
  * columns, n_pts each). */
 int pdfxc_get_grid(pdfxc_ctx* ctx, double* x, double* y, double* z, double* w);
@@ -190,7 +229,9 @@ int pdfxc_eval_functional_uks(int functional, int64_t n, const double* in, doubl
  * assignment is used); AO centres must coincide with atoms.  Outputs dE/dR,
  * n_atoms x 3 each: grad_functional = the reference's functional term
  * (AO-centre and point-translation parts, doubled for RKS); grad_weight = the
- * derivative of the size-adjusted Becke partition weights (the reference's
+ * derivative of the Becke partition weights, with or without the size
+ * adjustments as pdfxc_generate_grid_ex made the grid (SS-partitioned grids
+ * and dummy atoms are refused with PDFXC_ERR_UNSUPPORTED; the reference's
  * own weight term evaluates to zero, DfGenerateGrid.cpp:2355-2372).  Their
  * sum is the gradient of E_xc.  E_xc (may be NULL) is returned.  FP64
  * contractions whatever the context's precision.  The context's stored rho

@@ -61,6 +61,17 @@ def lib():
         L.oracle_becke_weight.restype = d
         L.oracle_generate_grid.argtypes = [i32, P, P, i32, i32, P, d, P, P, i32]
         L.oracle_generate_grid.restype = i64
+        L.oracle_radial_gc.argtypes = [d, i32, i32, i32, P, P]
+        L.oracle_sg1_radius.argtypes = [i32]
+        L.oracle_sg1_radius.restype = d
+        L.oracle_lebedev_floor.argtypes = [i32]
+        L.oracle_lebedev_floor.restype = i32
+        L.oracle_pruned_order.argtypes = [i32, i32, d, d, i32]
+        L.oracle_pruned_order.restype = i32
+        L.oracle_ss_weight.argtypes = [i32, P, i32, P]
+        L.oracle_ss_weight.restype = d
+        L.oracle_generate_grid_ex.argtypes = [i32, P, P, P, P, P, P, i32]
+        L.oracle_generate_grid_ex.restype = i64
         L.oracle_eval_ao_hessians.argtypes = [P, i64, P, P]
         L.oracle_becke_weight_gradient.argtypes = [i32, P, P, i32, P, P]
         L.oracle_becke_weight_gradient.restype = d
@@ -232,6 +243,59 @@ def becke_weight(Z, xyz, iatom, r, size_adjust=True):
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     r = np.ascontiguousarray(r, dtype=np.float64)
     return lib().oracle_becke_weight(len(Z), _p(Z), _p(xyz), iatom, _p(r), int(size_adjust))
+
+
+class GridOpts(ctypes.Structure):
+    """oracle_grid_opts: radial 0 EM / 1 GC; gc_mapping 0 Becke / 1 TA / 2 KK;
+    pruning 0 none / 1 MHL / 2 SG-1; partitioning 0 Becke / 1 SS; radii 0
+    Bragg-Slater / 1 SG-1."""
+    _fields_ = [("n_rad", ctypes.c_int), ("n_ang", ctypes.c_int), ("radial", ctypes.c_int),
+                ("gc_mapping", ctypes.c_int), ("pruning", ctypes.c_int), ("partitioning", ctypes.c_int),
+                ("size_adjust", ctypes.c_int), ("radii", ctypes.c_int), ("weight_cutoff", ctypes.c_double)]
+
+
+def grid_opts(n_rad=75, n_ang=302, radial=0, gc_mapping=1, pruning=0, partitioning=0, size_adjust=1,
+              radii=0, weight_cutoff=1e-16):
+    return GridOpts(n_rad, n_ang, radial, gc_mapping, pruning, partitioning, size_adjust, radii,
+                    weight_cutoff)
+
+
+def radial_gc(R, Nr, i, mapping):
+    r, w = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_radial_gc(R, Nr, i, mapping, ctypes.byref(r), ctypes.byref(w))
+    return r.value, w.value
+
+
+def sg1_radius(Z):
+    return lib().oracle_sg1_radius(Z)
+
+
+def lebedev_floor(n):
+    return lib().oracle_lebedev_floor(n)
+
+
+def pruned_order(pruning, Z, r, R, n_ang):
+    return lib().oracle_pruned_order(pruning, Z, r, R, n_ang)
+
+
+def ss_weight(xyz, iatom, r):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    return lib().oracle_ss_weight(len(xyz), _p(xyz), iatom, _p(r))
+
+
+def generate_grid_ex(Z, xyz, opts, O=None):
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    if O is None:
+        O = o_matrix(Z, xyz)
+    O = np.ascontiguousarray(O, dtype=np.float64)
+    L = lib()
+    n = L.oracle_generate_grid_ex(len(Z), _p(Z), _p(xyz), ctypes.byref(opts), _p(O), None, None, 0)
+    grid = np.zeros((n, 4))
+    atom = np.zeros(n, dtype=np.int32)
+    L.oracle_generate_grid_ex(len(Z), _p(Z), _p(xyz), ctypes.byref(opts), _p(O), _p(grid), _p(atom), 0)
+    return grid, atom
 
 
 def generate_grid(Z, xyz, n_rad, n_ang, O=None, weight_cutoff=1e-16):

@@ -833,67 +833,19 @@ double oracle_calc_xc_uks(const oracle_aos* aos, int functional, int64_t n_pts, 
 
 /* ------------------------------------------------------------------------ */
 /* Lebedev rules.  Each rule is a list of octahedral orbits (TlLebedevGrid.cpp */
-/* :93-269 lists the six orbit shapes; the per-order parameters :271-, 302    */
-/* at :649-697, 590 at :816-892).  The parameters are the published           */
-/* Lebedev-Laikov values.  The orbit is generated here by applying all 48      */
+/* :93-269 lists the six orbit shapes; the per-order parameters :271-1577).   */
+/* The parameters are the published Lebedev-Laikov values, all 22 orders of   */
+/* TlLebedevGrid.cpp:4-6 (lebedev_rules.inc, shared with the library).         */
+/* The orbit is generated here by applying all 48                             */
 /* signed permutations to a seed point and removing duplicates, so the point   */
 /* order differs from the reference (only the summation order changes).        */
 /* ------------------------------------------------------------------------ */
+#include "lebedev_rules.inc"
+
 typedef struct {
     int type;
     double a, b, w;
 } leb_orbit;
-
-static const leb_orbit LEB6[] = {{1, 0, 0, 0.1666666666666667}};
-static const leb_orbit LEB38[] = {{1, 0, 0, 0.9523809523809524E-2},
-                                  {3, 0, 0, 0.3214285714285714E-1},
-                                  {5, 0.4597008433809831E+0, 0, 0.2857142857142857E-1}};
-static const leb_orbit LEB86[] = {{1, 0, 0, 0.1154401154401154E-1},
-                                  {3, 0, 0, 0.1194390908585628E-1},
-                                  {4, 0.3696028464541502E+0, 0, 0.1111055571060340E-1},
-                                  {4, 0.6943540066026664E+0, 0, 0.1187650129453714E-1},
-                                  {5, 0.3742430390903412E+0, 0, 0.1181230374690448E-1}};
-static const leb_orbit LEB194[] = {{1, 0, 0, 0.1782340447244611E-2},
-                                   {2, 0, 0, 0.5716905949977102E-2},
-                                   {3, 0, 0, 0.5573383178848738E-2},
-                                   {4, 0.6712973442695226E+0, 0, 0.5608704082587997E-2},
-                                   {4, 0.2892465627575439E+0, 0, 0.5158237711805383E-2},
-                                   {4, 0.4446933178717437E+0, 0, 0.5518771467273614E-2},
-                                   {4, 0.1299335447650067E+0, 0, 0.4106777028169394E-2},
-                                   {5, 0.3457702197611283E+0, 0, 0.5051846064614808E-2},
-                                   {6, 0.1590417105383530E+0, 0.8360360154824589E+0, 0.5530248916233094E-2}};
-static const leb_orbit LEB302[] = {{1, 0, 0, 0.8545911725128148E-3},
-                                   {3, 0, 0, 0.3599119285025571E-2},
-                                   {4, 0.3515640345570105E+0, 0, 0.3449788424305883E-2},
-                                   {4, 0.6566329410219612E+0, 0, 0.3604822601419882E-2},
-                                   {4, 0.4729054132581005E+0, 0, 0.3576729661743367E-2},
-                                   {4, 0.9618308522614784E-1, 0, 0.2352101413689164E-2},
-                                   {4, 0.2219645236294178E+0, 0, 0.3108953122413675E-2},
-                                   {4, 0.7011766416089545E+0, 0, 0.3650045807677255E-2},
-                                   {5, 0.2644152887060663E+0, 0, 0.2982344963171804E-2},
-                                   {5, 0.5718955891878961E+0, 0, 0.3600820932216460E-2},
-                                   {6, 0.2510034751770465E+0, 0.8000727494073952E+0, 0.3571540554273387E-2},
-                                   {6, 0.1233548532583327E+0, 0.4127724083168531E+0, 0.3392312205006170E-2}};
-static const leb_orbit LEB590[] = {{1, 0, 0, 0.3095121295306187E-3},
-                                   {3, 0, 0, 0.1852379698597489E-2},
-                                   {4, 0.7040954938227469E+0, 0, 0.1871790639277744E-2},
-                                   {4, 0.6807744066455243E+0, 0, 0.1858812585438317E-2},
-                                   {4, 0.6372546939258752E+0, 0, 0.1852028828296213E-2},
-                                   {4, 0.5044419707800358E+0, 0, 0.1846715956151242E-2},
-                                   {4, 0.4215761784010967E+0, 0, 0.1818471778162769E-2},
-                                   {4, 0.3317920736472123E+0, 0, 0.1749564657281154E-2},
-                                   {4, 0.2384736701421887E+0, 0, 0.1617210647254411E-2},
-                                   {4, 0.1459036449157763E+0, 0, 0.1384737234851692E-2},
-                                   {4, 0.6095034115507196E-1, 0, 0.9764331165051050E-3},
-                                   {5, 0.6116843442009876E+0, 0, 0.1857161196774078E-2},
-                                   {5, 0.3964755348199858E+0, 0, 0.1705153996395864E-2},
-                                   {5, 0.1724782009907724E+0, 0, 0.1300321685886048E-2},
-                                   {6, 0.5610263808622060E+0, 0.3518280927733519E+0, 0.1842866472905286E-2},
-                                   {6, 0.4742392842551980E+0, 0.2634716655937950E+0, 0.1802658934377451E-2},
-                                   {6, 0.5984126497885380E+0, 0.1816640840360209E+0, 0.1849830560443660E-2},
-                                   {6, 0.3791035407695563E+0, 0.1720795225656878E+0, 0.1713904507106709E-2},
-                                   {6, 0.2778673190586244E+0, 0.8213021581932511E-1, 0.1555213603396808E-2},
-                                   {6, 0.5033564271075117E+0, 0.8999205842074875E-1, 0.1802239128008525E-2}};
 
 static int orbit_points(const leb_orbit* o, double* xyz) {
     double s[3];
@@ -930,28 +882,24 @@ static int orbit_points(const leb_orbit* o, double* xyz) {
 }
 
 int oracle_lebedev(int order, double* xyz, double* w) {
-    const leb_orbit* t = NULL;
+    const double(*t)[4] = NULL;
     int no = 0;
 #define LEB_CASE(N, T) case N: t = T; no = (int)(sizeof(T) / sizeof(T[0])); break;
     switch (order) {
-        LEB_CASE(6, LEB6)
-        LEB_CASE(38, LEB38)
-        LEB_CASE(86, LEB86)
-        LEB_CASE(194, LEB194)
-        LEB_CASE(302, LEB302)
-        LEB_CASE(590, LEB590)
+        LEBEDEV_RULE_LIST(LEB_CASE)
         default: return 0;
     }
 #undef LEB_CASE
     int n = 0;
     double buf[48 * 3];
     for (int k = 0; k < no; ++k) {
-        const int m = orbit_points(&t[k], buf);
+        const leb_orbit o = {(int)t[k][0], t[k][1], t[k][2], t[k][3]};
+        const int m = orbit_points(&o, buf);
         for (int i = 0; i < m; ++i) {
             xyz[3 * (n + i)] = buf[3 * i];
             xyz[3 * (n + i) + 1] = buf[3 * i + 1];
             xyz[3 * (n + i) + 2] = buf[3 * i + 2];
-            w[n + i] = t[k].w;
+            w[n + i] = o.w;
         }
         n += m;
     }
@@ -1096,18 +1044,131 @@ double oracle_becke_weight(int n_atoms, const int* Z, const double* xyz, int iat
     return mine / total;
 }
 
-int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz, int n_rad,
-                             int n_ang, const double* O, double weight_cutoff,
-                             double* out_grid, int* out_atom, int n_threads) {
-    double* leb = (double*)malloc(sizeof(double) * 4 * (size_t)n_ang);
-    double* lebw = leb + 3 * (size_t)n_ang;
-    if (oracle_lebedev(n_ang, leb, lebw) != n_ang) {
-        free(leb);
-        return -1;
+/* Gauss-Chebyshev (second kind) radial rule with the Becke, Treutler-Ahlrichs
+ * or Knowles-Kirby style mapping (getRadialAbscissaAndWeight_GaussChebyshev,
+ * DfGenerateGrid.cpp:1612-1655); the weight carries 4 pi r^2. */
+void oracle_radial_gc(double R, int Nr, int i, int mapping, double* r, double* w) {
+    const double inv_ln2 = 1.0 / log(2.0);
+    const double Nr1 = Nr + 1;
+    const double t = (double)i * M_PI / Nr1;
+    const double sn = sin(t), ss = sn * sn, cs = cos(t);
+    const double x = (Nr1 - 2.0 * (double)i) / Nr1 + (2.0 / M_PI) * (1.0 + (2.0 / 3.0) * ss) * cs * sn;
+    const double inv_x1 = 1.0 / (x - 1.0);
+    double ri = 0.0, jac = 0.0;
+    switch (mapping) {
+        case 0: /* Becke: r = R (1 + x) / (1 - x) */
+            ri = R * (1.0 + x) / (1.0 - x);
+            jac = R * ((x + 1.0) * inv_x1 * inv_x1 - inv_x1);
+            break;
+        case 1: /* TA: r = R / ln2 ln(2 / (1 - x)) */
+            ri = inv_ln2 * log(2.0 * (-inv_x1)) * R;
+            jac = R * inv_ln2 / (1.0 - x);
+            break;
+        default: /* KK: the TA form without R */
+            ri = inv_ln2 * log(2.0 * (-inv_x1));
+            jac = inv_ln2 / (1.0 - x);
+            break;
+    }
+    *r = ri;
+    *w = (16.0 / (3.0 * Nr1)) * ss * ss * jac * ri * ri * 4.0 * M_PI;
+}
+
+/* SG-1 atomic radii (setCellPara's SG-1 list, DfGenerateGrid.cpp:409-429) */
+double oracle_sg1_radius(int Z) {
+    static const double t[] = {0.0,    1.0000, 0.5882, 3.0769, 2.0513, 1.5385, 1.2308,
+                               1.0256, 0.8791, 0.7692, 0.6838, 4.0909, 3.1579, 2.5714,
+                               2.1687, 1.8750, 1.6514, 1.4754, 1.3333};
+    if (Z >= 1 && Z < (int)(sizeof(t) / sizeof(t[0]))) return t[Z];
+    return 0.0;
+}
+
+static const int LEB_ORDERS[] = {6,   14,  26,  38,  50,  74,  86,  110,  146,  170,  194,
+                                 230, 266, 302, 350, 434, 590, 770, 974, 1202, 1454, 1730};
+
+/* the largest supported order <= n, or 6 (getNumOfPrunedAnglarPoints' lookup, :1677-1684) */
+int oracle_lebedev_floor(int n) {
+    int best = LEB_ORDERS[0];
+    for (size_t k = 0; k < sizeof(LEB_ORDERS) / sizeof(LEB_ORDERS[0]); ++k)
+        if (LEB_ORDERS[k] <= n) best = LEB_ORDERS[k];
+    return best;
+}
+
+/* Angular points of a radial shell at r (DfGenerateGrid.cpp:1658-1713).
+ * pruning 0: the rule of n_ang (floored to a supported order); 1: Murray-Handy-
+ * Laming, n = min(int(5 r / R_Bragg * n_ang), n_ang) then floored; 2: SG-1
+ * regions of r / R by element row: 6, 38, 86, 194, 86. */
+int oracle_pruned_order(int pruning, int Z, double r, double R, int n_ang) {
+    if (pruning == 2) {
+        double al[4] = {0.0, 0.0, 0.0, 10000.0};
+        if (Z == 1 || Z == 2) {
+            al[0] = 0.25; al[1] = 0.5; al[2] = 1.0; al[3] = 4.5;
+        } else if (Z >= 3 && Z <= 10) {
+            al[0] = 0.1667; al[1] = 0.5; al[2] = 0.9; al[3] = 3.5;
+        } else if (Z >= 11 && Z <= 18) {
+            al[0] = 0.1; al[1] = 0.4; al[2] = 0.8; al[3] = 2.5;
+        }
+        const double q = r * (1.0 / R);
+        if (q < al[1]) return (q < al[0]) ? 6 : 38;
+        if (q < al[2]) return 86;
+        return (q < al[3]) ? 194 : 86;
+    }
+    int n = n_ang;
+    if (pruning == 1) {
+        const double v = (5.0 * r / oracle_bragg_slater(Z)) * (double)n_ang;
+        n = (v < (double)n_ang) ? (int)v : n_ang;
+    }
+    return oracle_lebedev_floor(n);
+}
+
+/* Stratmann-Scuseria partition weight (calcMultiCenterWeight_SS, :1878-1956):
+ * every atom, dummy atoms included, has a cell; no size adjustment. */
+double oracle_ss_weight(int n_atoms, const double* xyz, int iatom, const double* r) {
+    double* rr = (double*)malloc(sizeof(double) * (size_t)n_atoms);
+    for (int p = 0; p < n_atoms; ++p) {
+        const double dx = r[0] - xyz[3 * p], dy = r[1] - xyz[3 * p + 1], dz = r[2] - xyz[3 * p + 2];
+        rr[p] = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    double total = 0.0, mine = 0.0;
+    for (int m = 0; m < n_atoms; ++m) {
+        double ps = 1.0;
+        for (int n = 0; n < n_atoms; ++n) {
+            if (n == m) continue;
+            const double ex = xyz[3 * m] - xyz[3 * n], ey = xyz[3 * m + 1] - xyz[3 * n + 1],
+                         ez = xyz[3 * m + 2] - xyz[3 * n + 2];
+            const double u = (rr[m] - rr[n]) * (1.0 / sqrt(ex * ex + ey * ey + ez * ez));
+            double g;
+            if (u < -0.64) g = -1.0;
+            else if (u > 0.64) g = 1.0;
+            else {
+                const double q = u / 0.64, q2 = q * q, q3 = q2 * q, q5 = q2 * q3;
+                g = (1.0 / 16.0) * (35.0 * q - 35.0 * q3 + 21.0 * q5 - 5.0 * q5 * q2);
+            }
+            ps *= 0.5 * (1.0 - g);
+        }
+        if (m == iatom) mine = ps;
+        total += ps;
+    }
+    free(rr);
+    return mine / total;
+}
+
+int64_t oracle_generate_grid_ex(int n_atoms, const int* Z, const double* xyz, const oracle_grid_opts* opt,
+                                const double* O, double* out_grid, int* out_atom, int n_threads) {
+    /* the rules a shell can use: every supported order up to the largest */
+    const int top = (opt->pruning == 2) ? 194 : oracle_lebedev_floor(opt->n_ang);
+    double* leb[32] = {0};
+    double* lebw[32] = {0};
+    int nord = 0, ord[32];
+    for (size_t k = 0; k < sizeof(LEB_ORDERS) / sizeof(LEB_ORDERS[0]) && LEB_ORDERS[k] <= top; ++k) {
+        ord[nord] = LEB_ORDERS[k];
+        leb[nord] = (double*)malloc(sizeof(double) * 4 * (size_t)LEB_ORDERS[k]);
+        lebw[nord] = leb[nord] + 3 * (size_t)LEB_ORDERS[k];
+        oracle_lebedev(LEB_ORDERS[k], leb[nord], lebw[nord]);
+        ++nord;
     }
     /* each atom fills its own buffer (generateGrid_atom, :934-1091); the
      * blocks are then concatenated in atom order */
-    const size_t cap = (size_t)n_rad * (size_t)n_ang;
+    const size_t cap = (size_t)opt->n_rad * (size_t)top;
     double** buf = (double**)calloc((size_t)n_atoms, sizeof(double*));
     int64_t* cnt = (int64_t*)calloc((size_t)n_atoms, sizeof(int64_t));
     (void)n_threads;
@@ -1117,21 +1178,26 @@ int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz, int n
         if (Z[a] > 0) {
             double* B = (double*)malloc(sizeof(double) * 4 * cap);
             buf[a] = B;
-            const double R = oracle_bragg_slater(Z[a]);
-            for (int i = 1; i <= n_rad; ++i) {
+            const double R = (opt->radii == 1) ? oracle_sg1_radius(Z[a]) : oracle_bragg_slater(Z[a]);
+            for (int i = 1; i <= opt->n_rad; ++i) {
                 double ri, wr;
-                oracle_radial_em(R, n_rad, i, &ri, &wr);
+                if (opt->radial == 1) oracle_radial_gc(R, opt->n_rad, i, opt->gc_mapping, &ri, &wr);
+                else oracle_radial_em(R, opt->n_rad, i, &ri, &wr);
                 if (ri > 30.0) continue;
-                for (int k = 0; k < n_ang; ++k) {
+                const int na = oracle_pruned_order(opt->pruning, Z[a], ri, R, opt->n_ang);
+                int q = 0;
+                while (ord[q] != na) ++q;
+                for (int k = 0; k < na; ++k) {
                     double p[3];
                     for (int d = 0; d < 3; ++d) {
-                        const double v = O[3 * d + 0] * leb[3 * k] + O[3 * d + 1] * leb[3 * k + 1] +
-                                         O[3 * d + 2] * leb[3 * k + 2];
+                        const double v = O[3 * d + 0] * leb[q][3 * k] + O[3 * d + 1] * leb[q][3 * k + 1] +
+                                         O[3 * d + 2] * leb[q][3 * k + 2];
                         p[d] = v * ri + xyz[3 * a + d];
                     }
-                    double w = lebw[k] * wr;
-                    w *= oracle_becke_weight(n_atoms, Z, xyz, a, p, 1);
-                    if (fabs(w) > weight_cutoff) {
+                    double w = lebw[q][k] * wr;
+                    w *= (opt->partitioning == 1) ? oracle_ss_weight(n_atoms, xyz, a, p)
+                                                  : oracle_becke_weight(n_atoms, Z, xyz, a, p, opt->size_adjust);
+                    if (fabs(w) > opt->weight_cutoff) {
                         B[4 * c + 0] = p[0];
                         B[4 * c + 1] = p[1];
                         B[4 * c + 2] = p[2];
@@ -1155,8 +1221,16 @@ int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz, int n
     }
     free(buf);
     free(cnt);
-    free(leb);
+    for (int q = 0; q < nord; ++q) free(leb[q]);
     return total;
+}
+
+int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz, int n_rad,
+                             int n_ang, const double* O, double weight_cutoff,
+                             double* out_grid, int* out_atom, int n_threads) {
+    if (oracle_lebedev_floor(n_ang) != n_ang) return -1;
+    const oracle_grid_opts opt = {n_rad, n_ang, 0, 1, 0, 0, 1, 0, weight_cutoff};
+    return oracle_generate_grid_ex(n_atoms, Z, xyz, &opt, O, out_grid, out_atom, n_threads);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -90,7 +90,7 @@ double oracle_calc_xc_uks(const oracle_aos* aos, int functional, int64_t n_pts, 
                           const double* PA, const double* PB, double* FA, double* FB, double* rho_out,
                           double density_cutoff, int exact_gab, int n_threads);
 
-/* Lebedev rule of the given order (6, 38, 86, 194, 302, 590 supported);
+/* Lebedev rule of the given order (the 22 orders 6 ... 1730 of TlLebedevGrid);
  * returns number of points written (0 if unsupported). xyz: n x 3, w: n. */
 int oracle_lebedev(int order, double* xyz, double* w);
 
@@ -120,6 +120,29 @@ int64_t oracle_generate_grid(int n_atoms, const int* Z, const double* xyz,
                              int n_rad, int n_ang, const double* O,
                              double weight_cutoff, double* out_grid,
                              int* out_atom, int n_threads);
+
+/* Grid options, as DfGenerateGrid's constructor resolves them (DfGenerateGrid.cpp:39-170).
+ * radial 0 EM, 1 GC; gc_mapping 0 Becke, 1 TA, 2 KK; pruning 0 none, 1 MHL,
+ * 2 SG-1; partitioning 0 Becke, 1 SS; size_adjust: Becke atomic size
+ * adjustments; radii 0 Bragg-Slater, 1 SG-1. */
+typedef struct {
+    int n_rad, n_ang, radial, gc_mapping, pruning, partitioning, size_adjust, radii;
+    double weight_cutoff;
+} oracle_grid_opts;
+
+/* Gauss-Chebyshev radial abscissa/weight (DfGenerateGrid.cpp:1612-1655). */
+void oracle_radial_gc(double R, int Nr, int i, int mapping, double* r, double* w);
+/* SG-1 radius list (DfGenerateGrid.cpp:409-429). */
+double oracle_sg1_radius(int Z);
+/* the largest supported Lebedev order <= n (6 when n < 6). */
+int oracle_lebedev_floor(int n);
+/* angular order of a radial shell at r (DfGenerateGrid.cpp:1658-1713). */
+int oracle_pruned_order(int pruning, int Z, double r, double R, int n_ang);
+/* Stratmann-Scuseria partition weight (DfGenerateGrid.cpp:1878-1956). */
+double oracle_ss_weight(int n_atoms, const double* xyz, int iatom, const double* r);
+/* The molecular grid for any option set (DfGenerateGrid.cpp:934-1091). */
+int64_t oracle_generate_grid_ex(int n_atoms, const int* Z, const double* xyz, const oracle_grid_opts* opt,
+                                const double* O, double* out_grid, int* out_atom, int n_threads);
 
 /* AO second derivatives (getD2AOs_core, DfCalcGridX.cpp:771-863): h6 is
  * n_pts x n_aos x 6 (xx, xy, xz, yy, yz, zz). */

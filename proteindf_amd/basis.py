@@ -74,19 +74,25 @@ class OrbitalInfo:
         return len(self.ao_type)
 
     @classmethod
-    def build(cls, mol, normalizer, basis_for=basis_631gd):
+    def build(cls, mol, normalizer, basis_for=basis_631gd, labels=None):
+        """basis_for(symbol) gives an atom's CGTOs; with labels (one string per
+        atom) it is called as basis_for(symbol, label), the reference's
+        "symbol@label" basis keys (TlOrbitalInfoObject.cpp:199-212)."""
         cache = {}
         atom, center, typ, nprim, exps, coefs = [], [], [], [], [], []
         for a, (s, p) in enumerate(zip(mol.symbols, mol.xyz)):
             if s == "X":
                 continue
-            if s not in cache:
+            key = (s,) if labels is None else (s, labels[a])
+            if key not in cache:
                 shells = []
-                for cg in basis_for(s):
+                for cg in basis_for(*key):
+                    if cg.shell not in SHELL_L:
+                        raise ValueError(f"shell '{cg.shell}' of atom {a} ({s}): only s, p and d shells are supported")
                     e, c = normalizer(SHELL_L[cg.shell], cg.exps, cg.coefs)
                     shells.append((SHELL_L[cg.shell], np.asarray(e), np.asarray(c)))
-                cache[s] = shells
-            for l, e, c in cache[s]:
+                cache[key] = shells
+            for l, e, c in cache[key]:
                 for k in range(2 * l + 1):
                     atom.append(a)
                     center.append(p)

@@ -83,12 +83,18 @@ def orbital_info(mol, basis_for=basis_631gd):
 
 
 class DfGenerateGrid:
-    """grid-type USER, EM radial, no pruning, Becke partitioning."""
+    """The molecular grid.  Without opts: grid-type USER, EM radial, no
+    pruning, Becke partitioning with size adjustments.  opts: an
+    engine.GridOpts (engine.grid_opts_preset, or PdfParam.grid.to_opts()) for
+    any grid type the reference's constructor resolves (DfGenerateGrid.cpp:39-170)."""
 
-    def __init__(self, mol, n_rad=75, n_ang=302, weight_cutoff=1e-16):
+    def __init__(self, mol, n_rad=75, n_ang=302, weight_cutoff=1e-16, opts=None):
         self.mol, self.n_rad, self.n_ang, self.weight_cutoff = mol, n_rad, n_ang, weight_cutoff
+        self.opts = opts
 
     def generate(self, ctx):
+        if self.opts is not None:
+            return ctx.generate_grid_ex(self.mol.Z, self.mol.xyz, self.opts)
         return ctx.generate_grid(self.mol.Z, self.mol.xyz, self.n_rad, self.n_ang,
                                  self.weight_cutoff)
 
@@ -115,8 +121,9 @@ class WorkFiles:
 
 
 class DfCalcGridX:
-    """grid: an (n, 4) array, or a grid.mat path (its points, atoms and stored rho), or
-    None to generate the grid."""
+    """grid: an (n, 4) array, a grid.mat path (its points, atoms and stored rho),
+    an engine.GridOpts to generate that grid, or None to generate the USER / EM /
+    unpruned / Becke grid of n_rad x n_ang."""
 
     def __init__(self, mol, grid=None, n_rad=75, n_ang=302, basis_for=basis_631gd,
                  density_cutoff=1e-16, ao_screen=0.0, info=None, precision=engine.PDFXC_FP64):
@@ -127,11 +134,22 @@ class DfCalcGridX:
         self.ctx = engine.XcContext(self.info, opts)
         if grid is None:
             DfGenerateGrid(mol, n_rad, n_ang).generate(self.ctx)
+        elif isinstance(grid, engine.GridOpts):
+            DfGenerateGrid(mol, opts=grid).generate(self.ctx)
         elif isinstance(grid, (str, Path)):
             self.ctx.load_grid_matrix(grid)
         else:
             self.ctx.set_grid(grid)
         self._rho = None
+
+    @classmethod
+    def from_pdfparam(cls, param, grid=None, ao_screen=0.0, precision=engine.PDFXC_FP64):
+        """DfCalcGridX(pdfParam): the molecule, orbital basis, density cut-off
+        (xc-density-threshold) and molecular grid of a pdfparam.PdfParam;
+        grid overrides the generated grid as in the constructor."""
+        mol = param.molecule
+        return cls(mol, grid=param.grid.to_opts() if grid is None else grid, info=param.orbital_info(),
+                   density_cutoff=param.density_cutoff, ao_screen=ao_screen, precision=precision)
 
     @property
     def grid(self):

@@ -2,6 +2,7 @@
 // partitioning with per-block AO screening, per-class work lists, device
 // buffers and grid generation.  Kernels live in pdfxc_kernels.hip.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -814,85 +815,52 @@ static int build_blocks(Context& c) {
 // ---------------------------------------------------------------------------
 // Grid generation pieces (DfGenerateGrid.cpp).  Lebedev rules are lists of
 // octahedral orbits with the published Lebedev-Laikov parameters (the same
-// rules as TlLebedevGrid.cpp:271-892); each orbit is expanded by applying the
+// rules as TlLebedevGrid.cpp:271-1577); each orbit is expanded by applying the
 // 48 signed permutations to its seed point.
 // ---------------------------------------------------------------------------
+#include "lebedev_rules.inc"
+
 struct Orbit {
     int kind;
     double a, b, w;
 };
 
-static const std::vector<Orbit>* lebedev_rule(int order) {
-    static const std::vector<Orbit> r6 = {{1, 0, 0, 0.1666666666666667}};
-    static const std::vector<Orbit> r38 = {{1, 0, 0, 0.9523809523809524E-2},
-                                           {3, 0, 0, 0.3214285714285714E-1},
-                                           {5, 0.4597008433809831E+0, 0, 0.2857142857142857E-1}};
-    static const std::vector<Orbit> r86 = {{1, 0, 0, 0.1154401154401154E-1},
-                                           {3, 0, 0, 0.1194390908585628E-1},
-                                           {4, 0.3696028464541502E+0, 0, 0.1111055571060340E-1},
-                                           {4, 0.6943540066026664E+0, 0, 0.1187650129453714E-1},
-                                           {5, 0.3742430390903412E+0, 0, 0.1181230374690448E-1}};
-    static const std::vector<Orbit> r194 = {
-        {1, 0, 0, 0.1782340447244611E-2},
-        {2, 0, 0, 0.5716905949977102E-2},
-        {3, 0, 0, 0.5573383178848738E-2},
-        {4, 0.6712973442695226E+0, 0, 0.5608704082587997E-2},
-        {4, 0.2892465627575439E+0, 0, 0.5158237711805383E-2},
-        {4, 0.4446933178717437E+0, 0, 0.5518771467273614E-2},
-        {4, 0.1299335447650067E+0, 0, 0.4106777028169394E-2},
-        {5, 0.3457702197611283E+0, 0, 0.5051846064614808E-2},
-        {6, 0.1590417105383530E+0, 0.8360360154824589E+0, 0.5530248916233094E-2}};
-    static const std::vector<Orbit> r302 = {
-        {1, 0, 0, 0.8545911725128148E-3},
-        {3, 0, 0, 0.3599119285025571E-2},
-        {4, 0.3515640345570105E+0, 0, 0.3449788424305883E-2},
-        {4, 0.6566329410219612E+0, 0, 0.3604822601419882E-2},
-        {4, 0.4729054132581005E+0, 0, 0.3576729661743367E-2},
-        {4, 0.9618308522614784E-1, 0, 0.2352101413689164E-2},
-        {4, 0.2219645236294178E+0, 0, 0.3108953122413675E-2},
-        {4, 0.7011766416089545E+0, 0, 0.3650045807677255E-2},
-        {5, 0.2644152887060663E+0, 0, 0.2982344963171804E-2},
-        {5, 0.5718955891878961E+0, 0, 0.3600820932216460E-2},
-        {6, 0.2510034751770465E+0, 0.8000727494073952E+0, 0.3571540554273387E-2},
-        {6, 0.1233548532583327E+0, 0.4127724083168531E+0, 0.3392312205006170E-2}};
-    static const std::vector<Orbit> r590 = {
-        {1, 0, 0, 0.3095121295306187E-3},
-        {3, 0, 0, 0.1852379698597489E-2},
-        {4, 0.7040954938227469E+0, 0, 0.1871790639277744E-2},
-        {4, 0.6807744066455243E+0, 0, 0.1858812585438317E-2},
-        {4, 0.6372546939258752E+0, 0, 0.1852028828296213E-2},
-        {4, 0.5044419707800358E+0, 0, 0.1846715956151242E-2},
-        {4, 0.4215761784010967E+0, 0, 0.1818471778162769E-2},
-        {4, 0.3317920736472123E+0, 0, 0.1749564657281154E-2},
-        {4, 0.2384736701421887E+0, 0, 0.1617210647254411E-2},
-        {4, 0.1459036449157763E+0, 0, 0.1384737234851692E-2},
-        {4, 0.6095034115507196E-1, 0, 0.9764331165051050E-3},
-        {5, 0.6116843442009876E+0, 0, 0.1857161196774078E-2},
-        {5, 0.3964755348199858E+0, 0, 0.1705153996395864E-2},
-        {5, 0.1724782009907724E+0, 0, 0.1300321685886048E-2},
-        {6, 0.5610263808622060E+0, 0.3518280927733519E+0, 0.1842866472905286E-2},
-        {6, 0.4742392842551980E+0, 0.2634716655937950E+0, 0.1802658934377451E-2},
-        {6, 0.5984126497885380E+0, 0.1816640840360209E+0, 0.1849830560443660E-2},
-        {6, 0.3791035407695563E+0, 0.1720795225656878E+0, 0.1713904507106709E-2},
-        {6, 0.2778673190586244E+0, 0.8213021581932511E-1, 0.1555213603396808E-2},
-        {6, 0.5033564271075117E+0, 0.8999205842074875E-1, 0.1802239128008525E-2}};
+// supported orders, ascending (TlLebedevGrid.cpp:4-6)
+static const int kLebOrders[] = {6,   14,  26,  38,  50,  74,  86,  110,  146,  170,  194,
+                                 230, 266, 302, 350, 434, 590, 770, 974, 1202, 1454, 1730};
+
+static bool lebedev_rule(int order, std::vector<Orbit>& rule) {
+    const double(*t)[4] = nullptr;
+    int no = 0;
+#define LEB_CASE(N, T)                             \
+    case N:                                        \
+        t = T;                                     \
+        no = (int)(sizeof(T) / sizeof(T[0]));      \
+        break;
     switch (order) {
-        case 6: return &r6;
-        case 38: return &r38;
-        case 86: return &r86;
-        case 194: return &r194;
-        case 302: return &r302;
-        case 590: return &r590;
-        default: return nullptr;
+        LEBEDEV_RULE_LIST(LEB_CASE)
+        default: return false;
     }
+#undef LEB_CASE
+    rule.clear();
+    for (int k = 0; k < no; ++k) rule.push_back({(int)t[k][0], t[k][1], t[k][2], t[k][3]});
+    return true;
+}
+
+// the largest supported order <= n, or 6 (getNumOfPrunedAnglarPoints, DfGenerateGrid.cpp:1677-1684)
+static int lebedev_floor(int n) {
+    int best = kLebOrders[0];
+    for (int o : kLebOrders)
+        if (o <= n) best = o;
+    return best;
 }
 
 static bool lebedev_points(int order, std::vector<double>& xyz, std::vector<double>& w) {
-    const std::vector<Orbit>* rule = lebedev_rule(order);
-    if (!rule) return false;
+    std::vector<Orbit> rule;
+    if (!lebedev_rule(order, rule)) return false;
     xyz.clear();
     w.clear();
-    for (const Orbit& o : *rule) {
+    for (const Orbit& o : rule) {
         double s[3];
         switch (o.kind) {
             case 1: s[0] = 1.0; s[1] = s[2] = 0.0; break;
@@ -941,6 +909,65 @@ static double bragg_slater(int Z) {
     static const double t[] = {0.00, 0.25, 0.50, 1.45, 1.05, 0.85, 0.70, 0.65, 0.60, 0.50,
                                0.50, 1.80, 1.50, 1.25, 1.10, 1.00, 1.00, 1.00, 1.00};
     return (Z >= 1 && Z < (int)(sizeof(t) / sizeof(t[0]))) ? t[Z] : 0.0;
+}
+
+// Gauss-Chebyshev (second kind) radial abscissa and weight with the Becke,
+// TA or KK mapping (DfGenerateGrid.cpp:1612-1655); the weight carries 4 pi r^2.
+static void radial_gc(double R, int Nr, int i, int mapping, double* r, double* w) {
+    static const double inv_ln2 = 1.0 / std::log(2.0);
+    const double Nr1 = Nr + 1;
+    const double t = (double)i * kPi / Nr1;
+    const double sn = std::sin(t), ss = sn * sn, cs = std::cos(t);
+    const double x = (Nr1 - 2.0 * (double)i) / Nr1 + (2.0 / kPi) * (1.0 + (2.0 / 3.0) * ss) * cs * sn;
+    const double inv_x1 = 1.0 / (x - 1.0);
+    double ri, jac;
+    if (mapping == PDFXC_GC_BECKE) {
+        ri = R * (1.0 + x) / (1.0 - x);
+        jac = R * ((x + 1.0) * inv_x1 * inv_x1 - inv_x1);
+    } else if (mapping == PDFXC_GC_TA) {
+        ri = inv_ln2 * std::log(2.0 * (-inv_x1)) * R;
+        jac = R * inv_ln2 / (1.0 - x);
+    } else {
+        ri = inv_ln2 * std::log(2.0 * (-inv_x1));
+        jac = inv_ln2 / (1.0 - x);
+    }
+    *r = ri;
+    *w = (16.0 / (3.0 * Nr1)) * ss * ss * jac * ri * ri * 4.0 * kPi;
+}
+
+// SG-1 radial scales (setCellPara's SG-1 list, DfGenerateGrid.cpp:409-429)
+static double sg1_radius(int Z) {
+    static const double t[] = {0.0,    1.0000, 0.5882, 3.0769, 2.0513, 1.5385, 1.2308,
+                               1.0256, 0.8791, 0.7692, 0.6838, 4.0909, 3.1579, 2.5714,
+                               2.1687, 1.8750, 1.6514, 1.4754, 1.3333};
+    return (Z >= 1 && Z < (int)(sizeof(t) / sizeof(t[0]))) ? t[Z] : 0.0;
+}
+
+// Lebedev order of the radial shell at r (DfGenerateGrid.cpp:1658-1713, the
+// SG-1 regions :961-983): none -> n_ang floored to a supported order; MHL
+// (K = 5, Bragg-Slater radius) -> min(int(5 r / R_B * n_ang), n_ang) floored;
+// SG-1 -> 6, 38, 86, 194, 86 by r / R against the element row's alphas.
+static int pruned_order(int pruning, int Z, double r, double R, int n_ang) {
+    if (pruning == PDFXC_PRUNE_SG1) {
+        double al[4] = {0.0, 0.0, 0.0, 10000.0};
+        if (Z == 1 || Z == 2) {
+            al[0] = 0.25, al[1] = 0.5, al[2] = 1.0, al[3] = 4.5;
+        } else if (Z >= 3 && Z <= 10) {
+            al[0] = 0.1667, al[1] = 0.5, al[2] = 0.9, al[3] = 3.5;
+        } else if (Z >= 11 && Z <= 18) {
+            al[0] = 0.1, al[1] = 0.4, al[2] = 0.8, al[3] = 2.5;
+        }
+        const double q = r * (1.0 / R);
+        if (q < al[1]) return (q < al[0]) ? 6 : 38;
+        if (q < al[2]) return 86;
+        return (q < al[3]) ? 194 : 86;
+    }
+    int n = n_ang;
+    if (pruning == PDFXC_PRUNE_MHL) {
+        const double v = (5.0 * r / bragg_slater(Z)) * (double)n_ang;
+        n = (v < (double)n_ang) ? (int)v : n_ang;
+    }
+    return lebedev_floor(n);
 }
 
 // Standard orientation (DfGenerateGrid.cpp:1739-1809): eigenvectors of the
@@ -1028,6 +1055,8 @@ int pdfxc_set_grid(pdfxc_ctx* ctx, int64_t n_pts, const double* x, const double*
         return arg_error("pdfxc_set_grid: more than 2^31 points");
     c->n_pts = n_pts;
     c->gen_atom.clear();
+    c->grid_partition = PDFXC_PARTITION_BECKE;
+    c->grid_size_adjust = 1;
     c->gx.assign(x, x + n_pts);
     c->gy.assign(y, y + n_pts);
     c->gz.assign(z, z + n_pts);
@@ -1053,20 +1082,71 @@ int pdfxc_get_grid(pdfxc_ctx* ctx, double* x, double* y, double* z, double* w) {
     return PDFXC_OK;
 }
 
-int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz, int n_rad,
-                        int n_ang, double weight_cutoff, int64_t* n_out) {
-    Context* c = reinterpret_cast<Context*>(ctx);
-    if (!c) return arg_error("pdfxc_generate_grid: null context");
-    if (n_atoms <= 0 || !Z || !xyz || n_rad <= 0) return arg_error("pdfxc_generate_grid: bad arguments");
-    std::vector<double> leb, lebw;
-    if (!lebedev_points(n_ang, leb, lebw)) {
-        set_error("pdfxc_generate_grid: unsupported Lebedev order " + std::to_string(n_ang) +
-                  " (6, 38, 86, 194, 302, 590)");
+int pdfxc_grid_opts_preset(const char* grid_type, pdfxc_grid_opts* out) {
+    if (!grid_type || !out) return arg_error("pdfxc_grid_opts_preset: bad arguments");
+    std::string t(grid_type);
+    for (char& ch : t) ch = (ch == '_') ? '-' : (char)std::toupper((unsigned char)ch);
+    pdfxc_grid_opts o;
+    o.n_rad = 75;
+    o.n_ang = 302;
+    o.radial = PDFXC_RADIAL_EM;
+    o.gc_mapping = PDFXC_GC_TA;
+    o.pruning = PDFXC_PRUNE_MHL;
+    o.partitioning = PDFXC_PARTITION_SS;
+    o.size_adjust = 1;
+    o.radii = PDFXC_RADII_BRAGG;
+    o.weight_cutoff = 1.0e-16;
+    if (t == "COARSE") {
+        o.n_rad = 35, o.n_ang = 110;
+    } else if (t == "MEDIUM") {
+        o.n_rad = 32, o.n_ang = 146;
+    } else if (t == "MEDIUM-FINE") {
+        o.n_rad = 64, o.n_ang = 146;
+    } else if (t == "FINE" || t == "USER") {
+        o.n_rad = 75, o.n_ang = 302;
+    } else if (t == "ULTRA-FINE") {
+        o.n_rad = 99, o.n_ang = 590;
+    } else if (t == "SG-1" || t == "SG1") {
+        o.n_rad = 50, o.n_ang = 194;
+        o.pruning = PDFXC_PRUNE_SG1;
+        o.radii = PDFXC_RADII_SG1;
+        o.size_adjust = 0;
+    } else {
+        set_error(std::string("pdfxc_grid_opts_preset: unsupported grid type: ") + grid_type);
         return PDFXC_ERR_UNSUPPORTED;
     }
+    *out = o;
+    return PDFXC_OK;
+}
+
+int pdfxc_generate_grid_ex(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
+                           const pdfxc_grid_opts* opts, int64_t* n_out) {
+    Context* c = reinterpret_cast<Context*>(ctx);
+    if (!c) return arg_error("pdfxc_generate_grid_ex: null context");
+    if (n_atoms <= 0 || !Z || !xyz || !opts) return arg_error("pdfxc_generate_grid_ex: bad arguments");
+    const pdfxc_grid_opts o = *opts;
+    if (o.n_rad <= 0 || o.n_ang <= 0 || (o.radial != PDFXC_RADIAL_EM && o.radial != PDFXC_RADIAL_GC) ||
+        o.gc_mapping < PDFXC_GC_BECKE || o.gc_mapping > PDFXC_GC_KK || o.pruning < PDFXC_PRUNE_NONE ||
+        o.pruning > PDFXC_PRUNE_SG1 || (o.partitioning != PDFXC_PARTITION_BECKE && o.partitioning != PDFXC_PARTITION_SS) ||
+        (o.radii != PDFXC_RADII_BRAGG && o.radii != PDFXC_RADII_SG1))
+        return arg_error("pdfxc_generate_grid_ex: bad options");
     for (int a = 0; a < n_atoms; ++a)
         if (Z[a] < 0 || (Z[a] > 0 && bragg_slater(Z[a]) == 0.0)) {
-            set_error("pdfxc_generate_grid: no Bragg-Slater radius for Z = " + std::to_string(Z[a]));
+            set_error("pdfxc_generate_grid_ex: no Bragg-Slater radius for Z = " + std::to_string(Z[a]));
+            return PDFXC_ERR_UNSUPPORTED;
+        }
+    // Lebedev rules a shell can take: every supported order up to the largest
+    const int top = (o.pruning == PDFXC_PRUNE_SG1) ? 194 : lebedev_floor(o.n_ang);
+    std::vector<std::vector<double>> leb(sizeof(kLebOrders) / sizeof(kLebOrders[0])),
+        lebw(sizeof(kLebOrders) / sizeof(kLebOrders[0]));
+    auto rule_index = [](int order) {
+        int q = 0;
+        while (kLebOrders[q] != order) ++q;
+        return q;
+    };
+    for (int q = 0; q < (int)leb.size() && kLebOrders[q] <= top; ++q)
+        if (!lebedev_points(kLebOrders[q], leb[q], lebw[q])) {
+            set_error("pdfxc_generate_grid_ex: Lebedev rule " + std::to_string(kLebOrders[q]) + " is inconsistent");
             return PDFXC_ERR_UNSUPPORTED;
         }
     double O[9];
@@ -1076,25 +1156,31 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
     std::vector<int> atom;
     for (int a = 0; a < n_atoms; ++a) {
         if (Z[a] <= 0) continue;
-        const double R = bragg_slater(Z[a]);
-        for (int i = 1; i <= n_rad; ++i) {
+        const double R = (o.radii == PDFXC_RADII_SG1) ? sg1_radius(Z[a]) : bragg_slater(Z[a]);
+        for (int i = 1; i <= o.n_rad; ++i) {
             double ri, wr;
-            radial_em(R, n_rad, i, &ri, &wr);
+            if (o.radial == PDFXC_RADIAL_GC)
+                radial_gc(R, o.n_rad, i, o.gc_mapping, &ri, &wr);
+            else
+                radial_em(R, o.n_rad, i, &ri, &wr);
             if (ri > 30.0) continue;
-            for (int k = 0; k < n_ang; ++k) {
+            const int q = rule_index(pruned_order(o.pruning, Z[a], ri, R, o.n_ang));
+            const std::vector<double>& L = leb[q];
+            const int nq = (int)lebw[q].size();
+            for (int k = 0; k < nq; ++k) {
                 for (int d = 0; d < 3; ++d) {
-                    const double v = O[3 * d] * leb[3 * k] + O[3 * d + 1] * leb[3 * k + 1] +
-                                     O[3 * d + 2] * leb[3 * k + 2];
+                    const double v = O[3 * d] * L[3 * k] + O[3 * d + 1] * L[3 * k + 1] + O[3 * d + 2] * L[3 * k + 2];
                     pts.push_back(v * ri + xyz[3 * a + d]);
                 }
-                w.push_back(lebw[k] * wr);
+                w.push_back(lebw[q][k] * wr);
                 atom.push_back(a);
             }
         }
     }
     const int64_t n = (int64_t)w.size();
-    // atom table for the Becke kernel: (x, y, z, Z), 1/R_mn, size adjustments a_mn
-    // (getCovalentRadiiForBecke: H 0.35, else Bragg-Slater; |a| <= 1/2)
+    // atom table for the partition kernel: (x, y, z, Z), 1/R_mn, Becke size
+    // adjustments a_mn (getCovalentRadiiForBecke: H 0.35, else Bragg-Slater,
+    // a dummy 0; |a| <= 1/2; zero when the adjustments are off)
     std::vector<double> at(4 * (size_t)n_atoms + 2 * (size_t)n_atoms * n_atoms, 0.0);
     for (int m = 0; m < n_atoms; ++m) {
         for (int d = 0; d < 3; ++d) at[4 * m + d] = xyz[3 * m + d];
@@ -1109,14 +1195,16 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
             const double dx = xyz[3 * m] - xyz[3 * k], dy = xyz[3 * m + 1] - xyz[3 * k + 1],
                          dz = xyz[3 * m + 2] - xyz[3 * k + 2];
             const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
-            if (dist < 1e-10 && Z[m] > 0 && Z[k] > 0)
-                return arg_error("pdfxc_generate_grid: atoms " + std::to_string(m) + " and " +
+            if (dist < 1e-10)
+                return arg_error("pdfxc_generate_grid_ex: atoms " + std::to_string(m) + " and " +
                                  std::to_string(k) + " coincide");
             invd[(size_t)m * n_atoms + k] = 1.0 / dist;
-            if (Z[m] > 0 && Z[k] > 0) {
-                const double Rm = brad(Z[m]), Rk = brad(Z[k]);
+            if (o.size_adjust && Z[k] > 0) {
+                // a dummy m has R = 0: au = -1, a = -1/0 clamps to -1/2 as in the reference
+                const double Rm = Z[m] > 0 ? brad(Z[m]) : 0.0, Rk = brad(Z[k]);
                 const double au = (Rm - Rk) / (Rm + Rk);
-                double aa = au / (au * au - 1.0);
+                const double den = au * au - 1.0;
+                double aa = (den == 0.0) ? (au > 0.0 ? 0.5 : -0.5) : au / den;
                 aa = std::min(0.5, std::max(-0.5, aa));
                 amat[(size_t)m * n_atoms + k] = aa;
             }
@@ -1124,11 +1212,11 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
     double *d_at = nullptr, *d_p = nullptr, *d_w = nullptr;
     int* d_a = nullptr;
     int rc = dalloc(&d_at, at.size());
-    if (rc == PDFXC_OK) rc = dalloc(&d_p, pts.size());
-    if (rc == PDFXC_OK) rc = dalloc(&d_w, w.size());
-    if (rc == PDFXC_OK) rc = dalloc(&d_a, atom.size());
+    if (rc == PDFXC_OK && n > 0) rc = dalloc(&d_p, pts.size());
+    if (rc == PDFXC_OK && n > 0) rc = dalloc(&d_w, w.size());
+    if (rc == PDFXC_OK && n > 0) rc = dalloc(&d_a, atom.size());
     hipError_t e = hipSuccess;
-    if (rc == PDFXC_OK) {
+    if (rc == PDFXC_OK && n > 0) {
         e = hipMemcpyAsync(d_at, at.data(), at.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_p, pts.data(), pts.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -1136,13 +1224,13 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
             e = hipMemcpyAsync(d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(d_a, atom.data(), atom.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_generate_grid: upload");
-    }
-    if (rc == PDFXC_OK) rc = launch_becke(d_at, n_atoms, n, d_p, d_a, d_w, c->stream);
-    if (rc == PDFXC_OK) {
-        e = hipMemcpyAsync(w.data(), d_w, w.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_generate_grid: Becke weights");
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_generate_grid_ex: upload");
+        if (rc == PDFXC_OK) rc = launch_becke(d_at, n_atoms, n, d_p, d_a, d_w, o.partitioning, c->stream);
+        if (rc == PDFXC_OK) {
+            e = hipMemcpyAsync(w.data(), d_w, w.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "pdfxc_generate_grid_ex: partition weights");
+        }
     }
     dfree(d_at);
     dfree(d_p);
@@ -1153,17 +1241,44 @@ int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double*
     std::vector<double> X, Y, Zc, W;
     std::vector<int> kept_atom;
     for (int64_t i = 0; i < n; ++i)
-        if (std::fabs(w[i]) > weight_cutoff) {
+        if (std::fabs(w[i]) > o.weight_cutoff) {
             X.push_back(pts[3 * i]);
             Y.push_back(pts[3 * i + 1]);
             Zc.push_back(pts[3 * i + 2]);
             W.push_back(w[i]);
             kept_atom.push_back(atom[i]);
         }
+    if (W.empty()) return arg_error("pdfxc_generate_grid_ex: no grid point survives the weight screen");
     if (n_out) *n_out = (int64_t)W.size();
     rc = pdfxc_set_grid(ctx, (int64_t)W.size(), X.data(), Y.data(), Zc.data(), W.data());
-    if (rc == PDFXC_OK) c->gen_atom = std::move(kept_atom);
+    if (rc == PDFXC_OK) {
+        c->gen_atom = std::move(kept_atom);
+        c->grid_partition = o.partitioning;
+        c->grid_size_adjust = o.size_adjust ? 1 : 0;
+    }
     return rc;
+}
+
+int pdfxc_generate_grid(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz, int n_rad,
+                        int n_ang, double weight_cutoff, int64_t* n_out) {
+    if (!ctx) return arg_error("pdfxc_generate_grid: null context");
+    if (lebedev_floor(n_ang) != n_ang) {
+        set_error("pdfxc_generate_grid: unsupported Lebedev order " + std::to_string(n_ang) +
+                  " (6, 14, 26, 38, 50, 74, 86, 110, 146, 170, 194, 230, 266, 302, 350, 434, 590, 770, 974, "
+                  "1202, 1454, 1730)");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    pdfxc_grid_opts o;
+    o.n_rad = n_rad;
+    o.n_ang = n_ang;
+    o.radial = PDFXC_RADIAL_EM;
+    o.gc_mapping = PDFXC_GC_TA;
+    o.pruning = PDFXC_PRUNE_NONE;
+    o.partitioning = PDFXC_PARTITION_BECKE;
+    o.size_adjust = 1;
+    o.radii = PDFXC_RADII_BRAGG;
+    o.weight_cutoff = weight_cutoff;
+    return pdfxc_generate_grid_ex(ctx, n_atoms, Z, xyz, &o, n_out);
 }
 
 // Functional ids are the XC_TYPE values of DfXCFunctional.h:44; an unknown
@@ -1456,6 +1571,15 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
     }
     if (!point_atom && c->gen_atom.size() != (size_t)c->n_pts)
         return arg_error("pdfxc_gradient_rks: the grid was not made by pdfxc_generate_grid; pass point_atom");
+    if (c->grid_partition != PDFXC_PARTITION_BECKE) {
+        set_error("pdfxc_gradient_rks: the weight derivative is implemented for Becke partitioning only");
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    for (int a = 0; a < n_atoms; ++a)
+        if (Z[a] <= 0) {
+            set_error("pdfxc_gradient_rks: dummy atoms (Z = 0) are not supported");
+            return PDFXC_ERR_UNSUPPORTED;
+        }
     const int N = c->n_aos;
     const int na = n_atoms;
     if (functional == kHF) {
@@ -1514,7 +1638,8 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
                     }
                     const double Rm = brad(Z[m]), Rk = brad(Z[k]);
                     const double au = (Rm - Rk) / (Rm + Rk);
-                    amat[(size_t)m * na + k] = std::min(0.5, std::max(-0.5, au / (au * au - 1.0)));
+                    if (c->grid_size_adjust)
+                        amat[(size_t)m * na + k] = std::min(0.5, std::max(-0.5, au / (au * au - 1.0)));
                 }
             }
     }

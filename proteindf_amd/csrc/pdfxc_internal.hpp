@@ -86,6 +86,8 @@ struct Context {
     int64_t n_pts = 0;
     std::vector<double> gx, gy, gz, gw;
     std::vector<int> gen_atom;  // atom of each point when pdfxc_generate_grid made the grid
+    int grid_partition = 0;     // partition weights of a generated grid (PDFXC_PARTITION_*)
+    int grid_size_adjust = 1;   // ... and its Becke size adjustments
 
     // blocks
     std::vector<BlockDesc> blocks;     // launch order (descending cost)
@@ -184,7 +186,7 @@ int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d
 int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, double* d_out,
                            hipStream_t s);
 int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,
-                 const int* d_atom_of_pt, double* d_w, hipStream_t s);
+                 const int* d_atom_of_pt, double* d_w, int partition, hipStream_t s);
 int launch_mfma_selftest(double* d_out, hipStream_t s);
 // split precision (pdfxc_split.hip)
 int split_points_per_workgroup(int cls);

@@ -679,10 +679,15 @@ __global__ __launch_bounds__(256) void k_eval_aos_points(int64_t n, const double
 // n_atoms^2 inverse distances and size adjustments a_mn.  One thread per
 // point; the point's atom distances are kept in LDS.
 // ---------------------------------------------------------------------------
+// Partition weight of each point's atom (calcMultiCenterWeight_Becke /
+// Ps_uij, DfGenerateGrid.cpp:1811-1876, :2043-2086; calcMultiCenterWeight_SS,
+// :1878-1956).  atoms: (x, y, z, Z) per atom | 1/R_mk | a_mk (the Becke size
+// adjustment, zero when off).  Every atom m, a dummy (Z = 0) included, has a
+// cell; the Becke product skips dummy neighbours k, the SS product does not.
 __global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms, int na,
                                               int64_t n, const double* __restrict__ pts,
                                               const int* __restrict__ atom_of_pt,
-                                              double* __restrict__ w) {
+                                              double* __restrict__ w, int ss) {
     extern __shared__ double s_r[];  // blockDim.x x na
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -697,17 +702,30 @@ __global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms, 
     const int A = atom_of_pt[i];
     double total = 0.0, mine = 0.0;
     for (int m = 0; m < na; ++m) {
-        if (atoms[4 * m + 3] <= 0.0) continue;  // dummy atom: no cell of its own
         double ps = 1.0;
         for (int k = 0; k < na && ps != 0.0; ++k) {
-            if (k == m || atoms[4 * k + 3] <= 0.0) continue;
+            if (k == m) continue;
             double u = (r[m] - r[k]) * inv_d[(int64_t)m * na + k];
-            const double a = amat[(int64_t)m * na + k];
-            u = u + a * (1.0 - u * u);
-            double f = 0.5 * u * (3.0 - u * u);
-            f = 0.5 * f * (3.0 - f * f);
-            f = 0.5 * f * (3.0 - f * f);
-            ps *= 0.5 * (1.0 - f);
+            double g;
+            if (ss) {
+                // Stratmann-Scuseria g(u), a = 0.64
+                if (u < -0.64) {
+                    g = -1.0;
+                } else if (u > 0.64) {
+                    g = 1.0;
+                } else {
+                    const double q = u / 0.64, q2 = q * q, q3 = q2 * q, q5 = q2 * q3;
+                    g = (1.0 / 16.0) * (35.0 * q - 35.0 * q3 + 21.0 * q5 - 5.0 * q5 * q2);
+                }
+            } else {
+                if (atoms[4 * k + 3] <= 0.0) continue;  // Becke: dummy neighbours do not cut
+                const double a = amat[(int64_t)m * na + k];
+                u = u + a * (1.0 - u * u);
+                g = 0.5 * u * (3.0 - u * u);
+                g = 0.5 * g * (3.0 - g * g);
+                g = 0.5 * g * (3.0 - g * g);
+            }
+            ps *= 0.5 * (1.0 - g);
         }
         if (m == A) mine = ps;
         total += ps;
@@ -873,7 +891,7 @@ int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, dou
 }
 
 int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,
-                 const int* d_atom_of_pt, double* d_w, hipStream_t s) {
+                 const int* d_atom_of_pt, double* d_w, int partition, hipStream_t s) {
     if (n <= 0) return PDFXC_OK;
     // point-to-atom distances live in LDS: keep them within 64 KiB per workgroup
     int threads = 64;
@@ -885,7 +903,7 @@ int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_
     const int64_t g = (n + threads - 1) / threads;
     const size_t lds = sizeof(double) * (size_t)threads * (size_t)n_atoms;
     hipLaunchKernelGGL(k_becke, dim3((unsigned)g), dim3(threads), lds, s, d_atoms, n_atoms, n,
-                       d_pts, d_atom_of_pt, d_w);
+                       d_pts, d_atom_of_pt, d_w, partition == 1 ? 1 : 0);
     return check_launch("k_becke");
 }
 

@@ -42,7 +42,7 @@ EXPORTED = [
     "pdfxc_build_uks_device", "pdfxc_build_uks_update_device", "pdfxc_eval_functional_uks",
     "pdfxc_density_at_points", "pdfxc_batch_create", "pdfxc_batch_destroy", "pdfxc_batch_offsets",
     "pdfxc_batch_build_rks_device", "pdfxc_batch_build_rks", "pdfxc_batch_stats", "pdfxc_set_tuning",
-    "pdfxc_autotune",
+    "pdfxc_autotune", "pdfxc_grid_opts_preset", "pdfxc_generate_grid_ex",
 ]
 # matrix file types (TlMatrixObject::MatrixType)
 PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
@@ -66,6 +66,34 @@ class Tuning(ctypes.Structure):
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# molecular grid options (pdfxc_grid_opts)
+PDFXC_RADIAL_EM, PDFXC_RADIAL_GC = 0, 1
+PDFXC_GC_BECKE, PDFXC_GC_TA, PDFXC_GC_KK = 0, 1, 2
+PDFXC_PRUNE_NONE, PDFXC_PRUNE_MHL, PDFXC_PRUNE_SG1 = 0, 1, 2
+PDFXC_PARTITION_BECKE, PDFXC_PARTITION_SS = 0, 1
+PDFXC_RADII_BRAGG, PDFXC_RADII_SG1 = 0, 1
+
+
+class GridOpts(ctypes.Structure):
+    _fields_ = [("n_rad", ctypes.c_int), ("n_ang", ctypes.c_int), ("radial", ctypes.c_int),
+                ("gc_mapping", ctypes.c_int), ("pruning", ctypes.c_int), ("partitioning", ctypes.c_int),
+                ("size_adjust", ctypes.c_int), ("radii", ctypes.c_int), ("weight_cutoff", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+def grid_opts_preset(grid_type="sg-1", **overrides):
+    """pdfxc_grid_opts_preset for a grid type, then field overrides."""
+    o = GridOpts()
+    check(lib().pdfxc_grid_opts_preset(str(grid_type).encode(), ctypes.byref(o)))
+    for k, v in overrides.items():
+        if k not in o.as_dict():
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
 
 
 class Stats(ctypes.Structure):
@@ -93,6 +121,8 @@ def lib():
         L.pdfxc_ctx_destroy.argtypes = [P]
         L.pdfxc_set_grid.argtypes = [P, i64, P, P, P, P]
         L.pdfxc_generate_grid.argtypes = [P, i32, P, P, i32, i32, d, P]
+        L.pdfxc_grid_opts_preset.argtypes = [ctypes.c_char_p, P]
+        L.pdfxc_generate_grid_ex.argtypes = [P, i32, P, P, P, P]
         L.pdfxc_get_grid.argtypes = [P, P, P, P, P]
         L.pdfxc_build_rks.argtypes = [P, i32, P, P, P, P]
         L.pdfxc_build_rks_device.argtypes = [P, i32, P, P, P]
@@ -276,6 +306,16 @@ class XcContext:
         n = ctypes.c_int64()
         check(lib().pdfxc_generate_grid(self.h, len(Z), _p(Z), _p(xyz), n_rad, n_ang,
                                         weight_cutoff, ctypes.byref(n)))
+        self.n_pts = n.value
+        return n.value
+
+    def generate_grid_ex(self, Z, xyz, opts):
+        """The molecular grid for a GridOpts (see grid_opts_preset)."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = ctypes.c_int64()
+        check(lib().pdfxc_generate_grid_ex(self.h, len(Z), _p(Z), _p(xyz), ctypes.byref(opts),
+                                           ctypes.byref(n)))
         self.n_pts = n.value
         return n.value
 

@@ -20,7 +20,10 @@ from dataclasses import dataclass
 import numpy as np
 
 BOHR_ANGSTROM = 0.52917721067
-Z_OF = {"X": 0, "H": 1, "He": 2, "C": 6, "N": 7, "O": 8}
+ELEMENTS = ("X", "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne",
+            "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar")
+# atomic numbers up to Ar, the elements the grid's Bragg-Slater table covers; "X" is a dummy atom
+Z_OF = {s: z for z, s in enumerate(ELEMENTS)}
 
 
 @dataclass

@@ -33,20 +33,27 @@ def test_lebedev_rules_are_exact_on_monomials():
     def dfact(n):
         return prod(range(n, 0, -2)) if n > 0 else 1
 
-    for order, degree in ((6, 3), (38, 9), (86, 15), (194, 23), (302, 29), (590, 41)):
+    # every order TlLebedevGrid supports, with the polynomial degree it integrates exactly
+    for order, degree in ((6, 3), (14, 5), (26, 7), (38, 9), (50, 11), (74, 13), (86, 15), (110, 17),
+                          (146, 19), (170, 21), (194, 23), (230, 25), (266, 27), (302, 29), (350, 31),
+                          (434, 35), (590, 41), (770, 47), (974, 53), (1202, 59), (1454, 65), (1730, 71)):
         xyz, w = oracle.lebedev(order)
+        assert len(w) == order
         assert abs(w.sum() - 1.0) < 1e-14
         np.testing.assert_allclose(np.linalg.norm(xyz, axis=1), 1.0, atol=1e-15)
-        for a in range(0, degree + 1):
-            for b in range(0, degree + 1 - a):
-                for c in range(0, degree + 1 - a - b):
-                    q = (w * xyz[:, 0] ** a * xyz[:, 1] ** b * xyz[:, 2] ** c).sum()
-                    if a % 2 or b % 2 or c % 2:
-                        exact = 0.0
-                    else:
-                        # mean of x^a y^b z^c over the unit sphere
-                        exact = dfact(a - 1) * dfact(b - 1) * dfact(c - 1) / dfact(a + b + c + 1)
-                    assert abs(q - exact) < 2e-14, (order, a, b, c, q, exact)
+        k = np.arange(degree + 1)
+        X, Y, Zp = (xyz[:, d][None, :] ** k[:, None] for d in range(3))
+        # q[a, b, c] = sum_p w_p x^a y^b z^c
+        q = np.einsum("abp,cp->abc", X[:, None, :] * Y[None, :, :], Zp * w[None, :])
+        a, b, c = np.meshgrid(k, k, k, indexing="ij")
+        inside = a + b + c <= degree
+        even = (a % 2 == 0) & (b % 2 == 0) & (c % 2 == 0)
+        # mean of x^a y^b z^c over the unit sphere
+        exact = np.zeros_like(q)
+        for i, j, l in zip(*np.nonzero(inside & even)):
+            exact[i, j, l] = dfact(i - 1) * dfact(j - 1) * dfact(l - 1) / dfact(i + j + l + 1)
+        err = np.abs(q - exact)[inside]
+        assert err.max() < 2e-14, (order, err.max())
 
 
 def test_em_radial_rule_integrates_gaussians():
