@@ -5,9 +5,9 @@ spherical d (638 AOs), 75 radial x 302 Lebedev Becke grid (~1.48M points),
 128-point grid blocks with per-block AO screening at 1e-12.  Each run times
 the FP64 build and the split-precision builds (bf16 slices on MFMA, FP32
 accumulation, FP64 elsewhere) on the same grid and P; value is the mode named
-by --precision (default: split-BF16 hi+mid+lo), and "modes" lists every mode's
-time, throughput and, for the split modes, |dE_xc| and max|dK_xc| against the
-FP64 build.
+by --precision (default bf16x3k: FP64 rho pass, so E_xc is the FP64 build's,
+and a hi+mid+lo K pass), and "modes" lists every mode's time, throughput and,
+for the split modes, |dE_xc| and max|dK_xc| against the FP64 build.
 P is the deterministic synthetic guess of proteindf_amd.guess (the reference's
 own SCF is outside the XC path).  Inputs (grid, blocks, P) are resident in HBM
 before the timed region; a step is one pdfxc_build_rks_device call.
@@ -19,7 +19,7 @@ baseline runs the C oracle (the reference algorithm: all AOs at every point)
 on a bounded sample of grid points and scales to the whole grid.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload gly10|h2o8|gly20|gly30|h2o_batch]
-                    [--precision fp64|bf16x2|bf16x3] [--modes fp64,bf16x2,bf16x3]
+                    [--precision fp64|bf16x2|bf16x3|bf16x3k] [--modes fp64,bf16x2,bf16x3,bf16x3k]
 """
 import argparse
 import ctypes
@@ -108,8 +108,18 @@ def cpu_baseline(info, grid, PA, flops_total, target_s=12.0):
                       f"scaled to a {t_full:.0f} s whole-grid build"}
 
 
-MODES = {"fp64": engine.PDFXC_FP64, "bf16x2": engine.PDFXC_SPLIT_BF16, "bf16x3": engine.PDFXC_SPLIT_BF16X3}
-PEAK_TFLOPS = {"fp64": 78.6, "bf16x2": 2500.0, "bf16x3": 2500.0}
+MODES = {"fp64": engine.PDFXC_FP64, "bf16x2": engine.PDFXC_SPLIT_BF16, "bf16x3": engine.PDFXC_SPLIT_BF16X3,
+         "bf16x3k": engine.PDFXC_SPLIT_K_BF16X3}
+# (rho-pass kernel, peak TFLOP/s), (K-pass kernel, peak) per mode: dense FP64 and bf16 MFMA peaks
+KERNELS = {"fp64": (("k_rho", 78.6), ("k_kmat", 78.6)),
+           "bf16x2": (("k_rho_split", 2500.0), ("k_kmat_split", 2500.0)),
+           "bf16x3": (("k_rho_split", 2500.0), ("k_kmat_split", 2500.0)),
+           "bf16x3k": (("k_rho", 78.6), ("k_kmat_split", 2500.0))}
+LABELS = {"fp64": "FP64", "bf16x2": "split-BF16 hi+lo", "bf16x3": "split-BF16 hi+mid+lo",
+          "bf16x3k": "FP64 rho pass + split-BF16 hi+mid+lo K pass"}
+DTYPES = {"fp64": "f64", "bf16x2": "bf16 slices, f32 accumulate, f64 AO/functional/sums",
+          "bf16x3": "bf16 slices, f32 accumulate, f64 AO/functional/sums",
+          "bf16x3k": "f64 rho pass; K pass bf16 slices, f32 accumulate; f64 AO/functional/sums"}
 E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 build
 
 
@@ -233,7 +243,7 @@ def bench_batch(args):
     if not np.array_equal(Eb, Es):
         raise RuntimeError(f"batched and single energies differ: max {np.abs(Eb - Es).max()}")
     lat_ms = np.array(lat) * 1e3
-    label = {"fp64": "FP64", "bf16x2": "split-BF16 hi+lo", "bf16x3": "split-BF16 hi+mid+lo"}[args.precision]
+    label = LABELS[args.precision]
     line = {
         "metric": f"batched K_xc+E_xc build points/s, 64 water clusters, {label} GPU",
         "value": n_pts / t_batch,
@@ -245,7 +255,7 @@ def bench_batch(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "bf16 slices, f32 accumulate, f64 AO/functional/sums",
+        "dtype": DTYPES[args.precision],
         "data": "synthetic (64 (H2O)_n clusters, n ~ U{4..64}, seed 3; deterministic density guess)",
         "config": {"workload": "64 (H2O)_n clusters, 6-31G(d), B3LYP, 75x302 Becke, one graph-captured batch",
                    "precision": args.precision, "n_points": n_pts, "n_blocks": int(st.n_blocks),
@@ -269,9 +279,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="gly10", choices=sorted(WORKLOADS) + ["h2o_batch"])
-    ap.add_argument("--precision", default="bf16x3", choices=sorted(MODES),
+    ap.add_argument("--precision", default="bf16x3k", choices=sorted(MODES),
                     help="mode whose throughput is the reported value (the others are measured beside it)")
-    ap.add_argument("--modes", default="fp64,bf16x2,bf16x3", help="comma list of modes to measure")
+    ap.add_argument("--modes", default="fp64,bf16x2,bf16x3,bf16x3k", help="comma list of modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -339,18 +349,17 @@ def main():
     # dominant kernel of the reported mode: per-build algorithmic flops of its
     # contraction (half of 4 n_b N_b^2) over its HIP-event time
     k_ms = head["k_ms"]
-    names = ["k_rho", "k_kmat"] if args.precision == "fp64" else ["k_rho_split", "k_kmat_split"]
     dom = int(np.argmax(k_ms[:2]))
+    name, peak = KERNELS[args.precision][dom]
     achieved = st.flops / 2.0 / (k_ms[dom] / 1e3) / 1e12
-    peak = PEAK_TFLOPS[args.precision]
-    roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": peak,
+    roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak,
             "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(info, grid, PA, st.flops)
 
-    label = {"fp64": "FP64", "bf16x2": "split-BF16 hi+lo", "bf16x3": "split-BF16 hi+mid+lo"}[args.precision]
+    label = LABELS[args.precision]
     line = {
         "metric": f"K_xc+E_xc build GFLOP/s (algorithmic 4 n_b N_b^2), {label} GPU",
         "value": st.flops / dt / 1e9,
@@ -363,7 +372,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "bf16 slices, f32 accumulate, f64 AO/functional/sums",
+        "dtype": DTYPES[args.precision],
         "data": "synthetic (NeRF alpha-helix geometry, deterministic density guess)",
         "config": {"workload": wl["label"], "precision": args.precision, "n_atoms": mol.n_atoms, "n_aos": N,
                    "n_points": n_pts, "n_blocks": int(st.n_blocks), "block_points": 128,

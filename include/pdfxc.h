@@ -54,17 +54,19 @@ enum {
 
 /* arithmetic of the two contractions X = Phi.P and K = Phi^T.V:
  * FP64 MFMA, or bf16 MFMA with FP32 accumulation on operands split into
- * bf16 slices (hi + lo: 3 slice products; hi + mid + lo: 6).  AO values,
- * the functional, the rho / grad rho dots and the K_xc sums stay FP64 in
- * every mode.  No reference counterpart (the reference is FP64 only). */
-enum { PDFXC_FP64 = 0, PDFXC_SPLIT_BF16 = 1, PDFXC_SPLIT_BF16X3 = 2 };
+ * bf16 slices (hi + lo: 3 slice products; hi + mid + lo: 6).
+ * PDFXC_SPLIT_K_BF16X3 runs the rho pass in FP64 (so E_xc and rho are the
+ * FP64 build's) and only the K pass in hi + mid + lo.  AO values, the
+ * functional, the rho / grad rho dots and the K_xc sums stay FP64 in every
+ * mode.  No reference counterpart (the reference is FP64 only). */
+enum { PDFXC_FP64 = 0, PDFXC_SPLIT_BF16 = 1, PDFXC_SPLIT_BF16X3 = 2, PDFXC_SPLIT_K_BF16X3 = 3 };
 
 typedef struct pdfxc_opts {
     double density_cutoff;   /* rho_alpha cutoff, xc-density-threshold (DfCalcGridX.cpp:55-59), 1e-16 */
     double ao_screen;        /* per-block AO screening bound; 0 keeps every AO the reference
                                 evaluates as non-zero (only the alpha r^2 > 30 primitive cut,
                                 DfCalcGridX.cpp:42, :690-700) */
-    int precision;           /* PDFXC_FP64, PDFXC_SPLIT_BF16, PDFXC_SPLIT_BF16X3 */
+    int precision;           /* PDFXC_FP64, PDFXC_SPLIT_BF16, PDFXC_SPLIT_BF16X3, PDFXC_SPLIT_K_BF16X3 */
     int block_points;        /* points per grid block: 128 */
     int64_t scratch_bytes;   /* reserved, must be 0 (AO values are never stored in HBM) */
 } pdfxc_opts;

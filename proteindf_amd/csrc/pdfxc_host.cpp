@@ -140,7 +140,7 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
         return PDFXC_ERR_UNSUPPORTED;
     }
     if (c->opts.precision != PDFXC_FP64 && c->opts.precision != PDFXC_SPLIT_BF16 &&
-        c->opts.precision != PDFXC_SPLIT_BF16X3) {
+        c->opts.precision != PDFXC_SPLIT_BF16X3 && c->opts.precision != PDFXC_SPLIT_K_BF16X3) {
         delete c;
         set_error("pdfxc_ctx_create: unknown precision mode " + std::to_string(c->opts.precision));
         return PDFXC_ERR_ARGUMENT;
@@ -277,7 +277,7 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
     if (rc == PDFXC_OK) rc = dalloc(&c->d_K, (size_t)n_aos * n_aos);
     if (rc == PDFXC_OK) rc = dalloc(&c->d_E, 1);
     if (rc == PDFXC_OK && c->opts.precision != PDFXC_FP64) {
-        const int S = (c->opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
+        const int S = prec_slices(c->opts.precision);
         c->split_nc = (n_aos + 7) / 8 * 8;
         // split AO order: atoms (runs of shells on one centre) in input order,
         // each atom's shells by most diffuse exponent first, so that the shells
@@ -645,7 +645,7 @@ static int build_blocks(Context& c) {
     c.k_fp64_launches.clear();
     c.n_k_fp64_blocks = 0;
     if (c.opts.precision != PDFXC_FP64) {
-        const int S = (c.opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
+        const int S = prec_slices(c.opts.precision);
         c.sblocks.resize(nb);
         std::vector<int> ipos(c.n_aos);
         for (int q = 0; q < c.n_aos; ++q) ipos[c.split_perm[q]] = q;
@@ -672,7 +672,7 @@ static int build_blocks(Context& c) {
                 const int pos = (int)(std::lower_bound(ch.begin(), ch.end(), q0 / 4) - ch.begin());
                 c.sshell_col.push_back(4 * pos + q0 % 4);
             }
-            if (SB.ldnp > 1024) {
+            if (SB.ldnp > 1024 && prec_split_rho(c.opts.precision)) {
                 set_error("set_grid: a block keeps " + std::to_string(4 * SB.nchunk) +
                           " chunked AO columns; the split kernels take at most 1024 (raise ao_screen)");
                 return PDFXC_ERR_UNSUPPORTED;
@@ -683,7 +683,8 @@ static int build_blocks(Context& c) {
             while ((256 << base) < ldnp) ++base;
             return std::min(2, base + c.tune.split_rho_shift);
         };
-        for (int cls = 0; cls < 3; ++cls) {
+        // (no split rho launches when the rho pass runs in FP64)
+        for (int cls = 0; cls < 3 && prec_split_rho(c.opts.precision); ++cls) {
             const size_t pb = (size_t)split_points_per_workgroup(cls);
             LaunchRange r{cls, (int)srho_work.size(), 0, 0};
             for (int b = 0; b < nb; ++b) {
@@ -1297,7 +1298,7 @@ static int run_rho(Context* c, const double* dP, double* rho) {
     c->d_P = const_cast<double*>(dP);
     c->d_rho = rho;
     int rc = PDFXC_OK;
-    if (c->opts.precision != PDFXC_FP64) {
+    if (prec_split_rho(c->opts.precision)) {
         rc = launch_split_rho(*c);
     } else {
         for (const LaunchRange& r : c->rho_launches)
@@ -1743,9 +1744,12 @@ int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
     if (c->opts.precision == PDFXC_FP64) {
         for (int rs : {0, 1})
             for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 32, 0.0});
-    } else {
+    } else if (prec_split_rho(c->opts.precision)) {
         for (int rs : {0, 1})
             for (int kp : {32, 16}) cand.push_back(pdfxc_tuning{0, 96, rs, kp, 0.0});
+    } else {  // FP64 rho pass, split K pass
+        for (int rs : {0, 1})
+            for (int kp : {32, 16}) cand.push_back(pdfxc_tuning{rs, 96, 0, kp, 0.0});
     }
     pdfxc_tuning best = cand[0];
     best.ms = 1e30;
@@ -2019,7 +2023,7 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
     m.opts = c0->opts;
     b->n_mol = n_mol;
     const bool split = m.opts.precision != PDFXC_FP64;
-    const int S = (m.opts.precision == PDFXC_SPLIT_BF16X3) ? 3 : 2;
+    const int S = prec_slices(m.opts.precision);
     int RS = 0;
     for (int i = 0; i < n_mol; ++i) RS = std::max(RS, reinterpret_cast<const Context*>(ctxs[i])->rec_stride);
     m.rec_stride = RS;

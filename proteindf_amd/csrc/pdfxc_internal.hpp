@@ -12,6 +12,12 @@
 
 namespace pdfxc {
 
+// precision modes: every non-FP64 mode keeps the split structures and runs the
+// K pass on bf16 slices; the rho pass is split only in SPLIT_BF16 / SPLIT_BF16X3
+inline bool prec_split_rho(int p) { return p == PDFXC_SPLIT_BF16 || p == PDFXC_SPLIT_BF16X3; }
+inline int prec_slices(int p) { return p == PDFXC_SPLIT_BF16 ? 2 : 3; }
+
+
 constexpr int kBlockPts = 128;  // grid points per block (rows of the MFMA tiles)
 constexpr int kColPad = 16;     // AO columns of a block are padded to a multiple of this
 constexpr int kMaxPrim = 16;

@@ -674,7 +674,7 @@ int launch_split_kmat(Context& c) {
     for (const LaunchRange& r : c.sk_launches) {
         if (rc != PDFXC_OK) break;
         if (r.count <= 0) continue;
-        rc = (c.opts.precision == PDFXC_SPLIT_BF16X3) ? kmat_split_one<3>(c, r) : kmat_split_one<2>(c, r);
+        rc = (prec_slices(c.opts.precision) == 3) ? kmat_split_one<3>(c, r) : kmat_split_one<2>(c, r);
     }
     return rc;
 }

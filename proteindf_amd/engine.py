@@ -25,7 +25,9 @@ XC_TYPES = {"HF": PDFXC_HF, "HFS": PDFXC_HFS, "SVWN": PDFXC_SVWN, "HFB": PDFXC_H
 PDFXC_FP64 = 0
 PDFXC_SPLIT_BF16 = 1
 PDFXC_SPLIT_BF16X3 = 2
-PRECISIONS = {"fp64": PDFXC_FP64, "bf16x2": PDFXC_SPLIT_BF16, "bf16x3": PDFXC_SPLIT_BF16X3}
+PDFXC_SPLIT_K_BF16X3 = 3   # FP64 rho pass, hi + mid + lo K pass
+PRECISIONS = {"fp64": PDFXC_FP64, "bf16x2": PDFXC_SPLIT_BF16, "bf16x3": PDFXC_SPLIT_BF16X3,
+              "bf16x3k": PDFXC_SPLIT_K_BF16X3}
 ERROR_NAMES = {1: "ARGUMENT", 2: "HIP", 3: "NO_GRID", 4: "UNSUPPORTED", 5: "MEMORY"}
 ERROR_CODES = {v: k for k, v in ERROR_NAMES.items()}
 

@@ -16,7 +16,11 @@ pytestmark = pytest.mark.gpu
 # 8e-8.  A CPU emulation of the same slices with exact accumulation gives
 # |dE| ~ 7e-9 for hi+mid+lo: what remains is the FP32 accumulation inside each
 # 16x16x32 MFMA (the engine adds every K step's sum into FP64).
-BOUNDS = {engine.PDFXC_SPLIT_BF16: (2e-4, 2e-5), engine.PDFXC_SPLIT_BF16X3: (5e-6, 5e-7)}
+# PDFXC_SPLIT_K_BF16X3 runs the rho pass in FP64: E and rho are the FP64 build's
+# (the same kernels in the same order), K has the hi+mid+lo error.
+BOUNDS = {engine.PDFXC_SPLIT_BF16: (2e-4, 2e-5), engine.PDFXC_SPLIT_BF16X3: (5e-6, 5e-7),
+          engine.PDFXC_SPLIT_K_BF16X3: (1e-10, 5e-7)}
+SPLIT_MODES = [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3, engine.PDFXC_SPLIT_K_BF16X3]
 
 
 def say(*a):
@@ -43,7 +47,7 @@ def _build(info, grid, PA, precision, ao_screen=0.0):
     return E, guess.unpack_lower(K, info.n_aos), rho
 
 
-@pytest.mark.parametrize("precision", [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3])
+@pytest.mark.parametrize("precision", SPLIT_MODES)
 def test_split_build_h2o8_against_fp64(precision):
     mol = molecules.water_cluster(8, 0)
     info = calc_grid.orbital_info(mol)
@@ -62,9 +66,12 @@ def test_split_build_h2o8_against_fp64(precision):
     assert dE <= eb
     assert dK <= kb
     np.testing.assert_allclose(K1, K1.T, atol=0)  # symmetric by construction
+    if precision == engine.PDFXC_SPLIT_K_BF16X3:
+        assert E1 == E0
+        np.testing.assert_array_equal(rho1, rho0)
 
 
-@pytest.mark.parametrize("precision", [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3])
+@pytest.mark.parametrize("precision", SPLIT_MODES)
 def test_split_ragged_grids_against_oracle(precision):
     mol = molecules.water_cluster(2, seed=5)
     info = calc_grid.orbital_info(mol)
@@ -80,8 +87,8 @@ def test_split_ragged_grids_against_oracle(precision):
         say(f"precision {precision} npts {npts}: |dE| {dE:.3e} max|dK| {dK:.3e}")
         eb, kb = BOUNDS[precision]
         assert dE <= eb and dK <= kb
-        np.testing.assert_allclose(rho[:, 0], rho_ref[:, 0], rtol=2e-4 if precision == 1 else 2e-6,
-                                   atol=1e-12)
+        rtol = {engine.PDFXC_SPLIT_BF16: 2e-4, engine.PDFXC_SPLIT_BF16X3: 2e-6, engine.PDFXC_SPLIT_K_BF16X3: 1e-9}
+        np.testing.assert_allclose(rho[:, 0], rho_ref[:, 0], rtol=rtol[precision], atol=1e-12)
 
 
 def test_split_zero_density_and_accumulate():

@@ -25,7 +25,7 @@ def _case():
     return mol, info, PA, grid
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "bf16x3k"])
 def test_every_shape_gives_the_same_build(prec):
     mol, info, PA, grid = _case()
     ctx = engine.XcContext(info, engine.default_opts(precision=engine.PRECISIONS[prec]))
