@@ -706,7 +706,7 @@ static int build_blocks(Context& c) {
             if (r.count) c.srho_launches.push_back(r);
         }
         // k_kmat_split items: lower 2 x 2 tile patches of the compact columns,
-        // at most 32 per item (8 consumer waves x 4); two slab buffers if they fit
+        // at most 16 per item (8 consumer waves x 2); two slab buffers if they fit
         size_t k_lds_max = 0;
         if (int rc = split_kmat_max_dyn_lds(S, &k_lds_max)) return rc;
         std::vector<char> k_fp64_block(nb, 0);
@@ -717,7 +717,7 @@ static int build_blocks(Context& c) {
             const int T = B.ldn / 16;
             const int Tp = (T + 1) / 2;
             const int np = Tp * (Tp + 1) / 2;
-            const int per = c.tune.split_k_patches;  // at most 32 (8 consumer waves x 4)
+            const int per = c.tune.split_k_patches;  // at most 16 (8 consumer waves x 2)
             for (int q0 = 0; q0 < np; q0 += per) {
                 KWork w;
                 w.block = b;
@@ -1722,7 +1722,7 @@ int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* t) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c || !t) return arg_error("pdfxc_set_tuning: bad arguments");
     if (t->rho_shift < 0 || t->rho_shift > 3 || t->split_rho_shift < 0 || t->split_rho_shift > 2 ||
-        t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32)
+        t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 16)
         return arg_error("pdfxc_set_tuning: shape out of range");
     c->tune = *t;
     if (c->n_pts == 0) return PDFXC_OK;
@@ -1743,13 +1743,13 @@ int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
     std::vector<pdfxc_tuning> cand;
     if (c->opts.precision == PDFXC_FP64) {
         for (int rs : {0, 1})
-            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 32, 0.0});
+            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 16, 0.0});
     } else if (prec_split_rho(c->opts.precision)) {
         for (int rs : {0, 1})
-            for (int kp : {32, 16}) cand.push_back(pdfxc_tuning{0, 96, rs, kp, 0.0});
+            for (int kp : {16, 8}) cand.push_back(pdfxc_tuning{0, 96, rs, kp, 0.0});
     } else {  // FP64 rho pass, split K pass
         for (int rs : {0, 1})
-            for (int kp : {32, 16}) cand.push_back(pdfxc_tuning{rs, 96, 0, kp, 0.0});
+            for (int kp : {16, 8}) cand.push_back(pdfxc_tuning{rs, 96, 0, kp, 0.0});
     }
     pdfxc_tuning best = cand[0];
     best.ms = 1e30;

@@ -387,11 +387,14 @@ __device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, i
             }
         }
     }
-    f32x4 acc[NPW][4];
+    // hi x hi (the leading products) and the corrections in FP32 sums of
+    // their own: the corrections (at most 2^-8 of the leading term) do not
+    // add their roundings at the leading term's size
+    f32x4 acc[NPW][4], acc_c[NPW][4];
 #pragma unroll
     for (int m = 0; m < NPW; ++m)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < 4; ++t) acc[m][t] = acc_c[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
     __syncthreads();  // the first slab is in
     for (int sl = 0; sl < nslab; ++sl) {
         const unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
@@ -412,11 +415,13 @@ __device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, i
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int hi = t >> 1, hj = t & 1;
+                acc[m][t] = mfma_bf16_k16(fI[hi][0], vJ[hj][0], acc[m][t]);
+                acc[m][t] = mfma_bf16_k16(vI[hi][0], fJ[hj][0], acc[m][t]);
 #pragma unroll
-                for (int q = 0; q < SlicePairs<S>::n; ++q) {
+                for (int q = 1; q < SlicePairs<S>::n; ++q) {
                     const int sa = SlicePairs<S>::a[q], sb = SlicePairs<S>::b[q];
-                    acc[m][t] = mfma_bf16_k16(fI[hi][sa], vJ[hj][sb], acc[m][t]);
-                    acc[m][t] = mfma_bf16_k16(vI[hi][sa], fJ[hj][sb], acc[m][t]);
+                    acc_c[m][t] = mfma_bf16_k16(fI[hi][sa], vJ[hj][sb], acc_c[m][t]);
+                    acc_c[m][t] = mfma_bf16_k16(vI[hi][sa], fJ[hj][sb], acc_c[m][t]);
                 }
             }
         }
@@ -435,7 +440,8 @@ __device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, i
                 for (int r = 0; r < 4; ++r) {
                     const int lr = offI[m] + 16 * (t >> 1) + 4 * lq + r;
                     const int gi = s_cols[lr];
-                    if (gi >= 0 && gj >= 0 && lr >= lc) atomicAdd(K + (int64_t)gi * N + gj, (double)acc[m][t][r]);
+                    if (gi >= 0 && gj >= 0 && lr >= lc)
+                        atomicAdd(K + (int64_t)gi * N + gj, (double)acc[m][t][r] + (double)acc_c[m][t][r]);
                 }
             }
         }
@@ -562,12 +568,11 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
     const bool has = pstart < pend;
     const int pb = has ? pstart : W.t1 - 1;
     const int pe = has ? pend : pb;
-    switch (has ? pend - pstart : 1) {
-        case 1: kmat_split_consume<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
-        case 2: kmat_split_consume<S, 2>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
-        case 3: kmat_split_consume<S, 3>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
-        default: kmat_split_consume<S, 4>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp); break;
-    }
+    // the host keeps items within 8 consumer waves x 2 patches
+    if (has && pend - pstart == 2)
+        kmat_split_consume<S, 2>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp);
+    else
+        kmat_split_consume<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp);
 }
 
 // layout self test: A(i, k) = ((3 i + 5 k) % 11) - 5, B(k, j) = ((7 k + 2 j) % 13) - 6;

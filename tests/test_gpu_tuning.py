@@ -33,7 +33,7 @@ def test_every_shape_gives_the_same_build(prec):
     P = guess.pack_lower(PA)
     E0, K0, _ = ctx.build_rks(P)
     shapes = [dict(rho_shift=1), dict(rho_shift=3, k_tiles=8), dict(k_tiles=24), dict(split_rho_shift=1),
-              dict(split_rho_shift=2, split_k_patches=4), dict(split_k_patches=16)]
+              dict(split_rho_shift=2, split_k_patches=4), dict(split_k_patches=8)]
     for sh in shapes:
         ctx.set_tuning(**sh)
         E, K, _ = ctx.build_rks(P)
