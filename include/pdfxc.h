@@ -231,10 +231,11 @@ int pdfxc_eval_functional_uks(int functional, int64_t n, const double* in, doubl
  * assignment is used); AO centres must coincide with atoms.  Outputs dE/dR,
  * n_atoms x 3 each: grad_functional = the reference's functional term
  * (AO-centre and point-translation parts, doubled for RKS); grad_weight = the
- * derivative of the Becke partition weights, with or without the size
- * adjustments as pdfxc_generate_grid_ex made the grid (SS-partitioned grids
- * and dummy atoms are refused with PDFXC_ERR_UNSUPPORTED; the reference's
- * own weight term evaluates to zero, DfGenerateGrid.cpp:2355-2372).  Their
+ * derivative of the partition weights the grid was made with (Becke with or
+ * without size adjustments, or Stratmann-Scuseria, as pdfxc_generate_grid_ex
+ * made it; size-adjusted Becke for a caller's grid).  Dummy atoms are refused
+ * with PDFXC_ERR_UNSUPPORTED.  (The reference's
+ * own weight term evaluates to zero, DfGenerateGrid.cpp:2355-2372.)  Their
  * sum is the gradient of E_xc.  E_xc (may be NULL) is returned.  FP64
  * contractions whatever the context's precision.  The context's stored rho
  * becomes rho[P_alpha] (an update-mode SCF restarts from a full build). */

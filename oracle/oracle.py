@@ -77,6 +77,10 @@ def lib():
         L.oracle_becke_weight_gradient.restype = d
         L.oracle_xc_gradient_rks.argtypes = [P, P, i32, i64, P, P, i32, P, P, P, d, P, P, i32]
         L.oracle_xc_gradient_rks.restype = d
+        L.oracle_xc_gradient_rks_ex.argtypes = [P, P, i32, i64, P, P, i32, P, P, P, d, i32, P, P, i32]
+        L.oracle_xc_gradient_rks_ex.restype = d
+        L.oracle_partition_weight_gradient.argtypes = [i32, P, P, i32, P, i32, P]
+        L.oracle_partition_weight_gradient.restype = d
         _lib = L
     return _lib
 
@@ -184,19 +188,22 @@ def eval_ao_hessians(aos, xyz):
     return h
 
 
-def becke_weight_gradient(Z, xyz, iatom, r):
-    """(omega_A, (n_atoms, 3) grad_B omega_A with the point fixed)."""
+def becke_weight_gradient(Z, xyz, iatom, r, partition=0):
+    """(omega_A, (n_atoms, 3) grad_B omega_A with the point fixed); partition 0
+    Becke with size adjustments, 1 Becke without, 2 Stratmann-Scuseria."""
     Z = np.ascontiguousarray(Z, dtype=np.int32)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     r = np.ascontiguousarray(r, dtype=np.float64)
     out = np.zeros((len(Z), 3))
-    om = lib().oracle_becke_weight_gradient(len(Z), _p(Z), _p(xyz), iatom, _p(r), _p(out))
+    om = lib().oracle_partition_weight_gradient(len(Z), _p(Z), _p(xyz), iatom, _p(r), partition, _p(out))
     return om, out
 
 
 def xc_gradient_rks(aos, ao_atom, grid, pt_atom, Z, xyz, PA, fid=B3LYP, density_cutoff=1e-16,
-                    n_threads=0):
-    """Returns (E_xc, grad_functional (n_atoms, 3), grad_weight (n_atoms, 3))."""
+                    n_threads=0, partition=0):
+    """Returns (E_xc, grad_functional (n_atoms, 3), grad_weight (n_atoms, 3)).
+    partition: the grid's partition (0 Becke with size adjustments, 1 Becke
+    without, 2 Stratmann-Scuseria) for the weight term."""
     grid = np.ascontiguousarray(grid, dtype=np.float64).reshape(-1, 4)
     pt_atom = np.ascontiguousarray(pt_atom, dtype=np.int32)
     ao_atom = np.ascontiguousarray(ao_atom, dtype=np.int32)
@@ -205,9 +212,9 @@ def xc_gradient_rks(aos, ao_atom, grid, pt_atom, Z, xyz, PA, fid=B3LYP, density_
     PA = np.ascontiguousarray(PA, dtype=np.float64)
     gf = np.zeros((len(Z), 3))
     gw = np.zeros((len(Z), 3))
-    e = lib().oracle_xc_gradient_rks(aos.ref(), _p(ao_atom), fid, grid.shape[0], _p(grid), _p(pt_atom),
-                                     len(Z), _p(Z), _p(xyz), _p(PA), density_cutoff, _p(gf), _p(gw),
-                                     n_threads)
+    e = lib().oracle_xc_gradient_rks_ex(aos.ref(), _p(ao_atom), fid, grid.shape[0], _p(grid), _p(pt_atom),
+                                        len(Z), _p(Z), _p(xyz), _p(PA), density_cutoff, partition, _p(gf),
+                                        _p(gw), n_threads)
     return e, gf, gw
 
 

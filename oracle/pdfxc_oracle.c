@@ -1337,10 +1337,23 @@ static double prod_except(const double* s, int n, int i, int j) {
         if (m != j) q *= s[i * n + m];
     return q;
 }
+/* Stratmann-Scuseria step s(mu) = (1 - g(mu)) / 2 and ds/dmu (the g of
+ * calcMultiCenterWeight_SS, DfGenerateGrid.cpp:1923-1938, a = 0.64) */
+static void ss_s(double mu, double* s, double* ds) {
+    if (mu < -0.64 || mu > 0.64) {
+        *s = (mu < 0.0) ? 1.0 : 0.0;
+        *ds = 0.0;
+        return;
+    }
+    const double q = mu / 0.64, q2 = q * q, q3 = q2 * q, q5 = q2 * q3;
+    *s = 0.5 * (1.0 - (1.0 / 16.0) * (35.0 * q - 35.0 * q3 + 21.0 * q5 - 5.0 * q5 * q2));
+    *ds = -0.5 * (1.0 / 16.0) * (35.0 - 105.0 * q2 + 105.0 * q2 * q2 - 35.0 * q2 * q2 * q2) / 0.64;
+}
 /* grad_B omega_A at point r for every B (point held fixed); out: n_atoms x 3.
+ * partition: 0 Becke with size adjustments, 1 Becke without, 2 SS.
  * Returns omega_A. */
 static double becke_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
-                                    double* s, double* dsm, double* Pv, double* out) {
+                                    double* s, double* dsm, double* Pv, double* out, int partition) {
     const int n = n_atoms;
     double* rr = Pv + n;
     for (int i = 0; i < n; ++i) {
@@ -1357,11 +1370,17 @@ static double becke_weight_gradient(int n_atoms, const int* Z, const double* xyz
                          ez = xyz[3 * i + 2] - xyz[3 * j + 2];
             const double inv_dist = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
             const double mu = (rr[i] - rr[j]) * inv_dist;  /* as becke_ps forms it */
-            const double a = becke_a(Z[i], Z[j]);
             double sv, ds;
-            becke_s(mu + a * (1.0 - mu * mu), &sv, &ds);
-            s[i * n + j] = sv;
-            dsm[i * n + j] = ds * (1.0 - 2.0 * a * mu);
+            if (partition == 2) {
+                ss_s(mu, &sv, &ds);
+                s[i * n + j] = sv;
+                dsm[i * n + j] = ds;
+            } else {
+                const double a = (partition == 0) ? becke_a(Z[i], Z[j]) : 0.0;
+                becke_s(mu + a * (1.0 - mu * mu), &sv, &ds);
+                s[i * n + j] = sv;
+                dsm[i * n + j] = ds * (1.0 - 2.0 * a * mu);
+            }
         }
     double Zs = 0.0;
     for (int i = 0; i < n; ++i) {
@@ -1410,19 +1429,31 @@ static double becke_weight_gradient(int n_atoms, const int* Z, const double* xyz
     return omega;
 }
 
-double oracle_becke_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
-                                    double* out) {
+double oracle_partition_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
+                                        int partition, double* out) {
     const int n = n_atoms;
     double* s = (double*)malloc(sizeof(double) * (2 * (size_t)n * n + 2 * (size_t)n));
-    const double om = becke_weight_gradient(n, Z, xyz, A, r, s, s + (size_t)n * n, s + 2 * (size_t)n * n, out);
+    const double om =
+        becke_weight_gradient(n, Z, xyz, A, r, s, s + (size_t)n * n, s + 2 * (size_t)n * n, out, partition);
     free(s);
     return om;
+}
+double oracle_becke_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
+                                    double* out) {
+    return oracle_partition_weight_gradient(n_atoms, Z, xyz, A, r, 0, out);
 }
 
 double oracle_xc_gradient_rks(const oracle_aos* aos, const int* ao_atom, int functional, int64_t n_pts,
                               const double* grid, const int* pt_atom, int n_atoms, const int* Z,
                               const double* xyz, const double* PA, double density_cutoff,
                               double* grad_f, double* grad_w, int n_threads) {
+    return oracle_xc_gradient_rks_ex(aos, ao_atom, functional, n_pts, grid, pt_atom, n_atoms, Z, xyz, PA,
+                                     density_cutoff, 0, grad_f, grad_w, n_threads);
+}
+double oracle_xc_gradient_rks_ex(const oracle_aos* aos, const int* ao_atom, int functional, int64_t n_pts,
+                                 const double* grid, const int* pt_atom, int n_atoms, const int* Z,
+                                 const double* xyz, const double* PA, double density_cutoff, int partition,
+                                 double* grad_f, double* grad_w, int n_threads) {
     const int N = aos->n_aos;
     const int na = n_atoms;
     int nt = 1;
@@ -1503,7 +1534,7 @@ double oracle_xc_gradient_rks(const oracle_aos* aos, const int* ao_atom, int fun
                 }
             }
             /* weight term (a cell weight that rounds to 0 has ds/dnu = 0 there too) */
-            const double omega = becke_weight_gradient(na, Z, xyz, A, p, s, dsm, Pv, gom);
+            const double omega = becke_weight_gradient(na, Z, xyz, A, p, s, dsm, Pv, gom, partition);
             if (!(omega > 0.0)) continue;
             const double ws = w / omega;
             for (int B = 0; B < na; ++B) {

@@ -154,6 +154,11 @@ void oracle_eval_ao_hessians(const oracle_aos* aos, int64_t n_pts, const double*
 double oracle_becke_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
                                     double* out);
 
+/* The same for partition 0 (Becke with size adjustments), 1 (Becke without)
+ * or 2 (Stratmann-Scuseria). */
+double oracle_partition_weight_gradient(int n_atoms, const int* Z, const double* xyz, int A, const double* r,
+                                        int partition, double* out);
+
 /* XC nuclear gradient, RKS GGA (energyGradient, DfCalcGridX.cpp:1215-1415)
  * on the given grid: grid n_pts x 4 (x, y, z, w), pt_atom[p] = the atom the
  * point belongs to (the point moves with it), ao_atom[i] = atom of AO i,
@@ -166,6 +171,11 @@ double oracle_xc_gradient_rks(const oracle_aos* aos, const int* ao_atom, int fun
                               const double* grid, const int* pt_atom, int n_atoms, const int* Z,
                               const double* xyz, const double* PA, double density_cutoff,
                               double* grad_f, double* grad_w, int n_threads);
+/* ... with the weight term of partition 0, 1 or 2 (as above). */
+double oracle_xc_gradient_rks_ex(const oracle_aos* aos, const int* ao_atom, int functional, int64_t n_pts,
+                                 const double* grid, const int* pt_atom, int n_atoms, const int* Z,
+                                 const double* xyz, const double* PA, double density_cutoff, int partition,
+                                 double* grad_f, double* grad_w, int n_threads);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,8 @@
 //   k_grad_fn  per 16 points of a block: phi and W into LDS, X^T = P_bb phi^T
 //              and Z^T = P_bb W^T on v_mfma_f64_16x16x4, then grad phi and the
 //              AO second derivatives per (point, shell), summed per atom in LDS
-//   k_grad_w   per point: grad_B omega_A of the size-adjusted Becke partition
+//   k_grad_w   per point: grad_B omega_A of the grid's partition (Becke with
+//              or without size adjustments, or Stratmann-Scuseria)
 //              for every atom B (point fixed), times w_single e_xc, to B and
 //              with opposite sign to A (:1386-1404).  The reference's JGP_t
 //              (DfGenerateGrid.cpp:2355-2372) returns 0 for every |mu| <= 1,
@@ -238,10 +239,32 @@ __device__ inline void becke_step(double nu, double& s, double& ds) {
     ds = -0.5 * (1.5 * (1.0 - f2 * f2)) * (1.5 * (1.0 - f1 * f1)) * (1.5 * (1.0 - nu * nu));
 }
 
-// atoms: n x 4 (x, y, z, Z) | 1/R_mn [n][n] | a_mn [n][n] (the Becke kernel's table)
+// cell factor s(mu) of the grid's partition and ds/dmu: Becke with the size
+// adjustment a (nu = mu + a (1 - mu^2); a = 0 without adjustments), or the
+// Stratmann-Scuseria step, g(mu) of calcMultiCenterWeight_SS
+// (DfGenerateGrid.cpp:1923-1938, a = 0.64)
+__device__ inline void cell_step(int ss, double mu, double a, double& s, double& dsdmu) {
+    if (ss) {
+        if (mu < -0.64 || mu > 0.64) {
+            s = (mu < 0.0) ? 1.0 : 0.0;
+            dsdmu = 0.0;
+        } else {
+            const double q = mu / 0.64, q2 = q * q, q3 = q2 * q, q5 = q2 * q3;
+            s = 0.5 * (1.0 - (1.0 / 16.0) * (35.0 * q - 35.0 * q3 + 21.0 * q5 - 5.0 * q5 * q2));
+            dsdmu = -0.5 * (1.0 / 16.0) * (35.0 - 105.0 * q2 + 105.0 * q2 * q2 - 35.0 * q2 * q2 * q2) / 0.64;
+        }
+        return;
+    }
+    double ds;
+    becke_step(mu + a * (1.0 - mu * mu), s, ds);
+    dsdmu = ds * (1.0 - 2.0 * a * mu);
+}
+
+// atoms: n x 4 (x, y, z, Z) | 1/R_mn [n][n] | a_mn [n][n] (the partition kernel's table);
+// ss: the grid's partition is Stratmann-Scuseria (else Becke)
 __global__ void k_grad_w(const double* __restrict__ atoms, int na, int64_t n_slots, const double* __restrict__ pts,
                          const int* __restrict__ slot_atom, const double* __restrict__ ewf,
-                         double* __restrict__ gacc) {
+                         double* __restrict__ gacc, int ss) {
     extern __shared__ double s_buf[];  // r [blockDim][na] | P [blockDim][na] | sums [3 na]
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
@@ -269,9 +292,8 @@ __global__ void k_grad_w(const double* __restrict__ atoms, int na, int64_t n_slo
                 for (int j = 0; j < na && p != 0.0; ++j) {
                     if (j == i || atoms[4 * j + 3] <= 0.0) continue;
                     const double mu = (r[i] - r[j]) * invd[(size_t)i * na + j];
-                    const double a = amat[(size_t)i * na + j];
                     double s, ds;
-                    becke_step(mu + a * (1.0 - mu * mu), s, ds);
+                    cell_step(ss, mu, amat[(size_t)i * na + j], s, ds);
                     p *= s;
                 }
             }
@@ -288,9 +310,8 @@ __global__ void k_grad_w(const double* __restrict__ atoms, int na, int64_t n_slo
                 for (int m = 0; m < na; ++m) {
                     if (m == i || m == j || atoms[4 * m + 3] <= 0.0) continue;
                     const double mu = (r[i] - r[m]) * invd[(size_t)i * na + m];
-                    const double a = amat[(size_t)i * na + m];
                     double s, ds;
-                    becke_step(mu + a * (1.0 - mu * mu), s, ds);
+                    cell_step(ss, mu, amat[(size_t)i * na + m], s, ds);
                     q *= s;
                 }
                 return q;
@@ -309,20 +330,18 @@ __global__ void k_grad_w(const double* __restrict__ atoms, int na, int64_t n_slo
                     // cell B, factor of atom i: grad_B mu_Bi = (u_B - mu_Bi e_Bi) / R
                     {
                         const double mu = (r[Bt] - r[i]) * id;
-                        const double a = amat[(size_t)Bt * na + i];
                         double s, ds;
-                        becke_step(mu + a * (1.0 - mu * mu), s, ds);
-                        const double c = prod_ex(Bt, i, s) * ds * (1.0 - 2.0 * a * mu) * id;
+                        cell_step(ss, mu, amat[(size_t)Bt * na + i], s, ds);
+                        const double c = prod_ex(Bt, i, s) * ds * id;
 #pragma unroll
                         for (int d = 0; d < 3; ++d) gZ[d] += c * (uB[d] - mu * Rv[d]);
                     }
                     // cell i, factor of atom B: grad_B mu_iB = (-u_B - mu_iB e_Bi) / R
                     {
                         const double mu = (r[i] - r[Bt]) * id;
-                        const double a = amat[(size_t)i * na + Bt];
                         double s, ds;
-                        becke_step(mu + a * (1.0 - mu * mu), s, ds);
-                        const double c = prod_ex(i, Bt, s) * ds * (1.0 - 2.0 * a * mu) * id;
+                        cell_step(ss, mu, amat[(size_t)i * na + Bt], s, ds);
+                        const double c = prod_ex(i, Bt, s) * ds * id;
 #pragma unroll
                         for (int d = 0; d < 3; ++d) {
                             const double gm = c * (-uB[d] - mu * Rv[d]);
@@ -379,7 +398,7 @@ int launch_grad_fn(Context& c, const LaunchRange& r, const int* d_shell_atom, co
 }
 
 int launch_grad_w(Context& c, const double* d_atoms, int n_atoms, const int* d_slot_atom, const double* d_ewf,
-                  double* d_gacc) {
+                  double* d_gacc, int partition) {
     int threads = 64;
     while (threads > 1 && (size_t)threads * 2 * n_atoms * sizeof(double) + 3 * n_atoms * sizeof(double) > 65536)
         threads >>= 1;
@@ -390,7 +409,7 @@ int launch_grad_w(Context& c, const double* d_atoms, int n_atoms, const int* d_s
     }
     const int64_t g = (c.n_slots + threads - 1) / threads;
     hipLaunchKernelGGL(k_grad_w, dim3((unsigned)g), dim3(threads), lds, c.stream, d_atoms, n_atoms, c.n_slots,
-                       c.d_pts, d_slot_atom, d_ewf, d_gacc + 3 * n_atoms);
+                       c.d_pts, d_slot_atom, d_ewf, d_gacc + 3 * n_atoms, partition == PDFXC_PARTITION_SS ? 1 : 0);
     return check("k_grad_w");
 }
 

@@ -1572,10 +1572,6 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
     }
     if (!point_atom && c->gen_atom.size() != (size_t)c->n_pts)
         return arg_error("pdfxc_gradient_rks: the grid was not made by pdfxc_generate_grid; pass point_atom");
-    if (c->grid_partition != PDFXC_PARTITION_BECKE) {
-        set_error("pdfxc_gradient_rks: the weight derivative is implemented for Becke partitioning only");
-        return PDFXC_ERR_UNSUPPORTED;
-    }
     for (int a = 0; a < n_atoms; ++a)
         if (Z[a] <= 0) {
             set_error("pdfxc_gradient_rks: dummy atoms (Z = 0) are not supported");
@@ -1696,7 +1692,7 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
     if (rc == PDFXC_OK) rc = launch_xc_points(*c, functional, d_ewf);
     for (const LaunchRange& r : gl)
         if (rc == PDFXC_OK) rc = launch_grad_fn(*c, r, d_shat, d_slat, na, d_g);
-    if (rc == PDFXC_OK) rc = launch_grad_w(*c, d_at, na, d_slat, d_ewf, d_g);
+    if (rc == PDFXC_OK) rc = launch_grad_w(*c, d_at, na, d_slat, d_ewf, d_g, c->grid_partition);
     if (rc == PDFXC_OK) rc = launch_energy_sum(*c, c->d_E);
     std::vector<double> g(6 * (size_t)na);
     double E = 0.0;

@@ -204,7 +204,7 @@ int launch_mfma_split_selftest(float* d_out, hipStream_t s);
 int launch_grad_fn(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,
                    int n_atoms, double* d_gacc);
 int launch_grad_w(Context& c, const double* d_atoms, int n_atoms, const int* d_slot_atom, const double* d_ewf,
-                  double* d_gacc);
+                  double* d_gacc, int partition);
 int launch_copy(const double* src, double* dst, int64_t n, hipStream_t s);
 
 }  // namespace pdfxc

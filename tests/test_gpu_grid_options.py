@@ -92,24 +92,46 @@ def test_build_on_pruned_grid_matches_oracle(name):
     assert abs(E - e_ref) <= 1e-10 and dK <= 1e-10
 
 
-def test_gradient_partition_rules():
+@pytest.mark.parametrize("name,partition", [("sg-1", 2), ("fine", 2), ("user-gc-becke-noadj", 1), ("coarse-becke", 0)])
+def test_gradient_on_option_grids_matches_oracle(name, partition):
     mol = molecules.water_cluster(2, seed=6)
     info = calc_grid.orbital_info(mol)
-    PA = guess.pack_lower(0.5 * guess.density_guess(mol, info, seed=6, eps=0.05))
+    PA = 0.5 * guess.density_guess(mol, info, seed=6, eps=0.05)
     ctx = engine.XcContext(info)
-    ctx.generate_grid_ex(mol.Z, mol.xyz, _engine_opts(OPTION_SETS["sg-1"]))
-    say("gradient on an SS grid is refused")
+    ctx.generate_grid_ex(mol.Z, mol.xyz, _engine_opts(OPTION_SETS[name]))
+    say("gradient on the", name, "grid")
+    E, gf, gw = ctx.gradient_rks(mol.Z, mol.xyz, guess.pack_lower(PA))
+    ctx.close()
+    grid, atom = oracle.generate_grid_ex(mol.Z, mol.xyz, oracle.grid_opts(**OPTION_SETS[name]))
+    e_ref, gf_ref, gw_ref = oracle.xc_gradient_rks(oracle.AoTable.from_basis(info), info.ao_atom, grid, atom,
+                                                   mol.Z, mol.xyz, PA, partition=partition)
+    say(f"  |dE| {abs(E - e_ref):.2e} max|dgf| {np.abs(gf - gf_ref).max():.2e} max|dgw| {np.abs(gw - gw_ref).max():.2e}"
+        f" max|gw| {np.abs(gw_ref).max():.3f}")
+    assert abs(E - e_ref) <= 1e-9
+    np.testing.assert_allclose(gf, gf_ref, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(gw, gw_ref, rtol=0, atol=1e-9)
+    assert np.abs(gw_ref).max() > 1e-3  # the weight term is there
+
+
+def test_gradient_refuses_dummy_atoms():
+    mol = molecules.water_cluster(2, seed=6)
+    info = calc_grid.orbital_info(mol)
+    Z = np.append(mol.Z, 0)
+    xyz = np.vstack([mol.xyz, mol.xyz.mean(0) + [0.3, -0.2, 0.1]])
+    ctx = engine.XcContext(info)
+    ctx.generate_grid_ex(Z, xyz, _engine_opts(OPTION_SETS["sg-1"]))
     with pytest.raises(engine.PdfxcError) as ei:
-        ctx.gradient_rks(mol.Z, mol.xyz, PA)
+        ctx.gradient_rks(Z, xyz, guess.pack_lower(0.5 * guess.density_guess(mol, info, seed=6, eps=0.05)))
     assert ei.value.code == engine.ERROR_CODES["UNSUPPORTED"]
     ctx.close()
 
 
-def test_gradient_without_size_adjustments_matches_central_differences():
+@pytest.mark.parametrize("name", ["user-gc-becke-noadj", "sg-1"])
+def test_gradient_matches_central_differences_on_option_grids(name):
     mol = molecules.water_cluster(2, seed=7)
     info = calc_grid.orbital_info(mol)
     PA = 0.5 * guess.density_guess(mol, info, seed=7, eps=0.05)
-    o = OPTION_SETS["user-gc-becke-noadj"]
+    o = OPTION_SETS[name]
     O = oracle.o_matrix(mol.Z, mol.xyz)
     ctx = engine.XcContext(info)
     ctx.generate_grid_ex(mol.Z, mol.xyz, _engine_opts(o))
@@ -137,7 +159,7 @@ def test_gradient_without_size_adjustments_matches_central_differences():
             fd = (energy(xp) - energy(xm)) / (2 * h)
             worst = max(worst, abs(g[A, d] - fd))
             assert abs(g[A, d] - fd) <= 5e-8 + 1e-6 * abs(fd), (A, d, g[A, d], fd)
-    say(f"gradient (Becke, no size adjustments) vs central differences: max |diff| {worst:.2e}")
+    say(f"gradient ({name} grid) vs central differences: max |diff| {worst:.2e}")
 
 
 def test_build_from_the_reference_pdfparam():

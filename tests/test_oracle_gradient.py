@@ -32,17 +32,28 @@ def test_ao_hessians_match_finite_differences_of_gradients():
         np.testing.assert_allclose(h[:, :, q], fd, rtol=1e-6, atol=1e-7 * np.abs(fd).max())
 
 
-def test_becke_weight_gradient_matches_finite_differences():
+@pytest.mark.parametrize("partition", [0, 1, 2], ids=["becke", "becke-noadj", "ss"])
+def test_becke_weight_gradient_matches_finite_differences(partition):
+    # the weight itself comes from the grid code (becke_weight / ss_weight),
+    # its derivative from the gradient code: central differences tie them
     mol, _ = water(2, 4)
     Z, xyz = mol.Z, mol.xyz
     rng = np.random.default_rng(1)
     eps = 1e-6
+
+    def weight(x, A, r):
+        if partition == 2:
+            return oracle.ss_weight(x, A, r)
+        return oracle.becke_weight(Z, x, A, r, partition == 0)
+
+    nonzero = 0
     for trial in range(12):
         A = int(rng.integers(0, len(Z)))
         r = xyz[A] + rng.normal(0, 1.2, 3)
-        om, g = oracle.becke_weight_gradient(Z, xyz, A, r)
-        assert abs(om - oracle.becke_weight(Z, xyz, A, r)) < 1e-15
+        om, g = oracle.becke_weight_gradient(Z, xyz, A, r, partition)
+        assert abs(om - weight(xyz, A, r)) < 1e-15
         assert np.all(g[A] == 0.0)
+        nonzero += np.any(g != 0.0)
         for B in range(len(Z)):
             if B == A:
                 continue
@@ -50,8 +61,9 @@ def test_becke_weight_gradient_matches_finite_differences():
                 xp, xm = xyz.copy(), xyz.copy()
                 xp[B, d] += eps
                 xm[B, d] -= eps
-                fd = (oracle.becke_weight(Z, xp, A, r) - oracle.becke_weight(Z, xm, A, r)) / (2 * eps)
+                fd = (weight(xp, A, r) - weight(xm, A, r)) / (2 * eps)
                 assert abs(g[B, d] - fd) <= 1e-7 * max(1.0, abs(fd)), (trial, B, d, g[B, d], fd)
+    assert nonzero >= 6  # points in the switching regions, not only inside a cell
 
 
 def _energy(mol, PA, O, n_rad, n_ang):
