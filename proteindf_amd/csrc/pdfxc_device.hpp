@@ -105,6 +105,76 @@ __device__ inline void rec_values(const double* rec, double px, double py, doubl
     }
 }
 
+// Radial sums R0 = sum c e^{-a r^2}, R1 = sum -2a c e^{-a r^2} of a shell
+// record at offset (x, y, z) (ascending exponents, TOOBIG cut), and the
+// values / values and gradients formed from them: the same arithmetic as
+// rec_values0 / rec_values, split so that a caller can keep R0, R1 between
+// a values pass and a gradients pass.
+__device__ inline void rec_radial01(const double* rec, double x, double y, double z, double& R0, double& R1) {
+    const int nprim = ((int)rec[3]) >> 2;
+    const double r2 = x * x + y * y + z * z;
+    R0 = 0.0;
+    R1 = 0.0;
+    for (int k = 0; k < nprim; ++k) {
+        const double a = rec[4 + 2 * k];
+        const double sh = a * r2;
+        if (sh > kTooBig) break;
+        const double g = rec[5 + 2 * k] * exp_neg(sh);
+        R0 += g;
+        R1 += -2.0 * a * g;
+    }
+}
+__device__ inline void rec_values0_r(int l, double x, double y, double z, double R0, double* v) {
+    if (l == 0) {
+        v[0] = R0;
+    } else if (l == 1) {
+        v[0] = x * R0;
+        v[1] = y * R0;
+        v[2] = z * R0;
+    } else {
+        v[0] = x * y * R0;
+        v[1] = z * x * R0;
+        v[2] = y * z * R0;
+        v[3] = 0.5 * (x * x - y * y) * R0;
+        v[4] = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)) * R0;
+    }
+}
+__device__ inline void rec_values_r(int l, double x, double y, double z, double R0, double R1, double* v,
+                                    double* vx, double* vy, double* vz) {
+    if (l == 0) {
+        v[0] = R0;
+        vx[0] = x * R1;
+        vy[0] = y * R1;
+        vz[0] = z * R1;
+    } else if (l == 1) {
+        v[0] = x * R0;
+        v[1] = y * R0;
+        v[2] = z * R0;
+        vx[0] = R0 + x * x * R1;
+        vy[0] = x * y * R1;
+        vz[0] = x * z * R1;
+        vx[1] = y * x * R1;
+        vy[1] = R0 + y * y * R1;
+        vz[1] = y * z * R1;
+        vx[2] = z * x * R1;
+        vy[2] = z * y * R1;
+        vz[2] = R0 + z * z * R1;
+    } else {
+        const double f[5] = {x * y, z * x, y * z, 0.5 * (x * x - y * y),
+                             kInvSqrt3 * (z * z - 0.5 * (x * x + y * y))};
+        const double fx[5] = {y, z, 0.0, x, -kInvSqrt3 * x};
+        const double fy[5] = {x, 0.0, z, -y, -kInvSqrt3 * y};
+        const double fz[5] = {0.0, x, y, 0.0, 2.0 * kInvSqrt3 * z};
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            v[m] = f[m] * R0;
+            vx[m] = fx[m] * R0 + f[m] * x * R1;
+            vy[m] = fy[m] * R0 + f[m] * y * R1;
+            vz[m] = fz[m] * R0 + f[m] * z * R1;
+        }
+    }
+}
+
 // values only (first pass of k_rho)
 __device__ inline void rec_values0(const double* rec, double px, double py, double pz, double* v) {
     const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];

@@ -215,8 +215,28 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
     const int p = tid & (PB - 1);
     const int kk = tid / PB;
     const double px = s_pt[p][0], py = s_pt[p][1], pz = s_pt[p][2];
-    // 1. Phi plane
-    for (int j = kk; j < nsh; j += NSL) {
+    // 1. Phi plane.  The radial sums of a thread's first NC shells are kept
+    // in registers for step 3 (same thread-to-shell mapping), which then
+    // forms phi and grad phi for them without the exponentials.
+    constexpr int NC = (PB >= 32) ? 4 : 0;  // (the 16-point classes would lose occupancy)
+    double cR0[NC > 0 ? NC : 1], cR1[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int jj = 0; jj < NC; ++jj) {
+        const int j = kk + jj * NSL;
+        cR0[jj] = cR1[jj] = 0.0;
+        if (j < nsh) {
+            const int k = s_sord[j];
+            const double* rec = s_rec + k * RS;
+            const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+            rec_radial01(rec, x, y, z, cR0[jj], cR1[jj]);
+            double v[5];
+            rec_values0_r(((int)rec[3]) & 3, x, y, z, cR0[jj], v);
+            const int c0 = s_shcol[k];
+            const int nc = 2 * (((int)rec[3]) & 3) + 1;
+            for (int m = 0; m < nc; ++m) s_phi[plane_at<PB>(c0 + m, p)] = v[m];
+        }
+    }
+    for (int j = kk + NC * NSL; j < nsh; j += NSL) {
         const int k = s_sord[j];
         double v[5];
         rec_values0(s_rec + k * RS, px, py, pz, v);
@@ -268,7 +288,27 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
 
     // 3. rho and grad rho: phi, grad phi dotted with X
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    for (int j = kk; j < nsh; j += NSL) {
+#pragma unroll
+    for (int jj = 0; jj < NC; ++jj) {
+        const int j = kk + jj * NSL;
+        if (j < nsh) {
+            const int k = s_sord[j];
+            const double* rec = s_rec + k * RS;
+            const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+            double v[5], vx[5], vy[5], vz[5];
+            rec_values_r(((int)rec[3]) & 3, x, y, z, cR0[jj], cR1[jj], v, vx, vy, vz);
+            const int c0 = s_shcol[k];
+            const int nc = 2 * (((int)rec[3]) & 3) + 1;
+            for (int m = 0; m < nc; ++m) {
+                const double xv = s_phi[plane_at<PB>(c0 + m, p)];
+                r0 += xv * v[m];
+                r1 += xv * vx[m];
+                r2 += xv * vy[m];
+                r3 += xv * vz[m];
+            }
+        }
+    }
+    for (int j = kk + NC * NSL; j < nsh; j += NSL) {
         const int k = s_sord[j];
         double v[5], vx[5], vy[5], vz[5];
         rec_values(s_rec + k * RS, px, py, pz, v, vx, vy, vz);
