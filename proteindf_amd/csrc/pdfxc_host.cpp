@@ -706,7 +706,7 @@ static int build_blocks(Context& c) {
             if (r.count) c.srho_launches.push_back(r);
         }
         // k_kmat_split items: lower 2 x 2 tile patches of the compact columns,
-        // at most 16 per item (8 consumer waves x 2); two slab buffers if they fit
+        // at most 16 per item (one per wave); two slab buffers if they fit
         size_t k_lds_max = 0;
         if (int rc = split_kmat_max_dyn_lds(S, &k_lds_max)) return rc;
         std::vector<char> k_fp64_block(nb, 0);
@@ -717,7 +717,7 @@ static int build_blocks(Context& c) {
             const int T = B.ldn / 16;
             const int Tp = (T + 1) / 2;
             const int np = Tp * (Tp + 1) / 2;
-            const int per = c.tune.split_k_patches;  // at most 16 (8 consumer waves x 2)
+            const int per = c.tune.split_k_patches;  // at most 16 (one per wave)
             for (int q0 = 0; q0 < np; q0 += per) {
                 KWork w;
                 w.block = b;

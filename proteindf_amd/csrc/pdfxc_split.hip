@@ -343,13 +343,14 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
 // patches [q0, q1) of 16 x 16 tiles of one block (q = P (P + 1) / 2 + Q
 // covers tile rows 2P, 2P + 1 and columns 2Q, 2Q + 1; a diagonal patch's
 // upper tile is computed and not scattered), on the block's compact AO
-// columns.  Sixteen waves: 8 producers evaluate phi and V = (w vrho / 2) phi
-// + c . grad phi in FP64 for a 16-point slab and store their bf16 slices as
-// [s][column][16 points] (the 4-point group g of column c at
-// g ^ ((c >> 3) & 3)); 8 consumers hold up to NPW patches each in FP32
-// accumulators and run v_mfma_f32_16x16x16_bf16 on the slab (one K step),
-// the slice products with a + b < S.  Two slab buffers when they fit in
-// LDS, else one.  FP64 atomics into the dense lower triangle of K.
+// columns.  Sixteen waves, each both producer and consumer: all evaluate phi
+// and V = (w vrho / 2) phi + c . grad phi in FP64 for the next 16-point slab
+// and store their bf16 slices as [s][column][16 points] (the 4-point group g
+// of column c at g ^ ((c >> 3) & 3)), then each runs
+// v_mfma_f32_16x16x16_bf16 on the current slab (one K step) for its patch,
+// the hi x hi products and the corrections (a + b >= 1) in FP32
+// accumulators of their own.  Two slab buffers when they fit in LDS, else
+// one.  FP64 atomics into the dense lower triangle of K.
 // ---------------------------------------------------------------------------
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 
@@ -362,10 +363,10 @@ __device__ inline f32x4 mfma_bf16_k16(s16x4 a, s16x4 b, f32x4 c) {
 
 __device__ inline int kslab_at(int c, int g) { return c * 16 + ((g ^ (c >> 3)) & 3) * 4; }
 
-template <int S, int NPW>
-__device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, int ncols, int lane,
-                                          const unsigned short* s_slab, const int* s_cols,
-                                          double* __restrict__ K, int N) {
+template <int S, int NPW, typename Produce>
+__device__ inline void kmat_split_run(int pb, int pe, int nslab, int nbuf, int ncols, int lane,
+                                      const unsigned short* s_slab, const int* s_cols,
+                                      double* __restrict__ K, int N, Produce&& produce) {
     const int li = lane & 15, lq = lane >> 4;
     const int BUF = 2 * S * ncols * 16;  // bf16 per buffer: [phi, V][s][col][16]
     int offI[NPW], offJ[NPW];
@@ -395,8 +396,10 @@ __device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, i
     for (int m = 0; m < NPW; ++m)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[m][t] = acc_c[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    produce(0);
     __syncthreads();  // the first slab is in
     for (int sl = 0; sl < nslab; ++sl) {
+        if (nbuf == 2 && sl + 1 < nslab) produce(sl + 1);  // into the other buffer
         const unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
         const unsigned short* vv = phi + S * ncols * 16;
 #pragma unroll
@@ -425,8 +428,11 @@ __device__ inline void kmat_split_consume(int pb, int pe, int nslab, int nbuf, i
                 }
             }
         }
-        __syncthreads();                 // slab consumed
-        if (nbuf == 1) __syncthreads();  // single buffer: the next slab is in
+        __syncthreads();  // slab sl consumed (and, two buffers, slab sl + 1 in)
+        if (nbuf == 1) {
+            if (sl + 1 < nslab) produce(sl + 1);
+            __syncthreads();  // single buffer: the next slab is in
+        }
     }
     // scatter: register r of lane l is tile element (4 (l / 16) + r, l % 16)
 #pragma unroll
@@ -512,67 +518,46 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
     __syncthreads();
 
     const int nslab = (B.npts + 15) / 16;
-    if (wave >= 8) {
-        const int ptid = tid - 512;  // 8 producer waves, two per SIMD
-        auto produce = [&](int sl) {
-            unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
-            unsigned short* vv = phi + S * ncols * 16;
-            for (int it = ptid; it < nsh * 16; it += 512) {
-                const int p = it & 15;
-                const int k = it >> 4;
-                const int pp = sl * 16 + p;
-                // radial sums once, then phi and grad phi per component from the
-                // angular polynomial (keeps the producer within its registers)
-                double r3[3], R0, R1, R2;
-                const int l = rec_radial2(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], r3, R0, R1, R2);
-                const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
-                const int c0 = s_shcol[k];
-                for (int m = 0; m < 2 * l + 1; ++m) {
-                    const int c = c0 + m;
-                    if (c >= ncols) break;
-                    double f, fg[3], fh[6];
-                    angular2(l, m, r3, f, fg, fh);
-                    const double v = f * R0;
-                    const double vx = fg[0] * R0 + f * r3[0] * R1;
-                    const double vy = fg[1] * R0 + f * r3[1] * R1;
-                    const double vz = fg[2] * R0 + f * r3[2] * R1;
-                    const int off = kslab_at(c, p >> 2) + (p & 3);
-                    unsigned short h[S];
-                    split_bf16<S>(v, h);
+    // phi and V = a phi + c . grad phi of a 16-point slab, FP64, sliced into the slab buffer
+    auto produce = [&](int sl) {
+        unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
+        unsigned short* vv = phi + S * ncols * 16;
+        for (int it = tid; it < nsh * 16; it += 1024) {
+            const int p = it & 15;
+            const int k = it >> 4;
+            const int pp = sl * 16 + p;
+            // radial sums once, then phi and grad phi per component from the
+            // angular polynomial (keeps the producer within its registers)
+            double r3[3], R0, R1, R2;
+            const int l = rec_radial2(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], r3, R0, R1, R2);
+            const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
+            const int c0 = s_shcol[k];
+            for (int m = 0; m < 2 * l + 1; ++m) {
+                const int c = c0 + m;
+                if (c >= ncols) break;
+                double f, fg[3], fh[6];
+                angular2(l, m, r3, f, fg, fh);
+                const double v = f * R0;
+                const double vx = fg[0] * R0 + f * r3[0] * R1;
+                const double vy = fg[1] * R0 + f * r3[1] * R1;
+                const double vz = fg[2] * R0 + f * r3[2] * R1;
+                const int off = kslab_at(c, p >> 2) + (p & 3);
+                unsigned short h[S];
+                split_bf16<S>(v, h);
 #pragma unroll
-                    for (int s = 0; s < S; ++s) phi[(size_t)s * ncols * 16 + off] = h[s];
-                    split_bf16<S>(a * v + cx * vx + cy * vy + cz * vz, h);
+                for (int s = 0; s < S; ++s) phi[(size_t)s * ncols * 16 + off] = h[s];
+                split_bf16<S>(a * v + cx * vx + cy * vy + cz * vz, h);
 #pragma unroll
-                    for (int s = 0; s < S; ++s) vv[(size_t)s * ncols * 16 + off] = h[s];
-                }
-            }
-        };
-        // same barrier sequence as the consumers
-        produce(0);
-        __syncthreads();
-        for (int sl = 0; sl < nslab; ++sl) {
-            if (nbuf == 2) {
-                if (sl + 1 < nslab) produce(sl + 1);
-                __syncthreads();
-            } else {
-                __syncthreads();  // the consumers are done with slab sl
-                if (sl + 1 < nslab) produce(sl + 1);
-                __syncthreads();
+                for (int s = 0; s < S; ++s) vv[(size_t)s * ncols * 16 + off] = h[s];
             }
         }
-        return;
-    }
-    const int q8 = (W.t1 - W.t0 + 7) >> 3;
-    const int pstart = W.t0 + wave * q8;
-    const int pend = min(W.t1, pstart + q8);
-    const bool has = pstart < pend;
-    const int pb = has ? pstart : W.t1 - 1;
-    const int pe = has ? pend : pb;
-    // the host keeps items within 8 consumer waves x 2 patches
-    if (has && pend - pstart == 2)
-        kmat_split_consume<S, 2>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp);
-    else
-        kmat_split_consume<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp);
+    };
+    // the item's patches over the 16 waves, one each; every wave runs the
+    // same production and barrier sequence
+    const bool has = W.t0 + wave < W.t1;
+    const int pb = has ? W.t0 + wave : W.t1 - 1;
+    const int pe = has ? pb + 1 : pb;
+    kmat_split_run<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp, produce);
 }
 
 // layout self test: A(i, k) = ((3 i + 5 k) % 11) - 5, B(k, j) = ((7 k + 2 j) % 13) - 6;
