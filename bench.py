@@ -121,6 +121,14 @@ DTYPES = {"fp64": "f64", "bf16x2": "bf16 slices, f32 accumulate, f64 AO/function
           "bf16x3": "bf16 slices, f32 accumulate, f64 AO/functional/sums",
           "bf16x3k": "f64 rho pass; K pass bf16 slices, f32 accumulate; f64 AO/functional/sums"}
 E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 build
+# HBM bytes per launch of a dominant kernel, measured in separate rocprofv3 --pmc
+# runs (FETCH_SIZE x 2.00 and WRITE_SIZE x 1.00, the factors a plain copy kernel
+# of known size gives) of this benchmark with the default launch shapes; the
+# counters are not read inside the timed run.
+TRAFFIC_MEASURED = {
+    ("gly10", "bf16x3k", "k_kmat_split"): (115043.5 * 1024 * 2.00 + 3232877.25 * 1024 * 1.00,
+                                           "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; profiles/gly10_kernels.md"),
+}
 
 
 def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True):
@@ -352,8 +360,11 @@ def main():
     dom = int(np.argmax(k_ms[:2]))
     name, peak = KERNELS[args.precision][dom]
     achieved = st.flops / 2.0 / (k_ms[dom] / 1e3) / 1e12
+    tr = TRAFFIC_MEASURED.get((args.workload, args.precision, name))
     roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak,
-            "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
+            "unit": "TFLOP/s", "frac": achieved / peak, "traffic": tr[0] if tr else None}
+    if tr:
+        roof["traffic_source"] = tr[1]
 
     cpu = None
     if not args.no_cpu_baseline:
