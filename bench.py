@@ -99,9 +99,9 @@ def cpu_baseline(info, grid, PA, flops_total, target_s=12.0):
         t0 = time.perf_counter()
         oracle.calc_xc_rks(aos, grid[idx], PA, n_threads=threads)
         t = time.perf_counter() - t0
-        if t > target_s / 4 or n >= len(grid):
+        if t > 0.6 * target_s or n >= len(grid):
             break
-        n = int(min(len(grid), n * max(2.0, 0.8 * target_s / max(t, 1e-3))))
+        n = int(min(len(grid), n * max(2.0, 1.05 * target_s / max(t, 1e-3))))
     t_full = t * len(grid) / len(idx)
     return {"value": flops_total / t_full / 1e9, "unit": "GFLOP/s", "cores": threads, "kind": "port",
             "sample": f"{len(idx)} random grid points of {len(grid)} in {t:.1f} s, "
