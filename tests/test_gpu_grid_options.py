@@ -186,3 +186,25 @@ def test_build_from_the_reference_pdfparam():
         say(fn, "dE", abs(xc.XC_energy_ - e_ref), "max dK", dK)
         assert abs(xc.XC_energy_ - e_ref) <= 1e-10 and dK <= 1e-10
     cg.ctx.close()
+
+
+def test_option_errors_and_order_floor():
+    mol = molecules.water_cluster(1, seed=3)
+    info = calc_grid.orbital_info(mol)
+    ctx = engine.XcContext(info)
+    say("grid option errors")
+    for bad in (dict(radial=5), dict(gc_mapping=7), dict(pruning=3), dict(partitioning=2), dict(radii=4),
+                dict(n_rad=0)):
+        with pytest.raises(engine.PdfxcError) as ei:
+            ctx.generate_grid_ex(mol.Z, mol.xyz, _engine_opts({**OPTION_SETS["coarse-becke"], **bad}))
+        assert ei.value.code == engine.ERROR_CODES["ARGUMENT"], bad
+    with pytest.raises(engine.PdfxcError) as ei:
+        ctx.generate_grid_ex(np.array([8, 1, 1, 19]), np.vstack([mol.xyz, [5.0, 5.0, 5.0]]),
+                             _engine_opts(OPTION_SETS["coarse-becke"]))
+    assert ei.value.code == engine.ERROR_CODES["UNSUPPORTED"]  # no radius past Ar
+    # an unsupported n_ang is floored to the largest rule below it, as the reference does
+    o = dict(n_rad=30, n_ang=300, pruning=0, partitioning=0)
+    n = ctx.generate_grid_ex(mol.Z, mol.xyz, _engine_opts(o))
+    ref, _ = oracle.generate_grid_ex(mol.Z, mol.xyz, oracle.grid_opts(**{**o, "n_ang": 266}))
+    ctx.close()
+    assert n == len(ref)
