@@ -459,20 +459,17 @@ static int build_blocks(Context& c) {
                 mx[d] = std::max(mx[d], v[d]);
             }
         }
-        double ctr[3], rad = 0.0;
-        for (int d = 0; d < 3; ++d) ctr[d] = 0.5 * (mn[d] + mx[d]);
-        for (int64_t k = t.lo; k < t.hi; ++k) {
-            const int64_t p = order[k];
-            const double dx = c.gx[p] - ctr[0], dy = c.gy[p] - ctr[1], dz = c.gz[p] - ctr[2];
-            rad = std::max(rad, std::sqrt(dx * dx + dy * dy + dz * dz));
-        }
-        rad *= 1.0 + 1e-12;
         t.nao = 0;
         for (int s = 0; s < nsh; ++s) {
             const Shell& S = c.shells[s];
-            const double dx = S.center[0] - ctr[0], dy = S.center[1] - ctr[1], dz = S.center[2] - ctr[2];
-            const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
-            const double dmin = std::max(0.0, dist - rad);
+            // distance from the shell's centre to the block's bounding box: no
+            // point of the block is nearer (shrunk by a relative 1e-12 for rounding)
+            double d2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double e = std::max(0.0, std::max(mn[d] - S.center[d], S.center[d] - mx[d]));
+                d2 += e * e;
+            }
+            const double dmin = std::sqrt(d2) * (1.0 - 1e-12);
             // the reference skips a primitive with a r^2 > 30: a shell whose most
             // diffuse primitive is cut at the nearest point is exactly zero here
             if (S.exps[0] * dmin * dmin > 30.0) continue;
