@@ -87,8 +87,10 @@ typedef struct pdfxc_stats {
 /* Launch shapes of the build kernels.  rho_shift (0..3) / split_rho_shift
  * (0..2): move every block that many classes toward fewer points per
  * workgroup (rho pass); k_tiles (8..96): FP64 K pass work item size in lower
- * 16 x 16 tiles for blocks of more than 32 tiles; split_k_patches (4..16):
- * split K pass work item size in 2 x 2 tile patches.  Defaults 0, 96, 0, 16.
+ * 16 x 16 tiles for blocks of more than 32 tiles; split_k_patches (4..32):
+ * split K pass work item size in 2 x 2 tile patches (up to 16: workgroups of
+ * 16 waves, one patch each; up to 24: 12 waves of two; up to 32: 8 waves of
+ * four).  Defaults 0, 96, 0, 24.
  * ms: the build time measured for it (pdfxc_autotune), else 0. */
 typedef struct pdfxc_tuning {
     int rho_shift;

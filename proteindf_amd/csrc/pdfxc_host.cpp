@@ -714,7 +714,7 @@ static int build_blocks(Context& c) {
             const int T = B.ldn / 16;
             const int Tp = (T + 1) / 2;
             const int np = Tp * (Tp + 1) / 2;
-            const int per = c.tune.split_k_patches;  // at most 16 (one per wave)
+            const int per = c.tune.split_k_patches;  // at most 16, 24 or 32 (1, 2 or 4 per wave)
             for (int q0 = 0; q0 < np; q0 += per) {
                 KWork w;
                 w.block = b;
@@ -749,7 +749,10 @@ static int build_blocks(Context& c) {
                 ++c.n_k_fp64_blocks;
             }
         }
-        if (!skw.empty()) c.sk_launches.push_back(LaunchRange{0, 0, (int)skw.size(), sk_lds});
+        // launch class = workgroup shape for the item size (16 x 1, 12 x 2 or
+        // 8 x 4 waves x patches), so a batch keeps every item's shape
+        const int kshape = c.tune.split_k_patches <= 16 ? 0 : c.tune.split_k_patches <= 24 ? 1 : 2;
+        if (!skw.empty()) c.sk_launches.push_back(LaunchRange{kshape, 0, (int)skw.size(), sk_lds});
         // FP64 K items of those blocks, appended to the FP64 work list
         for (int cls = 0; cls < 2; ++cls) {
             LaunchRange r{cls, (int)kall.size(), 0, klds[cls]};
@@ -1715,7 +1718,7 @@ int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* t) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c || !t) return arg_error("pdfxc_set_tuning: bad arguments");
     if (t->rho_shift < 0 || t->rho_shift > 3 || t->split_rho_shift < 0 || t->split_rho_shift > 2 ||
-        t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 16)
+        t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32)
         return arg_error("pdfxc_set_tuning: shape out of range");
     c->tune = *t;
     if (c->n_pts == 0) return PDFXC_OK;
@@ -1736,13 +1739,13 @@ int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
     std::vector<pdfxc_tuning> cand;
     if (c->opts.precision == PDFXC_FP64) {
         for (int rs : {0, 1})
-            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 16, 0.0});
+            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 24, 0.0});
     } else if (prec_split_rho(c->opts.precision)) {
         for (int rs : {0, 1})
-            for (int kp : {16, 8}) cand.push_back(pdfxc_tuning{0, 96, rs, kp, 0.0});
+            for (int kp : {16, 24, 32}) cand.push_back(pdfxc_tuning{0, 96, rs, kp, 0.0});
     } else {  // FP64 rho pass, split K pass
         for (int rs : {0, 1})
-            for (int kp : {16, 8}) cand.push_back(pdfxc_tuning{rs, 96, 0, kp, 0.0});
+            for (int kp : {16, 24, 32}) cand.push_back(pdfxc_tuning{rs, 96, 0, kp, 0.0});
     }
     pdfxc_tuning best = cand[0];
     best.ms = 1e30;

@@ -81,7 +81,7 @@ struct DeviceShell {
 
 struct Context {
     pdfxc_opts opts;
-    pdfxc_tuning tune{0, 96, 0, 16, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
+    pdfxc_tuning tune{0, 96, 0, 24, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
     int n_aos = 0;
     std::vector<Shell> shells;
     int device = 0;

@@ -343,11 +343,12 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
 // patches [q0, q1) of 16 x 16 tiles of one block (q = P (P + 1) / 2 + Q
 // covers tile rows 2P, 2P + 1 and columns 2Q, 2Q + 1; a diagonal patch's
 // upper tile is computed and not scattered), on the block's compact AO
-// columns.  Sixteen waves, each both producer and consumer: all evaluate phi
-// and V = (w vrho / 2) phi + c . grad phi in FP64 for the next 16-point slab
-// and store their bf16 slices as [s][column][16 points] (the 4-point group g
-// of column c at g ^ ((c >> 3) & 3)), then each runs
-// v_mfma_f32_16x16x16_bf16 on the current slab (one K step) for its patch,
+// columns.  NW waves (16, 12 or 8), each both producer and consumer: all
+// evaluate phi and V = (w vrho / 2) phi + c . grad phi in FP64 for the next
+// 16-point slab and store their bf16 slices as [s][column][16 points] (the
+// 4-point group g of column c at g ^ ((c >> 3) & 3)), then each runs
+// v_mfma_f32_16x16x16_bf16 on the current slab (one K step) for its NPW
+// patches (1, 2 or 4: fewer, larger items produce each block fewer times),
 // the hi x hi products and the corrections (a + b >= 1) in FP32
 // accumulators of their own.  Two slab buffers when they fit in LDS, else
 // one.  FP64 atomics into the dense lower triangle of K.
@@ -454,8 +455,8 @@ __device__ inline void kmat_split_run(int pb, int pe, int nslab, int nbuf, int n
     }
 }
 
-template <int S>
-__global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+template <int S, int NW, int NPW>
+__global__ __launch_bounds__(NW * 64) void k_kmat_split(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                                     const double* __restrict__ pts, int64_t n_slots,
                                                     const double* __restrict__ shrec, int RS,
                                                     const int* __restrict__ bshells, const int* __restrict__ bshell_col,
@@ -480,8 +481,9 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
     int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
     int* s_shcol = s_cols + ncols;
     int* s_shid = s_shcol + nsh;
-    for (int c = tid; c < ncols; c += 1024) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
-    if (wave == 8) {  // shells this item needs, in (l, nprim) order
+    constexpr int NT = NW * 64;
+    for (int c = tid; c < ncols; c += NT) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
+    if (wave == NW / 2) {  // shells this item needs, in (l, nprim) order
         int base = 0;
         for (int i0 = 0; i0 < B.nshell; i0 += 64) {
             const int i = i0 + lane;
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
             base += __popcll(msk);
         }
     }
-    for (int q = tid; q < kBlockPts; q += 1024) {
+    for (int q = tid; q < kBlockPts; q += NT) {
         const int64_t slot = B.pt_begin + q;
         s_pt[q][0] = pts[slot];
         s_pt[q][1] = pts[n_slots + slot];
@@ -508,10 +510,10 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
     }
     {
         uint4* z = reinterpret_cast<uint4*>(s_dyn);  // columns past the AOs stay zero
-        for (int i = tid; i < slab_dbl / 2; i += 1024) z[i] = uint4{0u, 0u, 0u, 0u};
+        for (int i = tid; i < slab_dbl / 2; i += NT) z[i] = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
-    for (int idx = tid; idx < nsh * RS; idx += 1024) {
+    for (int idx = tid; idx < nsh * RS; idx += NT) {
         const int k = idx / RS;
         s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
     }
@@ -522,7 +524,7 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
     auto produce = [&](int sl) {
         unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
         unsigned short* vv = phi + S * ncols * 16;
-        for (int it = tid; it < nsh * 16; it += 1024) {
+        for (int it = tid; it < nsh * 16; it += NT) {
             const int p = it & 15;
             const int k = it >> 4;
             const int pp = sl * 16 + p;
@@ -552,12 +554,12 @@ __global__ __launch_bounds__(1024) void k_kmat_split(const KWork* __restrict__ w
             }
         }
     };
-    // the item's patches over the 16 waves, one each; every wave runs the
-    // same production and barrier sequence
-    const bool has = W.t0 + wave < W.t1;
-    const int pb = has ? W.t0 + wave : W.t1 - 1;
-    const int pe = has ? pb + 1 : pb;
-    kmat_split_run<S, 1>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp, produce);
+    // the item's patches over the waves, NPW each; every wave runs the same
+    // production and barrier sequence
+    const bool has = W.t0 + wave * NPW < W.t1;
+    const int pb = has ? W.t0 + wave * NPW : W.t1 - 1;
+    const int pe = has ? min(pb + NPW, W.t1) : pb;
+    kmat_split_run<S, NPW>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp, produce);
 }
 
 // layout self test: A(i, k) = ((3 i + 5 k) % 11) - 5, B(k, j) = ((7 k + 2 j) % 13) - 6;
@@ -627,13 +629,26 @@ static int rho_split_one(Context& c, const LaunchRange& r) {
     return check("k_rho_split");
 }
 
-template <int S>
-static int kmat_split_one(Context& c, const LaunchRange& r) {
-    hipLaunchKernelGGL((k_kmat_split<S>), dim3(r.count), dim3(1024), r.lds, c.stream,
+template <int S, int NW, int NPW>
+static int kmat_split_launch(Context& c, const LaunchRange& r) {
+    hipLaunchKernelGGL((k_kmat_split<S, NW, NPW>), dim3(r.count), dim3(NW * 64), r.lds, c.stream,
                        reinterpret_cast<const KWork*>(c.d_sk_work) + r.offset, c.d_blocks, c.d_pts, c.n_slots,
                        c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col, c.d_block_cols,
                        c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
     return check("k_kmat_split");
+}
+
+// the workgroup shape of a launch class: items of up to 16 patches, 16 waves
+// of one patch (127 VGPRs); up to 24, 12 waves of two (167); up to 32, 8
+// waves of four (247)
+template <int S>
+static int kmat_split_one(Context& c, const LaunchRange& r) {
+    switch (r.cls) {
+        case 0: return kmat_split_launch<S, 16, 1>(c, r);
+        case 1: return kmat_split_launch<S, 12, 2>(c, r);
+        case 2: return kmat_split_launch<S, 8, 4>(c, r);
+        default: set_error("k_kmat_split: bad class"); return PDFXC_ERR_ARGUMENT;
+    }
 }
 
 template <int S>
@@ -673,8 +688,8 @@ int launch_split_kmat(Context& c) {
 // workgroup less the kernel's static LDS, read from the code object
 int split_kmat_max_dyn_lds(int S, size_t* out) {
     hipFuncAttributes a;
-    const hipError_t e = (S == 3) ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<3>))
-                                  : hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<2>));
+    const hipError_t e = (S == 3) ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<3, 16, 1>))
+                                  : hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<2, 16, 1>));
     if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(k_kmat_split)");
     *out = (size_t)160 * 1024 - a.sharedSizeBytes;
     return PDFXC_OK;

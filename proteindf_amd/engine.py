@@ -331,7 +331,7 @@ class XcContext:
     def save_grid_matrix(self, path, n_rho_cols=4):
         check(lib().pdfxc_save_grid_matrix(self.h, _path(path), n_rho_cols))
 
-    def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=16):
+    def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=24):
         t = Tuning(rho_shift, k_tiles, split_rho_shift, split_k_patches, 0.0)
         check(lib().pdfxc_set_tuning(self.h, ctypes.byref(t)))
 

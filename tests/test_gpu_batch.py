@@ -37,6 +37,9 @@ def test_batch_equals_individual_builds(prec):
             ctx.set_grid(grid)
         else:
             ctx.generate_grid(mol.Z, mol.xyz, 40, 86)
+        # each molecule its own launch shapes: the batch keeps every item's
+        # workgroup shape (split K: 16 x 1, 12 x 2, 8 x 4 waves x patches)
+        ctx.set_tuning(k_tiles=(96, 48, 24)[k], split_k_patches=(16, 24, 32)[k])
         E, K, _ = ctx.build_rks(guess.pack_lower(P))
         single.append((E, K))
         ctxs.append(ctx)

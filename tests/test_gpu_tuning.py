@@ -33,16 +33,20 @@ def test_every_shape_gives_the_same_build(prec):
     P = guess.pack_lower(PA)
     E0, K0, _ = ctx.build_rks(P)
     shapes = [dict(rho_shift=1), dict(rho_shift=3, k_tiles=8), dict(k_tiles=24), dict(split_rho_shift=1),
-              dict(split_rho_shift=2, split_k_patches=4), dict(split_k_patches=8)]
+              dict(split_rho_shift=2, split_k_patches=4), dict(split_k_patches=8),
+              # 12 waves of two patches and 8 waves of four, full and partial items
+              dict(split_k_patches=24), dict(split_k_patches=21), dict(split_k_patches=32),
+              dict(split_k_patches=29)]
     for sh in shapes:
         ctx.set_tuning(**sh)
         E, K, _ = ctx.build_rks(P)
         say(prec, sh, "dE", abs(E - E0), "max dK", np.abs(K - K0).max())
         assert abs(E - E0) <= 1e-12 * abs(E0)
         np.testing.assert_allclose(K, K0, rtol=0, atol=1e-12)
-    with pytest.raises(engine.PdfxcError) as ei:
-        ctx.set_tuning(k_tiles=200)
-    assert ei.value.code == engine.ERROR_CODES["ARGUMENT"]
+    for bad in (dict(k_tiles=200), dict(split_k_patches=33), dict(split_k_patches=3)):
+        with pytest.raises(engine.PdfxcError) as ei:
+            ctx.set_tuning(**bad)
+        assert ei.value.code == engine.ERROR_CODES["ARGUMENT"]
     ctx.close()
 
 
@@ -72,3 +76,34 @@ def test_autotune_keeps_a_candidate_and_builds_correctly():
     e_ref, K_ref, _ = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), grid, PA)
     assert abs(E - e_ref) <= 1e-10
     assert np.abs(guess.unpack_lower(K, n) - K_ref).max() <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["bf16x3k", "bf16x2"])
+def test_split_k_shapes_on_wide_blocks(prec):
+    # blocks of more than 160 AO columns: items of 16 patches split such a
+    # block, items of 24 and 32 (12 waves of two patches, 8 waves of four)
+    # take it whole or nearly; every shape sums the same slab products
+    mol = molecules.polyglycine_helix(3, seed=5)
+    info = calc_grid.orbital_info(mol)
+    PA = 0.5 * guess.density_guess(mol, info, seed=5, eps=0.05)
+    ctx = engine.XcContext(info, engine.default_opts(precision=engine.PRECISIONS[prec]))
+    ctx.generate_grid(mol.Z, mol.xyz, 40, 146)
+    st = ctx.stats()
+    say(prec, "n_aos", info.n_aos, "max N_b", st.max_nao, "mean N_b", round(st.mean_nao, 1))
+    assert st.max_nao > 160
+    P = guess.pack_lower(PA)
+    E0, K0, _ = ctx.build_rks(P)
+    for kp in (24, 21, 32, 29, 16):
+        ctx.set_tuning(split_k_patches=kp)
+        E, K, _ = ctx.build_rks(P)
+        say(prec, "split_k_patches", kp, "dE", abs(E - E0), "max dK", np.abs(K - K0).max())
+        assert abs(E - E0) <= 1e-12 * abs(E0)
+        np.testing.assert_allclose(K, K0, rtol=0, atol=1e-12)
+    ctx.close()
+    ref = engine.XcContext(info, engine.default_opts())
+    ref.generate_grid(mol.Z, mol.xyz, 40, 146)
+    Ef, Kf, _ = ref.build_rks(P)
+    ref.close()
+    e_tol, k_tol = {"bf16x3k": (1e-10, 1e-7), "bf16x2": (2e-4, 2e-5)}[prec]
+    say(prec, "vs FP64: dE", abs(E0 - Ef), "max dK", np.abs(K0 - Kf).max())
+    assert abs(E0 - Ef) <= e_tol and np.abs(K0 - Kf).max() <= k_tol
