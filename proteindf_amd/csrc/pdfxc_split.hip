@@ -38,21 +38,23 @@ __device__ inline f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
 __device__ inline unsigned short bf16_bits(__bf16 h) { return __builtin_bit_cast(unsigned short, h); }
 
 // slices of x: s[0] = bf16(x), s[1] = bf16 of what s[0] leaves, and the last
-// slice from the exact FP64 residual x - s[0] - ... .  The middle slice of
-// S = 3 comes from f - s[0] in FP32 (f = float(x); exact, as s[0] keeps the top
-// 8 of f's 24 bits), which saves the FP64 round trips of a slice.
+// slice from what the others leave plus e = float(x - f), the FP64 part that
+// f = float(x) drops.  f - s[0] and r - s[1] are exact in FP32 (each slice
+// keeps the top 8 bits of a value with at most 24), so only the last sum
+// rounds, at 2^-24 of a value below 2^-16 |x|; one FP64 subtraction per x.
 template <int S>
 __device__ inline void split_bf16(double x, unsigned short* out) {
     const float f = (float)x;
+    const float e = (float)(x - (double)f);
     const __bf16 h0 = (__bf16)f;
     out[0] = bf16_bits(h0);
-    const double r1 = x - (double)(float)h0;  // exact
+    const float r1 = f - (float)h0;  // exact
     if constexpr (S == 2) {
-        out[1] = bf16_bits((__bf16)(float)r1);
+        out[1] = bf16_bits((__bf16)(r1 + e));
     } else {
-        const __bf16 h1 = (__bf16)(f - (float)h0);
+        const __bf16 h1 = (__bf16)r1;
         out[1] = bf16_bits(h1);
-        out[2] = bf16_bits((__bf16)(float)(r1 - (double)(float)h1));  // exact residual, rounded
+        out[2] = bf16_bits((__bf16)((r1 - (float)h1) + e));  // r1 - h1 exact
     }
 }
 
