@@ -121,17 +121,21 @@ DTYPES = {"fp64": "f64", "bf16x2": "bf16 slices, f32 accumulate, f64 AO/function
           "bf16x3": "bf16 slices, f32 accumulate, f64 AO/functional/sums",
           "bf16x3k": "f64 rho pass; K pass bf16 slices, f32 accumulate; f64 AO/functional/sums"}
 E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 build
-# HBM bytes per launch of a dominant kernel, measured in separate rocprofv3 --pmc
-# runs (FETCH_SIZE x 2.00 and WRITE_SIZE x 1.00, the factors a plain copy kernel
-# of known size gives) of this benchmark with the default launch shapes; the
-# counters are not read inside the timed run.
+# HBM bytes per build of a dominant kernel (the sum over its class launches),
+# measured in separate rocprofv3 --pmc runs (FETCH_SIZE x 2.00 and WRITE_SIZE
+# x 1.00, the factors a plain copy kernel of known size gives) of this benchmark
+# with the default launch shapes; the counters are not read inside the timed run.
 TRAFFIC_MEASURED = {
-    ("gly10", "bf16x3k", "k_kmat_split"): (110440.65 * 1024 * 2.00 + 3095591.0 * 1024 * 1.00,
+    ("gly10", "bf16x3k", "k_kmat_split"): (82418.6 * 1024 * 2.00 + 3095591.0 * 1024 * 1.00,
                                            "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; profiles/gly10_kernels.md"),
+    ("gly10", "bf16x3k", "k_rho"): ((90370.46 + 28065.10 + 11098.0) * 1024 * 2.00
+                                    + (33056.0 + 4412.0 + 5154.0) * 1024 * 1.00,
+                                    "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches; "
+                                    "profiles/gly10_kernels.md"),
 }
 
 
-def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True):
+def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True, k_patches=0):
     """Build a context with the given precision on the same grid, let the
     autotuner pick its launch shapes (set-up, untimed), time steps builds on
     HBM-resident P/K, return (ms per build, stats, E, K dense, tuning)."""
@@ -140,7 +144,9 @@ def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune
     ctx.set_grid(grid)
     dP, dK, dE = hip.malloc(PA.nbytes), hip.malloc(PA.nbytes), hip.malloc(8)
     hip.h2d(dP, PA)
-    tune = ctx.autotune(dP, dK, dE, reps=2).as_dict() if autotune else None
+    if k_patches:
+        ctx.set_tuning(split_k_patches=k_patches)
+    tune = ctx.autotune(dP, dK, dE, reps=2).as_dict() if autotune and not k_patches else None
     hip.memset(dK, PA.nbytes)
     for _ in range(warmup):
         ctx.build_rks_device(dP, dK, dE)
@@ -292,6 +298,9 @@ def main():
     ap.add_argument("--modes", default="fp64,bf16x2,bf16x3,bf16x3k", help="comma list of modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--split-k-patches", type=int, default=0, metavar="N",
+                    help="split K pass items of N patches (4..32: 16 waves x 1 up to 16, 12 x 2 up to 24, "
+                         "8 x 4 up to 32) instead of the autotuner's choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -306,6 +315,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; the build does not "
                                    "shard across GPUs (use --gpus 1)"}), flush=True)
+        sys.exit(2)
+    if args.split_k_patches and not 4 <= args.split_k_patches <= 32:
+        print(json.dumps({"error": "--split-k-patches takes 4..32"}), flush=True)
         sys.exit(2)
     if args.workload == "h2o_batch":
         bench_batch(args)
@@ -335,7 +347,7 @@ def main():
     res = {}
     for m in modes:
         dt, st, E, K, tune = time_mode(hip, info, grid, PA, MODES[m], wl["ao_screen"], args.steps, args.warmup,
-                                       not args.no_autotune)
+                                       not args.no_autotune, args.split_k_patches)
         res[m] = dict(dt=dt, st=st, E=E, K=K, k_ms=list(st.last_kernel_ms), tune=tune)
         print(f"[bench] {m:7s} E_xc {E:.10f}  build {dt * 1e3:.2f} ms  kernels ms: "
               + ", ".join(f"{t:.2f}" for t in res[m]["k_ms"][:3]), file=sys.stderr, flush=True)

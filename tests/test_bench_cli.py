@@ -1,5 +1,6 @@
-"""bench.py's command line without a GPU: more than one GPU is refused with a
-message (the build does not shard across GPUs), and the mode tables agree."""
+"""bench.py's command line without a GPU: more than one GPU and an out-of-range
+split K item size are refused with a message (the build does not shard across
+GPUs), and the mode tables agree."""
 import json
 import subprocess
 import sys
@@ -14,6 +15,13 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_split_k_item_size_out_of_range_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--split-k-patches", "40"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "4..32" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
 
 
 def test_mode_tables_cover_every_mode():
