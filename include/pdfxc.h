@@ -86,7 +86,8 @@ typedef struct pdfxc_stats {
 
 /* Launch shapes of the build kernels.  rho_shift (0..3) / split_rho_shift
  * (0..2): move every block that many classes toward fewer points per
- * workgroup (rho pass); k_tiles (8..96): FP64 K pass work item size in lower
+ * workgroup (rho pass; at rho_shift 0 a block of 129..256 padded AO columns
+ * takes 64 points per workgroup when they fit in LDS, else 32); k_tiles (8..96): FP64 K pass work item size in lower
  * 16 x 16 tiles for blocks of more than 32 tiles; split_k_patches (4..32):
  * split K pass work item size in 2 x 2 tile patches (up to 16: workgroups of
  * 16 waves, one patch each; up to 24: 12 waves of two; up to 32: 8 waves of

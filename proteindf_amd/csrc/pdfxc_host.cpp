@@ -550,26 +550,30 @@ static int build_blocks(Context& c) {
     c.mean_nao = nb ? sum_nao / nb : 0.0;
 
     // k_rho work lists by template class: ldn <= 128 (64 points per
-    // workgroup), <= 256 (32), <= 512, <= 1024 (16); the tuning may move every
-    // block up by rho_shift classes (fewer points per workgroup)
+    // workgroup), <= 256 (64 when the plane fits in LDS: class 4, else 32),
+    // <= 512, <= 1024 (16); the tuning may move every block up by rho_shift
+    // classes (fewer points per workgroup; ldn <= 256 then takes 16)
     std::vector<int> rho_work;
     c.rho_launches.clear();
     const int RS = c.rec_stride;
-    auto rho_class = [&](int ldn) {
-        int base = 0;
-        while ((128 << base) < ldn) ++base;
-        return std::min(3, base + c.tune.rho_shift);
+    auto rho_lds = [&](const BlockDesc& B, size_t pb) {
+        return sizeof(double) * (std::max<size_t>((size_t)B.ldn * pb, 4 * 512) + (size_t)B.nshell * RS) +
+               sizeof(int) * (size_t)(B.ldn + 2 * B.nshell);
     };
-    for (int cls = 0; cls < 4; ++cls) {
+    auto rho_class = [&](const BlockDesc& B) {
+        int base = 0;
+        while ((128 << base) < B.ldn) ++base;
+        const int k = std::min(3, base + c.tune.rho_shift);
+        return (k == 1 && c.tune.rho_shift == 0 && rho_lds(B, 64) <= (size_t)kMaxDynLds) ? 4 : k;
+    };
+    for (int cls = 0; cls < 5; ++cls) {
         const size_t pb = (size_t)rho_points_per_workgroup(cls);
         LaunchRange r{cls, (int)rho_work.size(), 0, 0};
         size_t maxdyn = 0;
         for (int b = 0; b < nb; ++b) {
             const BlockDesc& B = c.blocks[b];
-            if (rho_class(B.ldn) == cls) {
-                const size_t lds = sizeof(double) * (std::max<size_t>((size_t)B.ldn * pb, 4 * 512) +
-                                                     (size_t)B.nshell * RS) +
-                                   sizeof(int) * (size_t)(B.ldn + 2 * B.nshell);
+            if (rho_class(B) == cls) {
+                const size_t lds = rho_lds(B, pb);
                 if (lds > (size_t)kMaxDynLds) {
                     set_error("set_grid: a block with " + std::to_string(B.nao) + " AOs and " +
                               std::to_string(B.nshell) + " shells needs more LDS than a workgroup has "
@@ -1646,13 +1650,14 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
     for (const LaunchRange& r : c->rho_launches) {
         LaunchRange g = r;
         g.lds = 0;
+        if (g.cls == 4) g.cls = 1;  // the 64-point rho class of ldn <= 256 blocks: the gradient's ldn <= 256 kernel
         std::vector<int> ids(r.count);
         PDFXC_HIP(hipMemcpy(ids.data(), c->d_rho_work + r.offset, sizeof(int) * r.count, hipMemcpyDeviceToHost));
         for (int b : ids) {
             const BlockDesc& B = c->blocks[b];
             const size_t lds = sizeof(double) * (2 * (size_t)B.ldn * 16 + (size_t)B.nshell * RS + 3 * (size_t)na) +
                                sizeof(int) * (size_t)(B.ldn + 3 * B.nshell);
-            if (r.cls > 2 || lds > (size_t)kMaxDynLds) {
+            if (g.cls > 2 || lds > (size_t)kMaxDynLds) {
                 set_error("pdfxc_gradient_rks: a block with " + std::to_string(B.nao) + " AOs and " +
                           std::to_string(na) + " atoms does not fit the gradient kernel (raise ao_screen)");
                 return PDFXC_ERR_UNSUPPORTED;

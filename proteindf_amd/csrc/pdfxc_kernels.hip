@@ -792,8 +792,10 @@ static int rho_one(Context& c, const LaunchRange& r) {
     return check_launch("k_rho");
 }
 
-// k_rho classes by padded block width ldn: <= 128, 256, 512, 1024
-int rho_points_per_workgroup(int cls) { return cls == 0 ? 64 : cls == 1 ? 32 : 16; }
+// k_rho classes by padded block width ldn: <= 128, 256, 512, 1024, and class
+// 4, ldn <= 256 at 64 points per workgroup (half the workgroups per block, so
+// each block's P_bb and shell records are read half as often)
+int rho_points_per_workgroup(int cls) { return (cls == 0 || cls == 4) ? 64 : cls == 1 ? 32 : 16; }
 
 int launch_rho(Context& c, const LaunchRange& r) {
     if (r.count <= 0) return PDFXC_OK;
@@ -802,6 +804,7 @@ int launch_rho(Context& c, const LaunchRange& r) {
         case 1: return rho_one<32, 8, 2, 4>(c, r);
         case 2: return rho_one<16, 8, 4, 4>(c, r);
         case 3: return rho_one<16, 8, 8, 2>(c, r);
+        case 4: return rho_one<64, 8, 2, 4>(c, r);
         default: set_error("k_rho: bad class"); return PDFXC_ERR_ARGUMENT;
     }
 }
