@@ -560,11 +560,13 @@ static int build_blocks(Context& c) {
         return sizeof(double) * (std::max<size_t>((size_t)B.ldn * pb, 4 * 512) + (size_t)B.nshell * RS) +
                sizeof(int) * (size_t)(B.ldn + 2 * B.nshell);
     };
+    size_t rho64_lds_max = 0;
+    if (int rc = rho64_max_dyn_lds(&rho64_lds_max)) return rc;
     auto rho_class = [&](const BlockDesc& B) {
         int base = 0;
         while ((128 << base) < B.ldn) ++base;
         const int k = std::min(3, base + c.tune.rho_shift);
-        return (k == 1 && c.tune.rho_shift == 0 && rho_lds(B, 64) <= (size_t)kMaxDynLds) ? 4 : k;
+        return (k == 1 && c.tune.rho_shift == 0 && rho_lds(B, 64) <= rho64_lds_max) ? 4 : k;
     };
     for (int cls = 0; cls < 5; ++cls) {
         const size_t pb = (size_t)rho_points_per_workgroup(cls);
@@ -574,7 +576,7 @@ static int build_blocks(Context& c) {
             const BlockDesc& B = c.blocks[b];
             if (rho_class(B) == cls) {
                 const size_t lds = rho_lds(B, pb);
-                if (lds > (size_t)kMaxDynLds) {
+                if (lds > (cls == 4 ? rho64_lds_max : (size_t)kMaxDynLds)) {
                     set_error("set_grid: a block with " + std::to_string(B.nao) + " AOs and " +
                               std::to_string(B.nshell) + " shells needs more LDS than a workgroup has "
                               "(raise ao_screen)");

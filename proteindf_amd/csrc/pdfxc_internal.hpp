@@ -179,6 +179,9 @@ int hip_fail(hipError_t e, const char* what);
 // kernel launchers (pdfxc_kernels.hip)
 int launch_rho(Context& c, const LaunchRange& r);
 int rho_points_per_workgroup(int cls);
+// dynamic LDS the 64-point k_rho class of ldn <= 256 blocks may take: 160 KiB
+// less the kernel's static LDS, read from the code object
+int rho64_max_dyn_lds(size_t* out);
 int launch_xc_points(Context& c, int xc_type, double* d_ewf = nullptr);
 int launch_kmat(Context& c, const LaunchRange& r);
 int configure_kernels();

@@ -797,6 +797,14 @@ static int rho_one(Context& c, const LaunchRange& r) {
 // each block's P_bb and shell records are read half as often)
 int rho_points_per_workgroup(int cls) { return (cls == 0 || cls == 4) ? 64 : cls == 1 ? 32 : 16; }
 
+int rho64_max_dyn_lds(size_t* out) {
+    hipFuncAttributes a;
+    const hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_rho<64, 8, 2, 4>));
+    if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(k_rho)");
+    *out = (size_t)160 * 1024 - a.sharedSizeBytes;
+    return PDFXC_OK;
+}
+
 int launch_rho(Context& c, const LaunchRange& r) {
     if (r.count <= 0) return PDFXC_OK;
     switch (r.cls) {
