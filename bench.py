@@ -128,7 +128,7 @@ E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 
 TRAFFIC_MEASURED = {
     ("gly10", "bf16x3k", "k_kmat_split"): (82418.6 * 1024 * 2.00 + 3095591.0 * 1024 * 1.00,
                                            "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; profiles/gly10_kernels.md"),
-    ("gly10", "bf16x3k", "k_rho"): ((90370.46 + 28065.10 + 11098.0) * 1024 * 2.00
+    ("gly10", "bf16x3k", "k_rho"): ((57362.74 + 28064.03 + 11098.27) * 1024 * 2.00
                                     + (33056.0 + 4412.0 + 5154.0) * 1024 * 1.00,
                                     "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches; "
                                     "profiles/gly10_kernels.md"),
