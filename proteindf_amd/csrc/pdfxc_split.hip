@@ -526,9 +526,15 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KWork* __restrict_
     auto produce = [&](int sl) {
         unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
         unsigned short* vv = phi + S * ncols * 16;
-        for (int it = tid; it < nsh * 16; it += NT) {
+        // (shell, point) items in rounds of NT: shells from the costliest
+        // (l, nprim) down, and every other round's waves in reverse order, so
+        // a wave's rounds mix dear and cheap shells
+        for (int r0 = 0; r0 < nsh * 16; r0 += NT) {
+            const int ww = ((r0 / NT) & 1) ? (NW - 1 - wave) : wave;
+            const int it = r0 + ww * 64 + lane;
+            if (it >= nsh * 16) continue;
             const int p = it & 15;
-            const int k = it >> 4;
+            const int k = nsh - 1 - (it >> 4);
             const int pp = sl * 16 + p;
             // radial sums once, then phi and grad phi per component from the
             // angular polynomial (keeps the producer within its registers)
