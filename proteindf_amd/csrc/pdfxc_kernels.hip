@@ -740,10 +740,15 @@ __global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms, 
         r[m] = sqrt(dx * dx + dy * dy + dz * dz);
     }
     const int A = atom_of_pt[i];
+    // the point's own atom is the first neighbour of every cell: it is (near)
+    // the nearest, so a far cell's product reaches zero (Stratmann-Scuseria:
+    // exactly, u > 0.64) or underflows after few factors and the loop ends
+    const int A0 = (A >= 0 && A < na) ? A : 0;
     double total = 0.0, mine = 0.0;
     for (int m = 0; m < na; ++m) {
         double ps = 1.0;
-        for (int k = 0; k < na && ps != 0.0; ++k) {
+        for (int j = 0; j < na && ps != 0.0; ++j) {
+            const int k = (j == 0) ? A0 : (j <= A0 ? j - 1 : j);  // A0, then the others in index order
             if (k == m) continue;
             double u = (r[m] - r[k]) * inv_d[(int64_t)m * na + k];
             double g;
