@@ -1213,7 +1213,7 @@ int pdfxc_generate_grid_ex(pdfxc_ctx* ctx, int n_atoms, const int* Z, const doub
                 const double den = au * au - 1.0;
                 double aa = (den == 0.0) ? (au > 0.0 ? 0.5 : -0.5) : au / den;
                 aa = std::min(0.5, std::max(-0.5, aa));
-                amat[(size_t)m * n_atoms + k] = aa;
+                amat[(size_t)k * n_atoms + m] = aa;  // a_mk at [k][m]: the kernel's lanes run over m
             }
         }
     double *d_at = nullptr, *d_p = nullptr, *d_w = nullptr;

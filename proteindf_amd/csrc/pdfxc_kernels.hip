@@ -712,45 +712,48 @@ __global__ __launch_bounds__(256) void k_eval_aos_points(int64_t n, const double
 }
 
 // ---------------------------------------------------------------------------
-// Becke partition weights (calcMultiCenterWeight_Becke + Ps_uij,
-// DfGenerateGrid.cpp:1811-1876, :2043-2086): w *= P_A / sum_m P_m with
-// P_m = prod_{n != m} (1 - f3(nu_mn)) / 2, nu = mu + a_mn (1 - mu^2),
-// mu = (r_m - r_n) / R_mn.  atoms: n_atoms x 4 (x, y, z, Z) then the
-// n_atoms^2 inverse distances and size adjustments a_mn.  One thread per
-// point; the point's atom distances are kept in LDS.
-// ---------------------------------------------------------------------------
 // Partition weight of each point's atom (calcMultiCenterWeight_Becke /
 // Ps_uij, DfGenerateGrid.cpp:1811-1876, :2043-2086; calcMultiCenterWeight_SS,
-// :1878-1956).  atoms: (x, y, z, Z) per atom | 1/R_mk | a_mk (the Becke size
-// adjustment, zero when off).  Every atom m, a dummy (Z = 0) included, has a
-// cell; the Becke product skips dummy neighbours k, the SS product does not.
-__global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms, int na,
+// :1878-1956): w *= P_A / sum_m P_m with P_m = prod_{k != m} s(u_mk),
+// u = (r_m - r_k) / R_mk (Becke: nu = u + a_mk (1 - u^2) through f3).
+// atoms: (x, y, z, Z) per atom | 1/R_mk (symmetric) | a_mk stored at [k][m].
+// Every atom m, a dummy (Z = 0) included, has a cell; the Becke product skips
+// dummy neighbours k, the SS product does not.  One wave per point: the
+// point's atom distances in LDS, lanes over the cells m (so the [k][m] reads
+// of a neighbour k are contiguous), the sums over cells in a fixed butterfly.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_becke(const double* __restrict__ atoms, int na,
                                               int64_t n, const double* __restrict__ pts,
                                               const int* __restrict__ atom_of_pt,
                                               double* __restrict__ w, int ss) {
-    extern __shared__ double s_r[];  // blockDim.x x na
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    extern __shared__ double s_r[];  // [waves][na]
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+    const bool valid = i0 < n;
+    const int64_t i = valid ? i0 : n - 1;  // a spare wave repeats the last point and writes nothing
     const double* inv_d = atoms + 4 * na;
-    const double* amat = inv_d + (int64_t)na * na;
-    double* r = s_r + threadIdx.x * na;
+    const double* amat_t = inv_d + (int64_t)na * na;
+    double* r = s_r + (int64_t)wv * na;
     const double x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-    for (int m = 0; m < na; ++m) {
+    for (int m = lane; m < na; m += 64) {
         const double dx = x - atoms[4 * m], dy = y - atoms[4 * m + 1], dz = z - atoms[4 * m + 2];
         r[m] = sqrt(dx * dx + dy * dy + dz * dz);
     }
+    __syncthreads();
     const int A = atom_of_pt[i];
     // the point's own atom is the first neighbour of every cell: it is (near)
     // the nearest, so a far cell's product reaches zero (Stratmann-Scuseria:
     // exactly, u > 0.64) or underflows after few factors and the loop ends
     const int A0 = (A >= 0 && A < na) ? A : 0;
     double total = 0.0, mine = 0.0;
-    for (int m = 0; m < na; ++m) {
+    for (int m = lane; m < na; m += 64) {
+        const double rm = r[m];
         double ps = 1.0;
         for (int j = 0; j < na && ps != 0.0; ++j) {
             const int k = (j == 0) ? A0 : (j <= A0 ? j - 1 : j);  // A0, then the others in index order
             if (k == m) continue;
-            double u = (r[m] - r[k]) * inv_d[(int64_t)m * na + k];
+            double u = (rm - r[k]) * inv_d[(int64_t)k * na + m];
             double g;
             if (ss) {
                 // Stratmann-Scuseria g(u), a = 0.64
@@ -764,7 +767,7 @@ __global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms, 
                 }
             } else {
                 if (atoms[4 * k + 3] <= 0.0) continue;  // Becke: dummy neighbours do not cut
-                const double a = amat[(int64_t)m * na + k];
+                const double a = amat_t[(int64_t)k * na + m];
                 u = u + a * (1.0 - u * u);
                 g = 0.5 * u * (3.0 - u * u);
                 g = 0.5 * g * (3.0 - g * g);
@@ -775,7 +778,12 @@ __global__ __launch_bounds__(64) void k_becke(const double* __restrict__ atoms, 
         if (m == A) mine = ps;
         total += ps;
     }
-    w[i] *= (mine / total);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        total += __shfl_xor(total, off);
+        mine += __shfl_xor(mine, off);
+    }
+    if (valid && lane == 0) w[i] *= (mine / total);
 }
 
 // ---------------------------------------------------------------------------
@@ -949,16 +957,16 @@ int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, dou
 int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,
                  const int* d_atom_of_pt, double* d_w, int partition, hipStream_t s) {
     if (n <= 0) return PDFXC_OK;
-    // point-to-atom distances live in LDS: keep them within 64 KiB per workgroup
-    int threads = 64;
-    while (threads > 1 && (size_t)threads * (size_t)n_atoms * sizeof(double) > 65536) threads >>= 1;
+    // one wave per point, each wave's point-to-atom distances in LDS: four
+    // waves per workgroup while they fit in 64 KiB, else one
     if ((size_t)n_atoms * sizeof(double) > 65536) {
         set_error("Becke weights: too many atoms for the LDS distance table");
         return PDFXC_ERR_UNSUPPORTED;
     }
-    const int64_t g = (n + threads - 1) / threads;
-    const size_t lds = sizeof(double) * (size_t)threads * (size_t)n_atoms;
-    hipLaunchKernelGGL(k_becke, dim3((unsigned)g), dim3(threads), lds, s, d_atoms, n_atoms, n,
+    const int waves = ((size_t)4 * n_atoms * sizeof(double) <= 65536) ? 4 : 1;
+    const int64_t g = (n + waves - 1) / waves;
+    const size_t lds = sizeof(double) * (size_t)waves * (size_t)n_atoms;
+    hipLaunchKernelGGL(k_becke, dim3((unsigned)g), dim3(64 * waves), lds, s, d_atoms, n_atoms, n,
                        d_pts, d_atom_of_pt, d_w, partition == 1 ? 1 : 0);
     return check_launch("k_becke");
 }
