@@ -15,7 +15,7 @@ from pathlib import Path
 import numpy as np
 
 SHELL_L = {"s": 0, "p": 1, "d": 2}
-DEFAULT_BASIS2 = Path(__file__).resolve().parents[1] / "tests" / "golden" / "basis2_6-31G_HCNO.txt"
+DEFAULT_BASIS2 = Path(__file__).resolve().parent / "data" / "basis2_6-31G_HCNO.txt"
 
 
 @dataclass

@@ -7,10 +7,10 @@ lines.  The library has no 6-31G(d); the workloads add one uncontracted d
 shell (exponent 0.8, Hariharan-Pople) on C, N and O in the host code.
 
 This script copies the four entries verbatim (they are published basis-set
-data) into tests/golden/basis2_6-31G_HCNO.txt, in the same format, so that the
+data) into proteindf_amd/data/basis2_6-31G_HCNO.txt, in the same format, so that the
 GPU runs do not need the reference tree:
 
-    python tests/golden/extract_basis.py /root/reference > tests/golden/basis2_6-31G_HCNO.txt
+    python tests/golden/extract_basis.py /root/reference > proteindf_amd/data/basis2_6-31G_HCNO.txt
 """
 import sys
 from pathlib import Path
