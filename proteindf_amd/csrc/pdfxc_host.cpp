@@ -432,7 +432,56 @@ static double shell_bound(const Shell& s, double d) {
     return b;
 }
 
+// Drop every grid-dependent host list and device buffer: a context left like
+// this refuses builds with PDFXC_ERR_NO_GRID.
+static void reset_grid(Context& c) {
+    free_grid_buffers(&c);
+    c.n_pts = 0;
+    c.n_slots = 0;
+    c.gx.clear();
+    c.gy.clear();
+    c.gz.clear();
+    c.gw.clear();
+    c.gen_atom.clear();
+    c.blocks.clear();
+    c.block_shells.clear();
+    c.block_cols.clear();
+    c.block_shell_col.clear();
+    c.block_sorder.clear();
+    c.slot_of_point.clear();
+    c.rho_launches.clear();
+    c.kmat_launches.clear();
+    c.sblocks.clear();
+    c.chunks.clear();
+    c.sshell_col.clear();
+    c.srho_launches.clear();
+    c.sk_launches.clear();
+    c.k_fp64_launches.clear();
+    c.h_rho_work.clear();
+    c.h_k_work.clear();
+    c.h_srho_work.clear();
+    c.h_sk_work.clear();
+    c.flops = 0.0;
+    c.mean_nao = 0.0;
+    c.max_nao = 0;
+    c.n_k_fp64_blocks = 0;
+}
+
+static int build_blocks_impl(Context& c);
+
+// Blocks, work lists and device buffers of the context's grid; on failure the
+// context is left with no grid (nothing half-built stays reachable).
 static int build_blocks(Context& c) {
+    const int rc = build_blocks_impl(c);
+    if (rc != PDFXC_OK) {
+        const std::string msg = g_last_error;
+        reset_grid(c);
+        set_error(msg);
+    }
+    return rc;
+}
+
+static int build_blocks_impl(Context& c) {
     std::vector<int64_t> order;
     std::vector<std::pair<int64_t, int64_t>> leaves;
     partition_points(c, order, leaves);
@@ -1058,8 +1107,17 @@ int pdfxc_set_grid(pdfxc_ctx* ctx, int64_t n_pts, const double* x, const double*
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return arg_error("pdfxc_set_grid: null context");
     if (n_pts <= 0 || !x || !y || !z || !w) return arg_error("pdfxc_set_grid: bad arguments");
-    if (n_pts > ((int64_t)1 << 31) - kBlockPts)
+    // a rejected grid leaves the context without one (a later build reports PDFXC_ERR_NO_GRID)
+    if (c->stream) PDFXC_HIP(hipStreamSynchronize(c->stream));
+    if (n_pts > ((int64_t)1 << 31) - kBlockPts) {
+        reset_grid(*c);
         return arg_error("pdfxc_set_grid: more than 2^31 points");
+    }
+    for (int64_t i = 0; i < n_pts; ++i)
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i]) || !std::isfinite(z[i]) || !std::isfinite(w[i])) {
+            reset_grid(*c);
+            return arg_error("pdfxc_set_grid: non-finite grid value at point " + std::to_string(i));
+        }
     c->n_pts = n_pts;
     c->gen_atom.clear();
     c->grid_partition = PDFXC_PARTITION_BECKE;
@@ -1068,10 +1126,6 @@ int pdfxc_set_grid(pdfxc_ctx* ctx, int64_t n_pts, const double* x, const double*
     c->gy.assign(y, y + n_pts);
     c->gz.assign(z, z + n_pts);
     c->gw.assign(w, w + n_pts);
-    for (int64_t i = 0; i < n_pts; ++i)
-        if (!std::isfinite(c->gx[i]) || !std::isfinite(c->gy[i]) || !std::isfinite(c->gz[i]) ||
-            !std::isfinite(c->gw[i]))
-            return arg_error("pdfxc_set_grid: non-finite grid value at point " + std::to_string(i));
     return build_blocks(*c);
 }
 
@@ -1727,10 +1781,28 @@ int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* t) {
     if (t->rho_shift < 0 || t->rho_shift > 3 || t->split_rho_shift < 0 || t->split_rho_shift > 2 ||
         t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32)
         return arg_error("pdfxc_set_tuning: shape out of range");
+    const pdfxc_tuning prev = c->tune;
     c->tune = *t;
     if (c->n_pts == 0) return PDFXC_OK;
     PDFXC_HIP(hipStreamSynchronize(c->stream));
-    return build_blocks(*c);
+    // a shape the grid's blocks do not fit drops the grid (build_blocks): put
+    // the grid back with the previous shape and report the failure
+    const std::vector<double> gx = c->gx, gy = c->gy, gz = c->gz, gw = c->gw;
+    const std::vector<int> gen_atom = c->gen_atom;
+    const int64_t n_pts = c->n_pts;
+    const int rc = build_blocks(*c);
+    if (rc == PDFXC_OK) return rc;
+    const std::string msg = g_last_error;
+    c->tune = prev;
+    c->n_pts = n_pts;
+    c->gx = gx;
+    c->gy = gy;
+    c->gz = gz;
+    c->gw = gw;
+    c->gen_atom = gen_atom;
+    if (build_blocks(*c) != PDFXC_OK) c->tune = prev;
+    set_error(msg);
+    return rc;
 }
 
 int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK, double* dE, int reps,

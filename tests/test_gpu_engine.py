@@ -203,6 +203,32 @@ def test_errors_are_reported_not_faulted():
     ctx.close()
 
 
+def test_rejected_grid_leaves_no_grid():
+    # a grid that set_grid refuses drops the previous one: the next build is
+    # refused with PDFXC_ERR_NO_GRID before any kernel runs
+    mol, info = water(1, seed=8)
+    grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 20, 50)
+    ctx = engine.XcContext(info)
+    n = info.n_aos
+    PA = guess.pack_lower(0.5 * guess.density_guess(mol, info, seed=1, eps=0.05))
+    ctx.set_grid(grid)
+    E0 = ctx.build_rks(PA)[0]
+    bad = grid.copy()
+    bad[len(bad) // 2, 3] = np.nan
+    with pytest.raises(engine.PdfxcError) as ei:
+        ctx.set_grid(bad)
+    assert ei.value.code == 1
+    with pytest.raises(engine.PdfxcError) as ei:
+        ctx.build_rks(np.zeros(n * (n + 1) // 2))
+    assert ei.value.code == 3
+    # a shape the blocks cannot take is refused and the grid stays usable
+    ctx.set_grid(grid)
+    with pytest.raises(engine.PdfxcError):
+        ctx.set_tuning(split_k_patches=64)
+    assert ctx.build_rks(PA)[0] == E0
+    ctx.close()
+
+
 def test_kxc_is_derivative_of_exc_on_gpu():
     mol, info = water(2, seed=9)
     grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 40, 194)
