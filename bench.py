@@ -135,7 +135,7 @@ TRAFFIC_MEASURED = {
 }
 
 
-def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True, k_patches=0):
+def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True, k_patches=0, k_group=0):
     """Build a context with the given precision on the same grid, let the
     autotuner pick its launch shapes (set-up, untimed), time steps builds on
     HBM-resident P/K, return (ms per build, stats, E, K dense, tuning)."""
@@ -144,9 +144,9 @@ def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune
     ctx.set_grid(grid)
     dP, dK, dE = hip.malloc(PA.nbytes), hip.malloc(PA.nbytes), hip.malloc(8)
     hip.h2d(dP, PA)
-    if k_patches:
-        ctx.set_tuning(split_k_patches=k_patches)
-    tune = ctx.autotune(dP, dK, dE, reps=2).as_dict() if autotune and not k_patches else None
+    if k_patches or k_group:
+        ctx.set_tuning(split_k_patches=k_patches or 24, split_k_group=k_group or 4)
+    tune = ctx.autotune(dP, dK, dE, reps=2).as_dict() if autotune and not (k_patches or k_group) else None
     hip.memset(dK, PA.nbytes)
     for _ in range(warmup):
         ctx.build_rks_device(dP, dK, dE)
@@ -183,7 +183,7 @@ def _tally(tunes):
     out = {}
     for t in tunes:
         key = f"rho_shift={t['rho_shift']} k_tiles={t['k_tiles']} split_rho_shift={t['split_rho_shift']} " \
-              f"split_k_patches={t['split_k_patches']}"
+              f"split_k_patches={t['split_k_patches']} split_k_group={t['split_k_group']}"
         out[key] = out.get(key, 0) + 1
     return out
 
@@ -292,7 +292,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="gly10", choices=sorted(WORKLOADS) + ["h2o_batch"])
+    ap.add_argument("--workload", default="gly30", choices=sorted(WORKLOADS) + ["h2o_batch"])
     ap.add_argument("--precision", default="bf16x3k", choices=sorted(MODES),
                     help="mode whose throughput is the reported value (the others are measured beside it)")
     ap.add_argument("--modes", default="fp64,bf16x2,bf16x3,bf16x3k", help="comma list of modes to measure")
@@ -301,6 +301,9 @@ def main():
     ap.add_argument("--split-k-patches", type=int, default=0, metavar="N",
                     help="split K pass items of N patches (4..32: 16 waves x 1 up to 16, 12 x 2 up to 24, "
                          "8 x 4 up to 32) instead of the autotuner's choice")
+    ap.add_argument("--split-k-group", type=int, default=0, metavar="G",
+                    help="split K pass work units of G blocks (1, 2, 4 or 8; 128 G points) instead of the "
+                         "autotuner's choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -318,6 +321,9 @@ def main():
         sys.exit(2)
     if args.split_k_patches and not 4 <= args.split_k_patches <= 32:
         print(json.dumps({"error": "--split-k-patches takes 4..32"}), flush=True)
+        sys.exit(2)
+    if args.split_k_group and args.split_k_group not in (1, 2, 4, 8):
+        print(json.dumps({"error": "--split-k-group takes 1, 2, 4 or 8"}), flush=True)
         sys.exit(2)
     if args.workload == "h2o_batch":
         bench_batch(args)
@@ -347,7 +353,7 @@ def main():
     res = {}
     for m in modes:
         dt, st, E, K, tune = time_mode(hip, info, grid, PA, MODES[m], wl["ao_screen"], args.steps, args.warmup,
-                                       not args.no_autotune, args.split_k_patches)
+                                       not args.no_autotune, args.split_k_patches, args.split_k_group)
         res[m] = dict(dt=dt, st=st, E=E, K=K, k_ms=list(st.last_kernel_ms), tune=tune)
         print(f"[bench] {m:7s} E_xc {E:.10f}  build {dt * 1e3:.2f} ms  kernels ms: "
               + ", ".join(f"{t:.2f}" for t in res[m]["k_ms"][:3]), file=sys.stderr, flush=True)

@@ -91,13 +91,18 @@ typedef struct pdfxc_stats {
  * 16 x 16 tiles for blocks of more than 32 tiles; split_k_patches (4..32):
  * split K pass work item size in 2 x 2 tile patches (up to 16: workgroups of
  * 16 waves, one patch each; up to 24: 12 waves of two; up to 32: 8 waves of
- * four).  Defaults 0, 96, 0, 24.
+ * four); split_k_group (1, 2, 4 or 8): grid blocks (128 points each,
+ * neighbours in the point bisection) whose K contributions the split K pass
+ * sums on the union of their AO columns before one scatter into K, i.e.
+ * 128 x split_k_group points per K work unit (a group too wide for the
+ * kernel's LDS is halved).  Defaults 0, 96, 0, 24, 4.
  * ms: the build time measured for it (pdfxc_autotune), else 0. */
 typedef struct pdfxc_tuning {
     int rho_shift;
     int k_tiles;
     int split_rho_shift;
     int split_k_patches;
+    int split_k_group;
     double ms;
 } pdfxc_tuning;
 
@@ -174,6 +179,14 @@ int pdfxc_grid_opts_preset(const char* grid_type, pdfxc_grid_opts* out);
  * the context and returns the point count. */
 int pdfxc_generate_grid_ex(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
                            const pdfxc_grid_opts* opts, int64_t* n_pts);
+/* pdfxc_generate_grid_ex with the angular grids in a given orientation O
+ * (3 x 3 row-major, orthogonal: point = O . (Lebedev point) r_i + atom)
+ * instead of the one the reference derives from the geometry
+ * (DfGenerateGrid.cpp:1739-1809), so that a caller moving atoms (finite
+ * differences, geometry scans) keeps the grid frame fixed.  A matrix that is
+ * not orthogonal to 1e-10 is refused with PDFXC_ERR_ARGUMENT. */
+int pdfxc_generate_grid_oriented(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
+                                 const pdfxc_grid_opts* opts, const double* O, int64_t* n_pts);
 
 /* Copy the context's grid back in the caller's point order (x, y, z, w
  * columns, n_pts each). */

@@ -2,6 +2,7 @@
 // partitioning with per-block AO screening, per-class work lists, device
 // buffers and grid generation.  Kernels live in pdfxc_kernels.hip.
 #include <algorithm>
+#include <functional>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -341,7 +342,14 @@ static void free_grid_buffers(Context* c) {
     dfree(c->d_chunks);
     dfree(c->d_sshell_col);
     dfree(c->d_srho_work);
-    dfree(c->d_sk_work);
+    dfree(c->d_kg_work);
+    dfree(c->d_kgroups);
+    dfree(c->d_kg_blocks);
+    dfree(c->d_kg_shells);
+    dfree(c->d_kg_shell_col);
+    dfree(c->d_kg_sorder);
+    dfree(c->d_kg_cols);
+    dfree(c->d_kg_masks);
 }
 
 int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
@@ -460,7 +468,14 @@ static void reset_grid(Context& c) {
     c.h_rho_work.clear();
     c.h_k_work.clear();
     c.h_srho_work.clear();
-    c.h_sk_work.clear();
+    c.h_kg_work.clear();
+    c.kgroups.clear();
+    c.kg_blocks.clear();
+    c.kg_shells.clear();
+    c.kg_shell_col.clear();
+    c.kg_sorder.clear();
+    c.kg_cols.clear();
+    c.kg_masks.clear();
     c.flops = 0.0;
     c.mean_nao = 0.0;
     c.max_nao = 0;
@@ -687,7 +702,7 @@ static int build_blocks_impl(Context& c) {
     // split precision: chunked column layout and k_rho_split classes (ldnp <=
     // 256: 64 points per workgroup, <= 512: 32, <= 1024: 16); k_kmat_split items
     std::vector<int> srho_work;
-    std::vector<KWork> skw;
+    std::vector<KGWork> skw;
     size_t sk_lds = 0;
     c.sblocks.clear();
     c.chunks.clear();
@@ -696,6 +711,13 @@ static int build_blocks_impl(Context& c) {
     c.sk_launches.clear();
     c.k_fp64_launches.clear();
     c.n_k_fp64_blocks = 0;
+    c.kgroups.clear();
+    c.kg_blocks.clear();
+    c.kg_shells.clear();
+    c.kg_shell_col.clear();
+    c.kg_sorder.clear();
+    c.kg_cols.clear();
+    c.kg_masks.clear();
     if (c.opts.precision != PDFXC_FP64) {
         const int S = prec_slices(c.opts.precision);
         c.sblocks.resize(nb);
@@ -757,53 +779,155 @@ static int build_blocks_impl(Context& c) {
             r.count = (int)srho_work.size() - r.offset;
             if (r.count) c.srho_launches.push_back(r);
         }
-        // k_kmat_split items: lower 2 x 2 tile patches of the compact columns,
-        // at most 16 per item (one per wave); two slab buffers if they fit
+        // k_kmat_split: groups of split_k_group neighbouring blocks (in the
+        // bisection's leaf order, so siblings), summed on the union of their
+        // shells; items of at most split_k_patches lower 2 x 2 tile patches in
+        // column-major order; two slab buffers if they fit.  A group whose
+        // widest item does not fit in LDS is halved; a single block that does
+        // not fit takes the FP64 K pass.
         size_t k_lds_max = 0;
         if (int rc = split_kmat_max_dyn_lds(S, &k_lds_max)) return rc;
         std::vector<char> k_fp64_block(nb, 0);
-        for (int b = 0; b < nb; ++b) {
-            const BlockDesc& B = c.blocks[b];
-            const size_t first_item = skw.size();
-            bool wide = false;
-            const int T = B.ldn / 16;
-            const int Tp = (T + 1) / 2;
+        std::vector<int> block_of_leaf(nb);
+        for (int li = 0; li < nb; ++li) block_of_leaf[lorder[li]] = li;
+        auto item_lds = [&](int wcols, int nsh_item, int nbuf) {
+            return ((size_t)nbuf * 2 * S * wcols * 16 * 2 + 15) / 16 * 16 + sizeof(double) * (size_t)nsh_item * RS +
+                   sizeof(int) * (size_t)(wcols + 3 * nsh_item);
+        };
+        struct GroupCost {
+            double cost;
+            int first, count;  // items in gitems
+        };
+        std::vector<KGWork> gitems;
+        std::vector<GroupCost> gcost;
+        std::vector<int> ushells;
+        // union of the shells of blocks bl[0..n) (block shell lists ascend)
+        auto make_union = [&](const int* bl, int n) {
+            ushells.clear();
+            for (int j = 0; j < n; ++j) {
+                const BlockDesc& B = c.blocks[bl[j]];
+                ushells.insert(ushells.end(), c.block_shells.begin() + B.shell_begin,
+                               c.block_shells.begin() + B.shell_begin + B.nshell);
+            }
+            std::sort(ushells.begin(), ushells.end());
+            ushells.erase(std::unique(ushells.begin(), ushells.end()), ushells.end());
+        };
+        // the first item (c0 = 0) is the widest: all union shells, all columns
+        auto fits = [&](int nao, int nsh) {
+            const int wcols = 32 * ((nao + 31) / 32);
+            return nsh <= 32 * kMaxMaskWords && item_lds(wcols, nsh, 1) <= k_lds_max;
+        };
+        std::vector<int> bl;
+        std::function<void(int, int)> form = [&](int leaf0, int n) {
+            bl.clear();
+            for (int j = 0; j < n; ++j) bl.push_back(block_of_leaf[leaf0 + j]);
+            make_union(bl.data(), n);
+            int nao = 0;
+            for (int s : ushells) nao += 2 * c.shells[s].l + 1;
+            if (!fits(nao, (int)ushells.size())) {
+                if (n > 1) {
+                    const int h = n / 2;
+                    form(leaf0, h);
+                    form(leaf0 + h, n - h);
+                } else {
+                    // the 16-point bf16 slabs of this block exceed a workgroup's
+                    // LDS: its K pass runs in FP64 (k_kmat, whose slab depth
+                    // adapts), which is at least as accurate
+                    k_fp64_block[bl[0]] = 1;
+                    ++c.n_k_fp64_blocks;
+                }
+                return;
+            }
+            KGroup G{};
+            G.sub_begin = (int)c.kg_blocks.size();
+            G.nsub = n;
+            G.shell_begin = (int)c.kg_shells.size();
+            G.nshell = (int)ushells.size();
+            G.col_begin = (int)c.kg_cols.size();
+            G.nao = nao;
+            G.mask_begin = (int)c.kg_masks.size();
+            G.nword = (G.nshell + 31) / 32;
+            G.ldp = c.n_aos;
+            G.pad = 0;
+            G.poff = 0;
+            int npts = 0;
+            for (int j = 0; j < n; ++j) {
+                c.kg_blocks.push_back(bl[j]);
+                npts += c.blocks[bl[j]].npts;
+            }
+            int col = 0;
+            std::vector<int> ucol(ushells.size());
+            for (size_t k = 0; k < ushells.size(); ++k) {
+                const Shell& Sh = c.shells[ushells[k]];
+                c.kg_shells.push_back(ushells[k]);
+                c.kg_shell_col.push_back(col);
+                ucol[k] = col;
+                for (int m = 0; m < 2 * Sh.l + 1; ++m) c.kg_cols.push_back(Sh.first_ao + m);
+                col += 2 * Sh.l + 1;
+            }
+            for (int j = 0; j < n; ++j) {
+                const BlockDesc& B = c.blocks[bl[j]];
+                std::vector<unsigned> w(G.nword, 0u);
+                for (int q = 0; q < B.nshell; ++q) {
+                    const int k = (int)(std::lower_bound(ushells.begin(), ushells.end(),
+                                                         c.block_shells[B.shell_begin + q]) - ushells.begin());
+                    w[k >> 5] |= 1u << (k & 31);
+                }
+                c.kg_masks.insert(c.kg_masks.end(), w.begin(), w.end());
+            }
+            {
+                // production order: equal (l, nprim) side by side, costliest last
+                std::vector<int> so(ushells.size());
+                std::iota(so.begin(), so.end(), 0);
+                std::stable_sort(so.begin(), so.end(), [&](int x, int y) {
+                    const Shell& A = c.shells[ushells[x]];
+                    const Shell& Bs = c.shells[ushells[y]];
+                    return A.l != Bs.l ? A.l < Bs.l : A.nprim < Bs.nprim;
+                });
+                c.kg_sorder.insert(c.kg_sorder.end(), so.begin(), so.end());
+            }
+            const int gi = (int)c.kgroups.size();
+            c.kgroups.push_back(G);
+            const int Tp = (nao + 31) / 32;
             const int np = Tp * (Tp + 1) / 2;
             const int per = c.tune.split_k_patches;  // at most 16, 24 or 32 (1, 2 or 4 per wave)
+            GroupCost gc{0.0, (int)gitems.size(), 0};
             for (int q0 = 0; q0 < np; q0 += per) {
-                KWork w;
-                w.block = b;
-                w.t0 = q0;
-                w.t1 = std::min(np, q0 + per);
-                int P = 0;
-                while ((P + 1) * (P + 2) / 2 <= w.t1 - 1) ++P;
-                w.ncols = 32 * (P + 1);
-                int nsh = 0;
-                while (nsh < B.nshell && c.block_shell_col[B.shell_begin + nsh] < w.ncols) ++nsh;
-                w.nsh = nsh;
-                size_t lds = 0;
-                for (int nbuf = 2; nbuf >= 1; --nbuf) {
-                    w.ps = nbuf;
-                    lds = ((size_t)nbuf * 2 * S * w.ncols * 16 * 2 + 15) / 16 * 16 + sizeof(double) * (size_t)nsh * RS +
-                          sizeof(int) * (size_t)(w.ncols + 2 * nsh);
-                    if (lds <= k_lds_max) break;
+                KGWork w{};
+                w.group = gi;
+                w.q0 = q0;
+                w.q1 = std::min(np, q0 + per);
+                int Q = 0, q = q0, rem = Tp;
+                while (q >= rem) {
+                    q -= rem;
+                    ++Q;
+                    --rem;
                 }
-                if (lds > k_lds_max) {
-                    wide = true;
-                    break;
-                }
+                w.c0 = 32 * Q;
+                w.c1 = 32 * Tp;
+                int k0 = 0;
+                while (k0 < G.nshell && ucol[k0] + 2 * c.shells[ushells[k0]].l + 1 <= w.c0) ++k0;
+                w.k0 = k0;
+                w.nbuf = item_lds(w.c1 - w.c0, G.nshell - k0, 2) <= k_lds_max ? 2 : 1;
+                const size_t lds = item_lds(w.c1 - w.c0, G.nshell - k0, w.nbuf);
                 sk_lds = std::max(sk_lds, lds);
-                skw.push_back(w);
+                w.pad = 0;
+                gitems.push_back(w);
+                // production (points x columns) plus MFMA (points x patches)
+                gc.cost = std::max(gc.cost, (double)npts * ((w.c1 - w.c0) + 8.0 * (w.q1 - w.q0)));
             }
-            if (wide) {
-                // the 16-point bf16 slabs of this block's widest item exceed a
-                // workgroup's LDS: its K pass runs in FP64 (k_kmat, whose slab
-                // depth adapts), which is at least as accurate
-                skw.resize(first_item);
-                k_fp64_block[b] = 1;
-                ++c.n_k_fp64_blocks;
-            }
-        }
+            gc.count = (int)gitems.size() - gc.first;
+            gcost.push_back(gc);
+        };
+        const int G0 = c.tune.split_k_group;
+        for (int leaf0 = 0; leaf0 < nb; leaf0 += G0) form(leaf0, std::min(G0, nb - leaf0));
+        // launch order: groups by descending cost (their items side by side)
+        std::vector<int> gorder(gcost.size());
+        std::iota(gorder.begin(), gorder.end(), 0);
+        std::stable_sort(gorder.begin(), gorder.end(),
+                         [&](int x, int y) { return gcost[x].cost > gcost[y].cost; });
+        for (int g : gorder)
+            skw.insert(skw.end(), gitems.begin() + gcost[g].first, gitems.begin() + gcost[g].first + gcost[g].count);
         // launch class = workgroup shape for the item size (16 x 1, 12 x 2 or
         // 8 x 4 waves x patches), so a batch keeps every item's shape
         const int kshape = c.tune.split_k_patches <= 16 ? 0 : c.tune.split_k_patches <= 24 ? 1 : 2;
@@ -836,7 +960,14 @@ static int build_blocks_impl(Context& c) {
         if (rc == PDFXC_OK) rc = dalloc(&c.d_chunks, c.chunks.size());
         if (rc == PDFXC_OK) rc = dalloc(&c.d_sshell_col, c.sshell_col.size());
         if (rc == PDFXC_OK) rc = dalloc(&c.d_srho_work, srho_work.size());
-        if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&c.d_sk_work), skw.size() * sizeof(KWork));
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kg_work, skw.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kgroups, c.kgroups.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kg_blocks, c.kg_blocks.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kg_shells, c.kg_shells.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kg_shell_col, c.kg_shell_col.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kg_sorder, c.kg_sorder.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kg_cols, c.kg_cols.size());
+        if (rc == PDFXC_OK) rc = dalloc(&c.d_kg_masks, c.kg_masks.size());
     }
     if (rc != PDFXC_OK) return rc;
     auto up = [](void* d, const void* h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
@@ -855,7 +986,16 @@ static int build_blocks_impl(Context& c) {
         PDFXC_HIP(up(c.d_chunks, c.chunks.data(), c.chunks.size() * sizeof(int)));
         PDFXC_HIP(up(c.d_sshell_col, c.sshell_col.data(), c.sshell_col.size() * sizeof(int)));
         PDFXC_HIP(up(c.d_srho_work, srho_work.data(), srho_work.size() * sizeof(int)));
-        PDFXC_HIP(up(c.d_sk_work, skw.data(), skw.size() * sizeof(KWork)));
+        if (!skw.empty()) {
+            PDFXC_HIP(up(c.d_kg_work, skw.data(), skw.size() * sizeof(KGWork)));
+            PDFXC_HIP(up(c.d_kgroups, c.kgroups.data(), c.kgroups.size() * sizeof(KGroup)));
+            PDFXC_HIP(up(c.d_kg_blocks, c.kg_blocks.data(), c.kg_blocks.size() * sizeof(int)));
+            PDFXC_HIP(up(c.d_kg_shells, c.kg_shells.data(), c.kg_shells.size() * sizeof(int)));
+            PDFXC_HIP(up(c.d_kg_shell_col, c.kg_shell_col.data(), c.kg_shell_col.size() * sizeof(int)));
+            PDFXC_HIP(up(c.d_kg_sorder, c.kg_sorder.data(), c.kg_sorder.size() * sizeof(int)));
+            PDFXC_HIP(up(c.d_kg_cols, c.kg_cols.data(), c.kg_cols.size() * sizeof(int)));
+            PDFXC_HIP(up(c.d_kg_masks, c.kg_masks.data(), c.kg_masks.size() * sizeof(unsigned)));
+        }
     }
     PDFXC_HIP(hipMemset(c.d_coef, 0, 4 * (size_t)c.n_slots * sizeof(double)));
     PDFXC_HIP(hipMemset(c.d_rho, 0, 4 * (size_t)c.n_slots * sizeof(double)));
@@ -864,7 +1004,7 @@ static int build_blocks_impl(Context& c) {
     c.h_rho_work = std::move(rho_work);
     c.h_k_work = std::move(kall);
     c.h_srho_work = std::move(srho_work);
-    c.h_sk_work = std::move(skw);
+    c.h_kg_work = std::move(skw);
     return PDFXC_OK;
 }
 
@@ -1180,8 +1320,29 @@ int pdfxc_grid_opts_preset(const char* grid_type, pdfxc_grid_opts* out) {
     return PDFXC_OK;
 }
 
+static int generate_grid_impl(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
+                              const pdfxc_grid_opts* opts, const double* O_given, int64_t* n_out);
+
 int pdfxc_generate_grid_ex(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
                            const pdfxc_grid_opts* opts, int64_t* n_out) {
+    return generate_grid_impl(ctx, n_atoms, Z, xyz, opts, nullptr, n_out);
+}
+
+int pdfxc_generate_grid_oriented(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
+                                 const pdfxc_grid_opts* opts, const double* O, int64_t* n_out) {
+    if (!O) return arg_error("pdfxc_generate_grid_oriented: null orientation");
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double d = 0.0;  // O^T O = 1 (columns orthonormal)
+            for (int k = 0; k < 3; ++k) d += O[3 * k + i] * O[3 * k + j];
+            if (!(std::fabs(d - (i == j ? 1.0 : 0.0)) <= 1e-10))
+                return arg_error("pdfxc_generate_grid_oriented: the orientation is not orthogonal");
+        }
+    return generate_grid_impl(ctx, n_atoms, Z, xyz, opts, O, n_out);
+}
+
+static int generate_grid_impl(pdfxc_ctx* ctx, int n_atoms, const int* Z, const double* xyz,
+                              const pdfxc_grid_opts* opts, const double* O_given, int64_t* n_out) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c) return arg_error("pdfxc_generate_grid_ex: null context");
     if (n_atoms <= 0 || !Z || !xyz || !opts) return arg_error("pdfxc_generate_grid_ex: bad arguments");
@@ -1211,7 +1372,10 @@ int pdfxc_generate_grid_ex(pdfxc_ctx* ctx, int n_atoms, const int* Z, const doub
             return PDFXC_ERR_UNSUPPORTED;
         }
     double O[9];
-    orientation(n_atoms, Z, xyz, O);
+    if (O_given)
+        std::copy(O_given, O_given + 9, O);
+    else
+        orientation(n_atoms, Z, xyz, O);
     // single-centre points and weights, atom by atom, radial shell by shell
     std::vector<double> pts, w;
     std::vector<int> atom;
@@ -1779,7 +1943,8 @@ int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* t) {
     Context* c = reinterpret_cast<Context*>(ctx);
     if (!c || !t) return arg_error("pdfxc_set_tuning: bad arguments");
     if (t->rho_shift < 0 || t->rho_shift > 3 || t->split_rho_shift < 0 || t->split_rho_shift > 2 ||
-        t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32)
+        t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32 ||
+        (t->split_k_group != 1 && t->split_k_group != 2 && t->split_k_group != 4 && t->split_k_group != 8))
         return arg_error("pdfxc_set_tuning: shape out of range");
     const pdfxc_tuning prev = c->tune;
     c->tune = *t;
@@ -1818,17 +1983,22 @@ int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
     std::vector<pdfxc_tuning> cand;
     if (c->opts.precision == PDFXC_FP64) {
         for (int rs : {0, 1})
-            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 24, 0.0});
-    } else if (prec_split_rho(c->opts.precision)) {
+            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 24, 4, 0.0});
+    } else {
+        // rho pass classes, then the split K pass's item size and group size
+        // (points per K work unit) on the fastest rho shape
         for (int rs : {0, 1})
-            for (int kp : {16, 24, 32}) cand.push_back(pdfxc_tuning{0, 96, rs, kp, 0.0});
-    } else {  // FP64 rho pass, split K pass
-        for (int rs : {0, 1})
-            for (int kp : {16, 24, 32}) cand.push_back(pdfxc_tuning{rs, 96, 0, kp, 0.0});
+            cand.push_back(prec_split_rho(c->opts.precision) ? pdfxc_tuning{0, 96, rs, 24, 4, 0.0}
+                                                              : pdfxc_tuning{rs, 96, 0, 24, 4, 0.0});
+        for (int kp : {16, 24, 32})
+            for (int kg : {1, 2, 4, 8})
+                if (!(kp == 24 && kg == 4)) cand.push_back(pdfxc_tuning{-1, 96, -1, kp, kg, 0.0});
     }
     pdfxc_tuning best = cand[0];
     best.ms = 1e30;
     for (pdfxc_tuning t : cand) {
+        if (t.rho_shift < 0) t.rho_shift = best.rho_shift;  // keep the best rho shape found so far
+        if (t.split_rho_shift < 0) t.split_rho_shift = best.split_rho_shift;
         if (int rc = pdfxc_set_tuning(ctx, &t)) return rc;
         if (int rc = build_device(c, functional, dP, dK, dE)) return rc;  // warm-up
         double sum = 0.0;
@@ -2112,7 +2282,7 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
     int64_t psoff = 0;
     int shell_base = 0, nblocks = 0;
     int64_t slot_base = 0;
-    std::vector<int> block_base(n_mol), slot_first(n_mol);
+    std::vector<int> block_base(n_mol), slot_first(n_mol), group_base(n_mol, 0);
     for (int i = 0; i < n_mol; ++i) {
         const Context& c = *reinterpret_cast<const Context*>(ctxs[i]);
         const int N = c.n_aos;
@@ -2159,6 +2329,25 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
         if (split) {
             m.chunks.insert(m.chunks.end(), c.chunks.begin(), c.chunks.end());
             m.sshell_col.insert(m.sshell_col.end(), c.sshell_col.begin(), c.sshell_col.end());
+            // split K groups with their lists moved to the batch's offsets
+            group_base[i] = (int)m.kgroups.size();
+            const int kb = (int)m.kg_blocks.size(), ks = (int)m.kg_shells.size(), kc = (int)m.kg_cols.size(),
+                      km = (int)m.kg_masks.size();
+            for (KGroup G : c.kgroups) {
+                G.sub_begin += kb;
+                G.shell_begin += ks;
+                G.col_begin += kc;
+                G.mask_begin += km;
+                G.ldp = N;
+                G.poff = b->doff.back();
+                m.kgroups.push_back(G);
+            }
+            for (int x : c.kg_blocks) m.kg_blocks.push_back(x + block_base[i]);
+            for (int x : c.kg_shells) m.kg_shells.push_back(x + shell_base);
+            m.kg_shell_col.insert(m.kg_shell_col.end(), c.kg_shell_col.begin(), c.kg_shell_col.end());
+            m.kg_sorder.insert(m.kg_sorder.end(), c.kg_sorder.begin(), c.kg_sorder.end());
+            m.kg_cols.insert(m.kg_cols.end(), c.kg_cols.begin(), c.kg_cols.end());
+            m.kg_masks.insert(m.kg_masks.end(), c.kg_masks.begin(), c.kg_masks.end());
             m.psplit_jobs.push_back(Context::PsplitJob{b->doff.back(), psoff, (int)perm.size(), N});
             perm.insert(perm.end(), c.split_perm.begin(), c.split_perm.end());
             psoff += (int64_t)S * N * ((N + 7) / 8 * 8);
@@ -2248,9 +2437,23 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
         merge_int([](const Context& c) -> const std::vector<LaunchRange>& { return c.srho_launches; },
                   [](const Context& c) -> const std::vector<int>& { return c.h_srho_work; }, m.h_srho_work,
                   m.srho_launches);
-        merge_kw([](const Context& c) -> const std::vector<LaunchRange>& { return c.sk_launches; },
-                 [](const Context& c) -> const std::vector<KWork>& { return c.h_sk_work; }, m.h_sk_work,
-                 m.sk_launches);
+        for (int cls = 0; cls < 8; ++cls) {  // split K items, every molecule's of one shape together
+            LaunchRange r{cls, (int)m.h_kg_work.size(), 0, 0};
+            for (int i = 0; i < n_mol; ++i) {
+                const Context& c = *reinterpret_cast<const Context*>(ctxs[i]);
+                for (const LaunchRange& q : c.sk_launches) {
+                    if (q.cls != cls) continue;
+                    for (int k = 0; k < q.count; ++k) {
+                        KGWork w = c.h_kg_work[q.offset + k];
+                        w.group += group_base[i];
+                        m.h_kg_work.push_back(w);
+                    }
+                    r.lds = std::max(r.lds, q.lds);
+                }
+            }
+            r.count = (int)m.h_kg_work.size() - r.offset;
+            if (r.count) m.sk_launches.push_back(r);
+        }
         for (int i = 0; i < n_mol; ++i) m.n_k_fp64_blocks += reinterpret_cast<const Context*>(ctxs[i])->n_k_fp64_blocks;
     }
     (void)kw_cls;
@@ -2283,7 +2486,14 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
         if (rc == PDFXC_OK) rc = dalloc(&m.d_chunks, std::max<size_t>(1, m.chunks.size()));
         if (rc == PDFXC_OK) rc = dalloc(&m.d_sshell_col, std::max<size_t>(1, m.sshell_col.size()));
         if (rc == PDFXC_OK) rc = dalloc(&m.d_srho_work, std::max<size_t>(1, m.h_srho_work.size()));
-        if (rc == PDFXC_OK) rc = dalloc(reinterpret_cast<char**>(&m.d_sk_work), std::max<size_t>(1, m.h_sk_work.size()) * sizeof(KWork));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kg_work, std::max<size_t>(1, m.h_kg_work.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kgroups, std::max<size_t>(1, m.kgroups.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kg_blocks, std::max<size_t>(1, m.kg_blocks.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kg_shells, std::max<size_t>(1, m.kg_shells.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kg_shell_col, std::max<size_t>(1, m.kg_shell_col.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kg_sorder, std::max<size_t>(1, m.kg_sorder.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kg_cols, std::max<size_t>(1, m.kg_cols.size()));
+        if (rc == PDFXC_OK) rc = dalloc(&m.d_kg_masks, std::max<size_t>(1, m.kg_masks.size()));
         if (rc == PDFXC_OK) rc = dalloc(&m.d_perm, perm.size());
         if (rc == PDFXC_OK) rc = dalloc(&m.d_Ps, (size_t)psoff);
     }
@@ -2302,7 +2512,14 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
         up(m.d_chunks, m.chunks);
         up(m.d_sshell_col, m.sshell_col);
         up(m.d_srho_work, m.h_srho_work);
-        up(reinterpret_cast<KWork*>(m.d_sk_work), m.h_sk_work);
+        up(m.d_kg_work, m.h_kg_work);
+        up(m.d_kgroups, m.kgroups);
+        up(m.d_kg_blocks, m.kg_blocks);
+        up(m.d_kg_shells, m.kg_shells);
+        up(m.d_kg_shell_col, m.kg_shell_col);
+        up(m.d_kg_sorder, m.kg_sorder);
+        up(m.d_kg_cols, m.kg_cols);
+        up(m.d_kg_masks, m.kg_masks);
         up(m.d_perm, perm);
     }
     if (rc == PDFXC_OK) {

@@ -70,6 +70,33 @@ struct LaunchRange {
     size_t lds;  // dynamic LDS bytes
 };
 
+// A split K pass work unit: 1 to kMaxGroup spatially adjacent blocks
+// (siblings of the point bisection) whose K contributions are summed in
+// registers on the union of their AO columns before one scatter into K.
+constexpr int kMaxGroup = 8;
+constexpr int kMaxMaskWords = 32;  // union shells per group <= 32 x 32
+struct KGroup {
+    int sub_begin;    // the group's blocks: kg_blocks[sub_begin .. + nsub)
+    int nsub;
+    int shell_begin;  // union shells: kg_shells (global ids, ascending), kg_shell_col, kg_sorder
+    int nshell;
+    int col_begin;    // union columns: kg_cols (global AO index, ascending)
+    int nao;
+    int mask_begin;   // kg_masks: nsub x nword words; bit k of block j = union shell k kept by block j
+    int nword;
+    int ldp;          // P / K leading dimension (the molecule's AO count)
+    int pad;
+    int64_t poff;     // offset of the molecule's dense P and K
+};
+
+// A k_kmat_split item: lower 2 x 2 tile patches [q0, q1) of a group's union
+// in column-major order (patch column Q, then rows P >= Q); its slabs hold
+// the union columns [c0, c1) (c0 = 32 Q(q0), c1 = 32 Tp), which the union
+// shells k0 .. nshell - 1 cover; nbuf slab buffers (1 or 2).
+struct KGWork {
+    int group, q0, q1, c0, c1, k0, nbuf, pad;
+};
+
 struct DeviceShell {
     double cx, cy, cz;
     int l, nprim;
@@ -81,7 +108,7 @@ struct DeviceShell {
 
 struct Context {
     pdfxc_opts opts;
-    pdfxc_tuning tune{0, 96, 0, 24, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
+    pdfxc_tuning tune{0, 96, 0, 24, 4, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
     int n_aos = 0;
     std::vector<Shell> shells;
     int device = 0;
@@ -145,7 +172,18 @@ struct Context {
     int* d_chunks = nullptr;
     int* d_sshell_col = nullptr;
     int* d_srho_work = nullptr;
-    void* d_sk_work = nullptr;          // KWork items of k_kmat_split
+    // split K pass groups (pdfxc_tuning.split_k_group blocks each) and their items
+    std::vector<KGroup> kgroups;
+    std::vector<int> kg_blocks, kg_shells, kg_shell_col, kg_sorder, kg_cols;
+    std::vector<unsigned> kg_masks;
+    KGroup* d_kgroups = nullptr;
+    int* d_kg_blocks = nullptr;
+    int* d_kg_shells = nullptr;
+    int* d_kg_shell_col = nullptr;
+    int* d_kg_sorder = nullptr;
+    int* d_kg_cols = nullptr;
+    unsigned* d_kg_masks = nullptr;
+    KGWork* d_kg_work = nullptr;        // items of k_kmat_split
     std::vector<LaunchRange> k_fp64_launches;  // k_kmat items of blocks too wide for k_kmat_split
     int64_t n_k_fp64_blocks = 0;
 
@@ -153,7 +191,8 @@ struct Context {
 
     // host copies of the device work lists (d_rho_work, d_k_work, d_srho_work, d_sk_work)
     std::vector<int> h_rho_work, h_srho_work;
-    std::vector<KWork> h_k_work, h_sk_work;
+    std::vector<KWork> h_k_work;
+    std::vector<KGWork> h_kg_work;
 
     // batches: P slices made per molecule (empty for one molecule)
     struct PsplitJob {

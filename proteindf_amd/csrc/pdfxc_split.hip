@@ -14,9 +14,10 @@
 //   k_psplit      P (dense FP64 N x N) -> S bf16 slice planes [S][N][Nc], split order
 //   k_rho_split   X = Phi_b P_bb for PB points, then rho, grad rho
 //                 (calcRho_GGA_part, DfCalcGridX.cpp:1923-1962)
-//   k_kmat_split  K_bb += Phi_b^T V_b + V_b^T Phi_b on the block's compact AO
-//                 columns, lower 2 x 2 tile patches, FP64 atomics into the
-//                 lower triangle of K (build_XC_Matrix GGA, :2455-2500)
+//   k_kmat_split  K_uu += Phi^T V + V^T Phi over a group of adjacent blocks on
+//                 the union u of their AO columns, lower 2 x 2 tile patches,
+//                 FP64 atomics into the lower triangle of K (build_XC_Matrix
+//                 GGA, :2455-2500)
 #include <hip/hip_runtime.h>
 
 #include "pdfxc_device.hpp"
@@ -341,177 +342,147 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
 }
 
 // ---------------------------------------------------------------------------
-// K pass, split precision: K_bb += Phi^T V + V^T Phi for the lower 2 x 2
-// patches [q0, q1) of 16 x 16 tiles of one block (q = P (P + 1) / 2 + Q
-// covers tile rows 2P, 2P + 1 and columns 2Q, 2Q + 1; a diagonal patch's
-// upper tile is computed and not scattered), on the block's compact AO
-// columns.  NW waves (16, 12 or 8), each both producer and consumer: all
-// evaluate phi and V = (w vrho / 2) phi + c . grad phi in FP64 for the next
-// 16-point slab and store their bf16 slices as [s][column][16 points] (the
-// 4-point group g of column c at g ^ ((c >> 3) & 3)), then each runs
-// v_mfma_f32_16x16x16_bf16 on the current slab (one K step) for its NPW
-// patches (1, 2 or 4: fewer, larger items produce each block fewer times),
-// the hi x hi products and the corrections (a + b >= 1) in FP32
-// accumulators of their own.  Two slab buffers when they fit in LDS, else
-// one.  FP64 atomics into the dense lower triangle of K.
+// K pass, split precision, over groups of spatially adjacent blocks (siblings
+// of the point bisection, 1 to kMaxGroup of them): K_uu += Phi^T V + V^T Phi
+// summed over all the group's points on the union u of their AO columns, so
+// that the scatter adds one union-sized lower triangle per group instead of
+// one triangle per block.  An item takes the lower 2 x 2 tile patches
+// [q0, q1) of the union in column-major order (patch column Q, then rows
+// P >= Q), so its slabs hold only the union columns [c0, c1) = [32 Q(q0),
+// 32 Tp) that those patches touch.  NW waves (16, 12 or 8), each producer
+// and consumer: all evaluate phi and V = (w vrho / 2) phi + c . grad phi in
+// FP64 for the next 16-point slab of the current block (the shells the
+// block's screening kept; the union's other shells are written as zeros) and
+// store their bf16 slices as [phi, V][s][column][16 points]; then each runs
+// v_mfma_f32_16x16x32_bf16 on the current slab for its NPW patches, the
+// 32-long K index covering the slab's 16 points twice, (phi | V) against
+// (V | phi), so that one product gives phi_i V_j + V_i phi_j.  Every slab
+// sums from zero in FP32, the slice corrections (a + b >= 1, at most 2^-8 of
+// the leading term) in one sum and hi x hi in another, and both are added
+// into FP64 accumulators per slab: the MFMA's FP32 sums do not round to
+// nearest (their errors share a sign), so a sum held in FP32 over a block's
+// 8 slabs lost up to ~1.6e-7 of K on coarse grids.  FP64 atomics into the
+// dense lower triangle of K at the end of the item.
 // ---------------------------------------------------------------------------
-typedef short s16x4 __attribute__((ext_vector_type(4)));
 
-// v_mfma_f32_16x16x16_bf16 (checked by pdfxc_selftest_mfma_split): lane l
-// holds A[l % 16][4 (l / 16) + e], B[4 (l / 16) + e][l % 16], e = 0..3; the
-// result as for the 16x16x32 form.
-__device__ inline f32x4 mfma_bf16_k16(s16x4 a, s16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
+// slab address (bf16 units) of the 8-point half h of column c, halves of
+// every eighth column swapped so that 16 lanes reading 16 consecutive
+// columns hit distinct banks
+__device__ inline int kslab_at(int c, int h) { return c * 16 + ((h ^ (c >> 3)) & 1) * 8; }
 
-__device__ inline int kslab_at(int c, int g) { return c * 16 + ((g ^ (c >> 3)) & 3) * 4; }
-
-template <int S, int NPW, typename Produce>
-__device__ inline void kmat_split_run(int pb, int pe, int nslab, int nbuf, int ncols, int lane,
-                                      const unsigned short* s_slab, const int* s_cols,
-                                      double* __restrict__ K, int N, Produce&& produce) {
+template <int S, int NPW>
+__device__ inline void kmat_split_consume(int pb, int pe, int Tp, int c0, int wcols, int lane,
+                                          const unsigned short* s_buf, v4d (&acc)[NPW][4]) {
     const int li = lane & 15, lq = lane >> 4;
-    const int BUF = 2 * S * ncols * 16;  // bf16 per buffer: [phi, V][s][col][16]
-    int offI[NPW], offJ[NPW];
+    // operand of the 16x16x32 form: lane (li, lq) holds k = 8 lq .. 8 lq + 7;
+    // k < 16 are the slab's points of phi (A) or V (B), k >= 16 the other
+    const unsigned short* fa = s_buf + (size_t)(lq >> 1) * S * wcols * 16;         // A: phi, then V
+    const unsigned short* fb = s_buf + (size_t)(1 - (lq >> 1)) * S * wcols * 16;   // B: V, then phi
+    int P = 0, Q = 0;
     {
-        int P = 0;
-        while ((P + 1) * (P + 2) / 2 <= pb) ++P;
-        int Q = pb - P * (P + 1) / 2;
-#pragma unroll
-        for (int m = 0; m < NPW; ++m) {
-            offI[m] = P * 32;
-            offJ[m] = Q * 32;
-            if (pb + m + 1 < pe) {
-                if (Q == P) {
-                    ++P;
-                    Q = 0;
-                } else {
-                    ++Q;
-                }
-            }
+        int q = pb, rem = Tp;
+        while (q >= rem) {  // column-major lower patches: column Q has Tp - Q of them
+            q -= rem;
+            ++Q;
+            --rem;
         }
+        P = Q + q;
     }
-    // hi x hi (the leading products) and the corrections in FP32 sums of
-    // their own: the corrections (at most 2^-8 of the leading term) do not
-    // add their roundings at the leading term's size
-    f32x4 acc[NPW][4], acc_c[NPW][4];
 #pragma unroll
-    for (int m = 0; m < NPW; ++m)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) acc[m][t] = acc_c[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    produce(0);
-    __syncthreads();  // the first slab is in
-    for (int sl = 0; sl < nslab; ++sl) {
-        if (nbuf == 2 && sl + 1 < nslab) produce(sl + 1);  // into the other buffer
-        const unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
-        const unsigned short* vv = phi + S * ncols * 16;
-#pragma unroll
-        for (int m = 0; m < NPW; ++m) {
-            s16x4 fI[2][S], vI[2][S], fJ[2][S], vJ[2][S];
+    for (int m = 0; m < NPW; ++m) {
+        if (pb + m < pe) {
+            bf16x8 aI[2][S], bJ[2][S];
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const size_t so = (size_t)s * ncols * 16;
-                    fI[h][s] = *reinterpret_cast<const s16x4*>(phi + so + kslab_at(offI[m] + 16 * h + li, lq));
-                    vI[h][s] = *reinterpret_cast<const s16x4*>(vv + so + kslab_at(offI[m] + 16 * h + li, lq));
-                    fJ[h][s] = *reinterpret_cast<const s16x4*>(phi + so + kslab_at(offJ[m] + 16 * h + li, lq));
-                    vJ[h][s] = *reinterpret_cast<const s16x4*>(vv + so + kslab_at(offJ[m] + 16 * h + li, lq));
+                    const size_t so = (size_t)s * wcols * 16;
+                    aI[h][s] = *reinterpret_cast<const bf16x8*>(fa + so + kslab_at(32 * P + 16 * h + li - c0, lq & 1));
+                    bJ[h][s] = *reinterpret_cast<const bf16x8*>(fb + so + kslab_at(32 * Q + 16 * h + li - c0, lq & 1));
                 }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int hi = t >> 1, hj = t & 1;
-                acc[m][t] = mfma_bf16_k16(fI[hi][0], vJ[hj][0], acc[m][t]);
-                acc[m][t] = mfma_bf16_k16(vI[hi][0], fJ[hj][0], acc[m][t]);
+                f32x4 cs = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int q = 1; q < SlicePairs<S>::n; ++q) {
-                    const int sa = SlicePairs<S>::a[q], sb = SlicePairs<S>::b[q];
-                    acc_c[m][t] = mfma_bf16_k16(fI[hi][sa], vJ[hj][sb], acc_c[m][t]);
-                    acc_c[m][t] = mfma_bf16_k16(vI[hi][sa], fJ[hj][sb], acc_c[m][t]);
-                }
+                for (int q = 1; q < SlicePairs<S>::n; ++q)
+                    cs = mfma_bf16(aI[hi][SlicePairs<S>::a[q]], bJ[hj][SlicePairs<S>::b[q]], cs);
+                const f32x4 ls = mfma_bf16(aI[hi][0], bJ[hj][0], f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[m][t][r] += (double)ls[r] + (double)cs[r];
             }
         }
-        __syncthreads();  // slab sl consumed (and, two buffers, slab sl + 1 in)
-        if (nbuf == 1) {
-            if (sl + 1 < nslab) produce(sl + 1);
-            __syncthreads();  // single buffer: the next slab is in
-        }
-    }
-    // scatter: register r of lane l is tile element (4 (l / 16) + r, l % 16)
-#pragma unroll
-    for (int m = 0; m < NPW; ++m) {
-        if (pb + m < pe) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int lc = offJ[m] + 16 * (t & 1) + li;
-                const int gj = s_cols[lc];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int lr = offI[m] + 16 * (t >> 1) + 4 * lq + r;
-                    const int gi = s_cols[lr];
-                    if (gi >= 0 && gj >= 0 && lr >= lc)
-                        atomicAdd(K + (int64_t)gi * N + gj, (double)acc[m][t][r] + (double)acc_c[m][t][r]);
-                }
-            }
+        if (++P == Tp) {
+            ++Q;
+            P = Q;
         }
     }
 }
 
 template <int S, int NW, int NPW>
-__global__ __launch_bounds__(NW * 64) void k_kmat_split(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict__ work, const KGroup* __restrict__ groups,
+                                                    const BlockDesc* __restrict__ blocks, const int* __restrict__ gblocks,
                                                     const double* __restrict__ pts, int64_t n_slots,
                                                     const double* __restrict__ shrec, int RS,
-                                                    const int* __restrict__ bshells, const int* __restrict__ bshell_col,
-                                                    const int* __restrict__ bcols, const int* __restrict__ bsorder,
-                                                    const double* __restrict__ coef, double* __restrict__ K, int N) {
-    __shared__ double s_pt[kBlockPts][3];
-    __shared__ double s_cf[kBlockPts][4];
+                                                    const int* __restrict__ gshells, const int* __restrict__ gshell_col,
+                                                    const int* __restrict__ gcols, const int* __restrict__ gsorder,
+                                                    const unsigned* __restrict__ gmasks,
+                                                    const double* __restrict__ coef, double* __restrict__ K) {
+    __shared__ double s_pt[2][kBlockPts][3];
+    __shared__ double s_cf[2][kBlockPts][4];
+    __shared__ unsigned s_msk[2][kMaxMaskWords];
+    __shared__ int s_first[kMaxGroup + 1];  // first slab of each block of the group, then the slab count
+    __shared__ int s_blk[kMaxGroup];
     extern __shared__ __attribute__((aligned(16))) double s_dyn[];
-    const KWork W = work[blockIdx.x];
-    const BlockDesc B = blocks[W.block];
+    const KGWork W = work[blockIdx.x];
+    const KGroup G = groups[W.group];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int ncols = W.ncols;  // multiple of 32
-    const int nsh = W.nsh;
-    const int nbuf = W.ps;      // slab buffers: 1 or 2
-    const int BUF = 2 * S * ncols * 16;
-    // LDS: slab buffers (bf16) | records | cols [ncols] | shcol [nsh] | shid [nsh]
+    constexpr int NT = NW * 64;
+    const int c0 = W.c0;
+    const int wcols = W.c1 - W.c0;  // multiple of 32
+    const int Tp = W.c1 / 32;
+    const int nsh = G.nshell - W.k0;
+    const int nbuf = W.nbuf;
+    const int BUF = 2 * S * wcols * 16;
+    // LDS: slab buffers (bf16) | records | cols [wcols] | shcol, shid, shk [nsh]
     unsigned short* s_slab = reinterpret_cast<unsigned short*>(s_dyn);
     const int slab_dbl = (nbuf * BUF * 2 + 15) / 16 * 2;
     double* s_rec = s_dyn + slab_dbl;
     int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
-    int* s_shcol = s_cols + ncols;
+    int* s_shcol = s_cols + wcols;
     int* s_shid = s_shcol + nsh;
-    constexpr int NT = NW * 64;
-    for (int c = tid; c < ncols; c += NT) s_cols[c] = (c < B.nao) ? bcols[B.col_begin + c] : -1;
-    if (wave == NW / 2) {  // shells this item needs, in (l, nprim) order
+    int* s_shk = s_shid + nsh;
+    for (int c = tid; c < wcols; c += NT) s_cols[c] = (c0 + c < G.nao) ? gcols[G.col_begin + c0 + c] : -1;
+    if (tid == 0) {
+        int f = 0;
+        for (int j = 0; j < G.nsub; ++j) {
+            const int b = gblocks[G.sub_begin + j];
+            s_blk[j] = b;
+            s_first[j] = f;
+            f += (blocks[b].npts + 15) / 16;
+        }
+        s_first[G.nsub] = f;
+    }
+    if (wave == NW / 2) {  // the item's shells (union shells k0 ..), costliest (l, nprim) last
         int base = 0;
-        for (int i0 = 0; i0 < B.nshell; i0 += 64) {
+        for (int i0 = 0; i0 < G.nshell; i0 += 64) {
             const int i = i0 + lane;
-            const int sh = (i < B.nshell) ? bsorder[B.shell_begin + i] : B.nshell;
-            const bool keep = sh < nsh;
+            const int sh = (i < G.nshell) ? gsorder[G.shell_begin + i] : -1;
+            const bool keep = sh >= W.k0;
             const unsigned long long msk = __ballot(keep);
             const int pos = base + __popcll(msk & ((1ull << lane) - 1ull));
             if (keep) {
-                s_shid[pos] = bshells[B.shell_begin + sh];
-                s_shcol[pos] = bshell_col[B.shell_begin + sh];
+                s_shid[pos] = gshells[G.shell_begin + sh];
+                s_shcol[pos] = gshell_col[G.shell_begin + sh] - c0;  // may be < 0 for a shell across c0
+                s_shk[pos] = sh;
             }
             base += __popcll(msk);
         }
     }
-    for (int q = tid; q < kBlockPts; q += NT) {
-        const int64_t slot = B.pt_begin + q;
-        s_pt[q][0] = pts[slot];
-        s_pt[q][1] = pts[n_slots + slot];
-        s_pt[q][2] = pts[2 * n_slots + slot];
-        s_cf[q][0] = coef[4 * slot + 0];
-        s_cf[q][1] = coef[4 * slot + 1];
-        s_cf[q][2] = coef[4 * slot + 2];
-        s_cf[q][3] = coef[4 * slot + 3];
-    }
     {
-        uint4* z = reinterpret_cast<uint4*>(s_dyn);  // columns past the AOs stay zero
+        uint4* z = reinterpret_cast<uint4*>(s_dyn);  // columns past the union stay zero
         for (int i = tid; i < slab_dbl / 2; i += NT) z[i] = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
@@ -519,13 +490,35 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KWork* __restrict_
         const int k = idx / RS;
         s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
     }
+    // points, point terms and kept-shell masks of block j of the group, into
+    // the buffers of j's parity
+    auto load_block = [&](int j) {
+        const int par = j & 1;
+        const BlockDesc& B = blocks[s_blk[j]];
+        for (int q = tid; q < kBlockPts; q += NT) {
+            const int64_t slot = B.pt_begin + q;
+            s_pt[par][q][0] = pts[slot];
+            s_pt[par][q][1] = pts[n_slots + slot];
+            s_pt[par][q][2] = pts[2 * n_slots + slot];
+            s_cf[par][q][0] = coef[4 * slot + 0];
+            s_cf[par][q][1] = coef[4 * slot + 1];
+            s_cf[par][q][2] = coef[4 * slot + 2];
+            s_cf[par][q][3] = coef[4 * slot + 3];
+        }
+        for (int w = tid; w < G.nword; w += NT) s_msk[par][w] = gmasks[G.mask_begin + j * G.nword + w];
+    };
+    load_block(0);
     __syncthreads();
+    const int nslab = s_first[G.nsub];
 
-    const int nslab = (B.npts + 15) / 16;
-    // phi and V = a phi + c . grad phi of a 16-point slab, FP64, sliced into the slab buffer
-    auto produce = [&](int sl) {
-        unsigned short* phi = s_slab + (nbuf == 2 ? (sl & 1) * BUF : 0);
-        unsigned short* vv = phi + S * ncols * 16;
+    // phi and V of global slab gs, FP64, sliced into the slab buffer
+    auto produce = [&](int gs) {
+        int j = 0;
+        while (gs >= s_first[j + 1]) ++j;
+        const int par = j & 1;
+        const int p0 = (gs - s_first[j]) * 16;
+        unsigned short* phi = s_slab + (nbuf == 2 ? (gs & 1) * BUF : 0);
+        unsigned short* vv = phi + S * wcols * 16;
         // (shell, point) items in rounds of NT: shells from the costliest
         // (l, nprim) down, and every other round's waves in reverse order, so
         // a wave's rounds mix dear and cheap shells
@@ -535,39 +528,128 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KWork* __restrict_
             if (it >= nsh * 16) continue;
             const int p = it & 15;
             const int k = nsh - 1 - (it >> 4);
-            const int pp = sl * 16 + p;
+            const int pp = p0 + p;
+            const int kk = s_shk[k];
+            const bool kept = (s_msk[par][kk >> 5] >> (kk & 31)) & 1u;
+            const int cs = s_shcol[k];
+            if (!kept) {  // a shell this block's screening dropped: zero columns
+                const int l = ((int)s_rec[k * RS + 3]) & 3;
+                for (int m = 0; m < 2 * l + 1; ++m) {
+                    const int c = cs + m;
+                    if (c < 0) continue;
+                    if (c >= wcols) break;
+                    const int off = kslab_at(c, p >> 3) + (p & 7);
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        phi[(size_t)s * wcols * 16 + off] = 0;
+                        vv[(size_t)s * wcols * 16 + off] = 0;
+                    }
+                }
+                continue;
+            }
             // radial sums once, then phi and grad phi per component from the
             // angular polynomial (keeps the producer within its registers)
             double r3[3], R0, R1, R2;
-            const int l = rec_radial2(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], r3, R0, R1, R2);
-            const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
-            const int c0 = s_shcol[k];
+            const int l = rec_radial2(s_rec + k * RS, s_pt[par][pp][0], s_pt[par][pp][1], s_pt[par][pp][2], r3, R0,
+                                      R1, R2);
+            const double a = s_cf[par][pp][0], cx = s_cf[par][pp][1], cy = s_cf[par][pp][2], cz = s_cf[par][pp][3];
             for (int m = 0; m < 2 * l + 1; ++m) {
-                const int c = c0 + m;
-                if (c >= ncols) break;
+                const int c = cs + m;
+                if (c < 0) continue;
+                if (c >= wcols) break;
                 double f, fg[3], fh[6];
                 angular2(l, m, r3, f, fg, fh);
                 const double v = f * R0;
                 const double vx = fg[0] * R0 + f * r3[0] * R1;
                 const double vy = fg[1] * R0 + f * r3[1] * R1;
                 const double vz = fg[2] * R0 + f * r3[2] * R1;
-                const int off = kslab_at(c, p >> 2) + (p & 3);
+                const int off = kslab_at(c, p >> 3) + (p & 7);
                 unsigned short h[S];
                 split_bf16<S>(v, h);
 #pragma unroll
-                for (int s = 0; s < S; ++s) phi[(size_t)s * ncols * 16 + off] = h[s];
+                for (int s = 0; s < S; ++s) phi[(size_t)s * wcols * 16 + off] = h[s];
                 split_bf16<S>(a * v + cx * vx + cy * vy + cz * vz, h);
 #pragma unroll
-                for (int s = 0; s < S; ++s) vv[(size_t)s * ncols * 16 + off] = h[s];
+                for (int s = 0; s < S; ++s) vv[(size_t)s * wcols * 16 + off] = h[s];
             }
         }
     };
+
     // the item's patches over the waves, NPW each; every wave runs the same
     // production and barrier sequence
-    const bool has = W.t0 + wave * NPW < W.t1;
-    const int pb = has ? W.t0 + wave * NPW : W.t1 - 1;
-    const int pe = has ? min(pb + NPW, W.t1) : pb;
-    kmat_split_run<S, NPW>(pb, pe, nslab, nbuf, ncols, lane, s_slab, s_cols, K + B.poff, B.ldp, produce);
+    const bool has = W.q0 + wave * NPW < W.q1;
+    const int pb = has ? W.q0 + wave * NPW : W.q1 - 1;
+    const int pe = has ? min(pb + NPW, W.q1) : pb;
+    v4d acc[NPW][4];
+#pragma unroll
+    for (int m = 0; m < NPW; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[m][t] = v4d{0.0, 0.0, 0.0, 0.0};
+    produce(0);
+    __syncthreads();  // the first slab is in
+    int jn = 1;       // next block of the group to load
+    for (int gs = 0; gs < nslab; ++gs) {
+        if (gs + 1 < nslab && jn < G.nsub && gs + 1 == s_first[jn]) {
+            // the next slab starts block jn: its points into the other parity's
+            // buffers (their last reader, block jn - 2, finished production
+            // before an earlier barrier)
+            load_block(jn);
+            ++jn;
+            __syncthreads();
+        }
+        if (nbuf == 2 && gs + 1 < nslab) produce(gs + 1);  // into the other buffer
+        kmat_split_consume<S, NPW>(pb, pe, Tp, c0, wcols, lane, s_slab + (nbuf == 2 ? (gs & 1) * BUF : 0), acc);
+        __syncthreads();  // slab gs consumed (and, two buffers, slab gs + 1 in)
+        if (nbuf == 1) {
+            if (gs + 1 < nslab) produce(gs + 1);
+            __syncthreads();  // single buffer: the next slab is in
+        }
+    }
+    // scatter: register r of lane l is tile element (4 (l / 16) + r, l % 16)
+    {
+        const int li = lane & 15, lq = lane >> 4;
+        double* Km = K + G.poff;
+        const int N = G.ldp;
+        int P = 0, Q = 0;
+        {
+            int q = pb, rem = Tp;
+            while (q >= rem) {
+                q -= rem;
+                ++Q;
+                --rem;
+            }
+            P = Q + q;
+        }
+#pragma unroll
+        for (int m = 0; m < NPW; ++m) {
+            if (pb + m < pe) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int lc = 32 * Q + 16 * (t & 1) + li;
+                    const int gj = s_cols[lc - c0];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int lr = 32 * P + 16 * (t >> 1) + 4 * lq + r;
+                        const int gi = s_cols[lr - c0];
+                        if (gi >= 0 && gj >= 0 && lr >= lc) atomicAdd(Km + (int64_t)gi * N + gj, acc[m][t][r]);
+                    }
+                }
+            }
+            if (++P == Tp) {
+                ++Q;
+                P = Q;
+            }
+        }
+    }
+}
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x16_bf16: lane l holds A[l % 16][4 (l / 16) + e],
+// B[4 (l / 16) + e][l % 16], e = 0..3; the result as for the 16x16x32 form
+// (the self test below checks it beside the 16x16x32 form the kernels use)
+__device__ inline f32x4 mfma_bf16_k16(s16x4 a, s16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
 // layout self test: A(i, k) = ((3 i + 5 k) % 11) - 5, B(k, j) = ((7 k + 2 j) % 13) - 6;
@@ -640,9 +722,9 @@ static int rho_split_one(Context& c, const LaunchRange& r) {
 template <int S, int NW, int NPW>
 static int kmat_split_launch(Context& c, const LaunchRange& r) {
     hipLaunchKernelGGL((k_kmat_split<S, NW, NPW>), dim3(r.count), dim3(NW * 64), r.lds, c.stream,
-                       reinterpret_cast<const KWork*>(c.d_sk_work) + r.offset, c.d_blocks, c.d_pts, c.n_slots,
-                       c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col, c.d_block_cols,
-                       c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
+                       c.d_kg_work + r.offset, c.d_kgroups, c.d_blocks, c.d_kg_blocks, c.d_pts, c.n_slots,
+                       c.d_shrec, c.rec_stride, c.d_kg_shells, c.d_kg_shell_col, c.d_kg_cols, c.d_kg_sorder,
+                       c.d_kg_masks, c.d_coef, c.d_K);
     return check("k_kmat_split");
 }
 

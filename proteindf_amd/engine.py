@@ -44,7 +44,7 @@ EXPORTED = [
     "pdfxc_build_uks_device", "pdfxc_build_uks_update_device", "pdfxc_eval_functional_uks",
     "pdfxc_density_at_points", "pdfxc_batch_create", "pdfxc_batch_destroy", "pdfxc_batch_offsets",
     "pdfxc_batch_build_rks_device", "pdfxc_batch_build_rks", "pdfxc_batch_stats", "pdfxc_set_tuning",
-    "pdfxc_autotune", "pdfxc_grid_opts_preset", "pdfxc_generate_grid_ex",
+    "pdfxc_autotune", "pdfxc_grid_opts_preset", "pdfxc_generate_grid_ex", "pdfxc_generate_grid_oriented",
 ]
 # matrix file types (TlMatrixObject::MatrixType)
 PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
@@ -64,7 +64,7 @@ class Opts(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("rho_shift", ctypes.c_int), ("k_tiles", ctypes.c_int), ("split_rho_shift", ctypes.c_int),
-                ("split_k_patches", ctypes.c_int), ("ms", ctypes.c_double)]
+                ("split_k_patches", ctypes.c_int), ("split_k_group", ctypes.c_int), ("ms", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -125,6 +125,7 @@ def lib():
         L.pdfxc_generate_grid.argtypes = [P, i32, P, P, i32, i32, d, P]
         L.pdfxc_grid_opts_preset.argtypes = [ctypes.c_char_p, P]
         L.pdfxc_generate_grid_ex.argtypes = [P, i32, P, P, P, P]
+        L.pdfxc_generate_grid_oriented.argtypes = [P, i32, P, P, P, P, P]
         L.pdfxc_get_grid.argtypes = [P, P, P, P, P]
         L.pdfxc_build_rks.argtypes = [P, i32, P, P, P, P]
         L.pdfxc_build_rks_device.argtypes = [P, i32, P, P, P]
@@ -321,6 +322,17 @@ class XcContext:
         self.n_pts = n.value
         return n.value
 
+    def generate_grid_oriented(self, Z, xyz, opts, O):
+        """generate_grid_ex with the angular grids in the fixed orientation O (3 x 3)."""
+        Z = np.ascontiguousarray(Z, dtype=np.int32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        O = np.ascontiguousarray(O, dtype=np.float64).reshape(3, 3)
+        n = ctypes.c_int64()
+        check(lib().pdfxc_generate_grid_oriented(self.h, len(Z), _p(Z), _p(xyz), ctypes.byref(opts), _p(O),
+                                                 ctypes.byref(n)))
+        self.n_pts = n.value
+        return n.value
+
     def load_grid_matrix(self, path):
         """grid.mat -> the context's grid (and point atoms, and stored rho); returns its column count."""
         n, nc = ctypes.c_int64(), ctypes.c_int()
@@ -331,8 +343,8 @@ class XcContext:
     def save_grid_matrix(self, path, n_rho_cols=4):
         check(lib().pdfxc_save_grid_matrix(self.h, _path(path), n_rho_cols))
 
-    def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=24):
-        t = Tuning(rho_shift, k_tiles, split_rho_shift, split_k_patches, 0.0)
+    def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=24, split_k_group=4):
+        t = Tuning(rho_shift, k_tiles, split_rho_shift, split_k_patches, split_k_group, 0.0)
         check(lib().pdfxc_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, dP_ptr, dK_ptr, dE_ptr, reps=3, functional=PDFXC_B3LYP):

@@ -35,3 +35,20 @@ def test_mode_tables_cover_every_mode():
     for (workload, mode, kernel), (nbytes, source) in bench.TRAFFIC_MEASURED.items():
         assert workload in bench.WORKLOADS and mode in bench.MODES
         assert kernel in {k for k, _ in bench.KERNELS[mode]} and nbytes > 0 and "profiles/" in source
+
+
+def test_split_k_group_out_of_range_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--split-k-group", "3"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "1, 2, 4 or 8" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_default_workload_is_the_headline_configuration():
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    ap_default = [a for a in open(ROOT / "bench.py").read().splitlines() if '"--workload", default=' in a]
+    assert ap_default and 'default="gly30"' in ap_default[0]
+    wl = bench.WORKLOADS["gly30"]
+    assert (wl["n_rad"], wl["n_ang"]) == (99, 590)

@@ -1,0 +1,237 @@
+"""The benchmark configurations on the GPU, at their full sizes.
+
+* (Gly)10 alpha-helix, 6-31G(d) (638 AOs), B3LYP, 75 x 302 grid: the HIP build
+  against the oracle on a fixed 100,000-point subset of the generated grid
+  (FP64 within 1e-10 unscreened and 1e-9 with the 1e-12 AO screening; the FP64
+  rho + hi+mid+lo K mode with E equal to the FP64 build and K within 1e-7).
+* (Gly)30 alpha-helix (1878 AOs), 99 x 590 grid (10.1 M points): size-
+  independent properties of the full build -- the split modes against FP64,
+  the electron count of a diagonal density, K = dE/dP by central differences,
+  K symmetric -- and the oracle on a 4,000-point subset.
+* 64 water clusters (H2O)_n, n ~ U{4..64} (seed 3), 75 x 302: the one-launch
+  batch equals every cluster's own build, and the two smallest clusters equal
+  the oracle.
+* (Gly)20 alpha-helix (1258 AOs), 75 x 302: the XC nuclear gradient against
+  central differences (step 1e-4 bohr) of the GPU energy on 16 random atoms,
+  each displaced along a random unit vector with the grid frame held fixed.
+
+P is the deterministic guess of proteindf_amd.guess throughout (the
+reference's SCF iterations are outside the XC path)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from proteindf_amd import calc_grid, engine, guess, molecules
+
+pytestmark = pytest.mark.gpu
+
+THREADS = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+
+
+def _build(info, grid, PA, precision=engine.PDFXC_FP64, ao_screen=0.0, want_rho=False):
+    ctx = engine.XcContext(info, engine.default_opts(precision=precision, ao_screen=ao_screen))
+    ctx.set_grid(grid)
+    E, K, rho = ctx.build_rks(guess.pack_lower(PA), want_rho=want_rho)
+    ctx.close()
+    return E, guess.unpack_lower(K, info.n_aos), rho
+
+
+# ---------------------------------------------------------------------------
+# (Gly)10, 75 x 302
+# ---------------------------------------------------------------------------
+def test_gly10_subset_against_oracle():
+    mol = molecules.polyglycine_helix(10, seed=1)
+    info = calc_grid.orbital_info(mol)
+    g = engine.XcContext(info, engine.default_opts(ao_screen=1e-12))
+    n = g.generate_grid(mol.Z, mol.xyz, 75, 302)
+    grid = g.get_grid()
+    g.close()
+    rng = np.random.default_rng(10)
+    sub = grid[np.sort(rng.choice(n, 100_000, replace=False))]
+    PA = 0.5 * guess.density_guess(mol, info, seed=0, eps=0.05)
+    say(f"(Gly)10: {info.n_aos} AOs, {n} grid points, subset {len(sub)}")
+    e_ref, K_ref, _ = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), sub, PA, n_threads=THREADS)
+    E0, K0, _ = _build(info, sub, PA)
+    say(f"  FP64 unscreened: E {E0:.12f} oracle {e_ref:.12f} |dE| {abs(E0 - e_ref):.2e} "
+        f"max|dK| {np.abs(K0 - K_ref).max():.2e}")
+    assert abs(E0 - e_ref) <= 1e-10
+    assert np.abs(K0 - K_ref).max() <= 1e-10
+    E1, K1, _ = _build(info, sub, PA, ao_screen=1e-12)
+    say(f"  FP64 screened 1e-12: |dE| {abs(E1 - e_ref):.2e} max|dK| {np.abs(K1 - K_ref).max():.2e}")
+    assert abs(E1 - e_ref) <= 1e-9 and np.abs(K1 - K_ref).max() <= 1e-9
+    E2, K2, _ = _build(info, sub, PA, precision=engine.PDFXC_SPLIT_K_BF16X3, ao_screen=1e-12)
+    say(f"  FP64 rho + hi+mid+lo K: E - E(FP64) {E2 - E1:.2e} max|dK| vs oracle {np.abs(K2 - K_ref).max():.2e}")
+    assert E2 == E1
+    assert np.abs(K2 - K_ref).max() <= 1e-7
+
+
+# ---------------------------------------------------------------------------
+# (Gly)30, 99 x 590
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def gly30():
+    mol = molecules.polyglycine_helix(30, seed=2)
+    info = calc_grid.orbital_info(mol)
+    ctx = engine.XcContext(info, engine.default_opts(ao_screen=1e-12))
+    n = ctx.generate_grid(mol.Z, mol.xyz, 99, 590)
+    grid = ctx.get_grid()
+    PA = 0.5 * guess.density_guess(mol, info, seed=0, eps=0.05)
+    say(f"(Gly)30: {info.n_aos} AOs, {n} grid points, {ctx.stats().n_blocks} blocks")
+    yield mol, info, grid, PA, ctx
+    ctx.close()
+
+
+def test_gly30_split_modes_against_fp64(gly30):
+    mol, info, grid, PA, ctx = gly30
+    E0, K0, _ = ctx.build_rks(guess.pack_lower(PA))
+    K0 = guess.unpack_lower(K0, info.n_aos)
+    for prec, e_tol, k_tol in ((engine.PDFXC_SPLIT_K_BF16X3, 0.0, 1e-7), (engine.PDFXC_SPLIT_BF16X3, 5e-5, 1e-7)):
+        E, K, _ = _build(info, grid, PA, precision=prec, ao_screen=1e-12)
+        say(f"  precision {prec}: |dE| {abs(E - E0):.2e} max|dK| {np.abs(K - K0).max():.2e} "
+            f"max|K| {np.abs(K0).max():.3f}")
+        assert abs(E - E0) <= e_tol
+        assert np.abs(K - K0).max() <= k_tol
+        np.testing.assert_array_equal(K, K.T)
+
+
+def test_gly30_electron_count_of_a_diagonal_density(gly30):
+    mol, info, grid, PA, ctx = gly30
+    occ = guess.atomic_occupations(mol, info)
+    n_elec = occ.sum()
+    _, _, rho = _build(info, grid, 0.5 * np.diag(occ), ao_screen=1e-12, want_rho=True)
+    count = 2.0 * float(np.dot(grid[:, 3], rho[:, 0]))
+    say(f"  electrons on the grid {count:.10f} of {n_elec:.1f}: relative {abs(count - n_elec) / n_elec:.2e}")
+    assert abs(count - n_elec) <= 1e-5 * n_elec
+
+
+def test_gly30_k_is_the_derivative_of_e(gly30):
+    mol, info, grid, PA, ctx = gly30
+    n = info.n_aos
+    E0, K, _ = ctx.build_rks(guess.pack_lower(PA))
+    Kd = guess.unpack_lower(K, n)
+    rng = np.random.default_rng(30)
+    h = 1e-5
+    for k in range(2):
+        D = rng.normal(size=(n, n))
+        D = 0.5 * (D + D.T)
+        ep = ctx.build_rks(guess.pack_lower(PA + h * D))[0]
+        em = ctx.build_rks(guess.pack_lower(PA - h * D))[0]
+        fd = (ep - em) / (2 * h)
+        ana = 2.0 * float((D * Kd).sum())
+        say(f"  direction {k}: fd {fd:.10f} 2<D,K> {ana:.10f} rel {abs(fd - ana) / abs(ana):.2e}")
+        assert abs(fd - ana) <= 1e-6 * max(abs(ana), 1.0)
+
+
+def test_gly30_subset_against_oracle(gly30):
+    mol, info, grid, PA, ctx = gly30
+    rng = np.random.default_rng(31)
+    sub = grid[np.sort(rng.choice(len(grid), 4000, replace=False))]
+    e_ref, K_ref, _ = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), sub, PA, n_threads=THREADS)
+    E, K, _ = _build(info, sub, PA)
+    say(f"  4000-point subset: |dE| {abs(E - e_ref):.2e} max|dK| {np.abs(K - K_ref).max():.2e}")
+    assert abs(E - e_ref) <= 1e-10
+    assert np.abs(K - K_ref).max() <= 1e-10
+    E3, K3, _ = _build(info, sub, PA, precision=engine.PDFXC_SPLIT_K_BF16X3)
+    assert E3 == E and np.abs(K3 - K_ref).max() <= 1e-7
+
+
+# ---------------------------------------------------------------------------
+# 64 water clusters, 75 x 302, one batch
+# ---------------------------------------------------------------------------
+def test_64_cluster_batch_equals_each_cluster_and_the_oracle():
+    mols = molecules.water_cluster_batch(64, 4, 64, seed=3)
+    prec = engine.PDFXC_SPLIT_K_BF16X3
+    ctxs, Ps, single = [], [], []
+    for k, mol in enumerate(mols):
+        info = calc_grid.orbital_info(mol)
+        ctx = engine.XcContext(info, engine.default_opts(ao_screen=1e-12, precision=prec))
+        ctx.generate_grid(mol.Z, mol.xyz, 75, 302)
+        PA = 0.5 * guess.density_guess(mol, info, seed=0, eps=0.05)
+        E, K, _ = ctx.build_rks(guess.pack_lower(PA))
+        ctxs.append(ctx)
+        Ps.append(PA)
+        single.append((E, K))
+        if k % 16 == 15:
+            say(f"  {k + 1} clusters set up")
+    npts = sum(c.n_pts for c in ctxs)
+    batch = engine.XcBatch(ctxs)
+    say(f"64 clusters: {npts} points, {batch.stats().n_blocks} blocks, "
+        f"n {min(m.n_atoms // 3 for m in mols)}..{max(m.n_atoms // 3 for m in mols)}")
+    for rep in range(2):  # capture, then a replay
+        E, Ks = batch.build_rks([guess.pack_lower(P) for P in Ps])
+        worst = 0.0
+        for i, (Ei, Ki) in enumerate(single):
+            assert E[i] == Ei, (rep, i)
+            worst = max(worst, float(np.abs(Ks[i] - Ki).max()))
+        say(f"  batch build {rep}: every E equal, max|dK| vs own builds {worst:.2e}")
+        assert worst <= 1e-13
+    batch.close()
+    # the two smallest clusters against the oracle
+    for i in np.argsort([m.n_atoms for m in mols], kind="stable")[:2]:
+        mol, ctx = mols[i], ctxs[i]
+        info = calc_grid.orbital_info(mol)
+        grid = ctx.get_grid()
+        e_ref, K_ref, _ = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), grid, Ps[i], n_threads=THREADS)
+        Ei, Ki = single[i]
+        dK = np.abs(guess.unpack_lower(Ki, info.n_aos) - K_ref).max()
+        say(f"  cluster {i} ((H2O){mol.n_atoms // 3}, {len(grid)} points): |dE| {abs(Ei - e_ref):.2e} max|dK| {dK:.2e}")
+        assert abs(Ei - e_ref) <= 1e-9 and dK <= 1e-7
+    for c in ctxs:
+        c.close()
+
+
+# ---------------------------------------------------------------------------
+# (Gly)20, 75 x 302: the nuclear gradient against central differences
+# ---------------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def gly20_gradient():
+    mol = molecules.polyglycine_helix(20, seed=4)
+    info = calc_grid.orbital_info(mol)
+    PA = 0.5 * guess.density_guess(mol, info, seed=0, eps=0.05)
+    O = oracle.o_matrix(mol.Z, mol.xyz)
+    opts = engine.grid_opts_preset("user", n_rad=75, n_ang=302, pruning=engine.PDFXC_PRUNE_NONE,
+                                   partitioning=engine.PDFXC_PARTITION_BECKE, size_adjust=1)
+    ctx = engine.XcContext(info, engine.default_opts(ao_screen=1e-12))
+    n = ctx.generate_grid_oriented(mol.Z, mol.xyz, opts, O)
+    E, gf, gw = ctx.gradient_rks(mol.Z, mol.xyz, guess.pack_lower(PA))
+    say(f"(Gly)20: {info.n_aos} AOs, {n} points, E {E:.10f}, max|g| {np.abs(gf + gw).max():.3e}")
+    rng = np.random.default_rng(20)
+    atoms = rng.choice(mol.n_atoms, 16, replace=False)
+    dirs = rng.normal(size=(16, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    yield mol, info, PA, O, opts, ctx, gf + gw, atoms, dirs
+    ctx.close()
+
+
+@pytest.mark.parametrize("half", [0, 1])
+def test_gly20_gradient_against_central_differences(gly20_gradient, half):
+    mol, info, PA, O, opts, ctx, g, atoms, dirs = gly20_gradient
+    h = 1e-4
+    P = guess.pack_lower(PA)
+
+    def energy(xyz):
+        m = molecules.Molecule(mol.symbols, xyz, mol.name)
+        c = engine.XcContext(calc_grid.orbital_info(m), engine.default_opts(ao_screen=1e-12))
+        c.generate_grid_oriented(m.Z, m.xyz, opts, O)
+        e = c.build_rks(P)[0]
+        c.close()
+        return e
+
+    worst = 0.0
+    for k in range(8 * half, 8 * half + 8):
+        A, u = atoms[k], dirs[k]
+        xp, xm = mol.xyz.copy(), mol.xyz.copy()
+        xp[A] += h * u
+        xm[A] -= h * u
+        fd = (energy(xp) - energy(xm)) / (2 * h)
+        ana = float(g[A] @ u)
+        worst = max(worst, abs(fd - ana))
+        say(f"  atom {A} ({mol.symbols[A]}): dE/dh fd {fd:.10f} gradient {ana:.10f} diff {abs(fd - ana):.2e}")
+        assert abs(fd - ana) <= 5e-8 + 1e-6 * abs(fd), (A, fd, ana)
+    say(f"  worst |fd - g.u| {worst:.2e}")

@@ -36,14 +36,18 @@ def test_every_shape_gives_the_same_build(prec):
               dict(split_rho_shift=2, split_k_patches=4), dict(split_k_patches=8),
               # 12 waves of two patches and 8 waves of four, full and partial items
               dict(split_k_patches=24), dict(split_k_patches=21), dict(split_k_patches=32),
-              dict(split_k_patches=29)]
+              dict(split_k_patches=29),
+              # split K work units of 1, 2 and 8 blocks (the default is 4)
+              dict(split_k_group=1), dict(split_k_group=2), dict(split_k_group=8),
+              dict(split_k_group=8, split_k_patches=16), dict(split_k_group=1, split_k_patches=32)]
     for sh in shapes:
         ctx.set_tuning(**sh)
         E, K, _ = ctx.build_rks(P)
         say(prec, sh, "dE", abs(E - E0), "max dK", np.abs(K - K0).max())
         assert abs(E - E0) <= 1e-12 * abs(E0)
         np.testing.assert_allclose(K, K0, rtol=0, atol=1e-12)
-    for bad in (dict(k_tiles=200), dict(split_k_patches=33), dict(split_k_patches=3)):
+    for bad in (dict(k_tiles=200), dict(split_k_patches=33), dict(split_k_patches=3), dict(split_k_group=3),
+                dict(split_k_group=16), dict(split_k_group=0)):
         with pytest.raises(engine.PdfxcError) as ei:
             ctx.set_tuning(**bad)
         assert ei.value.code == engine.ERROR_CODES["ARGUMENT"]
@@ -93,10 +97,10 @@ def test_split_k_shapes_on_wide_blocks(prec):
     assert st.max_nao > 160
     P = guess.pack_lower(PA)
     E0, K0, _ = ctx.build_rks(P)
-    for kp in (24, 21, 32, 29, 16):
-        ctx.set_tuning(split_k_patches=kp)
+    for kp, kg in ((24, 4), (21, 4), (32, 4), (29, 4), (16, 4), (24, 1), (16, 2), (32, 8), (24, 8)):
+        ctx.set_tuning(split_k_patches=kp, split_k_group=kg)
         E, K, _ = ctx.build_rks(P)
-        say(prec, "split_k_patches", kp, "dE", abs(E - E0), "max dK", np.abs(K - K0).max())
+        say(prec, "split_k_patches", kp, "split_k_group", kg, "dE", abs(E - E0), "max dK", np.abs(K - K0).max())
         assert abs(E - E0) <= 1e-12 * abs(E0)
         np.testing.assert_allclose(K, K0, rtol=0, atol=1e-12)
     ctx.close()
