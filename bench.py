@@ -183,7 +183,8 @@ def _tally(tunes):
     out = {}
     for t in tunes:
         key = f"rho_shift={t['rho_shift']} k_tiles={t['k_tiles']} split_rho_shift={t['split_rho_shift']} " \
-              f"split_k_patches={t['split_k_patches']} split_k_group={t['split_k_group']}"
+              f"split_k_patches={t['split_k_patches']} split_k_group={t['split_k_group']} " \
+              f"persistent={t['persistent']}"
         out[key] = out.get(key, 0) + 1
     return out
 

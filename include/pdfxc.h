@@ -95,7 +95,11 @@ typedef struct pdfxc_stats {
  * neighbours in the point bisection) whose K contributions the split K pass
  * sums on the union of their AO columns before one scatter into K, i.e.
  * 128 x split_k_group points per K work unit (a group too wide for the
- * kernel's LDS is halved).  Defaults 0, 96, 0, 24, 4.
+ * kernel's LDS is halved); persistent (0 or 1): the rho and K kernels run as
+ * persistent launches, as many workgroups as stay resident on the device,
+ * each taking work items (blocks, K items; of every molecule in a batch) from
+ * one device counter until they run out, instead of one workgroup per item.
+ * Defaults 0, 96, 0, 24, 4, 1.
  * ms: the build time measured for it (pdfxc_autotune), else 0. */
 typedef struct pdfxc_tuning {
     int rho_shift;
@@ -103,6 +107,7 @@ typedef struct pdfxc_tuning {
     int split_rho_shift;
     int split_k_patches;
     int split_k_group;
+    int persistent;
     double ms;
 } pdfxc_tuning;
 
@@ -252,9 +257,13 @@ int pdfxc_eval_functional_uks(int functional, int64_t n, const double* in, doubl
  * made it; size-adjusted Becke for a caller's grid).  Dummy atoms are refused
  * with PDFXC_ERR_UNSUPPORTED.  (The reference's
  * own weight term evaluates to zero, DfGenerateGrid.cpp:2355-2372.)  Their
- * sum is the gradient of E_xc.  E_xc (may be NULL) is returned.  FP64
- * contractions whatever the context's precision.  The context's stored rho
- * becomes rho[P_alpha] (an update-mode SCF restarts from a full build). */
+ * sum is the gradient of E_xc.  E_xc (may be NULL) is returned.  The
+ * context's precision applies: FP64, or, in a split mode, the functional
+ * term's contractions X = P phi and Z = P W on bf16 MFMA with the mode's
+ * slices (hi+lo, or hi+mid+lo for PDFXC_SPLIT_BF16X3 and PDFXC_SPLIT_K_BF16X3)
+ * and rho from the build's own rho pass (FP64 in PDFXC_SPLIT_K_BF16X3); the
+ * rest stays FP64.  The context's stored rho becomes rho[P_alpha] (an
+ * update-mode SCF restarts from a full build). */
 int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z, const double* xyz,
                        const int* point_atom, const double* P_alpha_packed, double* grad_functional,
                        double* grad_weight, double* E_xc);

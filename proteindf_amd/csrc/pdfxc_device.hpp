@@ -18,6 +18,20 @@ __device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Work-queue step of a persistent launch: the workgroup's next item, taken
+// from *queue with one atomic (queue != nullptr; the launch has fewer
+// workgroups than items and the host zeroes *queue before it), or the single
+// item blockIdx.x (queue == nullptr; the second call returns n_items).  The
+// barrier before the fetch keeps the previous item's LDS use apart from the
+// next item's, the one after it publishes the index.  Every workgroup leaves
+// its loop once the counter passes n_items.
+__device__ inline int next_item(int* queue, int n_items, int iter, int* s_item) {
+    __syncthreads();
+    if (threadIdx.x == 0) *s_item = queue ? atomicAdd(queue, 1) : (iter == 0 ? (int)blockIdx.x : n_items);
+    __syncthreads();
+    return *s_item;
+}
+
 constexpr double kTooBig = 30.0;  // DfCalcGridX::TOOBIG, primitives with alpha r^2 > 30 are skipped
 constexpr double kInvSqrt3 = 0.57735026918962576451;
 

@@ -23,6 +23,7 @@
 
 #include "pdfxc_device.hpp"
 #include "pdfxc_internal.hpp"
+#include "pdfxc_split.hpp"
 
 namespace pdfxc {
 
@@ -84,7 +85,103 @@ __device__ inline void grad_tiles(int wave, int lane, int T, const int* s_cols, 
         }
 }
 
-template <int NIMAX>
+// The same X^T and Z^T tiles on v_mfma_f32_16x16x32_bf16 with S bf16 slices
+// per operand (P sliced as it is loaded, phi and W as they are read from the
+// planes; padding columns are zero): each 32-deep K step sums its hi x hi
+// products and its slice corrections in two FP32 accumulators from zero and
+// adds both into FP64, as the split rho pass does.  Register r of lane l
+// holds D[4 (l / 16) + r][l % 16] (AO row, point).
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <int S, int NI>
+__device__ inline void grad_tiles_split(int wave, int lane, int ldn, const int* s_cols, double* s_phi, double* s_w,
+                                        const double* __restrict__ P, int N) {
+    const int li = lane & 15, lq = lane >> 4;
+    v4d acc0[NI], acc1[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        acc0[i] = v4d{0.0, 0.0, 0.0, 0.0};
+        acc1[i] = v4d{0.0, 0.0, 0.0, 0.0};
+    }
+    int gi[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) gi[i] = max(s_cols[(wave + 8 * i) * 16 + li], 0);
+    const int nks = (ldn + 31) / 32;
+    for (int ks = 0; ks < nks; ++ks) {
+        u16x8 bp[S], bw[S];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = 32 * ks + 8 * lq + e;
+            const double vp = (k < ldn) ? s_phi[k * 16 + li] : 0.0;
+            const double vw = (k < ldn) ? s_w[k * 16 + li] : 0.0;
+            unsigned short h[S];
+            split_bf16<S>(vp, h);
+#pragma unroll
+            for (int q = 0; q < S; ++q) bp[q][e] = h[q];
+            split_bf16<S>(vw, h);
+#pragma unroll
+            for (int q = 0; q < S; ++q) bw[q][e] = h[q];
+        }
+        bf16x8 Bp[S], Bw[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) {
+            Bp[q] = __builtin_bit_cast(bf16x8, bp[q]);
+            Bw[q] = __builtin_bit_cast(bf16x8, bw[q]);
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            u16x8 ap[S];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = 32 * ks + 8 * lq + e;
+                const int col = (k < ldn) ? s_cols[k] : -1;
+                const double x = (col >= 0) ? P[(int64_t)col * N + gi[i]] : 0.0;
+                unsigned short h[S];
+                split_bf16<S>(x, h);
+#pragma unroll
+                for (int q = 0; q < S; ++q) ap[q][e] = h[q];
+            }
+            bf16x8 A[S];
+#pragma unroll
+            for (int q = 0; q < S; ++q) A[q] = __builtin_bit_cast(bf16x8, ap[q]);
+            const f32x4 z = f32x4{0.f, 0.f, 0.f, 0.f};
+            const f32x4 h0 = mfma_bf16(A[0], Bp[0], z);
+            const f32x4 h1 = mfma_bf16(A[0], Bw[0], z);
+            f32x4 c0 = z, c1 = z;
+#pragma unroll
+            for (int q = 1; q < SlicePairs<S>::n; ++q) {
+                c0 = mfma_bf16(A[SlicePairs<S>::a[q]], Bp[SlicePairs<S>::b[q]], c0);
+                c1 = mfma_bf16(A[SlicePairs<S>::a[q]], Bw[SlicePairs<S>::b[q]], c1);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc0[i][r] += (double)h0[r] + (double)c0[r];
+                acc1[i][r] += (double)h1[r] + (double)c1[r];
+            }
+        }
+    }
+    __syncthreads();  // every wave has stopped reading the planes
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int o = ((wave + 8 * i) * 16 + 4 * lq + r) * 16 + li;
+            s_phi[o] = acc0[i][r];
+            s_w[o] = acc1[i][r];
+        }
+}
+
+// X^T, Z^T tiles of one wave: FP64 MFMA (S = 0) or S bf16 slices
+template <int S, int NI>
+__device__ inline void grad_tiles_any(int wave, int lane, int T, int ldn, const int* s_cols, double* s_phi,
+                                      double* s_w, const double* __restrict__ P, int N) {
+    if constexpr (S == 0)
+        grad_tiles<NI>(wave, lane, T, s_cols, s_phi, s_w, P, N);
+    else
+        grad_tiles_split<S, NI>(wave, lane, ldn, s_cols, s_phi, s_w, P, N);
+}
+
+template <int S, int NIMAX>
 __global__ __launch_bounds__(512) void k_grad_fn(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                                  const double* __restrict__ pts, int64_t n_slots,
                                                  const double* __restrict__ shrec, int RS,
@@ -165,17 +262,17 @@ __global__ __launch_bounds__(512) void k_grad_fn(const int* __restrict__ work, c
     {
         const int nI = (T > wave) ? (T - wave + 7) / 8 : 0;
         switch (nI) {
-            case 1: grad_tiles<1>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp); break;
+            case 1: grad_tiles_any<S, 1>(wave, lane, T, ldn, s_cols, s_phi, s_w, P + B.poff, B.ldp); break;
             case 2:
-                if constexpr (NIMAX >= 2) grad_tiles<2>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp);
+                if constexpr (NIMAX >= 2) grad_tiles_any<S, 2>(wave, lane, T, ldn, s_cols, s_phi, s_w, P + B.poff, B.ldp);
                 else __syncthreads();  // unreachable: the host keeps T <= 8 NIMAX
                 break;
             case 3:
-                if constexpr (NIMAX >= 3) grad_tiles<3>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp);
+                if constexpr (NIMAX >= 3) grad_tiles_any<S, 3>(wave, lane, T, ldn, s_cols, s_phi, s_w, P + B.poff, B.ldp);
                 else __syncthreads();
                 break;
             case 4:
-                if constexpr (NIMAX >= 4) grad_tiles<4>(wave, lane, T, s_cols, s_phi, s_w, P + B.poff, B.ldp);
+                if constexpr (NIMAX >= 4) grad_tiles_any<S, 4>(wave, lane, T, ldn, s_cols, s_phi, s_w, P + B.poff, B.ldp);
                 else __syncthreads();
                 break;
             default: __syncthreads(); break;
@@ -375,25 +472,35 @@ static int check(const char* what) {
     return PDFXC_OK;
 }
 
-template <int NIMAX>
+template <int S, int NIMAX>
 static int grad_fn_one(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,
                        int n_atoms, double* d_gacc) {
-    hipLaunchKernelGGL((k_grad_fn<NIMAX>), dim3(r.count * (kBlockPts / 16)), dim3(512), r.lds, c.stream,
+    hipLaunchKernelGGL((k_grad_fn<S, NIMAX>), dim3(r.count * (kBlockPts / 16)), dim3(512), r.lds, c.stream,
                        c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
                        c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, d_shell_atom,
                        d_slot_atom, c.d_coef, c.d_P, c.n_aos, n_atoms, d_gacc);
     return check("k_grad_fn");
 }
 
+template <int S>
+static int grad_fn_cls(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,
+                       int n_atoms, double* d_gacc) {
+    switch (r.cls) {
+        case 0: return grad_fn_one<S, 1>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
+        case 1: return grad_fn_one<S, 2>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
+        case 2: return grad_fn_one<S, 4>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
+        default: set_error("gradient: blocks wider than 512 AO columns are not supported (raise ao_screen)");
+                 return PDFXC_ERR_UNSUPPORTED;
+    }
+}
+
 int launch_grad_fn(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,
                    int n_atoms, double* d_gacc) {
     if (r.count <= 0) return PDFXC_OK;
-    switch (r.cls) {
-        case 0: return grad_fn_one<1>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
-        case 1: return grad_fn_one<2>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
-        case 2: return grad_fn_one<4>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
-        default: set_error("gradient: blocks wider than 512 AO columns are not supported (raise ao_screen)");
-                 return PDFXC_ERR_UNSUPPORTED;
+    switch (c.opts.precision == PDFXC_FP64 ? 0 : prec_slices(c.opts.precision)) {
+        case 0: return grad_fn_cls<0>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
+        case 2: return grad_fn_cls<2>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
+        default: return grad_fn_cls<3>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
     }
 }
 

@@ -277,6 +277,11 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
     if (rc == PDFXC_OK) rc = dalloc(&c->d_P, (size_t)n_aos * n_aos);
     if (rc == PDFXC_OK) rc = dalloc(&c->d_K, (size_t)n_aos * n_aos);
     if (rc == PDFXC_OK) rc = dalloc(&c->d_E, 1);
+    if (rc == PDFXC_OK) rc = dalloc(&c->d_queue, kMaxQueues);
+    if (rc == PDFXC_OK) {
+        e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+        if (e != hipSuccess) rc = hip_fail(e, "pdfxc_ctx_create: compute unit count");
+    }
     if (rc == PDFXC_OK && c->opts.precision != PDFXC_FP64) {
         const int S = prec_slices(c->opts.precision);
         c->split_nc = (n_aos + 7) / 8 * 8;
@@ -361,6 +366,7 @@ int pdfxc_ctx_destroy(pdfxc_ctx* ctx) {
     dfree(c->d_shrec);
     dfree(c->d_P);
     dfree(c->d_K);
+    dfree(c->d_queue);
     dfree(c->d_P2);
     dfree(c->d_K2);
     dfree(c->d_E);
@@ -1553,6 +1559,14 @@ static int run_kmat(Context* c, double* coef, double* dK) {
     return rc;
 }
 
+// Closes a build sequence's work-queue counters however the sequence ends, so
+// that launches outside a sequence never take a counter begin_queues did not zero.
+struct QueueScope {
+    Context* c;
+    explicit QueueScope(Context* cc) : c(cc) {}
+    ~QueueScope() { c->queue_open = false; }
+};
+
 // One RKS (nspin 1) or UKS (nspin 2) build on device buffers: the rho pass of
 // each spin, the point terms, the K pass of each spin, the energy sum.
 static int build_device(Context* c, int xc, int nspin, const double* const* dP, double* const* dK, double* dE,
@@ -1572,10 +1586,11 @@ static int build_device(Context* c, int xc, int nspin, const double* const* dP, 
     }
     if (nspin == 2)
         if (int rc = ensure_beta_buffers(c)) return rc;
-    int rc = PDFXC_OK;
+    QueueScope queues(c);
     c->rho_accumulate = accumulate_rho;
     PDFXC_HIP(hipEventRecord(c->ev[0], c->stream));
-    rc = run_rho(c, dP[0], c->d_rho);
+    int rc = begin_queues(*c);
+    if (rc == PDFXC_OK) rc = run_rho(c, dP[0], c->d_rho);
     if (nspin == 2 && rc == PDFXC_OK) rc = run_rho(c, dP[1], c->d_rho_b);
     if (rc == PDFXC_OK) rc = nspin == 2 ? launch_xc_points_uks(*c, xc) : launch_xc_points(*c, xc);
     PDFXC_HIP(hipEventRecord(c->ev[1], c->stream));
@@ -1911,9 +1926,11 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
         e = hipMemcpyAsync(c->d_P, H, sizeof(double) * (size_t)N * N, hipMemcpyHostToDevice, c->stream);
         if (e != hipSuccess) rc = hip_fail(e, "pdfxc_gradient_rks: P upload");
     }
-    // rho, the point terms (with w e_xc per slot), the two gradient terms, E_xc
-    for (const LaunchRange& r : c->rho_launches)
-        if (rc == PDFXC_OK) rc = launch_rho(*c, r);
+    // rho (the build's rho pass: FP64, or split when the context's precision
+    // splits it), the point terms (with w e_xc per slot), the two gradient
+    // terms (the functional term's contractions split in every split mode), E_xc
+    c->rho_accumulate = 0;
+    if (rc == PDFXC_OK) rc = run_rho(c, c->d_P, c->d_rho);
     if (rc == PDFXC_OK) rc = launch_xc_points(*c, functional, d_ewf);
     for (const LaunchRange& r : gl)
         if (rc == PDFXC_OK) rc = launch_grad_fn(*c, r, d_shat, d_slat, na, d_g);
@@ -1944,11 +1961,15 @@ int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* t) {
     if (!c || !t) return arg_error("pdfxc_set_tuning: bad arguments");
     if (t->rho_shift < 0 || t->rho_shift > 3 || t->split_rho_shift < 0 || t->split_rho_shift > 2 ||
         t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32 ||
-        (t->split_k_group != 1 && t->split_k_group != 2 && t->split_k_group != 4 && t->split_k_group != 8))
+        (t->split_k_group != 1 && t->split_k_group != 2 && t->split_k_group != 4 && t->split_k_group != 8) ||
+        (t->persistent != 0 && t->persistent != 1))
         return arg_error("pdfxc_set_tuning: shape out of range");
     const pdfxc_tuning prev = c->tune;
     c->tune = *t;
     if (c->n_pts == 0) return PDFXC_OK;
+    if (t->rho_shift == prev.rho_shift && t->k_tiles == prev.k_tiles && t->split_rho_shift == prev.split_rho_shift &&
+        t->split_k_patches == prev.split_k_patches && t->split_k_group == prev.split_k_group)
+        return PDFXC_OK;  // same blocks and work lists (the launch mode is chosen per launch)
     PDFXC_HIP(hipStreamSynchronize(c->stream));
     // a shape the grid's blocks do not fit drops the grid (build_blocks): put
     // the grid back with the previous shape and report the failure
@@ -1983,20 +2004,27 @@ int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
     std::vector<pdfxc_tuning> cand;
     if (c->opts.precision == PDFXC_FP64) {
         for (int rs : {0, 1})
-            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 24, 4, 0.0});
+            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 24, 4, 1, 0.0});
     } else {
         // rho pass classes, then the split K pass's item size and group size
         // (points per K work unit) on the fastest rho shape
         for (int rs : {0, 1})
-            cand.push_back(prec_split_rho(c->opts.precision) ? pdfxc_tuning{0, 96, rs, 24, 4, 0.0}
-                                                              : pdfxc_tuning{rs, 96, 0, 24, 4, 0.0});
+            cand.push_back(prec_split_rho(c->opts.precision) ? pdfxc_tuning{0, 96, rs, 24, 4, 1, 0.0}
+                                                              : pdfxc_tuning{rs, 96, 0, 24, 4, 1, 0.0});
         for (int kp : {16, 24, 32})
             for (int kg : {1, 2, 4, 8})
-                if (!(kp == 24 && kg == 4)) cand.push_back(pdfxc_tuning{-1, 96, -1, kp, kg, 0.0});
+                if (!(kp == 24 && kg == 4)) cand.push_back(pdfxc_tuning{-1, 96, -1, kp, kg, 1, 0.0});
     }
+    // last: the fastest shape in the other launch mode (persistent or one
+    // workgroup per item)
+    cand.push_back(pdfxc_tuning{-2, -2, -2, -2, -2, -2, 0.0});
     pdfxc_tuning best = cand[0];
     best.ms = 1e30;
     for (pdfxc_tuning t : cand) {
+        if (t.k_tiles == -2) {
+            t = best;
+            t.persistent = 1 - best.persistent;
+        }
         if (t.rho_shift < 0) t.rho_shift = best.rho_shift;  // keep the best rho shape found so far
         if (t.split_rho_shift < 0) t.split_rho_shift = best.split_rho_shift;
         if (int rc = pdfxc_set_tuning(ctx, &t)) return rc;
@@ -2222,6 +2250,7 @@ static void batch_free(Batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     free_grid_buffers(&b->m);
     dfree(b->m.d_shrec);
+    dfree(b->m.d_queue);
     dfree(b->m.d_Ps);
     dfree(b->m.d_perm);
     dfree(b->d_ebounds);
@@ -2237,7 +2266,9 @@ static void batch_free(Batch* b) {
 // the build sequence on the batch's stream (no synchronisation: it is captured)
 static int batch_sequence(Batch* b, int xc, const double* dP, double* dK, double* dE) {
     Context& c = b->m;
-    int rc = run_rho(&c, dP, c.d_rho);
+    QueueScope queues(&c);
+    int rc = begin_queues(c);
+    if (rc == PDFXC_OK) rc = run_rho(&c, dP, c.d_rho);
     if (rc == PDFXC_OK) rc = launch_xc_points(c, xc);
     if (rc == PDFXC_OK) rc = run_kmat(&c, c.d_coef, dK);
     if (rc == PDFXC_OK) rc = launch_energy_sums(c, b->d_ebounds, b->n_mol, dE);
@@ -2460,6 +2491,7 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
     (void)slot_first;
     // device side
     hipError_t e = hipGetDevice(&m.device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&m.n_cu, hipDeviceAttributeMultiprocessorCount, m.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m.stream, hipStreamNonBlocking);
     for (int k = 0; k < 6 && e == hipSuccess; ++k) e = hipEventCreate(&m.ev[k]);
     int rc = (e == hipSuccess) ? PDFXC_OK : hip_fail(e, "pdfxc_batch_create: stream/event creation");
@@ -2469,6 +2501,7 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
         if (ee != hipSuccess) rc = hip_fail(ee, "pdfxc_batch_create: upload");
     };
     if (rc == PDFXC_OK) rc = dalloc(&m.d_shrec, rec.size());
+    if (rc == PDFXC_OK) rc = dalloc(&m.d_queue, kMaxQueues);
     if (rc == PDFXC_OK) rc = dalloc(&m.d_pts, soa.size());
     if (rc == PDFXC_OK) rc = dalloc(&m.d_blocks, m.blocks.size());
     if (rc == PDFXC_OK) rc = dalloc(&m.d_block_shells, std::max<size_t>(1, m.block_shells.size()));

@@ -108,7 +108,7 @@ struct DeviceShell {
 
 struct Context {
     pdfxc_opts opts;
-    pdfxc_tuning tune{0, 96, 0, 24, 4, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
+    pdfxc_tuning tune{0, 96, 0, 24, 4, 1, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
     int n_aos = 0;
     std::vector<Shell> shells;
     int device = 0;
@@ -189,6 +189,14 @@ struct Context {
 
     int rho_accumulate = 0;             // update mode: the rho kernels add rho[dP] to d_rho
 
+    // persistent launches: work-queue counters of one build sequence (zeroed
+    // by a fill kernel when the sequence starts), the next free one, and the
+    // device's compute units
+    int* d_queue = nullptr;             // kMaxQueues counters
+    int n_queue_used = 0;
+    bool queue_open = false;            // begin_queues ran for the current sequence
+    int n_cu = 0;
+
     // host copies of the device work lists (d_rho_work, d_k_work, d_srho_work, d_sk_work)
     std::vector<int> h_rho_work, h_srho_work;
     std::vector<KWork> h_k_work;
@@ -204,6 +212,8 @@ struct Context {
     double last_build_ms = 0.0;
     double last_kernel_ms[4] = {0, 0, 0, 0};
 };
+
+constexpr int kMaxQueues = 64;  // persistent launches per build sequence
 
 // error helpers
 void set_error(const std::string& msg);
@@ -226,6 +236,26 @@ int launch_kmat(Context& c, const LaunchRange& r);
 int configure_kernels();
 int launch_energy_sum(Context& c, double* dE);
 int launch_zero(double* d, int64_t n, hipStream_t s);
+// start a build sequence's persistent launches: zero the counters on the stream
+int begin_queues(Context& c);
+// grid and queue of a launch of n_items work items whose kernel keeps
+// `per_cu` workgroups resident per compute unit: a persistent launch of the
+// resident workgroups over the next counter when the items exceed them (and
+// the sequence's counters are open and tune.persistent is set), else one
+// workgroup per item (queue = nullptr)
+int* launch_queue(Context& c, int n_items, int per_cu, unsigned* grid);
+// workgroups of `kernel` resident per compute unit at `threads` and `lds` bytes
+// of dynamic LDS (at least 1)
+template <typename Kernel>
+inline int resident_per_cu(Kernel kernel, int threads, size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kernel), threads, lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        nb = 1;
+    }
+    return nb > 0 ? nb : 1;
+}
 int launch_energy_sums(Context& c, const int* d_bounds, int n_mol, double* dE);  // one E per molecule
 int launch_xc_points_uks(Context& c, int xc_type);
 int ensure_beta_buffers(Context* c);  // UKS beta rho / point-term buffers, zeroed when first made

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
                                              const int* __restrict__ bshells, const int* __restrict__ bshell_col,
                                              const int* __restrict__ bcols, const int* __restrict__ bsorder,
                                              const double* __restrict__ P, int N, double* __restrict__ rho_out,
-                                             int accumulate) {
+                                             int accumulate, int* __restrict__ queue, int n_items) {
     constexpr int SUBS = kBlockPts / PB;
     constexpr int NSL = 512 / PB;  // shell lanes per point
     constexpr int WP = 8 / WI;
@@ -176,12 +176,15 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
     static_assert(RT * 16 * WP == PB, "point tiles must cover the points");
     __shared__ double s_pt[PB][3];
     extern __shared__ double s_dyn[];
-
-    const int bidx = work[blockIdx.x / SUBS];
-    const int sub = blockIdx.x % SUBS;
+    __shared__ int s_item;
+    for (int iter = 0;; ++iter) {
+    const int item = next_item(queue, n_items, iter, &s_item);
+    if (item >= n_items) break;
+    const int bidx = work[item / SUBS];
+    const int sub = item % SUBS;
     const BlockDesc B = blocks[bidx];
     const int p0 = sub * PB;
-    if (p0 >= B.npts) return;  // uniform exit before any barrier: no real point here
+    if (p0 >= B.npts) continue;  // uniform: no real point in this part of the block
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -338,6 +341,7 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
         const double v = (q == 0) ? s : 2.0 * s;
         rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;  // update mode adds rho[dP]
     }
+    }  // work items
 }
 
 // ---------------------------------------------------------------------------
@@ -524,11 +528,16 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
                                               const double* __restrict__ shrec, int RS,
                                               const int* __restrict__ bshells, const int* __restrict__ bshell_col,
                                               const int* __restrict__ bcols, const int* __restrict__ bsorder,
-                                              const double* __restrict__ coef, double* __restrict__ K, int N) {
+                                              const double* __restrict__ coef, double* __restrict__ K, int N,
+                                              int* __restrict__ queue, int n_items) {
     __shared__ double s_pt[kBlockPts][3];
     __shared__ double s_cf[kBlockPts][4];
     extern __shared__ double s_dynd[];
-    const KWork W = work[blockIdx.x];
+    __shared__ int s_item;
+    for (int iter = 0;; ++iter) {
+    const int item = next_item(queue, n_items, iter, &s_item);
+    if (item >= n_items) break;
+    const KWork W = work[item];
     const BlockDesc B = blocks[W.block];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -621,7 +630,7 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
             }
             __syncthreads();
         }
-        return;
+        continue;  // to the next item's barrier, which the consumers reach after their scatter
     }
     // consumers: a contiguous run [tb, te) of the item's tiles, dispatched on
     // the run length rounded up to 4 (one barrier before the slab loop, one per slab)
@@ -637,6 +646,7 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
         case 2: kmat_consume<8>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K + B.poff, B.ldp); break;
         default: kmat_consume<12>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K + B.poff, B.ldp); break;
     }
+    }  // work items
 }
 
 // deterministic sum of the per-block energies (one workgroup)
@@ -797,11 +807,13 @@ static int check_launch(const char* what) {
 
 template <int PB, int WI, int NIMAX, int G>
 static int rho_one(Context& c, const LaunchRange& r) {
-    const int subs = kBlockPts / PB;
-    hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G>), dim3(r.count * subs), dim3(512), r.lds, c.stream,
+    const int items = r.count * (kBlockPts / PB);
+    unsigned grid = 0;
+    int* q = launch_queue(c, items, resident_per_cu(&k_rho<PB, WI, NIMAX, G>, 512, r.lds), &grid);
+    hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G>), dim3(grid), dim3(512), r.lds, c.stream,
                        c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
                        c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
-                       c.n_aos, c.d_rho, c.rho_accumulate);
+                       c.n_aos, c.d_rho, c.rho_accumulate, q, items);
     return check_launch("k_rho");
 }
 
@@ -869,10 +881,12 @@ int launch_xc_points_uks(Context& c, int xc_type) {
 }
 
 static int kmat_one(Context& c, const LaunchRange& r) {
-    hipLaunchKernelGGL(k_kmat, dim3(r.count), dim3(768), r.lds, c.stream,
+    unsigned grid = 0;
+    int* q = launch_queue(c, r.count, resident_per_cu(&k_kmat, 768, r.lds), &grid);
+    hipLaunchKernelGGL(k_kmat, dim3(grid), dim3(768), r.lds, c.stream,
                        reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
                        c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col,
-                       c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos);
+                       c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos, q, r.count);
     return check_launch("k_kmat");
 }
 
@@ -909,6 +923,32 @@ int launch_zero(double* d, int64_t n, hipStream_t s) {
     if (n <= 0) return PDFXC_OK;
     hipLaunchKernelGGL(k_zero, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, d, n);
     return check_launch("k_zero");
+}
+
+__global__ __launch_bounds__(64) void k_zero_int(int* __restrict__ p, int n) {
+    const int i = (int)blockIdx.x * 64 + (int)threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+int begin_queues(Context& c) {
+    c.n_queue_used = 0;
+    c.queue_open = false;
+    if (!c.d_queue || !c.tune.persistent) return PDFXC_OK;
+    // a fill kernel, not a memset: the sequence may be captured in a graph
+    hipLaunchKernelGGL(k_zero_int, dim3((kMaxQueues + 63) / 64), dim3(64), 0, c.stream, c.d_queue, kMaxQueues);
+    if (int rc = check_launch("k_zero_int")) return rc;
+    c.queue_open = true;
+    return PDFXC_OK;
+}
+
+int* launch_queue(Context& c, int n_items, int per_cu, unsigned* grid) {
+    const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(c.n_cu, 1);
+    if (!c.queue_open || c.n_queue_used >= kMaxQueues || n_items <= resident) {
+        *grid = (unsigned)n_items;
+        return nullptr;
+    }
+    *grid = (unsigned)resident;
+    return c.d_queue + c.n_queue_used++;
 }
 
 int launch_energy_sums(Context& c, const int* d_bounds, int n_mol, double* dE) {

@@ -22,58 +22,9 @@
 
 #include "pdfxc_device.hpp"
 #include "pdfxc_internal.hpp"
+#include "pdfxc_split.hpp"
 
 namespace pdfxc {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// v_mfma_f32_16x16x32_bf16 (checked on gfx950 by pdfxc_selftest_mfma_split):
-// lane l holds A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16],
-// e = 0..7; register r of the result holds D[4 (l / 16) + r][l % 16].
-__device__ inline f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-__device__ inline unsigned short bf16_bits(__bf16 h) { return __builtin_bit_cast(unsigned short, h); }
-
-// slices of x: s[0] = bf16(x), s[1] = bf16 of what s[0] leaves, and the last
-// slice from what the others leave plus e = float(x - f), the FP64 part that
-// f = float(x) drops.  f - s[0] and r - s[1] are exact in FP32 (each slice
-// keeps the top 8 bits of a value with at most 24), so only the last sum
-// rounds, at 2^-24 of a value below 2^-16 |x|; one FP64 subtraction per x.
-template <int S>
-__device__ inline void split_bf16(double x, unsigned short* out) {
-    const float f = (float)x;
-    const float e = (float)(x - (double)f);
-    const __bf16 h0 = (__bf16)f;
-    out[0] = bf16_bits(h0);
-    const float r1 = f - (float)h0;  // exact
-    if constexpr (S == 2) {
-        out[1] = bf16_bits((__bf16)(r1 + e));
-    } else {
-        const __bf16 h1 = (__bf16)r1;
-        out[1] = bf16_bits(h1);
-        out[2] = bf16_bits((__bf16)((r1 - (float)h1) + e));  // r1 - h1 exact
-    }
-}
-
-// slice products kept: (a, b) with a + b < S
-template <int S>
-struct SlicePairs;
-template <>
-struct SlicePairs<2> {
-    static constexpr int n = 3;
-    static constexpr int a[3] = {0, 0, 1};
-    static constexpr int b[3] = {0, 1, 0};
-};
-template <>
-struct SlicePairs<3> {
-    static constexpr int n = 6;
-    static constexpr int a[6] = {0, 0, 1, 0, 2, 1};
-    static constexpr int b[6] = {0, 1, 0, 2, 0, 1};
-};
 
 // ---------------------------------------------------------------------------
 // P slices in the split order: Ps[s][r][c] = slice s of P[perm r][perm c]
@@ -167,18 +118,22 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
                                                    const int* __restrict__ bshells, const int* __restrict__ bsorder,
                                                    const int* __restrict__ sshell_col, const int* __restrict__ chunks,
                                                    const unsigned short* __restrict__ Ps, int N, int Nc,
-                                                   double* __restrict__ rho_out, int accumulate) {
+                                                   double* __restrict__ rho_out, int accumulate,
+                                                   int* __restrict__ queue, int n_items) {
     constexpr int SUBS = kBlockPts / PB;
     constexpr int NSL = 512 / PB;  // shell lanes per point
     __shared__ double s_pt[PB][3];
     extern __shared__ __attribute__((aligned(16))) double s_dyn[];
-
-    const int bidx = work[blockIdx.x / SUBS];
-    const int sub = blockIdx.x % SUBS;
+    __shared__ int s_item;
+    for (int iter = 0;; ++iter) {
+    const int item = next_item(queue, n_items, iter, &s_item);
+    if (item >= n_items) break;
+    const int bidx = work[item / SUBS];
+    const int sub = item % SUBS;
     const BlockDesc B = blocks[bidx];
     const SplitBlock SB = sblocks[bidx];
     const int p0 = sub * PB;
-    if (p0 >= B.npts) return;  // uniform exit before any barrier
+    if (p0 >= B.npts) continue;  // uniform: no real point in this part of the block
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -339,6 +294,7 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
         const double v = (q == 0) ? sum : 2.0 * sum;
         rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;
     }
+    }  // work items
 }
 
 // ---------------------------------------------------------------------------
@@ -427,14 +383,19 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict
                                                     const int* __restrict__ gshells, const int* __restrict__ gshell_col,
                                                     const int* __restrict__ gcols, const int* __restrict__ gsorder,
                                                     const unsigned* __restrict__ gmasks,
-                                                    const double* __restrict__ coef, double* __restrict__ K) {
+                                                    const double* __restrict__ coef, double* __restrict__ K,
+                                                    int* __restrict__ queue, int n_items) {
     __shared__ double s_pt[2][kBlockPts][3];
     __shared__ double s_cf[2][kBlockPts][4];
     __shared__ unsigned s_msk[2][kMaxMaskWords];
     __shared__ int s_first[kMaxGroup + 1];  // first slab of each block of the group, then the slab count
     __shared__ int s_blk[kMaxGroup];
     extern __shared__ __attribute__((aligned(16))) double s_dyn[];
-    const KGWork W = work[blockIdx.x];
+    __shared__ int s_item;
+    for (int iter = 0;; ++iter) {
+    const int item = next_item(queue, n_items, iter, &s_item);
+    if (item >= n_items) break;
+    const KGWork W = work[item];
     const KGroup G = groups[W.group];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -641,6 +602,7 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict
             }
         }
     }
+    }  // work items
 }
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
@@ -711,20 +673,24 @@ static int psplit_one(Context& c) {
 
 template <int S, int PB, int NJMAX>
 static int rho_split_one(Context& c, const LaunchRange& r) {
-    const int subs = kBlockPts / PB;
-    hipLaunchKernelGGL((k_rho_split<S, PB, NJMAX>), dim3(r.count * subs), dim3(512), r.lds, c.stream,
+    const int items = r.count * (kBlockPts / PB);
+    unsigned grid = 0;
+    int* q = launch_queue(c, items, resident_per_cu(&k_rho_split<S, PB, NJMAX>, 512, r.lds), &grid);
+    hipLaunchKernelGGL((k_rho_split<S, PB, NJMAX>), dim3(grid), dim3(512), r.lds, c.stream,
                        c.d_srho_work + r.offset, c.d_blocks, c.d_sblocks, c.d_pts, c.n_slots, c.d_shrec,
                        c.rec_stride, c.d_block_shells, c.d_block_sorder, c.d_sshell_col, c.d_chunks, c.d_Ps,
-                       c.n_aos, c.split_nc, c.d_rho, c.rho_accumulate);
+                       c.n_aos, c.split_nc, c.d_rho, c.rho_accumulate, q, items);
     return check("k_rho_split");
 }
 
 template <int S, int NW, int NPW>
 static int kmat_split_launch(Context& c, const LaunchRange& r) {
-    hipLaunchKernelGGL((k_kmat_split<S, NW, NPW>), dim3(r.count), dim3(NW * 64), r.lds, c.stream,
+    unsigned grid = 0;
+    int* q = launch_queue(c, r.count, resident_per_cu(&k_kmat_split<S, NW, NPW>, NW * 64, r.lds), &grid);
+    hipLaunchKernelGGL((k_kmat_split<S, NW, NPW>), dim3(grid), dim3(NW * 64), r.lds, c.stream,
                        c.d_kg_work + r.offset, c.d_kgroups, c.d_blocks, c.d_kg_blocks, c.d_pts, c.n_slots,
                        c.d_shrec, c.rec_stride, c.d_kg_shells, c.d_kg_shell_col, c.d_kg_cols, c.d_kg_sorder,
-                       c.d_kg_masks, c.d_coef, c.d_K);
+                       c.d_kg_masks, c.d_coef, c.d_K, q, r.count);
     return check("k_kmat_split");
 }
 

@@ -64,7 +64,8 @@ class Opts(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("rho_shift", ctypes.c_int), ("k_tiles", ctypes.c_int), ("split_rho_shift", ctypes.c_int),
-                ("split_k_patches", ctypes.c_int), ("split_k_group", ctypes.c_int), ("ms", ctypes.c_double)]
+                ("split_k_patches", ctypes.c_int), ("split_k_group", ctypes.c_int), ("persistent", ctypes.c_int),
+                ("ms", ctypes.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -343,8 +344,9 @@ class XcContext:
     def save_grid_matrix(self, path, n_rho_cols=4):
         check(lib().pdfxc_save_grid_matrix(self.h, _path(path), n_rho_cols))
 
-    def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=24, split_k_group=4):
-        t = Tuning(rho_shift, k_tiles, split_rho_shift, split_k_patches, split_k_group, 0.0)
+    def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=24, split_k_group=4,
+                   persistent=1):
+        t = Tuning(rho_shift, k_tiles, split_rho_shift, split_k_patches, split_k_group, persistent, 0.0)
         check(lib().pdfxc_set_tuning(self.h, ctypes.byref(t)))
 
     def autotune(self, dP_ptr, dK_ptr, dE_ptr, reps=3, functional=PDFXC_B3LYP):

@@ -6,8 +6,10 @@
   rho + hi+mid+lo K mode with E equal to the FP64 build and K within 1e-7).
 * (Gly)30 alpha-helix (1878 AOs), 99 x 590 grid (10.1 M points): size-
   independent properties of the full build -- the split modes against FP64,
-  the electron count of a diagonal density, K = dE/dP by central differences,
-  K symmetric -- and the oracle on a 4,000-point subset.
+  K = dE/dP by central differences, K symmetric -- and the oracle on a
+  4,000-point subset; the electron count of a diagonal density at full size
+  on the 75 x 302 grid, and on 99 radial points the reference's own weight
+  wrap (the oracle grid's count) on (Gly)2.
 * 64 water clusters (H2O)_n, n ~ U{4..64} (seed 3), 75 x 302: the one-launch
   batch equals every cluster's own build, and the two smallest clusters equal
   the oracle.
@@ -100,14 +102,52 @@ def test_gly30_split_modes_against_fp64(gly30):
         np.testing.assert_array_equal(K, K.T)
 
 
-def test_gly30_electron_count_of_a_diagonal_density(gly30):
-    mol, info, grid, PA, ctx = gly30
+def test_gly30_electron_count_of_a_diagonal_density():
+    """The integral of rho for a diagonal density is its trace, at full (Gly)30
+    size on the 75 x 302 grid.  (On 99 radial points the reference's EM weights
+    are wrong for shells i >= 74, see the next test, so the 99 x 590 grid
+    integrates such a density a few per cent short, as the reference's does.)"""
+    mol = molecules.polyglycine_helix(30, seed=2)
+    info = calc_grid.orbital_info(mol)
+    ctx = engine.XcContext(info, engine.default_opts(ao_screen=1e-12))
+    n = ctx.generate_grid(mol.Z, mol.xyz, 75, 302)
     occ = guess.atomic_occupations(mol, info)
     n_elec = occ.sum()
-    _, _, rho = _build(info, grid, 0.5 * np.diag(occ), ao_screen=1e-12, want_rho=True)
+    _, _, rho = ctx.build_rks(guess.pack_lower(0.5 * np.diag(occ)), want_rho=True)
+    grid = ctx.get_grid()
+    ctx.close()
     count = 2.0 * float(np.dot(grid[:, 3], rho[:, 0]))
-    say(f"  electrons on the grid {count:.10f} of {n_elec:.1f}: relative {abs(count - n_elec) / n_elec:.2e}")
+    say(f"  (Gly)30 75x302 ({n} points): electrons on the grid {count:.10f} of {n_elec:.1f}, "
+        f"relative {abs(count - n_elec) / n_elec:.2e}")
     assert abs(count - n_elec) <= 1e-5 * n_elec
+
+
+def test_99_radial_points_keep_the_reference_weight_wrap():
+    """DfGenerateGrid::getRadialAbscissaAndWeight_EulerMaclaurin forms i^5 in
+    32-bit int arithmetic (DfGenerateGrid.cpp:1607), which wraps for i >= 74;
+    with 99 radial points those shells lie inside the r <= 30 cut-off, so the
+    reference's grid integrates a diagonal density 2-3 % short.  The engine
+    keeps that behaviour: its (Gly)2 99 x 590 grid gives the oracle grid's
+    electron count, not the trace."""
+    mol = molecules.polyglycine_helix(2, seed=2)
+    info = calc_grid.orbital_info(mol)
+    occ = guess.atomic_occupations(mol, info)
+    ctx = engine.XcContext(info, engine.default_opts())
+    n = ctx.generate_grid(mol.Z, mol.xyz, 99, 590)
+    _, _, rho = ctx.build_rks(guess.pack_lower(0.5 * np.diag(occ)), want_rho=True)
+    grid = ctx.get_grid()
+    ctx.close()
+    count = 2.0 * float(np.dot(grid[:, 3], rho[:, 0]))
+    ogrid, _ = oracle.generate_grid(mol.Z, mol.xyz, 99, 590)
+    aos = oracle.AoTable.from_basis(info)
+    ref = 0.0
+    for s in range(0, len(ogrid), 50_000):
+        g = ogrid[s:s + 50_000]
+        ref += float(np.dot(g[:, 3], (oracle.eval_aos(aos, g[:, :3], grad=False) ** 2) @ occ))
+    say(f"  (Gly)2 99x590: {n} points (oracle {len(ogrid)}), electrons {count:.10f}, oracle grid {ref:.10f}, "
+        f"trace {occ.sum():.1f}")
+    assert abs(count - ref) <= 1e-9 * ref
+    assert 0.95 * occ.sum() < count < 0.99 * occ.sum()
 
 
 def test_gly30_k_is_the_derivative_of_e(gly30):
@@ -235,3 +275,43 @@ def test_gly20_gradient_against_central_differences(gly20_gradient, half):
         say(f"  atom {A} ({mol.symbols[A]}): dE/dh fd {fd:.10f} gradient {ana:.10f} diff {abs(fd - ana):.2e}")
         assert abs(fd - ana) <= 5e-8 + 1e-6 * abs(fd), (A, fd, ana)
     say(f"  worst |fd - g.u| {worst:.2e}")
+
+
+@pytest.mark.parametrize("prec", [engine.PDFXC_SPLIT_K_BF16X3, engine.PDFXC_SPLIT_BF16X3])
+def test_gly20_split_gradient(gly20_gradient, prec):
+    """The XC gradient with the functional term's contractions in hi+mid+lo
+    bf16 against the FP64 gradient at full (Gly)20 size, and against central
+    differences of the energy on four of the atoms (in the FP64-rho mode the
+    energy is the FP64 build's, so the differences are those of the FP64
+    test)."""
+    mol, info, PA, O, opts, ctx64, g64, atoms, dirs = gly20_gradient
+    ctx = engine.XcContext(info, engine.default_opts(ao_screen=1e-12, precision=prec))
+    ctx.generate_grid_oriented(mol.Z, mol.xyz, opts, O)
+    E, gf, gw = ctx.gradient_rks(mol.Z, mol.xyz, guess.pack_lower(PA))
+    ctx.close()
+    g = gf + gw
+    d = np.abs(g - g64).max()
+    say(f"  split gradient precision {prec}: max|g - g(FP64)| {d:.2e} (max|g| {np.abs(g64).max():.3e})")
+    assert d <= 1e-7
+    if prec != engine.PDFXC_SPLIT_K_BF16X3:
+        return
+    h = 1e-4
+    P = guess.pack_lower(PA)
+
+    def energy(xyz):
+        m = molecules.Molecule(mol.symbols, xyz, mol.name)
+        c = engine.XcContext(calc_grid.orbital_info(m), engine.default_opts(ao_screen=1e-12, precision=prec))
+        c.generate_grid_oriented(m.Z, m.xyz, opts, O)
+        e = c.build_rks(P)[0]
+        c.close()
+        return e
+
+    for k in range(4):
+        A, u = atoms[k], dirs[k]
+        xp, xm = mol.xyz.copy(), mol.xyz.copy()
+        xp[A] += h * u
+        xm[A] -= h * u
+        fd = (energy(xp) - energy(xm)) / (2 * h)
+        ana = float(g[A] @ u)
+        say(f"  atom {A} ({mol.symbols[A]}): fd {fd:.10f} split gradient {ana:.10f} diff {abs(fd - ana):.2e}")
+        assert abs(fd - ana) <= 5e-8 + 1e-6 * abs(fd), (A, fd, ana)

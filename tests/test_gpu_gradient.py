@@ -87,3 +87,30 @@ def test_gradient_matches_central_differences_of_gpu_energy():
             worst = max(worst, abs(g[A, d] - fd))
             assert abs(g[A, d] - fd) <= 5e-8 + 1e-6 * abs(fd), (A, d, g[A, d], fd)
     say(f"gradient vs central differences: max |diff| {worst:.2e}")
+
+
+# split precision: the functional term's contractions X = P phi and Z = P W
+# on bf16 MFMA (hi+mid+lo, or hi+lo); against the oracle's FP64 gradient,
+# bounds (|dE|, max|dg|) per mode at about 4x the measured (H2O)4 50 x 194
+# errors (hi+mid+lo: |dg| 5-6e-9; hi+lo: |dE| 4e-5, |dg| 1.1e-6; the FP64-rho
+# mode's energy is the FP64 one)
+SPLIT_GRAD_BOUNDS = {engine.PDFXC_SPLIT_K_BF16X3: (0.0, 2e-8), engine.PDFXC_SPLIT_BF16X3: (2e-6, 2e-8),
+                     engine.PDFXC_SPLIT_BF16: (2e-4, 5e-6)}
+
+
+@pytest.mark.parametrize("prec", sorted(SPLIT_GRAD_BOUNDS))
+def test_split_gradient_matches_oracle(prec):
+    mol, info, PA, O, grid, atom = _case(4, 2, 50, 194)
+    ctx = engine.XcContext(info, engine.default_opts(precision=prec))
+    ctx.set_grid(grid)
+    E, gf, gw = ctx.gradient_rks(mol.Z, mol.xyz, guess.pack_lower(PA), point_atom=atom)
+    ctx.close()
+    e_ref, gf_ref, gw_ref = oracle.xc_gradient_rks(oracle.AoTable.from_basis(info), info.ao_atom, grid, atom,
+                                                   mol.Z, mol.xyz, PA)
+    e_tol, g_tol = SPLIT_GRAD_BOUNDS[prec]
+    say(f"[gpu-case] split gradient precision {prec}: |dE| {abs(E - e_ref):.2e} "
+        f"max|dgf| {np.abs(gf - gf_ref).max():.2e} max|dgw| {np.abs(gw - gw_ref).max():.2e}")
+    assert abs(E - e_ref) <= e_tol + 1e-10
+    np.testing.assert_allclose(gf, gf_ref, rtol=0, atol=g_tol)
+    np.testing.assert_allclose(gw, gw_ref, rtol=0, atol=g_tol)
+    np.testing.assert_allclose(gf.sum(0), 0.0, atol=10 * g_tol)

@@ -12,14 +12,16 @@ from proteindf_amd import calc_grid, engine, guess, molecules
 pytestmark = pytest.mark.gpu
 
 # |dE| and max|dK| bounds per mode on the (H2O)_8 / 6-31G(d) / 75 x 302 build.
-# Measured: hi+lo |dE| 2.6e-5, max|dK| 1.0e-6; hi+mid+lo |dE| 1.4e-6, max|dK|
-# 9.6e-8; FP64 rho + hi+mid+lo K |dE| 0, max|dK| 3.7e-8.  A CPU emulation of
+# Measured: hi+lo |dE| 2.6e-5, max|dK| 1.1e-6; hi+mid+lo |dE| 1.4e-6, max|dK|
+# 6.0e-8; FP64 rho + hi+mid+lo K |dE| 0, max|dK| 5.1e-9.  A CPU emulation of
 # the same slices with exact accumulation gives |dE| ~ 7e-9 for hi+mid+lo: what
-# remains is the FP32 accumulation inside each 16x16x32 MFMA (the engine adds
-# every K step's sum into FP64).  The FP64-rho mode is held to the 1e-7 target.
-# PDFXC_SPLIT_K_BF16X3 runs the rho pass in FP64: E and rho are the FP64 build's
-# (the same kernels in the same order), K has the hi+mid+lo error.
-BOUNDS = {engine.PDFXC_SPLIT_BF16: (2e-4, 2e-5), engine.PDFXC_SPLIT_BF16X3: (5e-6, 5e-7),
+# remains in E is the FP32 accumulation inside each 16x16x32 MFMA of the rho
+# pass (the engine adds every K step's sum into FP64), so the hi+mid+lo energy
+# is held to 5e-6, not the 1e-6 target.  Both hi+mid+lo K passes are held to
+# the 1e-7 target.  PDFXC_SPLIT_K_BF16X3 runs the rho pass in FP64: E and rho
+# are the FP64 build's (the same kernels in the same order), K has the
+# hi+mid+lo error.
+BOUNDS = {engine.PDFXC_SPLIT_BF16: (2e-4, 2e-5), engine.PDFXC_SPLIT_BF16X3: (5e-6, 1e-7),
           engine.PDFXC_SPLIT_K_BF16X3: (1e-10, 1e-7)}
 SPLIT_MODES = [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3, engine.PDFXC_SPLIT_K_BF16X3]
 
