@@ -95,11 +95,11 @@ typedef struct pdfxc_stats {
  * neighbours in the point bisection) whose K contributions the split K pass
  * sums on the union of their AO columns before one scatter into K, i.e.
  * 128 x split_k_group points per K work unit (a group too wide for the
- * kernel's LDS is halved); persistent (0 or 1): the rho and K kernels run as
- * persistent launches, as many workgroups as stay resident on the device,
- * each taking work items (blocks, K items; of every molecule in a batch) from
- * one device counter until they run out, instead of one workgroup per item.
- * Defaults 0, 96, 0, 24, 4, 1.
+ * kernel's LDS is halved); persistent (bit mask, 0..3): bit 0 the rho-pass
+ * kernels, bit 1 the K-pass kernels run as persistent launches, as many
+ * workgroups as stay resident on the device, each taking work items (blocks,
+ * K items; of every molecule in a batch) from one device counter until they
+ * run out, instead of one workgroup per item.  Defaults 0, 96, 0, 24, 4, 2.
  * ms: the build time measured for it (pdfxc_autotune), else 0. */
 typedef struct pdfxc_tuning {
     int rho_shift;

@@ -32,6 +32,7 @@ __device__ inline int next_item(int* queue, int n_items, int iter, int* s_item) 
     return *s_item;
 }
 
+
 constexpr double kTooBig = 30.0;  // DfCalcGridX::TOOBIG, primitives with alpha r^2 > 30 are skipped
 constexpr double kInvSqrt3 = 0.57735026918962576451;
 
@@ -153,6 +154,37 @@ __device__ inline void rec_values0_r(int l, double x, double y, double z, double
         v[4] = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)) * R0;
     }
 }
+// Dots of one shell's values at a point with per-component weights X
+// (X = P phi in the rho pass): sum_m X_m phi_m = R0 S and sum_m X_m grad
+// phi_m = R0 G + R1 r S, with S = sum_m X_m f_m and G = sum_m X_m grad f_m
+// over the angular polynomials f_m of getPrefactor (DfCalcGridX.cpp:103-141)
+// and R1 = (dR/dr) / r; added to (d0, dx, dy, dz).  The same sums as
+// forming phi and grad phi first, in far fewer operations.
+__device__ inline void shell_dots(int l, double x, double y, double z, double R0, double R1, const double* X,
+                                  double& d0, double& dx, double& dy, double& dz) {
+    double S, Gx, Gy, Gz;
+    if (l == 0) {
+        S = X[0];
+        Gx = Gy = Gz = 0.0;
+    } else if (l == 1) {
+        S = X[0] * x + X[1] * y + X[2] * z;
+        Gx = X[0];
+        Gy = X[1];
+        Gz = X[2];
+    } else {
+        const double x4 = kInvSqrt3 * X[4];
+        S = X[0] * (x * y) + X[1] * (z * x) + X[2] * (y * z) + X[3] * (0.5 * (x * x - y * y)) +
+            x4 * (z * z - 0.5 * (x * x + y * y));
+        Gx = X[0] * y + X[1] * z + (X[3] - x4) * x;
+        Gy = X[0] * x + X[2] * z - (X[3] + x4) * y;
+        Gz = X[1] * x + X[2] * y + 2.0 * x4 * z;
+    }
+    d0 += R0 * S;
+    dx += R0 * Gx + R1 * x * S;
+    dy += R0 * Gy + R1 * y * S;
+    dz += R0 * Gz + R1 * z * S;
+}
+
 __device__ inline void rec_values_r(int l, double x, double y, double z, double R0, double R1, double* v,
                                     double* vx, double* vy, double* vz) {
     if (l == 0) {

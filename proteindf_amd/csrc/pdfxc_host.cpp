@@ -1962,7 +1962,7 @@ int pdfxc_set_tuning(pdfxc_ctx* ctx, const pdfxc_tuning* t) {
     if (t->rho_shift < 0 || t->rho_shift > 3 || t->split_rho_shift < 0 || t->split_rho_shift > 2 ||
         t->k_tiles < 8 || t->k_tiles > 96 || t->split_k_patches < 4 || t->split_k_patches > 32 ||
         (t->split_k_group != 1 && t->split_k_group != 2 && t->split_k_group != 4 && t->split_k_group != 8) ||
-        (t->persistent != 0 && t->persistent != 1))
+        t->persistent < 0 || t->persistent > 3)
         return arg_error("pdfxc_set_tuning: shape out of range");
     const pdfxc_tuning prev = c->tune;
     c->tune = *t;
@@ -2004,26 +2004,28 @@ int pdfxc_autotune(pdfxc_ctx* ctx, int functional, const double* dP, double* dK,
     std::vector<pdfxc_tuning> cand;
     if (c->opts.precision == PDFXC_FP64) {
         for (int rs : {0, 1})
-            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 24, 4, 1, 0.0});
+            for (int kt : {96, 48, 24}) cand.push_back(pdfxc_tuning{rs, kt, 0, 24, 4, 2, 0.0});
     } else {
         // rho pass classes, then the split K pass's item size and group size
         // (points per K work unit) on the fastest rho shape
         for (int rs : {0, 1})
-            cand.push_back(prec_split_rho(c->opts.precision) ? pdfxc_tuning{0, 96, rs, 24, 4, 1, 0.0}
-                                                              : pdfxc_tuning{rs, 96, 0, 24, 4, 1, 0.0});
+            cand.push_back(prec_split_rho(c->opts.precision) ? pdfxc_tuning{0, 96, rs, 24, 4, 2, 0.0}
+                                                              : pdfxc_tuning{rs, 96, 0, 24, 4, 2, 0.0});
         for (int kp : {16, 24, 32})
             for (int kg : {1, 2, 4, 8})
-                if (!(kp == 24 && kg == 4)) cand.push_back(pdfxc_tuning{-1, 96, -1, kp, kg, 1, 0.0});
+                if (!(kp == 24 && kg == 4)) cand.push_back(pdfxc_tuning{-1, 96, -1, kp, kg, 2, 0.0});
     }
-    // last: the fastest shape in the other launch mode (persistent or one
-    // workgroup per item)
-    cand.push_back(pdfxc_tuning{-2, -2, -2, -2, -2, -2, 0.0});
+    // last: the fastest shape with the other launch modes (persistent rho
+    // and / or K kernels, or one workgroup per item)
+    for (int mask = 0; mask < 4; ++mask) cand.push_back(pdfxc_tuning{-2, -2, -2, -2, -2, mask, 0.0});
     pdfxc_tuning best = cand[0];
     best.ms = 1e30;
     for (pdfxc_tuning t : cand) {
         if (t.k_tiles == -2) {
+            if (t.persistent == best.persistent) continue;  // timed already
+            const int mask = t.persistent;
             t = best;
-            t.persistent = 1 - best.persistent;
+            t.persistent = mask;
         }
         if (t.rho_shift < 0) t.rho_shift = best.rho_shift;  // keep the best rho shape found so far
         if (t.split_rho_shift < 0) t.split_rho_shift = best.split_rho_shift;

@@ -108,7 +108,7 @@ struct DeviceShell {
 
 struct Context {
     pdfxc_opts opts;
-    pdfxc_tuning tune{0, 96, 0, 24, 4, 1, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
+    pdfxc_tuning tune{0, 96, 0, 24, 4, 2, 0.0};  // launch shapes (pdfxc_set_tuning / pdfxc_autotune)
     int n_aos = 0;
     std::vector<Shell> shells;
     int device = 0;
@@ -238,12 +238,15 @@ int launch_energy_sum(Context& c, double* dE);
 int launch_zero(double* d, int64_t n, hipStream_t s);
 // start a build sequence's persistent launches: zero the counters on the stream
 int begin_queues(Context& c);
-// grid and queue of a launch of n_items work items whose kernel keeps
-// `per_cu` workgroups resident per compute unit: a persistent launch of the
-// resident workgroups over the next counter when the items exceed them (and
-// the sequence's counters are open and tune.persistent is set), else one
-// workgroup per item (queue = nullptr)
-int* launch_queue(Context& c, int n_items, int per_cu, unsigned* grid);
+// kernel families of pdfxc_tuning.persistent (bit mask)
+constexpr int kQueueRho = 1, kQueueK = 2;
+// grid and queue of a launch of n_items work items of a kernel family whose
+// persistent instantiation keeps `per_cu` workgroups resident per compute
+// unit: a persistent launch of the resident workgroups over the next counter
+// when the items exceed them (and the sequence's counters are open and the
+// family's bit of tune.persistent is set), else one workgroup per item of the
+// plain instantiation (queue = nullptr)
+int* launch_queue(Context& c, int kind, int n_items, int per_cu, unsigned* grid);
 // workgroups of `kernel` resident per compute unit at `threads` and `lds` bytes
 // of dynamic LDS (at least 1)
 template <typename Kernel>

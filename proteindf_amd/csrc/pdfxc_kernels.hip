@@ -162,29 +162,24 @@ __device__ inline void rho_tiles(int wI, int WI, int wP, int T, int lane, const 
 // functional runs over all points in k_xc_points.
 // ---------------------------------------------------------------------------
 template <int PB, int WI, int NIMAX, int G>
-__global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
+__device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                              const double* __restrict__ pts, int64_t n_slots,
                                              const double* __restrict__ shrec, int RS,
                                              const int* __restrict__ bshells, const int* __restrict__ bshell_col,
                                              const int* __restrict__ bcols, const int* __restrict__ bsorder,
                                              const double* __restrict__ P, int N, double* __restrict__ rho_out,
-                                             int accumulate, int* __restrict__ queue, int n_items) {
+                                             int accumulate,
+        double (*s_pt)[3], double* s_dyn) {
     constexpr int SUBS = kBlockPts / PB;
     constexpr int NSL = 512 / PB;  // shell lanes per point
     constexpr int WP = 8 / WI;
     constexpr int RT = PB / 16 / WP;  // point tiles per wave
     static_assert(RT * 16 * WP == PB, "point tiles must cover the points");
-    __shared__ double s_pt[PB][3];
-    extern __shared__ double s_dyn[];
-    __shared__ int s_item;
-    for (int iter = 0;; ++iter) {
-    const int item = next_item(queue, n_items, iter, &s_item);
-    if (item >= n_items) break;
     const int bidx = work[item / SUBS];
     const int sub = item % SUBS;
     const BlockDesc B = blocks[bidx];
     const int p0 = sub * PB;
-    if (p0 >= B.npts) continue;  // uniform: no real point in this part of the block
+    if (p0 >= B.npts) return;  // uniform: no real point in this part of the block
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -289,41 +284,44 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
     }
     __syncthreads();
 
-    // 3. rho and grad rho: phi, grad phi dotted with X
+    // 3. rho and grad rho: X dotted with phi and grad phi, per shell from its
+    // radial sums (kept from step 1 for the first NC shells of a thread)
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    auto shell_x = [&](int k, double* X) {  // the shell's X values at point p (components past l: 0)
+        const int c0 = s_shcol[k];
+        const int l = ((int)s_rec[k * RS + 3]) & 3;
+        X[0] = s_phi[plane_at<PB>(c0, p)];
+        X[1] = X[2] = X[3] = X[4] = 0.0;
+        if (l >= 1) {
+            X[1] = s_phi[plane_at<PB>(c0 + 1, p)];
+            X[2] = s_phi[plane_at<PB>(c0 + 2, p)];
+        }
+        if (l == 2) {
+            X[3] = s_phi[plane_at<PB>(c0 + 3, p)];
+            X[4] = s_phi[plane_at<PB>(c0 + 4, p)];
+        }
+        return l;
+    };
 #pragma unroll
     for (int jj = 0; jj < NC; ++jj) {
         const int j = kk + jj * NSL;
         if (j < nsh) {
             const int k = s_sord[j];
             const double* rec = s_rec + k * RS;
-            const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
-            double v[5], vx[5], vy[5], vz[5];
-            rec_values_r(((int)rec[3]) & 3, x, y, z, cR0[jj], cR1[jj], v, vx, vy, vz);
-            const int c0 = s_shcol[k];
-            const int nc = 2 * (((int)rec[3]) & 3) + 1;
-            for (int m = 0; m < nc; ++m) {
-                const double xv = s_phi[plane_at<PB>(c0 + m, p)];
-                r0 += xv * v[m];
-                r1 += xv * vx[m];
-                r2 += xv * vy[m];
-                r3 += xv * vz[m];
-            }
+            double X[5];
+            const int l = shell_x(k, X);
+            shell_dots(l, px - rec[0], py - rec[1], pz - rec[2], cR0[jj], cR1[jj], X, r0, r1, r2, r3);
         }
     }
     for (int j = kk + NC * NSL; j < nsh; j += NSL) {
         const int k = s_sord[j];
-        double v[5], vx[5], vy[5], vz[5];
-        rec_values(s_rec + k * RS, px, py, pz, v, vx, vy, vz);
-        const int c0 = s_shcol[k];
-        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-        for (int m = 0; m < nc; ++m) {
-            const double x = s_phi[plane_at<PB>(c0 + m, p)];
-            r0 += x * v[m];
-            r1 += x * vx[m];
-            r2 += x * vy[m];
-            r3 += x * vz[m];
-        }
+        const double* rec = s_rec + k * RS;
+        const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+        double R0, R1;
+        rec_radial01(rec, x, y, z, R0, R1);
+        double X[5];
+        const int l = shell_x(k, X);
+        shell_dots(l, x, y, z, R0, R1, X, r0, r1, r2, r3);
     }
     __syncthreads();  // X no longer read: the plane holds the partial sums
     double* s_part = s_phi;  // [4][NSL][PB]
@@ -341,7 +339,28 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
         const double v = (q == 0) ? s : 2.0 * s;
         rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;  // update mode adds rho[dP]
     }
-    }  // work items
+}
+
+template <int PB, int WI, int NIMAX, int G, bool PERSIST>
+__global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
+                                             const double* __restrict__ pts, int64_t n_slots,
+                                             const double* __restrict__ shrec, int RS,
+                                             const int* __restrict__ bshells, const int* __restrict__ bshell_col,
+                                             const int* __restrict__ bcols, const int* __restrict__ bsorder,
+                                             const double* __restrict__ P, int N, double* __restrict__ rho_out,
+                                             int accumulate, int* __restrict__ queue, int n_items) {
+    __shared__ double s_pt[PB][3];
+    extern __shared__ double s_dyn[];
+    if constexpr (PERSIST) {
+        __shared__ int s_item;
+        for (int iter = 0;; ++iter) {
+            const int item = next_item(queue, n_items, iter, &s_item);
+            if (item >= n_items) break;
+            k_rho_item<PB, WI, NIMAX, G>(item, work, blocks, pts, n_slots, shrec, RS, bshells, bshell_col, bcols, bsorder, P, N, rho_out, accumulate, s_pt, s_dyn);
+        }
+    } else {
+        k_rho_item<PB, WI, NIMAX, G>((int)blockIdx.x, work, blocks, pts, n_slots, shrec, RS, bshells, bshell_col, bcols, bsorder, P, N, rho_out, accumulate, s_pt, s_dyn);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -523,20 +542,13 @@ __device__ inline void kmat_consume(int tb, int te, int nslab, int PS, int LD, i
 // the per-point coefficients (build_XC_Matrix GGA, DfCalcGridX.cpp:2455-2500).
 // Tiles go into the dense lower triangle of K with FP64 atomics.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+__device__ __forceinline__ void k_kmat_item(int item, const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                               const double* __restrict__ pts, int64_t n_slots,
                                               const double* __restrict__ shrec, int RS,
                                               const int* __restrict__ bshells, const int* __restrict__ bshell_col,
                                               const int* __restrict__ bcols, const int* __restrict__ bsorder,
                                               const double* __restrict__ coef, double* __restrict__ K, int N,
-                                              int* __restrict__ queue, int n_items) {
-    __shared__ double s_pt[kBlockPts][3];
-    __shared__ double s_cf[kBlockPts][4];
-    extern __shared__ double s_dynd[];
-    __shared__ int s_item;
-    for (int iter = 0;; ++iter) {
-    const int item = next_item(queue, n_items, iter, &s_item);
-    if (item >= n_items) break;
+        double (*s_pt)[3], double (*s_cf)[4], double* s_dynd) {
     const KWork W = work[item];
     const BlockDesc B = blocks[W.block];
     const int tid = threadIdx.x;
@@ -630,7 +642,7 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
             }
             __syncthreads();
         }
-        continue;  // to the next item's barrier, which the consumers reach after their scatter
+        return;  // to the next item's barrier, which the consumers reach after their scatter
     }
     // consumers: a contiguous run [tb, te) of the item's tiles, dispatched on
     // the run length rounded up to 4 (one barrier before the slab loop, one per slab)
@@ -646,7 +658,29 @@ __global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, co
         case 2: kmat_consume<8>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K + B.poff, B.ldp); break;
         default: kmat_consume<12>(tb, te, nslab, PS, LD, SLAB, lane, s_buf, s_cols, K + B.poff, B.ldp); break;
     }
-    }  // work items
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(768) void k_kmat(const KWork* __restrict__ work, const BlockDesc* __restrict__ blocks,
+                                              const double* __restrict__ pts, int64_t n_slots,
+                                              const double* __restrict__ shrec, int RS,
+                                              const int* __restrict__ bshells, const int* __restrict__ bshell_col,
+                                              const int* __restrict__ bcols, const int* __restrict__ bsorder,
+                                              const double* __restrict__ coef, double* __restrict__ K, int N,
+                                              int* __restrict__ queue, int n_items) {
+    __shared__ double s_pt[kBlockPts][3];
+    __shared__ double s_cf[kBlockPts][4];
+    extern __shared__ double s_dynd[];
+    if constexpr (PERSIST) {
+        __shared__ int s_item;
+        for (int iter = 0;; ++iter) {
+            const int item = next_item(queue, n_items, iter, &s_item);
+            if (item >= n_items) break;
+            k_kmat_item(item, work, blocks, pts, n_slots, shrec, RS, bshells, bshell_col, bcols, bsorder, coef, K, N, s_pt, s_cf, s_dynd);
+        }
+    } else {
+        k_kmat_item((int)blockIdx.x, work, blocks, pts, n_slots, shrec, RS, bshells, bshell_col, bcols, bsorder, coef, K, N, s_pt, s_cf, s_dynd);
+    }
 }
 
 // deterministic sum of the per-block energies (one workgroup)
@@ -809,11 +843,17 @@ template <int PB, int WI, int NIMAX, int G>
 static int rho_one(Context& c, const LaunchRange& r) {
     const int items = r.count * (kBlockPts / PB);
     unsigned grid = 0;
-    int* q = launch_queue(c, items, resident_per_cu(&k_rho<PB, WI, NIMAX, G>, 512, r.lds), &grid);
-    hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G>), dim3(grid), dim3(512), r.lds, c.stream,
-                       c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
-                       c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
-                       c.n_aos, c.d_rho, c.rho_accumulate, q, items);
+    int* q = launch_queue(c, kQueueRho, items, resident_per_cu(&k_rho<PB, WI, NIMAX, G, true>, 512, r.lds), &grid);
+    if (q)
+        hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G, true>), dim3(grid), dim3(512), r.lds, c.stream,
+                           c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
+                           c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
+                           c.n_aos, c.d_rho, c.rho_accumulate, q, items);
+    else
+        hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G, false>), dim3(grid), dim3(512), r.lds, c.stream,
+                           c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
+                           c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
+                           c.n_aos, c.d_rho, c.rho_accumulate, q, items);
     return check_launch("k_rho");
 }
 
@@ -824,7 +864,7 @@ int rho_points_per_workgroup(int cls) { return (cls == 0 || cls == 4) ? 64 : cls
 
 int rho64_max_dyn_lds(size_t* out) {
     hipFuncAttributes a;
-    const hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_rho<64, 8, 2, 4>));
+    const hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_rho<64, 8, 2, 4, false>));
     if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(k_rho)");
     *out = (size_t)160 * 1024 - a.sharedSizeBytes;
     return PDFXC_OK;
@@ -882,11 +922,17 @@ int launch_xc_points_uks(Context& c, int xc_type) {
 
 static int kmat_one(Context& c, const LaunchRange& r) {
     unsigned grid = 0;
-    int* q = launch_queue(c, r.count, resident_per_cu(&k_kmat, 768, r.lds), &grid);
-    hipLaunchKernelGGL(k_kmat, dim3(grid), dim3(768), r.lds, c.stream,
-                       reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
-                       c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col,
-                       c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos, q, r.count);
+    int* q = launch_queue(c, kQueueK, r.count, resident_per_cu(&k_kmat<true>, 768, r.lds), &grid);
+    if (q)
+        hipLaunchKernelGGL(k_kmat<true>, dim3(grid), dim3(768), r.lds, c.stream,
+                           reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
+                           c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col,
+                           c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos, q, r.count);
+    else
+        hipLaunchKernelGGL(k_kmat<false>, dim3(grid), dim3(768), r.lds, c.stream,
+                           reinterpret_cast<const KWork*>(c.d_k_work) + r.offset, c.d_blocks, c.d_pts,
+                           c.n_slots, c.d_shrec, c.rec_stride, c.d_block_shells, c.d_block_shell_col,
+                           c.d_block_cols, c.d_block_sorder, c.d_coef, c.d_K, c.n_aos, q, r.count);
     return check_launch("k_kmat");
 }
 
@@ -933,7 +979,7 @@ __global__ __launch_bounds__(64) void k_zero_int(int* __restrict__ p, int n) {
 int begin_queues(Context& c) {
     c.n_queue_used = 0;
     c.queue_open = false;
-    if (!c.d_queue || !c.tune.persistent) return PDFXC_OK;
+    if (!c.d_queue || !c.tune.persistent) return PDFXC_OK;  // no persistent kernel in this sequence
     // a fill kernel, not a memset: the sequence may be captured in a graph
     hipLaunchKernelGGL(k_zero_int, dim3((kMaxQueues + 63) / 64), dim3(64), 0, c.stream, c.d_queue, kMaxQueues);
     if (int rc = check_launch("k_zero_int")) return rc;
@@ -941,9 +987,9 @@ int begin_queues(Context& c) {
     return PDFXC_OK;
 }
 
-int* launch_queue(Context& c, int n_items, int per_cu, unsigned* grid) {
+int* launch_queue(Context& c, int kind, int n_items, int per_cu, unsigned* grid) {
     const int64_t resident = (int64_t)std::max(per_cu, 1) * std::max(c.n_cu, 1);
-    if (!c.queue_open || c.n_queue_used >= kMaxQueues || n_items <= resident) {
+    if (!c.queue_open || !(c.tune.persistent & kind) || c.n_queue_used >= kMaxQueues || n_items <= resident) {
         *grid = (unsigned)n_items;
         return nullptr;
     }

@@ -111,7 +111,7 @@ __device__ inline void rho_split_tiles(int wave, int lane, int ldnp, int nch, co
 }
 
 template <int S, int PB, int NJMAX>
-__global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
+__device__ __forceinline__ void k_rho_split_item(int item, const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                                    const SplitBlock* __restrict__ sblocks,
                                                    const double* __restrict__ pts, int64_t n_slots,
                                                    const double* __restrict__ shrec, int RS,
@@ -119,21 +119,15 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
                                                    const int* __restrict__ sshell_col, const int* __restrict__ chunks,
                                                    const unsigned short* __restrict__ Ps, int N, int Nc,
                                                    double* __restrict__ rho_out, int accumulate,
-                                                   int* __restrict__ queue, int n_items) {
+        double (*s_pt)[3], double* s_dyn) {
     constexpr int SUBS = kBlockPts / PB;
     constexpr int NSL = 512 / PB;  // shell lanes per point
-    __shared__ double s_pt[PB][3];
-    extern __shared__ __attribute__((aligned(16))) double s_dyn[];
-    __shared__ int s_item;
-    for (int iter = 0;; ++iter) {
-    const int item = next_item(queue, n_items, iter, &s_item);
-    if (item >= n_items) break;
     const int bidx = work[item / SUBS];
     const int sub = item % SUBS;
     const BlockDesc B = blocks[bidx];
     const SplitBlock SB = sblocks[bidx];
     const int p0 = sub * PB;
-    if (p0 >= B.npts) continue;  // uniform: no real point in this part of the block
+    if (p0 >= B.npts) return;  // uniform: no real point in this part of the block
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -294,7 +288,30 @@ __global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work,
         const double v = (q == 0) ? sum : 2.0 * sum;
         rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;
     }
-    }  // work items
+}
+
+template <int S, int PB, int NJMAX, bool PERSIST>
+__global__ __launch_bounds__(512) void k_rho_split(const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
+                                                   const SplitBlock* __restrict__ sblocks,
+                                                   const double* __restrict__ pts, int64_t n_slots,
+                                                   const double* __restrict__ shrec, int RS,
+                                                   const int* __restrict__ bshells, const int* __restrict__ bsorder,
+                                                   const int* __restrict__ sshell_col, const int* __restrict__ chunks,
+                                                   const unsigned short* __restrict__ Ps, int N, int Nc,
+                                                   double* __restrict__ rho_out, int accumulate,
+                                                   int* __restrict__ queue, int n_items) {
+    __shared__ double s_pt[PB][3];
+    extern __shared__ __attribute__((aligned(16))) double s_dyn[];
+    if constexpr (PERSIST) {
+        __shared__ int s_item;
+        for (int iter = 0;; ++iter) {
+            const int item = next_item(queue, n_items, iter, &s_item);
+            if (item >= n_items) break;
+            k_rho_split_item<S, PB, NJMAX>(item, work, blocks, sblocks, pts, n_slots, shrec, RS, bshells, bsorder, sshell_col, chunks, Ps, N, Nc, rho_out, accumulate, s_pt, s_dyn);
+        }
+    } else {
+        k_rho_split_item<S, PB, NJMAX>((int)blockIdx.x, work, blocks, sblocks, pts, n_slots, shrec, RS, bshells, bsorder, sshell_col, chunks, Ps, N, Nc, rho_out, accumulate, s_pt, s_dyn);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -376,7 +393,7 @@ __device__ inline void kmat_split_consume(int pb, int pe, int Tp, int c0, int wc
 }
 
 template <int S, int NW, int NPW>
-__global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict__ work, const KGroup* __restrict__ groups,
+__device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __restrict__ work, const KGroup* __restrict__ groups,
                                                     const BlockDesc* __restrict__ blocks, const int* __restrict__ gblocks,
                                                     const double* __restrict__ pts, int64_t n_slots,
                                                     const double* __restrict__ shrec, int RS,
@@ -384,17 +401,8 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict
                                                     const int* __restrict__ gcols, const int* __restrict__ gsorder,
                                                     const unsigned* __restrict__ gmasks,
                                                     const double* __restrict__ coef, double* __restrict__ K,
-                                                    int* __restrict__ queue, int n_items) {
-    __shared__ double s_pt[2][kBlockPts][3];
-    __shared__ double s_cf[2][kBlockPts][4];
-    __shared__ unsigned s_msk[2][kMaxMaskWords];
-    __shared__ int s_first[kMaxGroup + 1];  // first slab of each block of the group, then the slab count
-    __shared__ int s_blk[kMaxGroup];
-    extern __shared__ __attribute__((aligned(16))) double s_dyn[];
-    __shared__ int s_item;
-    for (int iter = 0;; ++iter) {
-    const int item = next_item(queue, n_items, iter, &s_item);
-    if (item >= n_items) break;
+        double (*s_pt)[kBlockPts][3], double (*s_cf)[kBlockPts][4], unsigned (*s_msk)[kMaxMaskWords],
+        int* s_first, int* s_blk, double* s_dyn) {
     const KGWork W = work[item];
     const KGroup G = groups[W.group];
     const int tid = threadIdx.x;
@@ -508,30 +516,43 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict
                 }
                 continue;
             }
-            // radial sums once, then phi and grad phi per component from the
-            // angular polynomial (keeps the producer within its registers)
-            double r3[3], R0, R1, R2;
-            const int l = rec_radial2(s_rec + k * RS, s_pt[par][pp][0], s_pt[par][pp][1], s_pt[par][pp][2], r3, R0,
-                                      R1, R2);
+            // radial sums once; then each component m in closed form: phi =
+            // f R0 and V = a phi + c . grad phi = a phi + R0 (c . grad f) +
+            // R1 f (c . r), with f the component's angular polynomial
+            // (getPrefactor, DfCalcGridX.cpp:103-141) and R1 = dR/dr / r
+            const double* rec = s_rec + k * RS;
+            const double x = s_pt[par][pp][0] - rec[0], y = s_pt[par][pp][1] - rec[1], z = s_pt[par][pp][2] - rec[2];
+            double R0, R1;
+            rec_radial01(rec, x, y, z, R0, R1);
+            const int l = ((int)rec[3]) & 3;
             const double a = s_cf[par][pp][0], cx = s_cf[par][pp][1], cy = s_cf[par][pp][2], cz = s_cf[par][pp][3];
-            for (int m = 0; m < 2 * l + 1; ++m) {
+            const double R1cr = R1 * (cx * x + cy * y + cz * z);
+            auto put = [&](int m, double f, double cgf) {
                 const int c = cs + m;
-                if (c < 0) continue;
-                if (c >= wcols) break;
-                double f, fg[3], fh[6];
-                angular2(l, m, r3, f, fg, fh);
+                if (c < 0 || c >= wcols) return;  // a shell across the item's first column, or past its last
                 const double v = f * R0;
-                const double vx = fg[0] * R0 + f * r3[0] * R1;
-                const double vy = fg[1] * R0 + f * r3[1] * R1;
-                const double vz = fg[2] * R0 + f * r3[2] * R1;
+                const double V = a * v + R0 * cgf + f * R1cr;
                 const int off = kslab_at(c, p >> 3) + (p & 7);
                 unsigned short h[S];
                 split_bf16<S>(v, h);
 #pragma unroll
                 for (int s = 0; s < S; ++s) phi[(size_t)s * wcols * 16 + off] = h[s];
-                split_bf16<S>(a * v + cx * vx + cy * vy + cz * vz, h);
+                split_bf16<S>(V, h);
 #pragma unroll
                 for (int s = 0; s < S; ++s) vv[(size_t)s * wcols * 16 + off] = h[s];
+            };
+            if (l == 0) {
+                put(0, 1.0, 0.0);
+            } else if (l == 1) {
+                put(0, x, cx);
+                put(1, y, cy);
+                put(2, z, cz);
+            } else {  // xy, zx, yz, (x^2 - y^2)/2, (z^2 - (x^2 + y^2)/2)/sqrt3
+                put(0, x * y, cx * y + cy * x);
+                put(1, z * x, cx * z + cz * x);
+                put(2, y * z, cy * z + cz * y);
+                put(3, 0.5 * (x * x - y * y), cx * x - cy * y);
+                put(4, kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)), kInvSqrt3 * (2.0 * cz * z - cx * x - cy * y));
             }
         }
     };
@@ -602,7 +623,34 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict
             }
         }
     }
-    }  // work items
+}
+
+template <int S, int NW, int NPW, bool PERSIST>
+__global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict__ work, const KGroup* __restrict__ groups,
+                                                    const BlockDesc* __restrict__ blocks, const int* __restrict__ gblocks,
+                                                    const double* __restrict__ pts, int64_t n_slots,
+                                                    const double* __restrict__ shrec, int RS,
+                                                    const int* __restrict__ gshells, const int* __restrict__ gshell_col,
+                                                    const int* __restrict__ gcols, const int* __restrict__ gsorder,
+                                                    const unsigned* __restrict__ gmasks,
+                                                    const double* __restrict__ coef, double* __restrict__ K,
+                                                    int* __restrict__ queue, int n_items) {
+    __shared__ double s_pt[2][kBlockPts][3];
+    __shared__ double s_cf[2][kBlockPts][4];
+    __shared__ unsigned s_msk[2][kMaxMaskWords];
+    __shared__ int s_first[kMaxGroup + 1];  // first slab of each block of the group, then the slab count
+    __shared__ int s_blk[kMaxGroup];
+    extern __shared__ __attribute__((aligned(16))) double s_dyn[];
+    if constexpr (PERSIST) {
+        __shared__ int s_item;
+        for (int iter = 0;; ++iter) {
+            const int item = next_item(queue, n_items, iter, &s_item);
+            if (item >= n_items) break;
+            k_kmat_split_item<S, NW, NPW>(item, work, groups, blocks, gblocks, pts, n_slots, shrec, RS, gshells, gshell_col, gcols, gsorder, gmasks, coef, K, s_pt, s_cf, s_msk, s_first, s_blk, s_dyn);
+        }
+    } else {
+        k_kmat_split_item<S, NW, NPW>((int)blockIdx.x, work, groups, blocks, gblocks, pts, n_slots, shrec, RS, gshells, gshell_col, gcols, gsorder, gmasks, coef, K, s_pt, s_cf, s_msk, s_first, s_blk, s_dyn);
+    }
 }
 
 typedef short s16x4 __attribute__((ext_vector_type(4)));
@@ -675,22 +723,35 @@ template <int S, int PB, int NJMAX>
 static int rho_split_one(Context& c, const LaunchRange& r) {
     const int items = r.count * (kBlockPts / PB);
     unsigned grid = 0;
-    int* q = launch_queue(c, items, resident_per_cu(&k_rho_split<S, PB, NJMAX>, 512, r.lds), &grid);
-    hipLaunchKernelGGL((k_rho_split<S, PB, NJMAX>), dim3(grid), dim3(512), r.lds, c.stream,
-                       c.d_srho_work + r.offset, c.d_blocks, c.d_sblocks, c.d_pts, c.n_slots, c.d_shrec,
-                       c.rec_stride, c.d_block_shells, c.d_block_sorder, c.d_sshell_col, c.d_chunks, c.d_Ps,
-                       c.n_aos, c.split_nc, c.d_rho, c.rho_accumulate, q, items);
+    int* q = launch_queue(c, kQueueRho, items, resident_per_cu(&k_rho_split<S, PB, NJMAX, true>, 512, r.lds), &grid);
+    if (q)
+        hipLaunchKernelGGL((k_rho_split<S, PB, NJMAX, true>), dim3(grid), dim3(512), r.lds, c.stream,
+                           c.d_srho_work + r.offset, c.d_blocks, c.d_sblocks, c.d_pts, c.n_slots, c.d_shrec,
+                           c.rec_stride, c.d_block_shells, c.d_block_sorder, c.d_sshell_col, c.d_chunks, c.d_Ps,
+                           c.n_aos, c.split_nc, c.d_rho, c.rho_accumulate, q, items);
+    else
+        hipLaunchKernelGGL((k_rho_split<S, PB, NJMAX, false>), dim3(grid), dim3(512), r.lds, c.stream,
+                           c.d_srho_work + r.offset, c.d_blocks, c.d_sblocks, c.d_pts, c.n_slots, c.d_shrec,
+                           c.rec_stride, c.d_block_shells, c.d_block_sorder, c.d_sshell_col, c.d_chunks, c.d_Ps,
+                           c.n_aos, c.split_nc, c.d_rho, c.rho_accumulate, q, items);
     return check("k_rho_split");
 }
 
 template <int S, int NW, int NPW>
 static int kmat_split_launch(Context& c, const LaunchRange& r) {
     unsigned grid = 0;
-    int* q = launch_queue(c, r.count, resident_per_cu(&k_kmat_split<S, NW, NPW>, NW * 64, r.lds), &grid);
-    hipLaunchKernelGGL((k_kmat_split<S, NW, NPW>), dim3(grid), dim3(NW * 64), r.lds, c.stream,
-                       c.d_kg_work + r.offset, c.d_kgroups, c.d_blocks, c.d_kg_blocks, c.d_pts, c.n_slots,
-                       c.d_shrec, c.rec_stride, c.d_kg_shells, c.d_kg_shell_col, c.d_kg_cols, c.d_kg_sorder,
-                       c.d_kg_masks, c.d_coef, c.d_K, q, r.count);
+    int* q = launch_queue(c, kQueueK, r.count, resident_per_cu(&k_kmat_split<S, NW, NPW, true>, NW * 64, r.lds),
+                          &grid);
+    if (q)
+        hipLaunchKernelGGL((k_kmat_split<S, NW, NPW, true>), dim3(grid), dim3(NW * 64), r.lds, c.stream,
+                           c.d_kg_work + r.offset, c.d_kgroups, c.d_blocks, c.d_kg_blocks, c.d_pts, c.n_slots,
+                           c.d_shrec, c.rec_stride, c.d_kg_shells, c.d_kg_shell_col, c.d_kg_cols, c.d_kg_sorder,
+                           c.d_kg_masks, c.d_coef, c.d_K, q, r.count);
+    else
+        hipLaunchKernelGGL((k_kmat_split<S, NW, NPW, false>), dim3(grid), dim3(NW * 64), r.lds, c.stream,
+                           c.d_kg_work + r.offset, c.d_kgroups, c.d_blocks, c.d_kg_blocks, c.d_pts, c.n_slots,
+                           c.d_shrec, c.rec_stride, c.d_kg_shells, c.d_kg_shell_col, c.d_kg_cols, c.d_kg_sorder,
+                           c.d_kg_masks, c.d_coef, c.d_K, q, r.count);
     return check("k_kmat_split");
 }
 
@@ -744,8 +805,8 @@ int launch_split_kmat(Context& c) {
 // workgroup less the kernel's static LDS, read from the code object
 int split_kmat_max_dyn_lds(int S, size_t* out) {
     hipFuncAttributes a;
-    const hipError_t e = (S == 3) ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<3, 16, 1>))
-                                  : hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<2, 16, 1>));
+    const hipError_t e = (S == 3) ? hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<3, 16, 1, false>))
+                                  : hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_kmat_split<2, 16, 1, false>));
     if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(k_kmat_split)");
     *out = (size_t)160 * 1024 - a.sharedSizeBytes;
     return PDFXC_OK;

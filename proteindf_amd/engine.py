@@ -345,7 +345,7 @@ class XcContext:
         check(lib().pdfxc_save_grid_matrix(self.h, _path(path), n_rho_cols))
 
     def set_tuning(self, rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=24, split_k_group=4,
-                   persistent=1):
+                   persistent=2):
         t = Tuning(rho_shift, k_tiles, split_rho_shift, split_k_patches, split_k_group, persistent, 0.0)
         check(lib().pdfxc_set_tuning(self.h, ctypes.byref(t)))
 

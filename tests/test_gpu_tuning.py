@@ -47,7 +47,7 @@ def test_every_shape_gives_the_same_build(prec):
         assert abs(E - E0) <= 1e-12 * abs(E0)
         np.testing.assert_allclose(K, K0, rtol=0, atol=1e-12)
     for bad in (dict(k_tiles=200), dict(split_k_patches=33), dict(split_k_patches=3), dict(split_k_group=3),
-                dict(split_k_group=16), dict(split_k_group=0), dict(persistent=2)):
+                dict(split_k_group=16), dict(split_k_group=0), dict(persistent=4), dict(persistent=-1)):
         with pytest.raises(engine.PdfxcError) as ei:
             ctx.set_tuning(**bad)
         assert ei.value.code == engine.ERROR_CODES["ARGUMENT"]
@@ -116,11 +116,12 @@ def test_split_k_shapes_on_wide_blocks(prec):
 @pytest.mark.parametrize("prec", ["fp64", "bf16x3", "bf16x3k"])
 def test_persistent_and_per_item_launches_agree(prec):
     """(H2O)8 on a 75 x 302 grid has more rho and K work items than the
-    device keeps workgroups resident, so the default launches are persistent
-    (workgroups take items from a device counter); one workgroup per item
-    gives the same energy bit for bit (every item's arithmetic is the same)
-    and K up to the order of its atomic adds, on repeated builds too (the
-    counters are reset by every build)."""
+    device keeps workgroups resident, so persistent launches (workgroups take
+    items from a device counter) engage: with the rho kernels, the K kernels
+    or both persistent, the build equals the one-workgroup-per-item build, the
+    energy bit for bit (every item's arithmetic is the same) and K up to the
+    order of its atomic adds, on repeated builds too (every build resets the
+    counters)."""
     mol = molecules.water_cluster(8, 0)
     info = calc_grid.orbital_info(mol)
     PA = 0.5 * guess.density_guess(mol, info, seed=0, eps=0.05)
@@ -130,10 +131,11 @@ def test_persistent_and_per_item_launches_agree(prec):
     P = guess.pack_lower(PA)
     ctx.set_tuning(persistent=0)
     E0, K0, _ = ctx.build_rks(P)
-    ctx.set_tuning(persistent=1)
-    for rep in range(3):
-        E, K, _ = ctx.build_rks(P)
-        say(prec, "persistent build", rep, "dE", E - E0, "max dK", np.abs(K - K0).max())
-        assert E == E0
-        np.testing.assert_allclose(K, K0, rtol=0, atol=1e-13)
+    for mask in (1, 2, 3):
+        ctx.set_tuning(persistent=mask)
+        for rep in range(2):
+            E, K, _ = ctx.build_rks(P)
+            say(prec, "persistent", mask, "build", rep, "dE", E - E0, "max dK", np.abs(K - K0).max())
+            assert E == E0
+            np.testing.assert_allclose(K, K0, rtol=0, atol=1e-13)
     ctx.close()
