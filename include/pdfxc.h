@@ -275,6 +275,17 @@ int pdfxc_gradient_rks(pdfxc_ctx* ctx, int functional, int n_atoms, const int* Z
  * Host buffers; runs on the current device. */
 int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* out);
 
+/* pdfxc_eval_functional without any device allocation, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  Inputs
+ * and outputs are staged through a fixed device array of the library, one per
+ * GPU, held until the process ends: n is at most PDFXC_NOALLOC_MAX_POINTS
+ * (larger n: PDFXC_ERR_ARGUMENT, nothing runs), i.e. 6 x 65536 doubles = 3 MiB
+ * per GPU.  Runs on the current device and the null stream; calls that overlap
+ * on one GPU share that array and race.  Same results as pdfxc_eval_functional. */
+#define PDFXC_NOALLOC_MAX_POINTS 65536
+int pdfxc_eval_functional_noalloc(int functional, int64_t n, const double* in, double* out);
+
 /* ProteinDF matrix files (TlMatrixObject::MatrixType, tl_matrix_object.h:50-61):
  * a header (type, rows, cols) and the doubles.  Readers accept every header
  * width the reference accepts (type 1/4/8 bytes, counts 4/8 bytes, the first

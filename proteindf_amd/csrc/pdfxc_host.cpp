@@ -2136,6 +2136,11 @@ int pdfxc_eval_functional(int functional, int64_t n, const double* in, double* o
     return rc;
 }
 
+int pdfxc_eval_functional_noalloc(int functional, int64_t n, const double* in, double* out) {
+    if (int rc = check_functional(functional, "pdfxc_eval_functional_noalloc")) return rc;
+    return eval_functional_noalloc(functional, n, in, out);
+}
+
 int pdfxc_eval_aos(pdfxc_ctx* ctx, int64_t n, const double* xyz, double* phi, double* gx, double* gy,
                    double* gz) {
     Context* c = reinterpret_cast<Context*>(ctx);

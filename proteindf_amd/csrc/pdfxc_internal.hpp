@@ -264,6 +264,8 @@ int launch_xc_points_uks(Context& c, int xc_type);
 int ensure_beta_buffers(Context* c);  // UKS beta rho / point-term buffers, zeroed when first made
 int launch_eval_functional_uks(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s);
 int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s);
+// pdfxc_eval_functional through the library's own device staging (no allocation)
+int eval_functional_noalloc(int xc_type, int64_t n, const double* in, double* out);
 int launch_eval_aos_points(const Context& c, int64_t n, const double* d_xyz, double* d_out,
                            hipStream_t s);
 int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_pts,

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "pdfxc_device.hpp"
 #include "pdfxc_internal.hpp"
@@ -1014,6 +1016,65 @@ int launch_eval_functional(int xc_type, int64_t n, const double* d_in, double* d
         default: set_error("k_eval_functional: no grid functional " + std::to_string(xc_type)); return PDFXC_ERR_UNSUPPORTED;
     }
     return check_launch("k_eval_functional");
+}
+
+// ---------------------------------------------------------------------------
+// Staging that the library does not allocate: pdfxc_eval_functional_noalloc
+// copies its inputs (n x 2) and outputs (n x 4) through this array, one copy
+// per GPU held for the life of the process.  Host code never passes the
+// array's name to a kernel or a copy (on the host it is only a stand-in for
+// the symbol): the device address comes from hipGetSymbolAddress, looked up
+// once per device on that device's first call and kept.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMax = PDFXC_NOALLOC_MAX_POINTS;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) double scratch[6 * kNoAllocMax];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+static int scratch_on_current_device(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "pdfxc_eval_functional_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        set_error("pdfxc_eval_functional_noalloc: device index " + std::to_string(dev) + " beyond " +
+                  std::to_string(kNoAllocMaxDevices));
+        return PDFXC_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "pdfxc_eval_functional_noalloc: hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return PDFXC_OK;
+}
+
+int eval_functional_noalloc(int xc_type, int64_t n, const double* in, double* out) {
+    if (n < 0 || n > kNoAllocMax) {
+        set_error("pdfxc_eval_functional_noalloc: " + std::to_string(n) + " points; this entry point takes 0 to " +
+                  std::to_string(kNoAllocMax) + " (PDFXC_NOALLOC_MAX_POINTS)");
+        return PDFXC_ERR_ARGUMENT;
+    }
+    if (n > 0 && (!in || !out)) {
+        set_error("pdfxc_eval_functional_noalloc: null input or output");
+        return PDFXC_ERR_ARGUMENT;
+    }
+    if (n == 0) return PDFXC_OK;
+    if (xc_type == kHF) {
+        std::fill(out, out + 4 * n, 0.0);
+        return PDFXC_OK;
+    }
+    double* s = nullptr;
+    if (int rc = scratch_on_current_device(&s)) return rc;
+    double* s_in = s;                  // [n][2]
+    double* s_out = s + 2 * kNoAllocMax;  // [n][4]
+    PDFXC_HIP(hipMemcpy(s_in, in, 2 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    if (int rc = launch_eval_functional(xc_type, n, s_in, s_out, nullptr)) return rc;
+    PDFXC_HIP(hipMemcpy(out, s_out, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    return PDFXC_OK;
 }
 
 int launch_eval_functional_uks(int xc_type, int64_t n, const double* d_in, double* d_out, hipStream_t s) {

@@ -45,6 +45,7 @@ EXPORTED = [
     "pdfxc_density_at_points", "pdfxc_batch_create", "pdfxc_batch_destroy", "pdfxc_batch_offsets",
     "pdfxc_batch_build_rks_device", "pdfxc_batch_build_rks", "pdfxc_batch_stats", "pdfxc_set_tuning",
     "pdfxc_autotune", "pdfxc_grid_opts_preset", "pdfxc_generate_grid_ex", "pdfxc_generate_grid_oriented",
+    "pdfxc_eval_functional_noalloc",
 ]
 # matrix file types (TlMatrixObject::MatrixType)
 PDFXC_MAT_RSFD, PDFXC_MAT_CSFD, PDFXC_MAT_RLHD = 0, 1, 2
@@ -131,6 +132,7 @@ def lib():
         L.pdfxc_build_rks.argtypes = [P, i32, P, P, P, P]
         L.pdfxc_build_rks_device.argtypes = [P, i32, P, P, P]
         L.pdfxc_eval_functional.argtypes = [i32, i64, P, P]
+        L.pdfxc_eval_functional_noalloc.argtypes = [i32, i64, P, P]
         L.pdfxc_eval_aos.argtypes = [P, i64, P, P, P, P, P]
         L.pdfxc_get_stats.argtypes = [P, P]
         L.pdfxc_stream.argtypes = [P]
@@ -199,6 +201,18 @@ def eval_functional(rho_a, gamma_aa, functional=PDFXC_B3LYP):
     x = np.ascontiguousarray(np.stack([np.ravel(rho_a), np.ravel(gamma_aa)], 1), dtype=np.float64)
     out = np.zeros((x.shape[0], 4))
     check(lib().pdfxc_eval_functional(functional, x.shape[0], _p(x), _p(out)))
+    return out
+
+
+NOALLOC_MAX_POINTS = 65536  # PDFXC_NOALLOC_MAX_POINTS
+
+
+def eval_functional_noalloc(rho_a, gamma_aa, functional=PDFXC_B3LYP):
+    """eval_functional through the library's fixed device staging (no device
+    allocation; at most NOALLOC_MAX_POINTS points per call)."""
+    x = np.ascontiguousarray(np.stack([np.ravel(rho_a), np.ravel(gamma_aa)], 1), dtype=np.float64)
+    out = np.zeros((x.shape[0], 4))
+    check(lib().pdfxc_eval_functional_noalloc(functional, x.shape[0], _p(x), _p(out)))
     return out
 
 
