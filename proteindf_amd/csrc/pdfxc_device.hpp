@@ -154,6 +154,35 @@ __device__ inline void rec_values0_r(int l, double x, double y, double z, double
         v[4] = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)) * R0;
     }
 }
+// phi_m and V_m = a phi_m + c . grad phi_m of the 2l+1 components of a shell
+// at offset (x, y, z) from its centre, from its radial sums R0 and R1 =
+// (dR/dr) / r: phi_m = f_m R0 and c . grad phi_m = R0 (c . grad f_m) +
+// R1 f_m (c . r), f_m the angular polynomials of getPrefactor
+// (DfCalcGridX.cpp:103-141: 1; x, y, z; xy, zx, yz, (x^2 - y^2)/2,
+// (z^2 - (x^2 + y^2)/2)/sqrt3).  out(m, phi_m, V_m) is called per component.
+template <typename Out>
+__device__ inline void shell_phi_v(int l, double x, double y, double z, double R0, double R1, double a, double cx,
+                                   double cy, double cz, Out out) {
+    const double R1cr = R1 * (cx * x + cy * y + cz * z);
+    auto put = [&](int m, double f, double cgf) {
+        const double v = f * R0;
+        out(m, v, a * v + R0 * cgf + f * R1cr);
+    };
+    if (l == 0) {
+        put(0, 1.0, 0.0);
+    } else if (l == 1) {
+        put(0, x, cx);
+        put(1, y, cy);
+        put(2, z, cz);
+    } else {
+        put(0, x * y, cx * y + cy * x);
+        put(1, z * x, cx * z + cz * x);
+        put(2, y * z, cy * z + cz * y);
+        put(3, 0.5 * (x * x - y * y), cx * x - cy * y);
+        put(4, kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)), kInvSqrt3 * (2.0 * cz * z - cx * x - cy * y));
+    }
+}
+
 // Dots of one shell's values at a point with per-component weights X
 // (X = P phi in the rho pass): sum_m X_m phi_m = R0 S and sum_m X_m grad
 // phi_m = R0 G + R1 r S, with S = sum_m X_m f_m and G = sum_m X_m grad f_m

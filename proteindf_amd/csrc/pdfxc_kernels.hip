@@ -609,19 +609,20 @@ __device__ __forceinline__ void k_kmat_item(int item, const KWork* __restrict__ 
             const int p = it & (PS - 1);  // PS is a power of two (16, 8 or 4)
             const int k = it >> ps_log;
             const int pp = pb + p;
-            double v[5], vx[5], vy[5], vz[5];
-            rec_values(s_rec + k * RS, s_pt[pp][0], s_pt[pp][1], s_pt[pp][2], v, vx, vy, vz);
-            const double a = s_cf[pp][0], cx = s_cf[pp][1], cy = s_cf[pp][2], cz = s_cf[pp][3];
+            const double* rec = s_rec + k * RS;
+            const double x = s_pt[pp][0] - rec[0], y = s_pt[pp][1] - rec[1], z = s_pt[pp][2] - rec[2];
+            double R0, R1;
+            rec_radial01(rec, x, y, z, R0, R1);
             const int c0 = s_shcol[k];
-            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
             const int sw = p & 15;
-            for (int m = 0; m < nc; ++m) {
-                const int c = c0 + m;
-                if (c < ncols) {
-                    phi[p * LD + (c ^ sw)] = v[m];
-                    vv[p * LD + (c ^ sw)] = a * v[m] + cx * vx[m] + cy * vy[m] + cz * vz[m];
-                }
-            }
+            shell_phi_v(((int)rec[3]) & 3, x, y, z, R0, R1, s_cf[pp][0], s_cf[pp][1], s_cf[pp][2], s_cf[pp][3],
+                        [&](int m, double v, double V) {
+                            const int c = c0 + m;
+                            if (c < ncols) {
+                                phi[p * LD + (c ^ sw)] = v;
+                                vv[p * LD + (c ^ sw)] = V;
+                            }
+                        });
         }
         for (int it = ptid; it < (ncols - min(ncols, B.nao)) * PS; it += 256) {
             const int c = B.nao + (it >> ps_log);

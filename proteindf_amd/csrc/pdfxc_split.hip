@@ -200,17 +200,22 @@ __device__ __forceinline__ void k_rho_split_item(int item, const int* __restrict
         double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
         for (int j = kk2; j < nsh; j += NSL2) {
             const int k = s_sord[j];
-            double v[5], vx[5], vy[5], vz[5];
-            rec_values(s_rec + k * RS, qx, qy, qz, v, vx, vy, vz);
-            const int c0 = s_shcol[k];
-            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-            for (int m = 0; m < nc; ++m) {
-                const double x = s_xd[pl * RXD + c0 + m];
-                r0 += x * v[m];
-                r1 += x * vx[m];
-                r2 += x * vy[m];
-                r3 += x * vz[m];
+            const double* rec = s_rec + k * RS;
+            const double x = qx - rec[0], y = qy - rec[1], z = qz - rec[2];
+            double R0, R1;
+            rec_radial01(rec, x, y, z, R0, R1);
+            const int l = ((int)rec[3]) & 3;
+            const double* xr = s_xd + pl * RXD + s_shcol[k];
+            double X[5] = {xr[0], 0.0, 0.0, 0.0, 0.0};
+            if (l >= 1) {
+                X[1] = xr[1];
+                X[2] = xr[2];
             }
+            if (l == 2) {
+                X[3] = xr[3];
+                X[4] = xr[4];
+            }
+            shell_dots(l, x, y, z, R0, R1, X, r0, r1, r2, r3);
         }
         part[round][0] = r0;
         part[round][1] = r1;
@@ -516,44 +521,24 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
                 }
                 continue;
             }
-            // radial sums once; then each component m in closed form: phi =
-            // f R0 and V = a phi + c . grad phi = a phi + R0 (c . grad f) +
-            // R1 f (c . r), with f the component's angular polynomial
-            // (getPrefactor, DfCalcGridX.cpp:103-141) and R1 = dR/dr / r
+            // radial sums once, then each component's phi and V in closed form
             const double* rec = s_rec + k * RS;
             const double x = s_pt[par][pp][0] - rec[0], y = s_pt[par][pp][1] - rec[1], z = s_pt[par][pp][2] - rec[2];
             double R0, R1;
             rec_radial01(rec, x, y, z, R0, R1);
-            const int l = ((int)rec[3]) & 3;
-            const double a = s_cf[par][pp][0], cx = s_cf[par][pp][1], cy = s_cf[par][pp][2], cz = s_cf[par][pp][3];
-            const double R1cr = R1 * (cx * x + cy * y + cz * z);
-            auto put = [&](int m, double f, double cgf) {
-                const int c = cs + m;
-                if (c < 0 || c >= wcols) return;  // a shell across the item's first column, or past its last
-                const double v = f * R0;
-                const double V = a * v + R0 * cgf + f * R1cr;
-                const int off = kslab_at(c, p >> 3) + (p & 7);
-                unsigned short h[S];
-                split_bf16<S>(v, h);
+            shell_phi_v(((int)rec[3]) & 3, x, y, z, R0, R1, s_cf[par][pp][0], s_cf[par][pp][1], s_cf[par][pp][2],
+                        s_cf[par][pp][3], [&](int m, double v, double V) {
+                            const int c = cs + m;
+                            if (c < 0 || c >= wcols) return;  // a shell across the item's first column, or past its last
+                            const int off = kslab_at(c, p >> 3) + (p & 7);
+                            unsigned short h[S];
+                            split_bf16<S>(v, h);
 #pragma unroll
-                for (int s = 0; s < S; ++s) phi[(size_t)s * wcols * 16 + off] = h[s];
-                split_bf16<S>(V, h);
+                            for (int s = 0; s < S; ++s) phi[(size_t)s * wcols * 16 + off] = h[s];
+                            split_bf16<S>(V, h);
 #pragma unroll
-                for (int s = 0; s < S; ++s) vv[(size_t)s * wcols * 16 + off] = h[s];
-            };
-            if (l == 0) {
-                put(0, 1.0, 0.0);
-            } else if (l == 1) {
-                put(0, x, cx);
-                put(1, y, cy);
-                put(2, z, cz);
-            } else {  // xy, zx, yz, (x^2 - y^2)/2, (z^2 - (x^2 + y^2)/2)/sqrt3
-                put(0, x * y, cx * y + cy * x);
-                put(1, z * x, cx * z + cz * x);
-                put(2, y * z, cy * z + cz * y);
-                put(3, 0.5 * (x * x - y * y), cx * x - cy * y);
-                put(4, kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)), kInvSqrt3 * (2.0 * cz * z - cx * x - cy * y));
-            }
+                            for (int s = 0; s < S; ++s) vv[(size_t)s * wcols * 16 + off] = h[s];
+                        });
         }
     };
 
