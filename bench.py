@@ -109,17 +109,24 @@ def cpu_baseline(info, grid, PA, flops_total, target_s=12.0):
 
 
 MODES = {"fp64": engine.PDFXC_FP64, "bf16x2": engine.PDFXC_SPLIT_BF16, "bf16x3": engine.PDFXC_SPLIT_BF16X3,
-         "bf16x3k": engine.PDFXC_SPLIT_K_BF16X3}
+         "bf16x3k": engine.PDFXC_SPLIT_K_BF16X3, "bfp": engine.PDFXC_SPLIT_BFP, "f16k": engine.PDFXC_SPLIT_K_F16}
 # (rho-pass kernel, peak TFLOP/s), (K-pass kernel, peak) per mode: dense FP64 and bf16 MFMA peaks
 KERNELS = {"fp64": (("k_rho", 78.6), ("k_kmat", 78.6)),
            "bf16x2": (("k_rho_split", 2500.0), ("k_kmat_split", 2500.0)),
            "bf16x3": (("k_rho_split", 2500.0), ("k_kmat_split", 2500.0)),
-           "bf16x3k": (("k_rho", 78.6), ("k_kmat_split", 2500.0))}
+           "bf16x3k": (("k_rho", 78.6), ("k_kmat_split", 2500.0)),
+           "bfp": (("k_rho_bfp", 2500.0), ("k_kmat_split", 2500.0)),
+           "f16k": (("k_rho", 78.6), ("k_kmat_split", 2500.0))}
 LABELS = {"fp64": "FP64", "bf16x2": "split-BF16 hi+lo", "bf16x3": "split-BF16 hi+mid+lo",
-          "bf16x3k": "FP64 rho pass + split-BF16 hi+mid+lo K pass"}
+          "bf16x3k": "FP64 rho pass + split-BF16 hi+mid+lo K pass",
+          "bfp": "block-FP BF16 rho pass (4 slices, exact sums) + split-BF16 hi+mid+lo K pass",
+          "f16k": "FP64 rho pass + split-FP16 hi+lo K pass (per-slab column scales)"}
 DTYPES = {"fp64": "f64", "bf16x2": "bf16 slices, f32 accumulate, f64 AO/functional/sums",
           "bf16x3": "bf16 slices, f32 accumulate, f64 AO/functional/sums",
-          "bf16x3k": "f64 rho pass; K pass bf16 slices, f32 accumulate; f64 AO/functional/sums"}
+          "bf16x3k": "f64 rho pass; K pass bf16 slices, f32 accumulate; f64 AO/functional/sums",
+          "bfp": "rho pass bf16 block-FP slices, exact f32 sums; K pass bf16 slices, f32 accumulate; "
+                 "f64 AO/functional/sums",
+          "f16k": "f64 rho pass; K pass fp16 slices, f32 accumulate; f64 AO/functional/sums"}
 E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 build
 # HBM bytes per build of a dominant kernel (the sum over its class launches),
 # measured in separate rocprofv3 --pmc runs (FETCH_SIZE x 2.00 and WRITE_SIZE
@@ -296,7 +303,7 @@ def main():
     ap.add_argument("--workload", default="gly30", choices=sorted(WORKLOADS) + ["h2o_batch"])
     ap.add_argument("--precision", default="bf16x3k", choices=sorted(MODES),
                     help="mode whose throughput is the reported value (the others are measured beside it)")
-    ap.add_argument("--modes", default="fp64,bf16x2,bf16x3,bf16x3k", help="comma list of modes to measure")
+    ap.add_argument("--modes", default="fp64,bf16x2,bf16x3,bf16x3k,bfp,f16k", help="comma list of modes to measure")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--split-k-patches", type=int, default=0, metavar="N",

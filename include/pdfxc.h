@@ -53,20 +53,35 @@ enum {
 };
 
 /* arithmetic of the two contractions X = Phi.P and K = Phi^T.V:
- * FP64 MFMA, or bf16 MFMA with FP32 accumulation on operands split into
- * bf16 slices (hi + lo: 3 slice products; hi + mid + lo: 6).
- * PDFXC_SPLIT_K_BF16X3 runs the rho pass in FP64 (so E_xc and rho are the
- * FP64 build's) and only the K pass in hi + mid + lo.  AO values, the
- * functional, the rho / grad rho dots and the K_xc sums stay FP64 in every
- * mode.  No reference counterpart (the reference is FP64 only). */
-enum { PDFXC_FP64 = 0, PDFXC_SPLIT_BF16 = 1, PDFXC_SPLIT_BF16X3 = 2, PDFXC_SPLIT_K_BF16X3 = 3 };
+ * FP64 MFMA, or bf16 / fp16 MFMA with FP32 accumulation on operands split
+ * into slices (hi + lo: 3 slice products; hi + mid + lo: 6).
+ *   PDFXC_SPLIT_BF16, PDFXC_SPLIT_BF16X3: both passes in bf16 hi + lo, or
+ *     hi + mid + lo;
+ *   PDFXC_SPLIT_K_BF16X3: the rho pass in FP64 (so E_xc and rho are the FP64
+ *     build's), the K pass in bf16 hi + mid + lo;
+ *   PDFXC_SPLIT_BFP: the rho pass in block floating point (4 bf16 slices of
+ *     8 bits sharing one exponent per point and per P row; the FP32 sums of
+ *     integer slice products are exact), the K pass in bf16 hi + mid + lo;
+ *   PDFXC_SPLIT_K_F16: the rho pass in FP64, the K pass in fp16 hi + lo with
+ *     one power-of-two scale per AO column and 16-point slab, so that the
+ *     values stay inside fp16's exponent range.
+ * AO values, the functional, the rho / grad rho dots and the K_xc sums stay
+ * FP64 in every mode.  No reference counterpart (the reference is FP64 only). */
+enum {
+    PDFXC_FP64 = 0,
+    PDFXC_SPLIT_BF16 = 1,
+    PDFXC_SPLIT_BF16X3 = 2,
+    PDFXC_SPLIT_K_BF16X3 = 3,
+    PDFXC_SPLIT_BFP = 4,
+    PDFXC_SPLIT_K_F16 = 5
+};
 
 typedef struct pdfxc_opts {
     double density_cutoff;   /* rho_alpha cutoff, xc-density-threshold (DfCalcGridX.cpp:55-59), 1e-16 */
     double ao_screen;        /* per-block AO screening bound; 0 keeps every AO the reference
                                 evaluates as non-zero (only the alpha r^2 > 30 primitive cut,
                                 DfCalcGridX.cpp:42, :690-700) */
-    int precision;           /* PDFXC_FP64, PDFXC_SPLIT_BF16, PDFXC_SPLIT_BF16X3, PDFXC_SPLIT_K_BF16X3 */
+    int precision;           /* PDFXC_FP64 or one of the PDFXC_SPLIT_* modes above */
     int block_points;        /* points per grid block: 128 */
     int64_t scratch_bytes;   /* reserved, must be 0 (AO values are never stored in HBM) */
 } pdfxc_opts;
@@ -87,7 +102,10 @@ typedef struct pdfxc_stats {
 /* Launch shapes of the build kernels.  rho_shift (0..3) / split_rho_shift
  * (0..2): move every block that many classes toward fewer points per
  * workgroup (rho pass; at rho_shift 0 a block of 129..256 padded AO columns
- * takes 64 points per workgroup when they fit in LDS, else 32); k_tiles (8..96): FP64 K pass work item size in lower
+ * takes 64 points per workgroup when they fit in LDS, else 32; in the block
+ * floating point rho pass a block takes the first of the 64-, 32- and
+ * 16-point classes from split_rho_shift on that holds its columns and fits
+ * in LDS); k_tiles (8..96): FP64 K pass work item size in lower
  * 16 x 16 tiles for blocks of more than 32 tiles; split_k_patches (4..32):
  * split K pass work item size in 2 x 2 tile patches (up to 16: workgroups of
  * 16 waves, one patch each; up to 24: 12 waves of two; up to 32: 8 waves of
@@ -370,12 +388,14 @@ const char* pdfxc_last_error(void);
 int pdfxc_selftest_mfma(double* out16x16);
 
 /* Diagnostics: the integer operands A[i][k] = ((3 i + 5 k) mod 11) - 5,
- * B[k][j] = ((7 k + 2 j) mod 13) - 6 (exact in bf16 and FP32) multiplied by
- * v_mfma_f32_16x16x32_bf16 (k < 32; out[0..255]) and by
- * v_mfma_f32_16x16x16_bf16 (k < 16; out[256..511]), each written as the
+ * B[k][j] = ((7 k + 2 j) mod 13) - 6 (exact in bf16, fp16 and FP32)
+ * multiplied by v_mfma_f32_16x16x32_bf16 (k < 32; out[0..255]), by
+ * v_mfma_f32_16x16x16_bf16 (k < 16; out[256..511]), by
+ * v_mfma_f32_16x16x32_f16 (k < 32; out[512..767]) and by
+ * v_mfma_f32_16x16x16_f16 (k < 16; out[768..1023]), each written as the
  * 16 x 16 row-major D the split-precision kernels' operand and accumulator
  * layouts assume. */
-int pdfxc_selftest_mfma_split(float* out2x16x16);
+int pdfxc_selftest_mfma_split(float* out4x16x16);
 
 /* Diagnostics: a plain streaming copy of n doubles between device buffers
  * (reads 8n bytes, writes 8n bytes), the reference point for HBM traffic

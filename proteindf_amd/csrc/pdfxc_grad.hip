@@ -497,7 +497,9 @@ static int grad_fn_cls(Context& c, const LaunchRange& r, const int* d_shell_atom
 int launch_grad_fn(Context& c, const LaunchRange& r, const int* d_shell_atom, const int* d_slot_atom,
                    int n_atoms, double* d_gacc) {
     if (r.count <= 0) return PDFXC_OK;
-    switch (c.opts.precision == PDFXC_FP64 ? 0 : prec_slices(c.opts.precision)) {
+    // the gradient's contractions: FP64, bf16 hi + lo in PDFXC_SPLIT_BF16,
+    // else bf16 hi + mid + lo (the FP16 K mode included)
+    switch (c.opts.precision == PDFXC_FP64 ? 0 : c.opts.precision == PDFXC_SPLIT_BF16 ? 2 : 3) {
         case 0: return grad_fn_cls<0>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
         case 2: return grad_fn_cls<2>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);
         default: return grad_fn_cls<3>(c, r, d_shell_atom, d_slot_atom, n_atoms, d_gacc);

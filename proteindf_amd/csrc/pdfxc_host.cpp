@@ -141,7 +141,8 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
         return PDFXC_ERR_UNSUPPORTED;
     }
     if (c->opts.precision != PDFXC_FP64 && c->opts.precision != PDFXC_SPLIT_BF16 &&
-        c->opts.precision != PDFXC_SPLIT_BF16X3 && c->opts.precision != PDFXC_SPLIT_K_BF16X3) {
+        c->opts.precision != PDFXC_SPLIT_BF16X3 && c->opts.precision != PDFXC_SPLIT_K_BF16X3 &&
+        c->opts.precision != PDFXC_SPLIT_BFP && c->opts.precision != PDFXC_SPLIT_K_F16) {
         delete c;
         set_error("pdfxc_ctx_create: unknown precision mode " + std::to_string(c->opts.precision));
         return PDFXC_ERR_ARGUMENT;
@@ -305,7 +306,8 @@ int pdfxc_ctx_create(pdfxc_ctx** out, int n_aos, const double* ao_center, const 
             for (int k : order)
                 for (int m = 0; m < 2 * c->shells[k].l + 1; ++m) c->split_perm.push_back(c->shells[k].first_ao + m);
         }
-        rc = dalloc(&c->d_Ps, (size_t)S * n_aos * c->split_nc);
+        (void)S;
+        rc = dalloc(&c->d_Ps, (size_t)psplit_elems(c->opts.precision, n_aos));
         if (rc == PDFXC_OK) rc = dalloc(&c->d_perm, (size_t)n_aos);
         if (rc == PDFXC_OK) {
             e = hipMemcpy(c->d_perm, c->split_perm.data(), sizeof(int) * n_aos, hipMemcpyHostToDevice);
@@ -752,7 +754,7 @@ static int build_blocks_impl(Context& c) {
                 const int pos = (int)(std::lower_bound(ch.begin(), ch.end(), q0 / 4) - ch.begin());
                 c.sshell_col.push_back(4 * pos + q0 % 4);
             }
-            if (SB.ldnp > 1024 && prec_split_rho(c.opts.precision)) {
+            if (SB.ldnp > 1024 && prec_split_rho(c.opts.precision) && !prec_bfp(c.opts.precision)) {
                 set_error("set_grid: a block keeps " + std::to_string(4 * SB.nchunk) +
                           " chunked AO columns; the split kernels take at most 1024 (raise ao_screen)");
                 return PDFXC_ERR_UNSUPPORTED;
@@ -763,8 +765,54 @@ static int build_blocks_impl(Context& c) {
             while ((256 << base) < ldnp) ++base;
             return std::min(2, base + c.tune.split_rho_shift);
         };
+        // block floating point: classes 0, 1, 2 (64, 32, 16 points per
+        // workgroup; blocks of at most 256, 384, 512 chunked columns), a block
+        // in the first class that takes it and whose LDS fits, the tuning's
+        // split_rho_shift moving blocks to narrower classes; blocks no class
+        // takes run the FP64 k_rho, listed here as class kRhoFp64Class + their
+        // k_rho class
+        std::vector<int> bfp_cls;
+        if (prec_bfp(c.opts.precision)) {
+            size_t bfp_lds_max[3];
+            for (int cls = 0; cls < 3; ++cls)
+                if (int rc = bfp_max_dyn_lds(cls, &bfp_lds_max[cls])) return rc;
+            auto bfp_lds = [&](int b, int cls) {
+                const size_t ra = (size_t)c.sblocks[b].ldnp + 8;
+                return std::max((size_t)kBfpPlanes * bfp_points_per_workgroup(cls) * ra * 2,
+                                (size_t)4 * 512 * sizeof(double)) +
+                       sizeof(double) * (size_t)c.blocks[b].nshell * RS +
+                       sizeof(int) * (size_t)(c.sblocks[b].ldnp / 4 + 2 * c.blocks[b].nshell);
+            };
+            bfp_cls.assign(nb, -1);
+            for (int b = 0; b < nb; ++b)
+                for (int cls = std::min(2, c.tune.split_rho_shift); cls < 3; ++cls)
+                    if (c.sblocks[b].ldnp <= bfp_max_columns(cls) && bfp_lds(b, cls) <= bfp_lds_max[cls]) {
+                        bfp_cls[b] = cls;
+                        break;
+                    }
+            for (int cls = 0; cls < 3; ++cls) {
+                LaunchRange r{cls, (int)srho_work.size(), 0, 0};
+                for (int b = 0; b < nb; ++b) {
+                    if (bfp_cls[b] != cls) continue;
+                    srho_work.push_back(b);
+                    r.lds = std::max(r.lds, bfp_lds(b, cls));
+                }
+                r.count = (int)srho_work.size() - r.offset;
+                if (r.count) c.srho_launches.push_back(r);
+            }
+        }
+        for (int fcls = 0; fcls < 5 && prec_bfp(c.opts.precision); ++fcls) {
+            LaunchRange r{kRhoFp64Class + fcls, (int)srho_work.size(), 0, 0};
+            for (int b = 0; b < nb; ++b) {
+                if (bfp_cls[b] >= 0 || rho_class(c.blocks[b]) != fcls) continue;
+                srho_work.push_back(b);
+                r.lds = std::max(r.lds, rho_lds(c.blocks[b], (size_t)rho_points_per_workgroup(fcls)));
+            }
+            r.count = (int)srho_work.size() - r.offset;
+            if (r.count) c.srho_launches.push_back(r);
+        }
         // (no split rho launches when the rho pass runs in FP64)
-        for (int cls = 0; cls < 3 && prec_split_rho(c.opts.precision); ++cls) {
+        for (int cls = 0; cls < 3 && prec_split_rho(c.opts.precision) && !prec_bfp(c.opts.precision); ++cls) {
             const size_t pb = (size_t)split_points_per_workgroup(cls);
             LaunchRange r{cls, (int)srho_work.size(), 0, 0};
             for (int b = 0; b < nb; ++b) {
@@ -796,9 +844,10 @@ static int build_blocks_impl(Context& c) {
         std::vector<char> k_fp64_block(nb, 0);
         std::vector<int> block_of_leaf(nb);
         for (int li = 0; li < nb; ++li) block_of_leaf[lorder[li]] = li;
+        const bool k_f16 = prec_k_f16(c.opts.precision);
         auto item_lds = [&](int wcols, int nsh_item, int nbuf) {
             return ((size_t)nbuf * 2 * S * wcols * 16 * 2 + 15) / 16 * 16 + sizeof(double) * (size_t)nsh_item * RS +
-                   sizeof(int) * (size_t)(wcols + 3 * nsh_item);
+                   sizeof(int) * (size_t)(wcols + 3 * nsh_item + (k_f16 ? 2 * nbuf * wcols : 0));
         };
         struct GroupCost {
             double cost;
@@ -2199,14 +2248,14 @@ int pdfxc_selftest_mfma(double* out16x16) {
     return rc;
 }
 
-int pdfxc_selftest_mfma_split(float* out2x16x16) {
-    if (!out2x16x16) return arg_error("pdfxc_selftest_mfma_split: null output");
+int pdfxc_selftest_mfma_split(float* out4x16x16) {
+    if (!out4x16x16) return arg_error("pdfxc_selftest_mfma_split: null output");
     float* d = nullptr;
-    int rc = dalloc(&d, 512);
+    int rc = dalloc(&d, 1024);
     if (rc) return rc;
     rc = launch_mfma_split_selftest(d, nullptr);
     if (rc == PDFXC_OK) {
-        hipError_t e = hipMemcpy(out2x16x16, d, 512 * sizeof(float), hipMemcpyDeviceToHost);
+        hipError_t e = hipMemcpy(out4x16x16, d, 1024 * sizeof(float), hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = hip_fail(e, "pdfxc_selftest_mfma_split");
     }
     dfree(d);
@@ -2306,7 +2355,6 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
     m.opts = c0->opts;
     b->n_mol = n_mol;
     const bool split = m.opts.precision != PDFXC_FP64;
-    const int S = prec_slices(m.opts.precision);
     int RS = 0;
     for (int i = 0; i < n_mol; ++i) RS = std::max(RS, reinterpret_cast<const Context*>(ctxs[i])->rec_stride);
     m.rec_stride = RS;
@@ -2388,7 +2436,7 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
             m.kg_masks.insert(m.kg_masks.end(), c.kg_masks.begin(), c.kg_masks.end());
             m.psplit_jobs.push_back(Context::PsplitJob{b->doff.back(), psoff, (int)perm.size(), N});
             perm.insert(perm.end(), c.split_perm.begin(), c.split_perm.end());
-            psoff += (int64_t)S * N * ((N + 7) / 8 * 8);
+            psoff += psplit_elems(m.opts.precision, N);
         }
         pts.resize(4 * (size_t)(slot_base + c.n_slots), 0.0);
         for (int64_t p = 0; p < c.n_pts; ++p) {
@@ -2429,7 +2477,7 @@ int pdfxc_batch_create(pdfxc_batch** out, int n_mol, pdfxc_ctx* const* ctxs) {
     }
     // work lists: every launch class over all molecules
     auto merge_int = [&](auto get_ranges, auto get_list, std::vector<int>& out_list, std::vector<LaunchRange>& out_r) {
-        for (int cls = 0; cls < 8; ++cls) {
+        for (int cls = 0; cls < kRhoFp64Class + 8; ++cls) {
             LaunchRange r{cls, (int)out_list.size(), 0, 0};
             for (int i = 0; i < n_mol; ++i) {
                 const Context& c = *reinterpret_cast<const Context*>(ctxs[i]);

@@ -14,8 +14,21 @@ namespace pdfxc {
 
 // precision modes: every non-FP64 mode keeps the split structures and runs the
 // K pass on bf16 slices; the rho pass is split only in SPLIT_BF16 / SPLIT_BF16X3
-inline bool prec_split_rho(int p) { return p == PDFXC_SPLIT_BF16 || p == PDFXC_SPLIT_BF16X3; }
-inline int prec_slices(int p) { return p == PDFXC_SPLIT_BF16 ? 2 : 3; }
+inline bool prec_split_rho(int p) { return p == PDFXC_SPLIT_BF16 || p == PDFXC_SPLIT_BF16X3 || p == PDFXC_SPLIT_BFP; }
+// the K pass in FP16 hi + lo with per-slab column exponents (PDFXC_SPLIT_K_F16)
+inline bool prec_k_f16(int p) { return p == PDFXC_SPLIT_K_F16; }
+// the K pass's slices: bf16 hi + lo, FP16 hi + lo, else bf16 hi + mid + lo
+inline int prec_slices(int p) { return (p == PDFXC_SPLIT_BF16 || p == PDFXC_SPLIT_K_F16) ? 2 : 3; }
+// the rho pass in block floating point: 4 slices with shared exponents (PDFXC_SPLIT_BFP)
+inline bool prec_bfp(int p) { return p == PDFXC_SPLIT_BFP; }
+constexpr int kBfpPlanes = 4;
+constexpr int kRhoFp64Class = 16;  // split rho launch classes from here on: FP64 k_rho class - kRhoFp64Class
+// bf16 elements of one molecule's P slices (and, in block FP, its row exponents after the planes)
+inline int64_t psplit_elems(int p, int n) {
+    const int64_t nc = (n + 7) / 8 * 8;
+    if (prec_bfp(p)) return (kBfpPlanes * (int64_t)n * nc + 2 * (int64_t)n + 7) / 8 * 8;
+    return (int64_t)prec_slices(p) * n * nc;
+}
 
 
 constexpr int kBlockPts = 128;  // grid points per block (rows of the MFMA tiles)
@@ -273,6 +286,11 @@ int launch_becke(const double* d_atoms, int n_atoms, int64_t n, const double* d_
 int launch_mfma_selftest(double* d_out, hipStream_t s);
 // split precision (pdfxc_split.hip)
 int split_points_per_workgroup(int cls);
+// block-FP rho classes 0, 1, 2: 64, 32, 16 points per workgroup, blocks of at
+// most 256, 384, 512 chunked columns; the dynamic LDS a class may take
+int bfp_points_per_workgroup(int cls);
+int bfp_max_columns(int cls);
+int bfp_max_dyn_lds(int cls, size_t* out);
 int launch_split_rho(Context& c);   // P slices + k_rho_split
 int launch_split_kmat(Context& c);  // k_kmat_split
 int split_kmat_max_dyn_lds(int S, size_t* out);

@@ -19,6 +19,30 @@ __device__ inline f32x4 mfma_bf16(bf16x8 a, bf16x8 b, f32x4 c) {
 
 __device__ inline unsigned short bf16_bits(__bf16 h) { return __builtin_bit_cast(unsigned short, h); }
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x32_f16: the lane layout of mfma_bf16 (checked by the same
+// self-test)
+__device__ inline f32x4 mfma_f16(f16x8 a, f16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f32_16x16x16_f16: lane l holds A[l % 16][4 (l / 16) + e] and
+// B[4 (l / 16) + e][l % 16], e = 0..3; the result as for the 16x16x32 forms
+__device__ inline f32x4 mfma_f16_k16(f16x4 a, f16x4 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// FP16 hi + lo of t, |t| <= 2^15: hi = f16(t), lo = f16(t - hi) (t - hi is
+// exact in FP64); the pair keeps 22 bits of t down to |t| ~ 2^-3, and an
+// absolute 2^-25 below that (FP16 subnormals)
+__device__ inline void split_f16(double t, unsigned short* out) {
+    const _Float16 h = (_Float16)(float)t;
+    out[0] = __builtin_bit_cast(unsigned short, h);
+    out[1] = __builtin_bit_cast(unsigned short, (_Float16)(float)(t - (double)(float)h));
+}
+
 // slices of x: s[0] = bf16(x), s[1] = bf16 of what s[0] leaves, and the last
 // slice from what the others leave plus e = float(x - f), the FP64 part that
 // f = float(x) drops.  f - s[0] and r - s[1] are exact in FP32 (each slice

@@ -26,8 +26,10 @@ PDFXC_FP64 = 0
 PDFXC_SPLIT_BF16 = 1
 PDFXC_SPLIT_BF16X3 = 2
 PDFXC_SPLIT_K_BF16X3 = 3   # FP64 rho pass, hi + mid + lo K pass
+PDFXC_SPLIT_BFP = 4        # block floating point rho pass (4 slices, exact sums), hi + mid + lo K pass
+PDFXC_SPLIT_K_F16 = 5      # FP64 rho pass, fp16 hi + lo K pass (per-slab column scales)
 PRECISIONS = {"fp64": PDFXC_FP64, "bf16x2": PDFXC_SPLIT_BF16, "bf16x3": PDFXC_SPLIT_BF16X3,
-              "bf16x3k": PDFXC_SPLIT_K_BF16X3}
+              "bf16x3k": PDFXC_SPLIT_K_BF16X3, "bfp": PDFXC_SPLIT_BFP, "f16k": PDFXC_SPLIT_K_F16}
 ERROR_NAMES = {1: "ARGUMENT", 2: "HIP", 3: "NO_GRID", 4: "UNSUPPORTED", 5: "MEMORY"}
 ERROR_CODES = {v: k for k, v in ERROR_NAMES.items()}
 
@@ -273,8 +275,8 @@ def selftest_mfma():
 
 
 def selftest_mfma_split():
-    """(2, 16, 16): the 16x16x32 and the 16x16x16 bf16 products."""
-    out = np.zeros((2, 16, 16), dtype=np.float32)
+    """(4, 16, 16): the 16x16x32 and 16x16x16 bf16 products, then the same two fp16 products."""
+    out = np.zeros((4, 16, 16), dtype=np.float32)
     check(lib().pdfxc_selftest_mfma_split(_p(out)))
     return out
 

@@ -65,9 +65,9 @@ def test_precision_modes_are_validated_before_any_device_work():
     from proteindf_amd import calc_grid, molecules
 
     assert (engine.PDFXC_FP64, engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3,
-            engine.PDFXC_SPLIT_K_BF16X3) == (0, 1, 2, 3)
+            engine.PDFXC_SPLIT_K_BF16X3, engine.PDFXC_SPLIT_BFP, engine.PDFXC_SPLIT_K_F16) == (0, 1, 2, 3, 4, 5)
     info = calc_grid.orbital_info(molecules.water_cluster(1, 0))
-    for bad in (-1, 4, 7):
+    for bad in (-1, 6, 7):
         with pytest.raises(engine.PdfxcError) as ei:
             engine.XcContext(info, engine.default_opts(precision=bad))
         assert ei.value.code == 1 and "precision" in str(ei.value)

@@ -93,7 +93,8 @@ def test_gly30_split_modes_against_fp64(gly30):
     mol, info, grid, PA, ctx = gly30
     E0, K0, _ = ctx.build_rks(guess.pack_lower(PA))
     K0 = guess.unpack_lower(K0, info.n_aos)
-    for prec, e_tol, k_tol in ((engine.PDFXC_SPLIT_K_BF16X3, 0.0, 1e-7), (engine.PDFXC_SPLIT_BF16X3, 5e-5, 1e-7)):
+    for prec, e_tol, k_tol in ((engine.PDFXC_SPLIT_K_BF16X3, 0.0, 1e-7), (engine.PDFXC_SPLIT_BF16X3, 5e-5, 1e-7),
+                               (engine.PDFXC_SPLIT_BFP, 1e-6, 1e-7), (engine.PDFXC_SPLIT_K_F16, 0.0, 1e-7)):
         E, K, _ = _build(info, grid, PA, precision=prec, ao_screen=1e-12)
         say(f"  precision {prec}: |dE| {abs(E - E0):.2e} max|dK| {np.abs(K - K0).max():.2e} "
             f"max|K| {np.abs(K0).max():.3f}")

@@ -21,9 +21,16 @@ pytestmark = pytest.mark.gpu
 # the 1e-7 target.  PDFXC_SPLIT_K_BF16X3 runs the rho pass in FP64: E and rho
 # are the FP64 build's (the same kernels in the same order), K has the
 # hi+mid+lo error.
+# PDFXC_SPLIT_BFP (block floating point rho pass: 4 slices with one exponent
+# per point and per P row, exact FP32 sums; hi+mid+lo K pass) is held to the
+# 1e-6 / 1e-7 targets; a CPU emulation of its slicing gives |dE| ~ 7e-9 here.
+# PDFXC_SPLIT_K_F16 (FP64 rho pass, fp16 hi+lo K pass with per-slab column
+# scales) is held to the FP64 energy and the 1e-7 K target.
 BOUNDS = {engine.PDFXC_SPLIT_BF16: (2e-4, 2e-5), engine.PDFXC_SPLIT_BF16X3: (5e-6, 1e-7),
-          engine.PDFXC_SPLIT_K_BF16X3: (1e-10, 1e-7)}
-SPLIT_MODES = [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3, engine.PDFXC_SPLIT_K_BF16X3]
+          engine.PDFXC_SPLIT_K_BF16X3: (1e-10, 1e-7), engine.PDFXC_SPLIT_BFP: (1e-6, 1e-7),
+          engine.PDFXC_SPLIT_K_F16: (1e-10, 1e-7)}
+SPLIT_MODES = [engine.PDFXC_SPLIT_BF16, engine.PDFXC_SPLIT_BF16X3, engine.PDFXC_SPLIT_K_BF16X3,
+               engine.PDFXC_SPLIT_BFP, engine.PDFXC_SPLIT_K_F16]
 
 
 def say(*a):
@@ -32,7 +39,7 @@ def say(*a):
 
 def test_mfma_split_operand_and_accumulator_layout():
     d = engine.selftest_mfma_split()
-    for n, (K, D) in enumerate(((32, d[0]), (16, d[1]))):
+    for n, (K, D) in enumerate(((32, d[0]), (16, d[1]), (32, d[2]), (16, d[3]))):
         i = np.arange(16)[:, None]
         k = np.arange(K)[None, :]
         A = ((3 * i + 5 * k) % 11) - 5
@@ -69,7 +76,7 @@ def test_split_build_h2o8_against_fp64(precision):
     assert dE <= eb
     assert dK <= kb
     np.testing.assert_allclose(K1, K1.T, atol=0)  # symmetric by construction
-    if precision == engine.PDFXC_SPLIT_K_BF16X3:
+    if precision in (engine.PDFXC_SPLIT_K_BF16X3, engine.PDFXC_SPLIT_K_F16):
         assert E1 == E0
         np.testing.assert_array_equal(rho1, rho0)
 
@@ -90,7 +97,8 @@ def test_split_ragged_grids_against_oracle(precision):
         say(f"precision {precision} npts {npts}: |dE| {dE:.3e} max|dK| {dK:.3e}")
         eb, kb = BOUNDS[precision]
         assert dE <= eb and dK <= kb
-        rtol = {engine.PDFXC_SPLIT_BF16: 2e-4, engine.PDFXC_SPLIT_BF16X3: 2e-6, engine.PDFXC_SPLIT_K_BF16X3: 1e-9}
+        rtol = {engine.PDFXC_SPLIT_BF16: 2e-4, engine.PDFXC_SPLIT_BF16X3: 2e-6, engine.PDFXC_SPLIT_K_BF16X3: 1e-9,
+                engine.PDFXC_SPLIT_BFP: 2e-6, engine.PDFXC_SPLIT_K_F16: 1e-9}
         np.testing.assert_allclose(rho[:, 0], rho_ref[:, 0], rtol=rtol[precision], atol=1e-12)
 
 
@@ -141,7 +149,9 @@ def test_wide_blocks_take_the_fp64_k_pass():
             break
     assert grid is not None
     out = {}
-    for prec in (engine.PDFXC_FP64, engine.PDFXC_SPLIT_BF16X3):
+    # (block floating point: these blocks are wider than the block-FP rho
+    # kernel's 512 chunked columns, so their rho pass runs in FP64)
+    for prec in (engine.PDFXC_FP64, engine.PDFXC_SPLIT_BF16X3, engine.PDFXC_SPLIT_BFP, engine.PDFXC_SPLIT_K_F16):
         ctx = engine.XcContext(info, engine.default_opts(precision=prec, ao_screen=1e-12))
         ctx.set_grid(grid)
         st = ctx.stats()
@@ -150,7 +160,8 @@ def test_wide_blocks_take_the_fp64_k_pass():
         out[prec] = (E, K, st)
         ctx.close()
     assert out[engine.PDFXC_FP64][2].n_k_fp64_blocks == 0
-    dE = abs(out[engine.PDFXC_SPLIT_BF16X3][0] - out[engine.PDFXC_FP64][0])
-    dK = np.abs(out[engine.PDFXC_SPLIT_BF16X3][1] - out[engine.PDFXC_FP64][1]).max()
-    say(f"  |dE| {dE:.3e} max|dK| {dK:.3e}")
-    assert dE <= BOUNDS[engine.PDFXC_SPLIT_BF16X3][0] and dK <= BOUNDS[engine.PDFXC_SPLIT_BF16X3][1]
+    for prec in (engine.PDFXC_SPLIT_BF16X3, engine.PDFXC_SPLIT_BFP, engine.PDFXC_SPLIT_K_F16):
+        dE = abs(out[prec][0] - out[engine.PDFXC_FP64][0])
+        dK = np.abs(out[prec][1] - out[engine.PDFXC_FP64][1]).max()
+        say(f"  precision {prec}: |dE| {dE:.3e} max|dK| {dK:.3e}")
+        assert dE <= BOUNDS[prec][0] and dK <= BOUNDS[prec][1]
