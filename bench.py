@@ -130,16 +130,34 @@ DTYPES = {"fp64": "f64", "bf16x2": "bf16 slices, f32 accumulate, f64 AO/function
 E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 build
 # HBM bytes per build of a dominant kernel (the sum over its class launches),
 # measured in separate rocprofv3 --pmc runs (FETCH_SIZE x 2.00 and WRITE_SIZE
-# x 1.00, the factors a plain copy kernel of known size gives) of this benchmark
-# with the default launch shapes; the counters are not read inside the timed run.
-TRAFFIC_MEASURED = {
-    ("gly10", "bf16x3k", "k_kmat_split"): (82418.6 * 1024 * 2.00 + 3095591.0 * 1024 * 1.00,
-                                           "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE; profiles/gly10_kernels.md"),
-    ("gly10", "bf16x3k", "k_rho"): ((57362.74 + 28064.03 + 11098.27) * 1024 * 2.00
-                                    + (33056.0 + 4412.0 + 5154.0) * 1024 * 1.00,
-                                    "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches; "
-                                    "profiles/gly10_kernels.md"),
-}
+# x 1.00, the factors a plain copy kernel of known size gives) of this benchmark;
+# the counters are not read inside the timed run.  Each entry holds for the
+# launch shapes it names (the tuning fields that change that kernel's traffic)
+# and is reported only when the run's tuning matches them.
+_KIB = 1024.0
+TRAFFIC_MEASURED = [
+    # (workload, modes, kernel, tuning fields, bytes per build, source)
+    ("gly30", ("bf16x3k", "f16k", "fp64"), "k_rho", {"rho_shift": 0},
+     (631813.9 + 514037.4 + 79570.8) * _KIB * 2.00 + (235992.0 + 60472.0 + 19658.6) * _KIB * 1.00,
+     "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches; profiles/gly30_kernels.md"),
+    ("gly30", ("bf16x3k", "bfp"), "k_kmat_split", {"split_k_patches": 24, "split_k_group": 2},
+     575575.2 * _KIB * 2.00 + 14948334.8 * _KIB * 1.00,
+     "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, groups of 2 blocks; profiles/gly30_kernels.md"),
+    ("gly30", ("bf16x3k", "bfp"), "k_kmat_split", {"split_k_patches": 24, "split_k_group": 4},
+     546979.2 * _KIB * 2.00 + 7839451.1 * _KIB * 1.00,
+     "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, groups of 4 blocks; profiles/gly30_kernels.md"),
+    ("gly30", ("bfp",), "k_rho_bfp", {"split_rho_shift": 0},
+     (334056.1 + 1030113.3 + 210071.5) * _KIB * 2.00 + (65830.0 + 246764.0 + 3528.0) * _KIB * 1.00,
+     "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches; profiles/gly30_kernels.md"),
+]
+
+
+def measured_traffic(workload, mode, kernel, tuning):
+    """(bytes, source) of the counter run that matches this launch shape, else None."""
+    for wl, modes, k, need, nbytes, src in TRAFFIC_MEASURED:
+        if wl == workload and mode in modes and k == kernel and all(tuning.get(f) == v for f, v in need.items()):
+            return nbytes, src
+    return None
 
 
 def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True, k_patches=0, k_group=0):
@@ -386,11 +404,13 @@ def main():
     dom = int(np.argmax(k_ms[:2]))
     name, peak = KERNELS[args.precision][dom]
     achieved = st.flops / 2.0 / (k_ms[dom] / 1e3) / 1e12
-    tr = TRAFFIC_MEASURED.get((args.workload, args.precision, name))
+    tr = measured_traffic(args.workload, args.precision, name, head["tune"])
     roof = {"kernel": name, "bound": "mfma", "achieved": achieved, "peak": peak,
             "unit": "TFLOP/s", "frac": achieved / peak, "traffic": tr[0] if tr else None}
     if tr:
         roof["traffic_source"] = tr[1]
+    else:
+        roof["traffic_note"] = "no counter run for this workload and launch shape"
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -217,8 +217,14 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
     const double px = s_pt[p][0], py = s_pt[p][1], pz = s_pt[p][2];
     // 1. Phi plane.  The radial sums of a thread's first NC shells are kept
     // in registers for step 3 (same thread-to-shell mapping), which then
-    // forms phi and grad phi for them without the exponentials.
-    constexpr int NC = (PB >= 32) ? 4 : 0;  // (the 16-point classes would lose occupancy)
+    // forms phi and grad phi for them without the exponentials (the rest are
+    // recomputed).  NC covers most of a thread's share of its class's blocks
+    // within the registers the class's occupancy leaves: the 64-point class of
+    // 256 columns and the 16-point class of 1024 run one workgroup per CU
+    // (2 waves per SIMD); the 16-point class of 512 columns runs two, and
+    // keeping even 4 shells there takes it past 128 VGPRs.
+    constexpr int kColsMax = NIMAX * WI * 16;
+    constexpr int NC = (PB == 64 && kColsMax == 256) ? 16 : (PB == 16 && kColsMax == 512) ? 0 : 8;
     double cR0[NC > 0 ? NC : 1], cR1[NC > 0 ? NC : 1];
 #pragma unroll
     for (int jj = 0; jj < NC; ++jj) {

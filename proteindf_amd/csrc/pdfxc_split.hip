@@ -711,8 +711,8 @@ __global__ __launch_bounds__(512) void k_rho_bfp(const int* __restrict__ work, c
 // 32-long K index covering the slab's 16 points twice, (phi | V) against
 // (V | phi), so that one product gives phi_i V_j + V_i phi_j.  Every slab
 // sums from zero in FP32, the slice corrections (a + b >= 1, at most 2^-8 of
-// the leading term) in one sum and hi x hi in another, and both are added
-// into FP64 accumulators per slab: the MFMA's FP32 sums do not round to
+// the leading term) in one sum and hi x hi in another, and their FP32 sum is
+// added into FP64 accumulators per slab: the MFMA's FP32 sums do not round to
 // nearest (their errors share a sign), so a sum held in FP32 over a block's
 // 8 slabs lost up to ~1.6e-7 of K on coarse grids.  FP64 atomics into the
 // dense lower triangle of K at the end of the item.
@@ -761,8 +761,10 @@ __device__ inline void kmat_split_consume(int pb, int pe, int Tp, int c0, int wc
                 for (int q = 1; q < SlicePairs<S>::n; ++q)
                     cs = mfma_bf16(aI[hi][SlicePairs<S>::a[q]], bJ[hj][SlicePairs<S>::b[q]], cs);
                 const f32x4 ls = mfma_bf16(aI[hi][0], bJ[hj][0], f32x4{0.f, 0.f, 0.f, 0.f});
+                // one FP32 rounding of the slab's sum (2^-24 of it, the size of
+                // those inside the MFMA's own sum) for one conversion less
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[m][t][r] += (double)ls[r] + (double)cs[r];
+                for (int r = 0; r < 4; ++r) acc[m][t][r] += (double)(ls[r] + cs[r]);
             }
         }
         if (++P == Tp) {

@@ -32,9 +32,25 @@ def test_mode_tables_cover_every_mode():
     from proteindf_amd import engine
 
     assert {bench.MODES[m] for m in bench.MODES} == set(engine.PRECISIONS.values())
-    for (workload, mode, kernel), (nbytes, source) in bench.TRAFFIC_MEASURED.items():
-        assert workload in bench.WORKLOADS and mode in bench.MODES
-        assert kernel in {k for k, _ in bench.KERNELS[mode]} and nbytes > 0 and "profiles/" in source
+    for workload, modes, kernel, need, nbytes, source in bench.TRAFFIC_MEASURED:
+        assert workload in bench.WORKLOADS and nbytes > 0 and "profiles/" in source
+        for mode in modes:
+            assert mode in bench.MODES and kernel in {k for k, _ in bench.KERNELS[mode]}
+        assert need and all(isinstance(v, int) for v in need.values())
+
+
+def test_traffic_is_reported_only_for_the_profiled_launch_shapes():
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    tune = {"rho_shift": 0, "k_tiles": 96, "split_rho_shift": 0, "split_k_patches": 24, "split_k_group": 2,
+            "persistent": 2}
+    g2 = bench.measured_traffic("gly30", "bf16x3k", "k_kmat_split", tune)
+    g4 = bench.measured_traffic("gly30", "bf16x3k", "k_kmat_split", dict(tune, split_k_group=4))
+    assert g2 and g4 and g2[0] > g4[0]  # a wider group scatters fewer atomics
+    assert bench.measured_traffic("gly30", "bf16x3k", "k_kmat_split", dict(tune, split_k_group=8)) is None
+    assert bench.measured_traffic("gly30", "bf16x3k", "k_rho", dict(tune, rho_shift=1)) is None
+    assert bench.measured_traffic("gly10", "bf16x3k", "k_rho", tune) is None
 
 
 def test_split_k_group_out_of_range_is_refused():
