@@ -214,6 +214,38 @@ def _tally(tunes):
     return out
 
 
+def cpu_baseline_batch(ctxs, infos, Ps, target_s=12.0):
+    """Oracle (reference algorithm, OpenMP) on a uniform random sample of the
+    batch's points (each cluster gets a share in proportion to its points, at
+    least one), scaled to the whole batch: points/s."""
+    from oracle import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sizes = np.array([c.n_pts for c in ctxs], dtype=np.float64)
+    total = sizes.sum()
+    rng = np.random.default_rng(0)
+    n = 2048
+    while True:
+        t = 0.0
+        taken = 0
+        for ctx, info, PA, npts in zip(ctxs, infos, Ps, sizes):
+            k = int(min(npts, max(1, round(n * npts / total))))
+            grid = ctx.get_grid()
+            idx = rng.choice(len(grid), size=k, replace=False)
+            aos = oracle.AoTable.from_basis(info)
+            t0 = time.perf_counter()
+            oracle.calc_xc_rks(aos, grid[idx], PA, n_threads=threads)
+            t += time.perf_counter() - t0
+            taken += k
+        if t > 0.6 * target_s or n >= total:
+            break
+        n = int(min(total, n * max(2.0, 1.05 * target_s / max(t, 1e-3))))
+    t_full = t * total / taken
+    return {"value": total / t_full, "unit": "points/s", "cores": threads, "kind": "port",
+            "sample": f"{taken} random points of the {int(total)} (each cluster's share in proportion to its points) "
+                      f"in {t:.1f} s, scaled to a {t_full:.0f} s whole-batch build"}
+
+
 def bench_batch(args):
     """64 water clusters (H2O)_n, n ~ U{4..64} from one stream (seed 3), 6-31G(d),
     B3LYP, 75 x 302: every cluster's blocks in one graph-captured batch vs one
@@ -307,8 +339,13 @@ def bench_batch(args):
                                    "cluster_latency_ms_p99": float(np.percentile(lat_ms, 99))},
         "speedup_batch_vs_single": t_single / t_batch,
         "tuning_chosen": _tally(tunes),
-        "roofline": None,
-        "cpu_baseline": None,
+        # the captured graph gives no per-pass times: the whole build against
+        # the FP64 MFMA peak of its rho pass (a lower bound of that pass's fraction)
+        "roofline": {"kernel": "batch graph (rho and K passes of all 64 clusters)", "bound": "mfma",
+                     "achieved": st.flops / 2.0 / t_batch / 1e12, "peak": KERNELS[args.precision][0][1],
+                     "unit": "TFLOP/s", "frac": st.flops / 2.0 / t_batch / 1e12 / KERNELS[args.precision][0][1],
+                     "traffic": None, "traffic_note": "no counter run for the batch"},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline_batch(ctxs, infos, Ps),
     }
     print(json.dumps(line), flush=True)
 

@@ -68,3 +68,35 @@ def test_default_workload_is_the_headline_configuration():
     assert ap_default and 'default="gly30"' in ap_default[0]
     wl = bench.WORKLOADS["gly30"]
     assert (wl["n_rad"], wl["n_ang"]) == (99, 590)
+
+
+def test_batch_cpu_baseline_samples_every_cluster():
+    """The batch's CPU baseline draws points from every cluster in proportion
+    to its size and reports points/s for the whole batch (oracle on the CPU,
+    stand-in contexts that only hold a grid)."""
+    sys.path.insert(0, str(ROOT))
+    import bench
+    import numpy as np
+    from oracle import oracle
+    from proteindf_amd import calc_grid, guess, molecules
+
+    class Grid:
+        def __init__(self, g):
+            self.g = g
+            self.n_pts = len(g)
+
+        def get_grid(self):
+            return self.g
+
+    ctxs, infos, Ps = [], [], []
+    for n in (1, 2, 3):
+        mol = molecules.water_cluster(n, seed=n)
+        info = calc_grid.orbital_info(mol)
+        g, _ = oracle.generate_grid(mol.Z, mol.xyz, 20, 26)
+        ctxs.append(Grid(g))
+        infos.append(info)
+        Ps.append(0.5 * guess.density_guess(mol, info, seed=0, eps=0.05))
+    out = bench.cpu_baseline_batch(ctxs, infos, Ps, target_s=0.2)
+    assert out["unit"] == "points/s" and out["value"] > 0 and out["kind"] == "port"
+    assert str(sum(c.n_pts for c in ctxs)) in out["sample"]
+    assert np.isfinite(out["value"])
