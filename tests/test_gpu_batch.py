@@ -26,7 +26,7 @@ def _molecules():
     return out
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "bf16x3k"])
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "bf16x3k", "bfp", "f16k"])
 def test_batch_equals_individual_builds(prec):
     mols = _molecules()
     ctxs, single = [], []

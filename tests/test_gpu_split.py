@@ -165,3 +165,26 @@ def test_wide_blocks_take_the_fp64_k_pass():
         dK = np.abs(out[prec][1] - out[engine.PDFXC_FP64][1]).max()
         say(f"  precision {prec}: |dE| {dE:.3e} max|dK| {dK:.3e}")
         assert dE <= BOUNDS[prec][0] and dK <= BOUNDS[prec][1]
+
+
+@pytest.mark.parametrize("precision", [engine.PDFXC_SPLIT_K_BF16X3, engine.PDFXC_SPLIT_BFP, engine.PDFXC_SPLIT_K_F16])
+def test_split_update_mode_accumulates_rho(precision):
+    """Update mode in split precision: rho[P1] then rho[P2 - P1] accumulated on
+    the device give E and K within the mode's bounds of the oracle's FP64 build
+    of P2 (the block floating point rho pass slices P2 - P1 with its own row
+    exponents)."""
+    mol = molecules.water_cluster(2, seed=9)
+    info = calc_grid.orbital_info(mol)
+    grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 40, 194)
+    P1 = 0.5 * guess.density_guess(mol, info, seed=1, eps=0.05)
+    P2 = 0.5 * guess.density_guess(mol, info, seed=2, eps=0.08)
+    ctx = engine.XcContext(info, engine.default_opts(precision=precision))
+    ctx.set_grid(grid)
+    ctx.build_rks_update(guess.pack_lower(P1))
+    Eu, Ku, _ = ctx.build_rks_update(guess.pack_lower(P2 - P1))
+    ctx.close()
+    e_ref, K_ref, _ = oracle.calc_xc_rks(oracle.AoTable.from_basis(info), grid, P2)
+    dE, dK = abs(Eu - e_ref), np.abs(guess.unpack_lower(Ku, info.n_aos) - K_ref).max()
+    say(f"update mode, precision {precision}: |dE| {dE:.3e} max|dK| {dK:.3e}")
+    eb, kb = BOUNDS[precision]
+    assert dE <= max(eb, 1e-10) and dK <= kb

@@ -123,7 +123,7 @@ def test_uks_closed_shell_equals_rks_and_update_mode():
     ctx.close()
 
 
-@pytest.mark.parametrize("prec", ["bf16x2", "bf16x3", "bf16x3k"])
+@pytest.mark.parametrize("prec", ["bf16x2", "bf16x3", "bf16x3k", "bfp", "f16k"])
 def test_uks_split_build_within_bounds(prec):
     mol, info, PA, PB = _case(2, 5)
     grid, _ = oracle.generate_grid(mol.Z, mol.xyz, 40, 194)
@@ -137,7 +137,8 @@ def test_uks_split_build_within_bounds(prec):
     dE = abs(out[prec][0] - out["fp64"][0])
     dK = max(np.abs(out[prec][1] - out["fp64"][1]).max(), np.abs(out[prec][2] - out["fp64"][2]).max())
     say("  dE", dE, "max dK", dK)
-    bound = {"bf16x2": (2e-4, 2e-5), "bf16x3": (5e-6, 5e-7), "bf16x3k": (1e-12, 5e-7)}[prec]
+    bound = {"bf16x2": (2e-4, 2e-5), "bf16x3": (5e-6, 5e-7), "bf16x3k": (1e-12, 5e-7), "bfp": (1e-6, 1e-7),
+             "f16k": (1e-12, 1e-7)}[prec]
     assert dE <= bound[0] and dK <= bound[1]
 
 
