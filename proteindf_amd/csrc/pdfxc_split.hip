@@ -527,19 +527,43 @@ __device__ __forceinline__ void k_rho_bfp_item(int item, const int* __restrict__
     const int p = tid & (PB - 1);
     const int kk = tid / PB;
     const double px = s_pt[p][0], py = s_pt[p][1], pz = s_pt[p][2];
-    // 1. phi (FP64) and the point's largest |phi|
+    // 1. phi (FP64) and the point's largest |phi|.  The radial sums of a
+    // thread's first NC shells stay in registers for step 4 (same thread-to-
+    // shell mapping); the rest are recomputed there.  These classes run 2
+    // waves per SIMD, so NC is what fits in 256 VGPRs beside the level
+    // accumulators without spilling in the persistent forms (the 64-point
+    // class has no room to spare).
+    constexpr int NC = (PB == 64) ? 0 : (PB == 32) ? 6 : 8;
+    double cR0[NC > 0 ? NC : 1], cR1[NC > 0 ? NC : 1];
     {
         float m = 0.0f;
-        for (int j = kk; j < nsh; j += NSL) {
-            const int k = s_sord[j];
-            double v[5];
-            rec_values0(s_rec + k * RS, px, py, pz, v);
+        auto put = [&](int k, const double* v) {
             const int c0 = s_shcol[k];
             const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
             for (int q = 0; q < nc; ++q) {
                 s_stage[p * RA + c0 + q] = v[q];
                 m = fmaxf(m, (float)fabs(v[q]));  // rounding is monotone: 2^E above the float bounds every |v|
             }
+        };
+#pragma unroll
+        for (int jj = 0; jj < NC; ++jj) {
+            const int j = kk + jj * NSL;
+            cR0[jj] = cR1[jj] = 0.0;
+            if (j < nsh) {
+                const int k = s_sord[j];
+                const double* rec = s_rec + k * RS;
+                const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
+                rec_radial01(rec, x, y, z, cR0[jj], cR1[jj]);
+                double v[5];
+                rec_values0_r(((int)rec[3]) & 3, x, y, z, cR0[jj], v);
+                put(k, v);
+            }
+        }
+        for (int j = kk + NC * NSL; j < nsh; j += NSL) {
+            const int k = s_sord[j];
+            double v[5];
+            rec_values0(s_rec + k * RS, px, py, pz, v);
+            put(k, v);
         }
         atomicMax(&s_emax[p], __float_as_uint(m));  // non-negative floats order as their bits
     }
@@ -628,14 +652,10 @@ __device__ __forceinline__ void k_rho_bfp_item(int item, const int* __restrict__
         }
     }
     __syncthreads();  // X is stored
-    // 4. rho and grad rho
+    // 4. rho and grad rho (radial sums kept from step 1 for the first NC shells)
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    for (int j = kk; j < nsh; j += NSL) {
-        const int k = s_sord[j];
+    auto dot = [&](int k, double R0, double R1) {
         const double* rec = s_rec + k * RS;
-        const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
-        double R0, R1;
-        rec_radial01(rec, x, y, z, R0, R1);
         const int l = ((int)rec[3]) & 3;
         const double* xr = s_xd + p * RXD + s_shcol[k];
         double X[5] = {xr[0], 0.0, 0.0, 0.0, 0.0};
@@ -647,7 +667,19 @@ __device__ __forceinline__ void k_rho_bfp_item(int item, const int* __restrict__
             X[3] = xr[3];
             X[4] = xr[4];
         }
-        shell_dots(l, x, y, z, R0, R1, X, r0, r1, r2, r3);
+        shell_dots(l, px - rec[0], py - rec[1], pz - rec[2], R0, R1, X, r0, r1, r2, r3);
+    };
+#pragma unroll
+    for (int jj = 0; jj < NC; ++jj) {
+        const int j = kk + jj * NSL;
+        if (j < nsh) dot(s_sord[j], cR0[jj], cR1[jj]);
+    }
+    for (int j = kk + NC * NSL; j < nsh; j += NSL) {
+        const int k = s_sord[j];
+        const double* rec = s_rec + k * RS;
+        double R0, R1;
+        rec_radial01(rec, px - rec[0], py - rec[1], pz - rec[2], R0, R1);
+        dot(k, R0, R1);
     }
     __syncthreads();  // X no longer read: the plane holds the partial sums
     double* s_part = s_dyn;  // [4][NSL][PB]
