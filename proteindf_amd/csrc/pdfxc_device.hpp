@@ -24,10 +24,14 @@ __device__ inline v4d mfma16x16x4(double a, double b, v4d c) {
 // item blockIdx.x (queue == nullptr; the second call returns n_items).  The
 // barrier before the fetch keeps the previous item's LDS use apart from the
 // next item's, the one after it publishes the index.  Every workgroup leaves
-// its loop once the counter passes n_items.
+// its loop once the counter passes n_items.  The fetch is made by the first
+// lane of the last wave: reading the atomic's result waits for every memory
+// write its wave still has in flight, and the kernels that use this loop
+// leave their writes (the K scatter, the rho stores) to the first waves.
 __device__ inline int next_item(int* queue, int n_items, int iter, int* s_item) {
     __syncthreads();
-    if (threadIdx.x == 0) *s_item = queue ? atomicAdd(queue, 1) : (iter == 0 ? (int)blockIdx.x : n_items);
+    if (threadIdx.x == blockDim.x - 64)
+        *s_item = queue ? atomicAdd(queue, 1) : (iter == 0 ? (int)blockIdx.x : n_items);
     __syncthreads();
     return *s_item;
 }

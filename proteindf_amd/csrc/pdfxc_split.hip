@@ -910,15 +910,10 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
     if constexpr (F16)
         for (int i = tid; i < nbuf * 2 * wcols; i += NT) s_ex[i] = 0;  // columns past the union: zero slices
     for (int c = tid; c < wcols; c += NT) s_cols[c] = (c0 + c < G.nao) ? gcols[G.col_begin + c0 + c] : -1;
-    if (tid == 0) {
-        int f = 0;
-        for (int j = 0; j < G.nsub; ++j) {
-            const int b = gblocks[G.sub_begin + j];
-            s_blk[j] = b;
-            s_first[j] = f;
-            f += (blocks[b].npts + 15) / 16;
-        }
-        s_first[G.nsub] = f;
+    if (tid < G.nsub) {  // the group's blocks and their slab counts, one lane each (a prefix after the barrier)
+        const int b = gblocks[G.sub_begin + tid];
+        s_blk[tid] = b;
+        s_first[tid + 1] = (blocks[b].npts + 15) / 16;
     }
     if (wave == NW / 2) {  // the item's shells (union shells k0 ..), costliest (l, nprim) last
         int base = 0;
@@ -941,6 +936,10 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
         for (int i = tid; i < slab_dbl / 2; i += NT) z[i] = uint4{0u, 0u, 0u, 0u};
     }
     __syncthreads();
+    if (tid == 0) {  // first slab of each block of the group, then the slab count
+        s_first[0] = 0;
+        for (int j = 0; j < G.nsub; ++j) s_first[j + 1] += s_first[j];
+    }
     for (int idx = tid; idx < nsh * RS; idx += NT) {
         const int k = idx / RS;
         s_rec[idx] = shrec[(int64_t)s_shid[k] * RS + (idx - k * RS)];
