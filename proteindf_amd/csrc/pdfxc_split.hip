@@ -884,7 +884,7 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
                                                     const unsigned* __restrict__ gmasks,
                                                     const double* __restrict__ coef, double* __restrict__ K,
         double (*s_pt)[kBlockPts][3], double (*s_cf)[kBlockPts][4], unsigned (*s_msk)[kMaxMaskWords],
-        int* s_first, int* s_blk, double* s_dyn) {
+        int* s_first, int* s_blk, int* s_pcount, double* s_dyn) {
     const KGWork W = work[item];
     const KGroup G = groups[W.group];
     const int tid = threadIdx.x;
@@ -910,6 +910,7 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
     if constexpr (F16)
         for (int i = tid; i < nbuf * 2 * wcols; i += NT) s_ex[i] = 0;  // columns past the union: zero slices
     for (int c = tid; c < wcols; c += NT) s_cols[c] = (c0 + c < G.nao) ? gcols[G.col_begin + c0 + c] : -1;
+    if (tid == 0) s_pcount[0] = s_pcount[1] = 0;
     if (tid < G.nsub) {  // the group's blocks and their slab counts, one lane each (a prefix after the barrier)
         const int b = gblocks[G.sub_begin + tid];
         s_blk[tid] = b;
@@ -974,13 +975,17 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
         unsigned short* phi = s_slab + (nbuf == 2 ? (gs & 1) * BUF : 0);
         unsigned short* vv = phi + S * wcols * 16;
         int* ex = s_ex + (nbuf == 2 ? (gs & 1) * 2 * wcols : 0);
-        // (shell, point) items in rounds of NT: shells from the costliest
-        // (l, nprim) down, and every other round's waves in reverse order, so
-        // a wave's rounds mix dear and cheap shells
-        for (int r0 = 0; r0 < nsh * 16; r0 += NT) {
-            const int ww = ((r0 / NT) & 1) ? (NW - 1 - wave) : wave;
-            const int it = r0 + ww * 64 + lane;
-            if (it >= nsh * 16) continue;
+        // (shell, point) items in chunks of 64 (four shells x 16 points) taken
+        // from an LDS counter, shells from the costliest (l, nprim) down: a
+        // wave that consumes fewer patches produces more
+        int* pc = s_pcount + (gs & 1);
+        for (;;) {
+            int chunk = 0;
+            if (lane == 0) chunk = atomicAdd(pc, 1);
+            const int r0 = __shfl(chunk, 0) * 64;
+            if (r0 >= nsh * 16) break;  // wave-uniform
+            const int it = r0 + lane;
+            if (it >= nsh * 16) continue;  // whole 16-lane groups
             const int p = it & 15;
             const int k = nsh - 1 - (it >> 4);
             const int pp = p0 + p;
@@ -1050,7 +1055,7 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
     };
 
     // the item's patches over the waves, NPW each; every wave runs the same
-    // production and barrier sequence
+    // barrier sequence
     const bool has = W.q0 + wave * NPW < W.q1;
     const int pb = has ? W.q0 + wave * NPW : W.q1 - 1;
     const int pe = has ? min(pb + NPW, W.q1) : pb;
@@ -1063,6 +1068,8 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
     __syncthreads();  // the first slab is in
     int jn = 1;       // next block of the group to load
     for (int gs = 0; gs < nslab; ++gs) {
+        // slab gs's production counter is done with (an earlier barrier): zero it for slab gs + 2
+        if (tid == 0) s_pcount[gs & 1] = 0;
         if (gs + 1 < nslab && jn < G.nsub && gs + 1 == s_first[jn]) {
             // the next slab starts block jn: its points into the other parity's
             // buffers (their last reader, block jn - 2, finished production
@@ -1071,12 +1078,14 @@ __device__ __forceinline__ void k_kmat_split_item(int item, const KGWork* __rest
             ++jn;
             __syncthreads();
         }
-        if (nbuf == 2 && gs + 1 < nslab) produce(gs + 1);  // into the other buffer
+        // a wave consumes its patches of slab gs, then takes production of
+        // slab gs + 1 (the other buffer) while any is left
         if constexpr (F16)
             kmat_f16_consume<NPW>(pb, pe, Tp, c0, wcols, lane, s_slab + (nbuf == 2 ? (gs & 1) * BUF : 0),
                                   s_ex + (nbuf == 2 ? (gs & 1) * 2 * wcols : 0), acc);
         else
             kmat_split_consume<S, NPW>(pb, pe, Tp, c0, wcols, lane, s_slab + (nbuf == 2 ? (gs & 1) * BUF : 0), acc);
+        if (nbuf == 2 && gs + 1 < nslab) produce(gs + 1);
         __syncthreads();  // slab gs consumed (and, two buffers, slab gs + 1 in)
         if (nbuf == 1) {
             if (gs + 1 < nslab) produce(gs + 1);
@@ -1136,16 +1145,17 @@ __global__ __launch_bounds__(NW * 64) void k_kmat_split(const KGWork* __restrict
     __shared__ unsigned s_msk[2][kMaxMaskWords];
     __shared__ int s_first[kMaxGroup + 1];  // first slab of each block of the group, then the slab count
     __shared__ int s_blk[kMaxGroup];
+    __shared__ int s_pcount[2];  // production chunks taken, per slab parity
     extern __shared__ __attribute__((aligned(16))) double s_dyn[];
     if constexpr (PERSIST) {
         __shared__ int s_item;
         for (int iter = 0;; ++iter) {
             const int item = next_item(queue, n_items, iter, &s_item);
             if (item >= n_items) break;
-            k_kmat_split_item<S, F16, NW, NPW>(item, work, groups, blocks, gblocks, pts, n_slots, shrec, RS, gshells, gshell_col, gcols, gsorder, gmasks, coef, K, s_pt, s_cf, s_msk, s_first, s_blk, s_dyn);
+            k_kmat_split_item<S, F16, NW, NPW>(item, work, groups, blocks, gblocks, pts, n_slots, shrec, RS, gshells, gshell_col, gcols, gsorder, gmasks, coef, K, s_pt, s_cf, s_msk, s_first, s_blk, s_pcount, s_dyn);
         }
     } else {
-        k_kmat_split_item<S, F16, NW, NPW>((int)blockIdx.x, work, groups, blocks, gblocks, pts, n_slots, shrec, RS, gshells, gshell_col, gcols, gsorder, gmasks, coef, K, s_pt, s_cf, s_msk, s_first, s_blk, s_dyn);
+        k_kmat_split_item<S, F16, NW, NPW>((int)blockIdx.x, work, groups, blocks, gblocks, pts, n_slots, shrec, RS, gshells, gshell_col, gcols, gsorder, gmasks, coef, K, s_pt, s_cf, s_msk, s_first, s_blk, s_pcount, s_dyn);
     }
 }
 
