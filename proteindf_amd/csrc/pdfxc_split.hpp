@@ -34,23 +34,33 @@ __device__ inline f32x4 mfma_f16_k16(f16x4 a, f16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
-// FP16 hi + lo of t, |t| <= 2^15: hi = f16(t), lo = f16(t - hi) (t - hi is
-// exact in FP64); the pair keeps 22 bits of t down to |t| ~ 2^-3, and an
-// absolute 2^-25 below that (FP16 subnormals)
-__device__ inline void split_f16(double t, unsigned short* out) {
-    const _Float16 h = (_Float16)(float)t;
-    out[0] = __builtin_bit_cast(unsigned short, h);
-    out[1] = __builtin_bit_cast(unsigned short, (_Float16)(float)(t - (double)(float)h));
+// Keeps a float as computed: without it the compiler folds a double ->
+// float -> bf16 / fp16 pair of conversions into one double -> bf16 / fp16
+// conversion, which gfx950 has no instruction for (7 extra VALU per bf16
+// value, about 35 per fp16 value); the hardware float -> bf16 / fp16
+// conversions then round the float
+__device__ inline float keep_f32(float f) {
+    __asm__ volatile("" : "+v"(f));
+    return f;
 }
 
-// slices of x: s[0] = bf16(x), s[1] = bf16 of what s[0] leaves, and the last
+// FP16 hi + lo of t, |t| <= 2^15: hi = f16(float(t)), lo = f16(float(t - hi))
+// (t - hi is exact in FP64); the pair keeps 22 bits of t down to |t| ~ 2^-3,
+// and an absolute 2^-25 below that (FP16 subnormals)
+__device__ inline void split_f16(double t, unsigned short* out) {
+    const _Float16 h = (_Float16)keep_f32((float)t);
+    out[0] = __builtin_bit_cast(unsigned short, h);
+    out[1] = __builtin_bit_cast(unsigned short, (_Float16)keep_f32((float)(t - (double)(float)h)));
+}
+
+// slices of x: s[0] = bf16(f), f = float(x), s[1] = bf16 of what s[0] leaves, and the last
 // slice from what the others leave plus e = float(x - f), the FP64 part that
 // f = float(x) drops.  f - s[0] and r - s[1] are exact in FP32 (each slice
 // keeps the top 8 bits of a value with at most 24), so only the last sum
 // rounds, at 2^-24 of a value below 2^-16 |x|; one FP64 subtraction per x.
 template <int S>
 __device__ inline void split_bf16(double x, unsigned short* out) {
-    const float f = (float)x;
+    const float f = keep_f32((float)x);
     const float e = (float)(x - (double)f);
     const __bf16 h0 = (__bf16)f;
     out[0] = bf16_bits(h0);
