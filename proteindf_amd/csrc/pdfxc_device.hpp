@@ -158,6 +158,26 @@ __device__ inline void rec_values0_r(int l, double x, double y, double z, double
         v[4] = kInvSqrt3 * (z * z - 0.5 * (x * x + y * y)) * R0;
     }
 }
+// f(m) for the 2l + 1 components m of a shell, each call with a constant m:
+// a loop to a run-time bound that reads a register array v[m] compiles to a
+// chain of conditional moves per element
+template <typename F>
+__device__ inline void for_components(int l, F f) {
+    if (l == 0) {
+        f(0);
+    } else if (l == 1) {
+        f(0);
+        f(1);
+        f(2);
+    } else {
+        f(0);
+        f(1);
+        f(2);
+        f(3);
+        f(4);
+    }
+}
+
 // phi_m and V_m = a phi_m + c . grad phi_m of the 2l+1 components of a shell
 // at offset (x, y, z) from its centre, from its radial sums R0 and R1 =
 // (dR/dr) / r: phi_m = f_m R0 and c . grad phi_m = R0 (c . grad f_m) +

@@ -236,10 +236,10 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
             const double x = px - rec[0], y = py - rec[1], z = pz - rec[2];
             rec_radial01(rec, x, y, z, cR0[jj], cR1[jj]);
             double v[5];
-            rec_values0_r(((int)rec[3]) & 3, x, y, z, cR0[jj], v);
+            const int l = ((int)rec[3]) & 3;
+            rec_values0_r(l, x, y, z, cR0[jj], v);
             const int c0 = s_shcol[k];
-            const int nc = 2 * (((int)rec[3]) & 3) + 1;
-            for (int m = 0; m < nc; ++m) s_phi[plane_at<PB>(c0 + m, p)] = v[m];
+            for_components(l, [&](int m) { s_phi[plane_at<PB>(c0 + m, p)] = v[m]; });
         }
     }
     for (int j = kk + NC * NSL; j < nsh; j += NSL) {
@@ -247,8 +247,7 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
         double v[5];
         rec_values0(s_rec + k * RS, px, py, pz, v);
         const int c0 = s_shcol[k];
-        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-        for (int m = 0; m < nc; ++m) s_phi[plane_at<PB>(c0 + m, p)] = v[m];
+        for_components(((int)s_rec[k * RS + 3]) & 3, [&](int m) { s_phi[plane_at<PB>(c0 + m, p)] = v[m]; });
     }
     for (int it = tid; it < (ldn - nao) * PB; it += 512) s_phi[nao * PB + it] = 0.0;
     __syncthreads();

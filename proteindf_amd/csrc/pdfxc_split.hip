@@ -175,13 +175,12 @@ __device__ __forceinline__ void k_rho_split_item(int item, const int* __restrict
         double v[5];
         rec_values0(s_rec + k * RS, px, py, pz, v);
         const int c0 = s_shcol[k];
-        const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-        for (int m = 0; m < nc; ++m) {
+        for_components(((int)s_rec[k * RS + 3]) & 3, [&](int m) {
             unsigned short h[S];
             split_bf16<S>(v[m], h);
 #pragma unroll
             for (int s = 0; s < S; ++s) s_a[(size_t)(s * PB + p) * RA + c0 + m] = h[s];
-        }
+        });
     }
     __syncthreads();
 
@@ -539,11 +538,10 @@ __device__ __forceinline__ void k_rho_bfp_item(int item, const int* __restrict__
         float m = 0.0f;
         auto put = [&](int k, const double* v) {
             const int c0 = s_shcol[k];
-            const int nc = 2 * (((int)s_rec[k * RS + 3]) & 3) + 1;
-            for (int q = 0; q < nc; ++q) {
+            for_components(((int)s_rec[k * RS + 3]) & 3, [&](int q) {
                 s_stage[p * RA + c0 + q] = v[q];
                 m = fmaxf(m, (float)fabs(v[q]));  // rounding is monotone: 2^E above the float bounds every |v|
-            }
+            });
         };
 #pragma unroll
         for (int jj = 0; jj < NC; ++jj) {
