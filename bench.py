@@ -171,7 +171,11 @@ def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune
     hip.h2d(dP, PA)
     if k_patches or k_group:
         ctx.set_tuning(split_k_patches=k_patches or 24, split_k_group=k_group or 4)
-    tune = ctx.autotune(dP, dK, dE, reps=2).as_dict() if autotune and not (k_patches or k_group) else None
+    if autotune and not (k_patches or k_group):
+        tune = ctx.autotune(dP, dK, dE, reps=2).as_dict()
+    else:  # the launch shapes in effect: the library's defaults, with the K item and group sizes given
+        tune = dict(rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=k_patches or 24,
+                    split_k_group=k_group or 4, persistent=2, ms=None)
     hip.memset(dK, PA.nbytes)
     for _ in range(warmup):
         ctx.build_rks_device(dP, dK, dE)
