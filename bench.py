@@ -1,7 +1,7 @@
 """Benchmark: one closed-shell B3LYP K_xc + E_xc build per step on one MI355X.
 
-Workload (default): alpha-helical H-(Gly)_10-OH, 73 atoms, 6-31G(d) with 5
-spherical d (638 AOs), 75 radial x 302 Lebedev Becke grid (~1.48M points),
+Workload (default): alpha-helical H-(Gly)_30-OH, 213 atoms, 6-31G(d) with 5
+spherical d (1878 AOs), 99 radial x 590 Lebedev Becke grid (~10.1M points),
 128-point grid blocks with per-block AO screening at 1e-12.  Each run times
 the FP64 build and the split-precision builds (bf16 slices on MFMA, FP32
 accumulation, FP64 elsewhere) on the same grid and P; value is the mode named
@@ -19,7 +19,8 @@ baseline runs the C oracle (the reference algorithm: all AOs at every point)
 on a bounded sample of grid points and scales to the whole grid.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--workload gly10|h2o8|gly20|gly30|h2o_batch]
-                    [--precision fp64|bf16x2|bf16x3|bf16x3k] [--modes fp64,bf16x2,bf16x3,bf16x3k]
+                    [--precision fp64|bf16x2|bf16x3|bf16x3k|bfp|f16k] [--modes fp64,bf16x2,...]
+                    [--no-autotune] [--split-k-patches N] [--split-k-group G] [--persistent MASK]
 """
 import argparse
 import ctypes
@@ -160,7 +161,8 @@ def measured_traffic(workload, mode, kernel, tuning):
     return None
 
 
-def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True, k_patches=0, k_group=0):
+def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune=True, k_patches=0, k_group=0,
+              persistent=-1):
     """Build a context with the given precision on the same grid, let the
     autotuner pick its launch shapes (set-up, untimed), time steps builds on
     HBM-resident P/K, return (ms per build, stats, E, K dense, tuning)."""
@@ -169,13 +171,15 @@ def time_mode(hip, info, grid, PA, precision, ao_screen, steps, warmup, autotune
     ctx.set_grid(grid)
     dP, dK, dE = hip.malloc(PA.nbytes), hip.malloc(PA.nbytes), hip.malloc(8)
     hip.h2d(dP, PA)
-    if k_patches or k_group:
-        ctx.set_tuning(split_k_patches=k_patches or 24, split_k_group=k_group or 4)
-    if autotune and not (k_patches or k_group):
+    fixed = bool(k_patches or k_group or persistent >= 0)
+    pmask = persistent if persistent >= 0 else 2
+    if fixed:
+        ctx.set_tuning(split_k_patches=k_patches or 24, split_k_group=k_group or 4, persistent=pmask)
+    if autotune and not fixed:
         tune = ctx.autotune(dP, dK, dE, reps=2).as_dict()
     else:  # the launch shapes in effect: the library's defaults, with the K item and group sizes given
         tune = dict(rho_shift=0, k_tiles=96, split_rho_shift=0, split_k_patches=k_patches or 24,
-                    split_k_group=k_group or 4, persistent=2, ms=None)
+                    split_k_group=k_group or 4, persistent=pmask, ms=None)
     hip.memset(dK, PA.nbytes)
     for _ in range(warmup):
         ctx.build_rks_device(dP, dK, dE)
@@ -371,6 +375,9 @@ def main():
     ap.add_argument("--split-k-group", type=int, default=0, metavar="G",
                     help="split K pass work units of G blocks (1, 2, 4 or 8; 128 G points) instead of the "
                          "autotuner's choice")
+    ap.add_argument("--persistent", type=int, default=-1, metavar="MASK",
+                    help="persistent launches (bit 0 the rho kernels, bit 1 the K kernels) instead of the "
+                         "autotuner's choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -391,6 +398,9 @@ def main():
         sys.exit(2)
     if args.split_k_group and args.split_k_group not in (1, 2, 4, 8):
         print(json.dumps({"error": "--split-k-group takes 1, 2, 4 or 8"}), flush=True)
+        sys.exit(2)
+    if not -1 <= args.persistent <= 3:
+        print(json.dumps({"error": "--persistent takes 0..3"}), flush=True)
         sys.exit(2)
     if args.workload == "h2o_batch":
         bench_batch(args)
@@ -420,7 +430,8 @@ def main():
     res = {}
     for m in modes:
         dt, st, E, K, tune = time_mode(hip, info, grid, PA, MODES[m], wl["ao_screen"], args.steps, args.warmup,
-                                       not args.no_autotune, args.split_k_patches, args.split_k_group)
+                                       not args.no_autotune, args.split_k_patches, args.split_k_group,
+                                       args.persistent)
         res[m] = dict(dt=dt, st=st, E=E, K=K, k_ms=list(st.last_kernel_ms), tune=tune)
         print(f"[bench] {m:7s} E_xc {E:.10f}  build {dt * 1e3:.2f} ms  kernels ms: "
               + ", ".join(f"{t:.2f}" for t in res[m]["k_ms"][:3]), file=sys.stderr, flush=True)

@@ -113,11 +113,14 @@ typedef struct pdfxc_stats {
  * neighbours in the point bisection) whose K contributions the split K pass
  * sums on the union of their AO columns before one scatter into K, i.e.
  * 128 x split_k_group points per K work unit (a group too wide for the
- * kernel's LDS is halved); persistent (bit mask, 0..3): bit 0 the rho-pass
- * kernels, bit 1 the K-pass kernels run as persistent launches, as many
- * workgroups as stay resident on the device, each taking work items (blocks,
- * K items; of every molecule in a batch) from one device counter until they
- * run out, instead of one workgroup per item.  Defaults 0, 96, 0, 24, 4, 2.
+ * kernel's LDS is halved); persistent (bit mask, 0..3): bit 0 the FP64
+ * rho-pass kernels of the widest classes (those that run one workgroup per CU:
+ * their persistent form loads the next item's set-up data while the current
+ * one runs) and the split rho-pass kernels, bit 1 the K-pass kernels run as
+ * persistent launches, as many workgroups as stay resident on the device,
+ * each taking work items (blocks, K items; of every molecule in a batch) from
+ * one device counter until they run out, instead of one workgroup per item.
+ * Defaults 0, 96, 0, 24, 4, 2.
  * ms: the build time measured for it (pdfxc_autotune), else 0. */
 typedef struct pdfxc_tuning {
     int rho_shift;

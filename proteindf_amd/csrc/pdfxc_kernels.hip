@@ -162,16 +162,30 @@ __device__ inline void rho_tiles(int wI, int WI, int wP, int T, int lane, const 
 // rho_out receives (rho, grad rho) per slot -- or adds them (update mode,
 // calcRho_GGA_part's accumulation, DfCalcGridX.cpp:1925-1961); the
 // functional runs over all points in k_xc_points.
+// The persistent kernel (AHEAD) loads the next item's set-up data while the
+// current item runs: its index during step 1, its descriptor during step 2,
+// its list entries at the start of step 3 and its first shell records at the
+// end, one load level per phase, each thread into registers (RhoAhead; the
+// LDS is the current item's).  A workgroup runs alone on its CU, so without
+// this the four dependent load levels of the set-up stall the whole CU.
 // ---------------------------------------------------------------------------
-template <int PB, int WI, int NIMAX, int G>
-__device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
+struct RhoAhead {
+    static constexpr int kRec = 4;  // record doubles per thread held (the rest load at the item's start)
+    int col, shcol, sord;           // entry tid of the column and shell lists
+    int shid[kRec];                 // shell of record double tid + 512 j
+    double rec[kRec];
+    double pt[3];                   // point tid (tid < PB)
+};
+
+template <int PB, int WI, int NIMAX, int G, bool AHEAD>
+__device__ __forceinline__ int k_rho_item(int item, const int* __restrict__ work, const BlockDesc* __restrict__ blocks,
                                              const double* __restrict__ pts, int64_t n_slots,
                                              const double* __restrict__ shrec, int RS,
                                              const int* __restrict__ bshells, const int* __restrict__ bshell_col,
                                              const int* __restrict__ bcols, const int* __restrict__ bsorder,
                                              const double* __restrict__ P, int N, double* __restrict__ rho_out,
                                              int accumulate,
-        double (*s_pt)[3], double* s_dyn) {
+        double (*s_pt)[3], double* s_dyn, RhoAhead& ah, bool have, int* queue, int n_items, int* s_next) {
     constexpr int SUBS = kBlockPts / PB;
     constexpr int NSL = 512 / PB;  // shell lanes per point
     constexpr int WP = 8 / WI;
@@ -181,7 +195,7 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
     const int sub = item % SUBS;
     const BlockDesc B = blocks[bidx];
     const int p0 = sub * PB;
-    if (p0 >= B.npts) return;  // uniform: no real point in this part of the block
+    if (p0 >= B.npts) return -1;  // uniform: no real point in this part of the block
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -195,22 +209,41 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
     int* s_cols = reinterpret_cast<int*>(s_rec + (size_t)nsh * RS);
     int* s_shcol = s_cols + ldn;
     int* s_sord = s_shcol + nsh;
-    for (int c = tid; c < ldn; c += 512) s_cols[c] = (c < nao) ? bcols[B.col_begin + c] : -1;
-    for (int c = tid; c < nsh; c += 512) {
+    const bool pre = AHEAD && have;  // the first entries of every list are in ah
+    if (pre) {
+        if (tid < ldn) s_cols[tid] = ah.col;
+        if (tid < nsh) {
+            s_shcol[tid] = ah.shcol;
+            s_sord[tid] = ah.sord;
+        }
+#pragma unroll
+        for (int j = 0; j < RhoAhead::kRec; ++j)
+            if (tid + 512 * j < nsh * RS) s_rec[tid + 512 * j] = ah.rec[j];
+        if (tid < PB) {
+            s_pt[tid][0] = ah.pt[0];
+            s_pt[tid][1] = ah.pt[1];
+            s_pt[tid][2] = ah.pt[2];
+        }
+    }
+    for (int c = tid + (pre ? 512 : 0); c < ldn; c += 512) s_cols[c] = (c < nao) ? bcols[B.col_begin + c] : -1;
+    for (int c = tid + (pre ? 512 : 0); c < nsh; c += 512) {
         s_shcol[c] = bshell_col[B.shell_begin + c];
         s_sord[c] = bsorder[B.shell_begin + c];
     }
-    for (int idx = tid; idx < nsh * RS; idx += 512) {
+    for (int idx = tid + (pre ? 512 * RhoAhead::kRec : 0); idx < nsh * RS; idx += 512) {
         const int k = idx / RS;
         s_rec[idx] = shrec[(int64_t)bshells[B.shell_begin + k] * RS + (idx - k * RS)];
     }
-    if (tid < PB) {
+    if (!pre && tid < PB) {
         const int64_t slot = B.pt_begin + p0 + tid;
         s_pt[tid][0] = pts[slot];
         s_pt[tid][1] = pts[n_slots + slot];
         s_pt[tid][2] = pts[2 * n_slots + slot];
     }
     __syncthreads();
+    // the next item's index, taken now and published at the end of step 1
+    int nxt = n_items;
+    if (AHEAD && tid == 512 - 64) nxt = atomicAdd(queue, 1);
 
     const int p = tid & (PB - 1);
     const int kk = tid / PB;
@@ -223,8 +256,9 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
     // 256 columns and the 16-point class of 1024 run one workgroup per CU
     // (2 waves per SIMD); the 16-point class of 512 columns runs two, and
     // keeping even 4 shells there takes it past 128 VGPRs.
+    // (14 in the persistent form, whose registers also hold the next item's set-up data)
     constexpr int kColsMax = NIMAX * WI * 16;
-    constexpr int NC = (PB == 64 && kColsMax == 256) ? 16 : (PB == 16 && kColsMax == 512) ? 0 : 8;
+    constexpr int NC = (PB == 64 && kColsMax == 256) ? (AHEAD ? 14 : 16) : (PB == 16 && kColsMax == 512) ? 0 : 8;
     double cR0[NC > 0 ? NC : 1], cR1[NC > 0 ? NC : 1];
 #pragma unroll
     for (int jj = 0; jj < NC; ++jj) {
@@ -250,7 +284,17 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
         for_components(((int)s_rec[k * RS + 3]) & 3, [&](int m) { s_phi[plane_at<PB>(c0 + m, p)] = v[m]; });
     }
     for (int it = tid; it < (ldn - nao) * PB; it += 512) s_phi[nao * PB + it] = 0.0;
+    if (AHEAD && tid == 512 - 64) *s_next = nxt;
     __syncthreads();
+    BlockDesc NB{};
+    int np0 = 0;
+    if constexpr (AHEAD) {
+        nxt = __builtin_amdgcn_readfirstlane(*s_next);
+        if (nxt < n_items) {
+            NB = blocks[work[nxt / SUBS]];
+            np0 = (nxt % SUBS) * PB;
+        }
+    }
 
     // 2. X^T = P_bb Phi^T, stored over the plane
     {
@@ -291,6 +335,25 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
     }
     __syncthreads();
 
+    // the next item's list entries (its descriptor has arrived during step 2)
+    const bool ahead = AHEAD && nxt < n_items && np0 < NB.npts;  // uniform
+    if (ahead) {
+        ah.col = (tid < NB.nao) ? bcols[NB.col_begin + tid] : -1;
+        ah.shcol = (tid < NB.nshell) ? bshell_col[NB.shell_begin + tid] : 0;
+        ah.sord = (tid < NB.nshell) ? bsorder[NB.shell_begin + tid] : 0;
+#pragma unroll
+        for (int j = 0; j < RhoAhead::kRec; ++j) {
+            const int idx = tid + 512 * j;
+            ah.shid[j] = (idx < NB.nshell * RS) ? bshells[NB.shell_begin + idx / RS] : 0;
+        }
+        if (tid < PB) {
+            const int64_t slot = NB.pt_begin + np0 + tid;
+            ah.pt[0] = pts[slot];
+            ah.pt[1] = pts[n_slots + slot];
+            ah.pt[2] = pts[2 * n_slots + slot];
+        }
+    }
+
     // 3. rho and grad rho: X dotted with phi and grad phi, per shell from its
     // radial sums (kept from step 1 for the first NC shells of a thread)
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
@@ -330,6 +393,13 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
         const int l = shell_x(k, X);
         shell_dots(l, x, y, z, R0, R1, X, r0, r1, r2, r3);
     }
+    if (ahead) {  // the next item's first shell records (their shells have arrived during step 3)
+#pragma unroll
+        for (int j = 0; j < RhoAhead::kRec; ++j) {
+            const int idx = tid + 512 * j;
+            if (idx < NB.nshell * RS) ah.rec[j] = shrec[(int64_t)ah.shid[j] * RS + (idx - (idx / RS) * RS)];
+        }
+    }
     __syncthreads();  // X no longer read: the plane holds the partial sums
     double* s_part = s_phi;  // [4][NSL][PB]
     s_part[(0 * NSL + kk) * PB + p] = r0;
@@ -346,6 +416,7 @@ __device__ __forceinline__ void k_rho_item(int item, const int* __restrict__ wor
         const double v = (q == 0) ? s : 2.0 * s;
         rho_out[4 * slot + q] = accumulate ? rho_out[4 * slot + q] + v : v;  // update mode adds rho[dP]
     }
+    return ahead ? nxt : (AHEAD && nxt >= n_items ? n_items : -1);
 }
 
 template <int PB, int WI, int NIMAX, int G, bool PERSIST>
@@ -358,15 +429,25 @@ __global__ __launch_bounds__(512) void k_rho(const int* __restrict__ work, const
                                              int accumulate, int* __restrict__ queue, int n_items) {
     __shared__ double s_pt[PB][3];
     extern __shared__ double s_dyn[];
+    RhoAhead ah;
     if constexpr (PERSIST) {
-        __shared__ int s_item;
-        for (int iter = 0;; ++iter) {
-            const int item = next_item(queue, n_items, iter, &s_item);
-            if (item >= n_items) break;
-            k_rho_item<PB, WI, NIMAX, G>(item, work, blocks, pts, n_slots, shrec, RS, bshells, bshell_col, bcols, bsorder, P, N, rho_out, accumulate, s_pt, s_dyn);
+        // an item returns the next one with its set-up data in ah, n_items when
+        // the queue is empty, or -1 when it took no next item (an empty part of
+        // a block, or the next one empty): then the queue is asked here
+        __shared__ int s_item, s_next;
+        int item = next_item(queue, n_items, 0, &s_item);
+        bool have = false;
+        while (item < n_items) {
+            const int nxt = k_rho_item<PB, WI, NIMAX, G, true>(item, work, blocks, pts, n_slots, shrec, RS, bshells,
+                                                               bshell_col, bcols, bsorder, P, N, rho_out, accumulate,
+                                                               s_pt, s_dyn, ah, have, queue, n_items, &s_next);
+            have = nxt >= 0 && nxt < n_items;
+            item = nxt >= 0 ? nxt : next_item(queue, n_items, 1, &s_item);
         }
     } else {
-        k_rho_item<PB, WI, NIMAX, G>((int)blockIdx.x, work, blocks, pts, n_slots, shrec, RS, bshells, bshell_col, bcols, bsorder, P, N, rho_out, accumulate, s_pt, s_dyn);
+        k_rho_item<PB, WI, NIMAX, G, false>((int)blockIdx.x, work, blocks, pts, n_slots, shrec, RS, bshells, bshell_col,
+                                            bcols, bsorder, P, N, rho_out, accumulate, s_pt, s_dyn, ah, false, nullptr,
+                                            n_items, nullptr);
     }
 }
 
@@ -850,18 +931,27 @@ static int check_launch(const char* what) {
 template <int PB, int WI, int NIMAX, int G>
 static int rho_one(Context& c, const LaunchRange& r) {
     const int items = r.count * (kBlockPts / PB);
-    unsigned grid = 0;
-    int* q = launch_queue(c, kQueueRho, items, resident_per_cu(&k_rho<PB, WI, NIMAX, G, true>, 512, r.lds), &grid);
-    if (q)
-        hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G, true>), dim3(grid), dim3(512), r.lds, c.stream,
-                           c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
-                           c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
-                           c.n_aos, c.d_rho, c.rho_accumulate, q, items);
-    else
-        hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G, false>), dim3(grid), dim3(512), r.lds, c.stream,
-                           c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
-                           c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
-                           c.n_aos, c.d_rho, c.rho_accumulate, q, items);
+    unsigned grid = (unsigned)items;
+    // the persistent form (the next item's set-up loaded during the current
+    // one) only for the classes whose widest plane takes more than half of the
+    // LDS: they run one workgroup per CU either way, while the others run two
+    // as plain launches and would lose one to the persistent form's registers
+    constexpr bool kAlone = (size_t)PB * NIMAX * WI * 16 * sizeof(double) > 80 * 1024;
+    if constexpr (kAlone) {
+        int* q = launch_queue(c, kQueueRho, items, resident_per_cu(&k_rho<PB, WI, NIMAX, G, true>, 512, r.lds),
+                              &grid);
+        if (q) {
+            hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G, true>), dim3(grid), dim3(512), r.lds, c.stream,
+                               c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
+                               c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P,
+                               c.n_aos, c.d_rho, c.rho_accumulate, q, items);
+            return check_launch("k_rho");
+        }
+    }
+    hipLaunchKernelGGL((k_rho<PB, WI, NIMAX, G, false>), dim3(grid), dim3(512), r.lds, c.stream,
+                       c.d_rho_work + r.offset, c.d_blocks, c.d_pts, c.n_slots, c.d_shrec, c.rec_stride,
+                       c.d_block_shells, c.d_block_shell_col, c.d_block_cols, c.d_block_sorder, c.d_P, c.n_aos,
+                       c.d_rho, c.rho_accumulate, nullptr, items);
     return check_launch("k_rho");
 }
 
@@ -871,10 +961,11 @@ static int rho_one(Context& c, const LaunchRange& r) {
 int rho_points_per_workgroup(int cls) { return (cls == 0 || cls == 4) ? 64 : cls == 1 ? 32 : 16; }
 
 int rho64_max_dyn_lds(size_t* out) {
-    hipFuncAttributes a;
-    const hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_rho<64, 8, 2, 4, false>));
+    hipFuncAttributes a, b;  // the plain and the persistent form (a few bytes more of static LDS)
+    hipError_t e = hipFuncGetAttributes(&a, reinterpret_cast<const void*>(&k_rho<64, 8, 2, 4, false>));
+    if (e == hipSuccess) e = hipFuncGetAttributes(&b, reinterpret_cast<const void*>(&k_rho<64, 8, 2, 4, true>));
     if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes(k_rho)");
-    *out = (size_t)160 * 1024 - a.sharedSizeBytes;
+    *out = (size_t)160 * 1024 - std::max(a.sharedSizeBytes, b.sharedSizeBytes);
     return PDFXC_OK;
 }
 

@@ -60,6 +60,13 @@ def test_split_k_group_out_of_range_is_refused():
     assert "1, 2, 4 or 8" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
 
 
+def test_persistent_mask_out_of_range_is_refused():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--persistent", "4"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "0..3" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
 def test_default_workload_is_the_headline_configuration():
     sys.path.insert(0, str(ROOT))
     import bench
