@@ -146,6 +146,86 @@ __device__ inline void rho_tiles(int wI, int WI, int wP, int T, int lane, const 
                 s_phi[plane_at<PB>((wI + WI * i) * 16 + 4 * r + lq, (wP * RT + rt) * 16 + li)] = acc[i][rt][r];
 }
 
+// X^T tiles of one wave of the 64-point, 8-wave form, balanced over the four
+// SIMDs (waves w and w + 4 share SIMD w mod 4).  Of the T row tiles the first
+// 4q (q = T / 4) go whole to SIMD I mod 4 (its two waves alternate rows: row
+// s + 4 (2 m + h) to wave s + 4 h), and each of the last r = T mod 4 rows is
+// split by point tile, SIMD s taking point tile s in the wave of its two that
+// holds fewer whole rows; so every SIMD runs T row x point tiles, where whole
+// rows alone give one SIMD ceil(T / 4) x 4.  NF whole rows, NR split rows.
+template <int NF, int NR, int G>
+__device__ inline void rho_tiles_bal(int s, int h, int q, int T, int lane, const int* s_cols, double* s_phi,
+                                     const double* __restrict__ P, int N) {
+    constexpr int PB = 64, RT = 4, NA = NF + NR;
+    const int li = lane & 15, lq = lane >> 4;
+    v4d accF[NF > 0 ? NF : 1][RT], accR[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int i = 0; i < NF; ++i)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) accF[i][rt] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < NR; ++j) accR[j] = v4d{0.0, 0.0, 0.0, 0.0};
+    auto row = [&](int i) { return i < NF ? s + 4 * (2 * i + h) : 4 * q + (i - NF); };
+    int gi[NA > 0 ? NA : 1];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) gi[i] = max(s_cols[row(i) * 16 + li], 0);
+    const int nsteps = T * 4;
+    const int ngroups = (nsteps + G - 1) / G;
+    double aC[G][NA > 0 ? NA : 1], aN[G][NA > 0 ? NA : 1];
+    auto load_a = [&](int g, double (&a)[G][NA > 0 ? NA : 1]) {
+#pragma unroll
+        for (int d = 0; d < G; ++d) {
+            const int step = min(g * G + d, nsteps - 1);
+            const double* prow = P + (int64_t)max(s_cols[4 * step + lq], 0) * N;
+#pragma unroll
+            for (int i = 0; i < NA; ++i) a[d][i] = prow[gi[i]];
+        }
+    };
+    if constexpr (NA > 0) {
+        load_a(0, aC);
+        for (int g = 0; g < ngroups; ++g) {
+            if (g + 1 < ngroups) load_a(g + 1, aN);
+#pragma unroll
+            for (int d = 0; d < G; ++d) {
+                const int step = g * G + d;
+                if (step < nsteps) {
+                    const int c = 4 * step + lq;
+                    if constexpr (NF > 0) {
+                        double b[RT];
+#pragma unroll
+                        for (int rt = 0; rt < RT; ++rt) b[rt] = s_phi[plane_at<PB>(c, rt * 16 + li)];
+#pragma unroll
+                        for (int i = 0; i < NF; ++i)
+#pragma unroll
+                            for (int rt = 0; rt < RT; ++rt) accF[i][rt] = mfma16x16x4(aC[d][i], b[rt], accF[i][rt]);
+                    }
+                    if constexpr (NR > 0) {
+                        const double bs = s_phi[plane_at<PB>(c, s * 16 + li)];
+#pragma unroll
+                        for (int j = 0; j < NR; ++j) accR[j] = mfma16x16x4(aC[d][NF + j], bs, accR[j]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < G; ++d)
+#pragma unroll
+                for (int i = 0; i < NA; ++i) aC[d][i] = aN[d][i];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NF; ++i)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                s_phi[plane_at<PB>(row(i) * 16 + 4 * r + lq, rt * 16 + li)] = accF[i][rt][r];
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s_phi[plane_at<PB>((4 * q + j) * 16 + 4 * r + lq, s * 16 + li)] = accR[j][r];
+}
+
 // ---------------------------------------------------------------------------
 // rho pass for PB points of a block (512 threads, 8 waves):
 //   1. Phi of all ldn columns at the PB points into an LDS plane; thread
@@ -256,9 +336,9 @@ __device__ __forceinline__ int k_rho_item(int item, const int* __restrict__ work
     // 256 columns and the 16-point class of 1024 run one workgroup per CU
     // (2 waves per SIMD); the 16-point class of 512 columns runs two, and
     // keeping even 4 shells there takes it past 128 VGPRs.
-    // (14 in the persistent form, whose registers also hold the next item's set-up data)
+    // (12 in the persistent form, whose registers also hold the next item's set-up data)
     constexpr int kColsMax = NIMAX * WI * 16;
-    constexpr int NC = (PB == 64 && kColsMax == 256) ? (AHEAD ? 14 : 16) : (PB == 16 && kColsMax == 512) ? 0 : 8;
+    constexpr int NC = (PB == 64 && kColsMax == 256) ? (AHEAD ? 12 : 16) : (PB == 16 && kColsMax == 512) ? 0 : 8;
     double cR0[NC > 0 ? NC : 1], cR1[NC > 0 ? NC : 1];
 #pragma unroll
     for (int jj = 0; jj < NC; ++jj) {
@@ -297,7 +377,23 @@ __device__ __forceinline__ int k_rho_item(int item, const int* __restrict__ work
     }
 
     // 2. X^T = P_bb Phi^T, stored over the plane
-    {
+    if constexpr (PB == 64 && WI == 8 && NIMAX == 2) {  // T <= 16: balanced over the SIMDs
+        const int sd = wave & 3, h = wave >> 2, q = T >> 2;
+        const int nf = (q > h) ? (q - h + 1) / 2 : 0;
+        const int nr = (h == (q & 1)) ? (T & 3) : 0;
+        switch (nf * 4 + nr) {
+            case 0: __syncthreads(); break;  // no tiles: only the barrier before the store
+            case 1: rho_tiles_bal<0, 1, 4>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            case 2: rho_tiles_bal<0, 2, 4>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            case 3: rho_tiles_bal<0, 3, 2>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            case 4: rho_tiles_bal<1, 0, 4>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            case 5: rho_tiles_bal<1, 1, 4>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            case 6: rho_tiles_bal<1, 2, 2>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            case 7: rho_tiles_bal<1, 3, 2>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            case 8: rho_tiles_bal<2, 0, 4>(sd, h, q, T, lane, s_cols, s_phi, Pb, ldp); break;
+            default: __syncthreads(); break;  // unreachable: T <= 16 gives nf = 2 only with nr = 0
+        }
+    } else {
         const int wI = wave % WI, wP = wave / WI;
         const int nI = (T > wI) ? (T - wI + WI - 1) / WI : 0;
         switch (nI) {
