@@ -101,35 +101,43 @@ __device__ inline void rho_tiles(int wI, int WI, int wP, int T, int lane, const 
     int gi[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) gi[i] = max(s_cols[(wI + WI * i) * 16 + li], 0);
-    const int nsteps = T * 4;
-    const int ngroups = (nsteps + G - 1) / G;
+    const int ngroups = T * 4 / G;  // G (2 or 4) divides the 4T steps: every group is whole
     double aC[G][NI], aN[G][NI];
     auto load_a = [&](int g, double (&a)[G][NI]) {
 #pragma unroll
         for (int d = 0; d < G; ++d) {
-            const int step = min(g * G + d, nsteps - 1);
-            const double* prow = P + (int64_t)max(s_cols[4 * step + lq], 0) * N;
+            const double* prow = P + (int64_t)max(s_cols[4 * (g * G + d) + lq], 0) * N;
 #pragma unroll
             for (int i = 0; i < NI; ++i) a[d][i] = prow[gi[i]];
         }
     };
-    load_a(0, aC);
+    // the first group through aN as well, so that aC never holds a pending
+    // load inside the loop (else its first MFMA waits for every load)
+    load_a(0, aN);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int d = 0; d < G; ++d)
+#pragma unroll
+        for (int i = 0; i < NI; ++i) aC[d][i] = aN[d][i];
     for (int g = 0; g < ngroups; ++g) {
         if (g + 1 < ngroups) load_a(g + 1, aN);
 #pragma unroll
         for (int d = 0; d < G; ++d) {
-            const int step = g * G + d;
-            if (step < nsteps) {
-                const int c = 4 * step + lq;
-                double b[RT];
+            const int c = 4 * (g * G + d) + lq;
+            double b[RT];
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) b[rt] = s_phi[plane_at<PB>(c, (wP * RT + rt) * 16 + li)];
+            for (int rt = 0; rt < RT; ++rt) b[rt] = s_phi[plane_at<PB>(c, (wP * RT + rt) * 16 + li)];
 #pragma unroll
-                for (int i = 0; i < NI; ++i)
+            for (int i = 0; i < NI; ++i)
 #pragma unroll
-                    for (int rt = 0; rt < RT; ++rt) acc[i][rt] = mfma16x16x4(aC[d][i], b[rt], acc[i][rt]);
-            }
+                for (int rt = 0; rt < RT; ++rt) acc[i][rt] = mfma16x16x4(aC[d][i], b[rt], acc[i][rt]);
         }
+        // the next group's operands are taken over here, after this group's
+        // MFMAs: without the scheduling barrier the compiler moves each copy
+        // up to the last use of its aC element, and the copy then waits for
+        // the loads just issued for the next group, before this group's
+        // first MFMA
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int d = 0; d < G; ++d)
 #pragma unroll
@@ -169,27 +177,29 @@ __device__ inline void rho_tiles_bal(int s, int h, int q, int T, int lane, const
     int gi[NA > 0 ? NA : 1];
 #pragma unroll
     for (int i = 0; i < NA; ++i) gi[i] = max(s_cols[row(i) * 16 + li], 0);
-    const int nsteps = T * 4;
-    const int ngroups = (nsteps + G - 1) / G;
+    const int ngroups = T * 4 / G;  // G (2 or 4) divides the 4T steps: every group is whole
     double aC[G][NA > 0 ? NA : 1], aN[G][NA > 0 ? NA : 1];
     auto load_a = [&](int g, double (&a)[G][NA > 0 ? NA : 1]) {
 #pragma unroll
         for (int d = 0; d < G; ++d) {
-            const int step = min(g * G + d, nsteps - 1);
-            const double* prow = P + (int64_t)max(s_cols[4 * step + lq], 0) * N;
+            const double* prow = P + (int64_t)max(s_cols[4 * (g * G + d) + lq], 0) * N;
 #pragma unroll
             for (int i = 0; i < NA; ++i) a[d][i] = prow[gi[i]];
         }
     };
     if constexpr (NA > 0) {
-        load_a(0, aC);
+        load_a(0, aN);  // as in rho_tiles
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int d = 0; d < G; ++d)
+#pragma unroll
+            for (int i = 0; i < NA; ++i) aC[d][i] = aN[d][i];
         for (int g = 0; g < ngroups; ++g) {
             if (g + 1 < ngroups) load_a(g + 1, aN);
 #pragma unroll
             for (int d = 0; d < G; ++d) {
-                const int step = g * G + d;
-                if (step < nsteps) {
-                    const int c = 4 * step + lq;
+                {
+                    const int c = 4 * (g * G + d) + lq;
                     if constexpr (NF > 0) {
                         double b[RT];
 #pragma unroll
@@ -206,6 +216,7 @@ __device__ inline void rho_tiles_bal(int s, int h, int q, int T, int lane, const
                     }
                 }
             }
+            __builtin_amdgcn_sched_barrier(0);  // as in rho_tiles
 #pragma unroll
             for (int d = 0; d < G; ++d)
 #pragma unroll
