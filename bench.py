@@ -138,9 +138,15 @@ E_TARGET, K_TARGET = 1e-6, 1e-7   # split-precision accuracy target vs the FP64 
 _KIB = 1024.0
 TRAFFIC_MEASURED = [
     # (workload, modes, kernel, tuning fields, bytes per build, source)
-    ("gly30", ("bf16x3k", "f16k", "fp64"), "k_rho", {"rho_shift": 0},
-     (631813.9 + 514037.4 + 79570.8) * _KIB * 2.00 + (235992.0 + 60472.0 + 19658.6) * _KIB * 1.00,
-     "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches; profiles/gly30_kernels.md"),
+    # k_rho: the persistent form of the widest class (persistent bit 0) reads less than the plain one
+    *[("gly30", ("bf16x3k", "f16k", "fp64"), "k_rho", {"rho_shift": 0, "persistent": pm},
+       (510766.1 + 519605.6 + 79248.1) * _KIB * 2.00 + (239687.4 + 60472.0 + 19658.0) * _KIB * 1.00,
+       "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches, persistent widest class; "
+       "profiles/gly30_kernels.md") for pm in (1, 3)],
+    *[("gly30", ("bf16x3k", "f16k", "fp64"), "k_rho", {"rho_shift": 0, "persistent": pm},
+       (631813.9 + 514037.4 + 79570.8) * _KIB * 2.00 + (235992.0 + 60472.0 + 19658.6) * _KIB * 1.00,
+       "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, 3 class launches, plain; profiles/gly30_kernels.md")
+      for pm in (0, 2)],
     ("gly30", ("bf16x3k", "bfp"), "k_kmat_split", {"split_k_patches": 24, "split_k_group": 2},
      575575.2 * _KIB * 2.00 + 14948334.8 * _KIB * 1.00,
      "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, groups of 2 blocks; profiles/gly30_kernels.md"),

@@ -51,6 +51,9 @@ def test_traffic_is_reported_only_for_the_profiled_launch_shapes():
     assert bench.measured_traffic("gly30", "bf16x3k", "k_kmat_split", dict(tune, split_k_group=8)) is None
     assert bench.measured_traffic("gly30", "bf16x3k", "k_rho", dict(tune, rho_shift=1)) is None
     assert bench.measured_traffic("gly10", "bf16x3k", "k_rho", tune) is None
+    plain = bench.measured_traffic("gly30", "bf16x3k", "k_rho", tune)
+    persistent = bench.measured_traffic("gly30", "bf16x3k", "k_rho", dict(tune, persistent=3))
+    assert plain and persistent and persistent[0] < plain[0]
 
 
 def test_split_k_group_out_of_range_is_refused():
